@@ -1,0 +1,220 @@
+/*
+ * sphx — C ABI of the MI355X (gfx950) SPH step library (libsphx.so).
+ *
+ * Drop-in boundary for the per-particle SPH step of hassanfv/SPH_2's
+ * GPU_sph / Evrard_collapse drivers.  Reference paths below are relative to
+ * 11_Dec_2022/ of that repository:
+ *   LIBSX  = GPU_sph/IC/libsx2_2t.py            (FP64 Python/Numba library)
+ *   LIBSX3 = GPU_sph/work_on_smoothing/libsx2_2t.py
+ *   SHEAR  = GPU_sph/work_on_smoothing/shear.py
+ *   PHOTOPY= GPU_sph/Cooling_v1/photolibs3.py
+ *   LIBV2  = GPU_sph/Important_Test_Simulations/Evrard_collapse/myCppSPHLibs_v2.h
+ *   DRV2   = GPU_sph/Important_Test_Simulations/Evrard_collapse/hfv_sph_gpu_v2_evrard.cu
+ *
+ * Conventions
+ *   - Every array pointer is a DEVICE pointer (hipMalloc'd or a torch-ROCm
+ *     tensor's data_ptr); positions and velocities are C-contiguous (n, 3)
+ *     float64 like the reference's numpy arrays; per-particle scalars are (n,)
+ *     float64; counts are int32.
+ *   - `stream` is a hipStream_t (NULL = legacy default stream).  Calls are
+ *     asynchronous unless stated; results are valid after the stream syncs.
+ *   - Status: 0 ok; > 0 completed with capped iterations (see the counters);
+ *     < 0 error (bad argument, out of memory, HIP failure).  Nothing is
+ *     printed; sphx_last_error() returns a message for the calling thread.
+ *   - Precision mode: SPHX_MIXED runs Morton-bucketed pair loops with FP32
+ *     pair arithmetic and compensated FP32 accumulation, FP64 neighbour
+ *     decisions, integrator and cooling solve.  SPHX_FP64 runs the plain
+ *     baseline: one thread per particle, brute-force FP64 pair loops in the
+ *     style of LIBV2, no LDS and no cells.
+ *   - Rows [row_beg, row_end) select the target particles, as the reference's
+ *     `*_mpi(nbeg, nend, ...)` variants do (e.g. LIBSX:58-95); outputs hold
+ *     row_end - row_beg rows.
+ */
+#ifndef SPHX_H
+#define SPHX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPHX_OK 0
+#define SPHX_CAPPED 1
+#define SPHX_ERR_ARG (-1)
+#define SPHX_ERR_NOMEM (-2)
+#define SPHX_ERR_HIP (-3)
+#define SPHX_ERR_STATE (-4)
+
+#define SPHX_MIXED 0
+#define SPHX_FP64 1
+
+/* Physics constants of the step (augsphx6.5x.py:472-484; DRV2:283-300). */
+typedef struct sphx_params {
+    double gamma;        /* adiabatic index, 5/3 */
+    double alpha, beta;  /* artificial-viscosity coefficients (1, 2) */
+    double eta;          /* M83 viscosity softening, 0.1 */
+    double G;            /* gravitational constant, 1 */
+    int32_t nngb;        /* target neighbour count, 64 */
+    int32_t ndown, nup;  /* accepted band, 59..69 (DRV2:283-285) */
+    int32_t max_iter;    /* h-iteration cap, 100 (LIBV2:55) */
+    int32_t av_form;     /* 0 Monaghan-Gingold 1983, 1 Gadget signal velocity */
+    int32_t balsara;     /* 1: Balsara switch on Pi_ij (LIBV2:400-405) */
+    int32_t eps_from_h;  /* 1: gravitational softening eps_i = h_i */
+    int32_t h_predictor; /* 1: Gadget-2 eq. 31 start of the h iteration (LIBSX3:960) */
+} sphx_params;
+
+/* Tabulated cooling: Lambda(T) on increasing T[ntab] (K) in erg cm^3 s^-1,
+ * linear in T inside a bin; heating `heat` erg/s per hydrogen atom; code ->
+ * cgs unit factors.  T and L are HOST pointers, copied by the callee. */
+typedef struct sphx_cooling {
+    double unit_u_cgs, unit_rho_cgs, unit_time_s;
+    double XH, mu, gamma, heat, tol;
+    int32_t max_iter, max_bracket, ntab, pad;
+    const double *T;
+    const double *L;
+} sphx_cooling;
+
+typedef struct sphx_counters {
+    int64_t n;              /* particles resident */
+    int64_t steps;          /* steps taken since load */
+    int64_t h_iters_max;    /* largest h-iteration count in the last step */
+    int64_t h_iters_sum;    /* total h-iteration counts in the last step */
+    int64_t h_capped;       /* particles whose h iteration did not converge */
+    int64_t cool_capped;    /* particles whose cooling solve hit its cap */
+    int64_t cand_buckets;   /* candidate bucket pairs in the last neighbour build */
+    int64_t buckets;        /* number of 64-particle buckets */
+} sphx_counters;
+
+typedef struct sphx_diag {
+    double ke, ut, pe;      /* kinetic, thermal, potential energy */
+    double px, py, pz;      /* linear momentum */
+    double lx, ly, lz;      /* angular momentum about the origin */
+    double mass;
+} sphx_diag;
+
+typedef struct sphx_ctx sphx_ctx;
+typedef struct sphx_plan sphx_plan;
+typedef struct sphx_batch sphx_batch;
+
+const char *sphx_last_error(void);
+const char *sphx_version(void);
+void sphx_default_params(sphx_params *p);
+
+/* ---------------- stateless drop-ins (scratch owned by a context) ---------- */
+int sphx_ctx_create(int device, sphx_ctx **out);
+int sphx_ctx_destroy(sphx_ctx *ctx);
+
+/* getDensity(pos, m, h) -> rho (LIBSX:17-51; LIBV2:69-114); nngb (nullable)
+ * receives the neighbour count under r_ij <= h_i + h_j. */
+int sphx_getDensity(sphx_ctx *ctx, int mode, int64_t n, const double *pos, const double *m,
+                    const double *h, int64_t row_beg, int64_t row_end, double *rho,
+                    int32_t *nngb, void *stream);
+
+/* Adiabatic EOS: getPressure (LIBSX:100-105) and getCsound (LIBV2:783-789). */
+int sphx_eos(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, double gamma,
+             double *P, double *c, void *stream);
+
+/* smoothing_fast_new_mpi3(nbeg, nend, pos, h, Nngb_previous, divV, dt)
+ * (LIBSX3:946-1010; LIBV7:255-330; LIBV2:7-64).  k_prev/divV nullable: without
+ * them the iteration starts at 2*h_prev.  Outputs h and the neighbour count k
+ * (self included, r_ij <= 2h_i). */
+int sphx_smoothing(sphx_ctx *ctx, int mode, int64_t n, const double *pos, const double *h_prev,
+                   const int32_t *k_prev, const double *divV, double dt, const sphx_params *p,
+                   int64_t row_beg, int64_t row_end, double *h_out, int32_t *k_out,
+                   int32_t *iters_out, void *stream);
+
+/* div_curlVel(pos, v, rho, m, h) -> (divV signed, |curlV|) (SHEAR:8-69; LIBV2:247-322). */
+int sphx_div_curlVel(sphx_ctx *ctx, int mode, int64_t n, const double *pos, const double *v,
+                     const double *rho, const double *m, const double *h, int64_t row_beg,
+                     int64_t row_end, double *divV, double *curlV, void *stream);
+
+/* getAcc_sph and get_dU in one pair pass (LIBSX:110-186 and 363-430;
+ * SHEAR:151-235 and 334-406 with Balsara; LIBV2:327-417 and 634-719).
+ * divV/curlV are read only when p->balsara; acc (rows x 3) and dudt nullable. */
+int sphx_getAcc_sph(sphx_ctx *ctx, int mode, int64_t n, const double *pos, const double *v,
+                    const double *rho, const double *P, const double *c, const double *h,
+                    const double *m, const double *divV, const double *curlV,
+                    const sphx_params *p, int64_t row_beg, int64_t row_end, double *acc,
+                    double *dudt, void *stream);
+
+/* getAcc_g_smth(pos, mass, G, epsilon) (LIBSX:633-671; LIBV2:119-168);
+ * pot (nullable) receives phi_i = -G sum_j m_j phi_ij (the pair term of getPE,
+ * LIBSX:593-629). */
+int sphx_getAcc_g_smth(sphx_ctx *ctx, int mode, int64_t n, const double *pos, const double *m,
+                       const double *eps, double G, int64_t row_beg, int64_t row_end,
+                       double *acc, double *pot, void *stream);
+
+/* Neighbour lists under r_ij <= h_i + h_j for rows [row_beg,row_end), ascending
+ * j per row (the gate of LIBV2:89).  offsets (rows+1, int64) always written;
+ * idx (int32) written when capacity suffices; *total receives the pair count.
+ * Synchronous. */
+int sphx_neighbors(sphx_ctx *ctx, int64_t n, const double *pos, const double *h,
+                   int64_t row_beg, int64_t row_end, int64_t *offsets, int32_t *idx,
+                   int64_t capacity, int64_t *total, void *stream);
+
+/* Implicit per-particle cooling solve over the tabulated Lambda(T)
+ * (DoCooling_h, PHOTOPY:489-543).  iters nullable.  Returns SPHX_CAPPED when
+ * any particle hit max_iter. Synchronous. */
+int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, double dt,
+              const sphx_cooling *cool, double *u_out, int32_t *iters, void *stream);
+
+/* getKE (LIBSX:579-588), thermal energy sum m u, momentum and angular
+ * momentum; pe only when pos/eps given (getPE, LIBSX:593-629).  Synchronous. */
+int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, const double *v,
+                     const double *m, const double *u, const double *eps, double G,
+                     sphx_diag *out, void *stream);
+
+/* Morton keys (21 bits per axis, 63-bit key) over the bounding box of pos,
+ * then a stable LSD radix sort; keys_out[i] and perm_out[i] = original index of
+ * the i-th smallest key.  Synchronous. */
+int sphx_morton_sort(sphx_ctx *ctx, int64_t n, const double *pos, uint64_t *keys_out,
+                     int32_t *perm_out, void *stream);
+
+/* ---------------- resident plan: the whole step on device ------------------ */
+/* State stays on the device in Morton order between steps.  The step order is
+ * DRV2:407-482: half kick, drift, h iteration, density + EOS, gravity,
+ * div/curl, hydro acceleration + du/dt (same velocity), half kick, u
+ * trapezoid, optional implicit cooling. */
+int sphx_plan_create(int device, int mode, const sphx_params *p, const sphx_cooling *cool,
+                     int64_t capacity, sphx_plan **out);
+/* pos, vel (n,3), u, m, h_guess (n,), eps (nullable; used when !eps_from_h). */
+int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, const double *vel,
+                   const double *u, const double *m, const double *h, const double *eps,
+                   void *stream);
+/* Forces at t = 0 (DRV2:311-392 without its Euler u update). */
+int sphx_plan_init(sphx_plan *plan, void *stream);
+/* n_steps KDK steps at fixed dt (cooling on when the plan has a table and
+ * use_cooling != 0). */
+int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32_t use_cooling,
+                   void *stream);
+/* Copy state out in the original particle order; any pointer may be NULL.
+ * acc = total acceleration (n,3); acc_g = gravity part (n,3). */
+int sphx_plan_store(sphx_plan *plan, double *pos, double *vel, double *u, double *h,
+                    double *rho, double *acc, double *acc_g, double *dudt, int32_t *k,
+                    void *stream);
+int sphx_plan_query(sphx_plan *plan, sphx_counters *out);
+/* Energies and momenta of the resident state (pe needs the potential pass). */
+int sphx_plan_diagnostics(sphx_plan *plan, int32_t with_pe, sphx_diag *out, void *stream);
+int sphx_plan_destroy(sphx_plan *plan);
+
+/* Gravity launch shape of a plan or context: ipl particles per lane (1, 2, 4),
+ * j-tile tj (256, 512, 1024), j split js >= 1 (more workgroups for small N).
+ * No reference counterpart: the reference launches acc_g with a fixed
+ * <<<ceil(N/256), 256>>> (DRV2:280-281); the autotuner sets this. */
+int sphx_plan_set_gravity_config(sphx_plan *plan, int32_t ipl, int32_t tj, int32_t js);
+int sphx_ctx_set_gravity_config(sphx_ctx *ctx, int32_t ipl, int32_t tj, int32_t js);
+
+/* Per-phase HIP-event timing of sphx_plan_step (the reference times its
+ * passes with std::chrono around synchronised launches,
+ * GPU_sph_Type/Evrard_collapse/individual_dt_not_completed/AWS/main.cu:548-602).
+ * Phases: 0 kick+drift+sort, 1 h iteration, 2 density+EOS, 3 gravity,
+ * 4 div/curl, 5 hydro+du/dt, 6 kick+u+cooling.  set_timing resets the sums;
+ * timing returns summed milliseconds and step counts per phase. */
+int sphx_plan_set_timing(sphx_plan *plan, int32_t on);
+int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int32_t nphase);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHX_H */

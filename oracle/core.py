@@ -1,0 +1,369 @@
+"""ctypes bindings of the FP64 C oracle and the oracle's composition of a full step.
+
+Each wrapper names the reference function it restates (paths relative to
+11_Dec_2022/ of hassanfv/SPH_2; aliases as in sph_oracle.c).  All arrays are
+float64 numpy arrays; ``rows`` selects a subset of target particles (the
+reference's ``*_mpi(nbeg, nend, ...)`` row ranges generalised to index lists).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile next to this file."""
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-s", "-B" if force else "liboracle.so"], cwd=_HERE, check=True)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int32)
+        lp = C.POINTER(C.c_int64)
+        i64 = C.c_int64
+        i32 = C.c_int32
+        L.orc_h_factor.restype = C.c_double
+        L.orc_h_factor.argtypes = [i32, i32]
+        L.orc_count_within.argtypes = [i64, dp, dp, lp, i64, ip]
+        L.orc_smoothing.argtypes = [i64, dp, dp, ip, dp, C.c_double, i32, i32, i32, i32, lp, i64,
+                                    dp, ip, ip, ip]
+        L.orc_neighbors.argtypes = [i64, dp, dp, lp, i64, lp, lp]
+        L.orc_density.argtypes = [i64, dp, dp, dp, lp, i64, dp, ip]
+        L.orc_eos.argtypes = [i64, dp, dp, C.c_double, dp, dp]
+        L.orc_divcurl.argtypes = [i64, dp, dp, dp, dp, dp, lp, i64, dp, dp]
+        L.orc_hydro.argtypes = [i64, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_void_p, lp, i64, dp, dp]
+        L.orc_gravity.argtypes = [i64, dp, dp, dp, C.c_double, lp, i64, dp, dp]
+        L.orc_lambda.restype = C.c_double
+        L.orc_lambda.argtypes = [C.c_double, dp, dp, i32]
+        L.orc_cool_one.restype = C.c_double
+        L.orc_cool_one.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p, dp, dp, ip]
+        L.orc_cool.argtypes = [i64, dp, dp, C.c_double, C.c_void_p, dp, dp, dp, ip]
+        L.orc_ke.restype = C.c_double
+        L.orc_ke.argtypes = [i64, dp, dp]
+        L.orc_pe.restype = C.c_double
+        L.orc_pe.argtypes = [i64, dp, dp, dp, C.c_double]
+        _LIB = L
+    return _LIB
+
+
+class _Params(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
+                ("eta", C.c_double), ("G", C.c_double), ("nngb", C.c_int32), ("ndown", C.c_int32),
+                ("nup", C.c_int32), ("max_iter", C.c_int32), ("av_form", C.c_int32),
+                ("balsara", C.c_int32)]
+
+
+class _CoolParams(C.Structure):
+    _fields_ = [("unit_u_cgs", C.c_double), ("unit_rho_cgs", C.c_double),
+                ("unit_time_s", C.c_double), ("XH", C.c_double), ("mu", C.c_double),
+                ("gamma", C.c_double), ("heat", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int32), ("max_bracket", C.c_int32), ("ntab", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+@dataclass
+class Params:
+    """Physics constants of the step (AUGX:472-484, hfv_sph_gpu_v2_evrard.cu:283-300)."""
+    gamma: float = 5.0 / 3.0
+    alpha: float = 1.0
+    beta: float = 2.0
+    eta: float = 0.1
+    G: float = 1.0
+    nngb: int = 64
+    ndown: int = 59
+    nup: int = 69
+    max_iter: int = 100
+    av_form: int = 0          # 0 Monaghan-Gingold 1983, 1 Gadget signal velocity
+    balsara: int = 0
+    eps_from_h: bool = True   # softening eps_i = h_i
+    h_predictor: bool = True  # Gadget-2 eq. 31 start for the h iteration
+
+    def c(self) -> _Params:
+        return _Params(self.gamma, self.alpha, self.beta, self.eta, self.G, self.nngb, self.ndown,
+                       self.nup, self.max_iter, self.av_form, self.balsara)
+
+
+@dataclass
+class CoolParams:
+    unit_u_cgs: float
+    unit_rho_cgs: float
+    unit_time_s: float
+    XH: float = 0.76
+    mu: float = 0.62
+    gamma: float = 5.0 / 3.0
+    heat: float = 0.0
+    tol: float = 1e-10
+    max_iter: int = 100
+    max_bracket: int = 200
+    T: np.ndarray = field(default=None)
+    L: np.ndarray = field(default=None)
+
+    def c(self) -> _CoolParams:
+        return _CoolParams(self.unit_u_cgs, self.unit_rho_cgs, self.unit_time_s, self.XH, self.mu,
+                           self.gamma, self.heat, self.tol, self.max_iter, self.max_bracket,
+                           len(self.T), 0)
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def _rows(rows, n):
+    if rows is None:
+        return None, None, n
+    r = np.ascontiguousarray(rows, dtype=np.int64)
+    return r, r.ctypes.data_as(C.POINTER(C.c_int64)), len(r)
+
+
+def h_factor(nngb: int, k: int) -> float:
+    return lib().orc_h_factor(nngb, k)
+
+
+def count_within(pos, radius, rows=None):
+    """k_i = #{j : |x_i - x_j|^2 <= s_i^2}, self included (LIBSX:906-915)."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    rr, rp, nr = _rows(rows, n)
+    rad, radp = _d(np.broadcast_to(radius, (nr,)))
+    k = np.zeros(nr, np.int32)
+    lib().orc_count_within(n, pp, radp, rp, nr, _i(k)[1])
+    return k
+
+
+def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(), rows=None):
+    """Smoothing-length iteration (LIBSX3:946-1010, LIBV7:255-330).
+
+    Returns (h, k, iterations, status) per row; status 1 = not converged."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    hp, hpp = _d(h_prev)
+    rr, rp, nr = _rows(rows, n)
+    if k_prev is not None and divV is not None:
+        kp, kpp = _i(k_prev)
+        dv, dvp = _d(divV)
+    else:
+        kp = dv = None
+        kpp = C.POINTER(C.c_int32)()
+        dvp = C.POINTER(C.c_double)()
+    h = np.zeros(nr)
+    k = np.zeros(nr, np.int32)
+    it = np.zeros(nr, np.int32)
+    st = np.zeros(nr, np.int32)
+    lib().orc_smoothing(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter, rp, nr,
+                        _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
+    return h, k, it, st
+
+
+def neighbors(pos, h, rows=None):
+    """Neighbour lists under r_ij^2 <= (h_i + h_j)^2 (gate of LIBV2:89); CSR (offsets, idx)."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    hh, hp = _d(h)
+    rr, rp, nr = _rows(rows, n)
+    off = np.zeros(nr + 1, np.int64)
+    lp = C.POINTER(C.c_int64)
+    lib().orc_neighbors(n, pp, hp, rp, nr, off.ctypes.data_as(lp), lp())
+    idx = np.zeros(max(int(off[-1]), 1), np.int64)
+    lib().orc_neighbors(n, pp, hp, rp, nr, off.ctypes.data_as(lp), idx.ctypes.data_as(lp))
+    return off, idx[: off[-1]]
+
+
+def density(pos, m, h, rows=None):
+    """getDensity (LIBSX:17-51); returns (rho, neighbour count)."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    m, mp = _d(m)
+    h, hp = _d(h)
+    rr, rp, nr = _rows(rows, n)
+    rho = np.zeros(nr)
+    k = np.zeros(nr, np.int32)
+    lib().orc_density(n, pp, mp, hp, rp, nr, _d(rho)[1], _i(k)[1])
+    return rho, k
+
+
+def eos(rho, u, gamma):
+    """getPressure (LIBSX:100-105) and the adiabatic sound speed (LIBV2:783-789)."""
+    n = len(rho)
+    rho, rp = _d(rho)
+    u, up = _d(u)
+    P = np.zeros(n)
+    c = np.zeros(n)
+    lib().orc_eos(n, rp, up, gamma, _d(P)[1], _d(c)[1])
+    return P, c
+
+
+def divcurl(pos, v, rho, m, h, rows=None):
+    """div_curlVel (SHEAR:8-69): signed divergence and |curl|."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    v, vp = _d(v)
+    rho, rhop = _d(rho)
+    m, mp = _d(m)
+    h, hp = _d(h)
+    rr, rp, nr = _rows(rows, n)
+    dv = np.zeros(nr)
+    cv = np.zeros(nr)
+    lib().orc_divcurl(n, pp, vp, rhop, mp, hp, rp, nr, _d(dv)[1], _d(cv)[1])
+    return dv, cv
+
+
+def hydro(pos, v, rho, P, c, h, m, divV=None, curlV=None, p: Params = Params(), rows=None):
+    """getAcc_sph (LIBSX:110-186) and get_dU (LIBSX:363-430) in one pair loop."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    v, vp = _d(v)
+    rho, rhop = _d(rho)
+    P, Pp = _d(P)
+    c, cp = _d(c)
+    h, hp = _d(h)
+    m, mp = _d(m)
+    dv, dvp = _d(divV if divV is not None else np.zeros(n))
+    cv, cvp = _d(curlV if curlV is not None else np.zeros(n))
+    rr, rp, nr = _rows(rows, n)
+    acc = np.zeros((nr, 3))
+    du = np.zeros(nr)
+    pc = p.c()
+    lib().orc_hydro(n, pp, vp, rhop, Pp, cp, hp, mp, dvp, cvp, C.byref(pc), rp, nr, _d(acc)[1],
+                    _d(du)[1])
+    return acc, du
+
+
+def gravity(pos, m, eps, G=1.0, rows=None, potential=False):
+    """getAcc_g_smth (LIBSX:633-671); optionally the pair potential of getPE (LIBSX:593-629)."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    m, mp = _d(m)
+    e, ep = _d(np.broadcast_to(eps, (n,)))
+    rr, rp, nr = _rows(rows, n)
+    acc = np.zeros((nr, 3))
+    pot = np.zeros(nr) if potential else None
+    lib().orc_gravity(n, pp, mp, ep, G, rp, nr, _d(acc)[1],
+                      _d(pot)[1] if potential else C.POINTER(C.c_double)())
+    return (acc, pot) if potential else acc
+
+
+def lam(T, cool: CoolParams):
+    Tt, Tp = _d(cool.T)
+    Lt, Lp = _d(cool.L)
+    return lib().orc_lambda(float(T), Tp, Lp, len(Tt))
+
+
+def cool(rho, u, dt, cp: CoolParams):
+    """Implicit per-particle cooling solve (PHOTOPY:489-543 structure) on a tabulated Lambda(T)."""
+    n = len(rho)
+    rho, rp = _d(rho)
+    u, up = _d(u)
+    Tt, Tp = _d(cp.T)
+    Lt, Lp = _d(cp.L)
+    out = np.zeros(n)
+    it = np.zeros(n, np.int32)
+    c = cp.c()
+    lib().orc_cool(n, rp, up, dt, C.byref(c), Tp, Lp, _d(out)[1], _i(it)[1])
+    return out, it
+
+
+def ke(v, m):
+    """getKE (LIBSX:579-588)."""
+    v, vp = _d(v)
+    m, mp = _d(m)
+    return lib().orc_ke(len(m), vp, mp)
+
+
+def pe(pos, m, eps, G=1.0):
+    """getPE (LIBSX:593-629)."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    m, mp = _d(m)
+    e, ep = _d(np.broadcast_to(eps, (n,)))
+    return lib().orc_pe(n, pp, mp, ep, G)
+
+
+# ---------------------------------------------------------------------------
+# Full step, composed from the functions above in the order of the GPU driver
+# (hfv_sph_gpu_v2_evrard.cu:407-482), with du/dt evaluated at the same
+# velocity as the hydro acceleration (augsphx6.5.py:514-528).
+# ---------------------------------------------------------------------------
+@dataclass
+class State:
+    pos: np.ndarray
+    vel: np.ndarray
+    u: np.ndarray
+    m: np.ndarray
+    h: np.ndarray
+    eps: np.ndarray | None = None
+    k: np.ndarray | None = None
+    rho: np.ndarray | None = None
+    P: np.ndarray | None = None
+    c: np.ndarray | None = None
+    divV: np.ndarray | None = None
+    curlV: np.ndarray | None = None
+    acc_g: np.ndarray | None = None
+    acc_sph: np.ndarray | None = None
+    acc: np.ndarray | None = None
+    dudt: np.ndarray | None = None
+    ut_prev: np.ndarray | None = None
+    h_iters: np.ndarray | None = None
+    h_status: np.ndarray | None = None
+
+    def copy(self):
+        return State(**{k: (None if v is None else np.array(v, copy=True))
+                        for k, v in self.__dict__.items()})
+
+
+def forces(s: State, p: Params, dt: float, use_pred: bool, cool_p: CoolParams | None = None):
+    if use_pred and p.h_predictor and s.k is not None and s.divV is not None:
+        h, k, it, st = smoothing(s.pos, s.h, s.k, s.divV, dt, p)
+    else:
+        h, k, it, st = smoothing(s.pos, s.h, p=p)
+    s.h, s.k, s.h_iters, s.h_status = h, k, it, st
+    s.rho, _ = density(s.pos, s.m, s.h)
+    s.P, s.c = eos(s.rho, s.u, p.gamma)
+    eps = s.h if p.eps_from_h else s.eps
+    s.acc_g = gravity(s.pos, s.m, eps, p.G)
+    s.divV, s.curlV = divcurl(s.pos, s.vel, s.rho, s.m, s.h)
+    s.acc_sph, s.dudt = hydro(s.pos, s.vel, s.rho, s.P, s.c, s.h, s.m, s.divV, s.curlV, p)
+    s.acc = s.acc_g + s.acc_sph
+
+
+def bootstrap(s: State, p: Params) -> State:
+    """Forces at t = 0 (hfv_sph_gpu_v2_evrard.cu:311-392, without its Euler u update)."""
+    s = s.copy()
+    forces(s, p, 0.0, use_pred=False)
+    s.ut_prev = s.dudt.copy()
+    return s
+
+
+def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None) -> State:
+    """One KDK step (hfv_sph_gpu_v2_evrard.cu:410-482)."""
+    s = s.copy()
+    s.vel = s.vel + s.acc * (0.5 * dt)
+    s.pos = s.pos + s.vel * dt
+    forces(s, p, dt, use_pred=True)
+    s.vel = s.vel + s.acc * (0.5 * dt)
+    s.u = s.u + (0.5 * dt) * (s.dudt + s.ut_prev)
+    s.ut_prev = s.dudt.copy()
+    if cool_p is not None:
+        s.u, _ = cool(s.rho, s.u, dt, cool_p)
+    return s

@@ -1,0 +1,520 @@
+/*
+ * FP64 CPU oracle for the per-particle SPH step (test infrastructure only).
+ *
+ * This file is the checker, not the product: only tests/, the smoke check and
+ * the bench's cpu_baseline leg load it.  The shipped path is the HIP library
+ * under sph_2_amd/csrc and never calls into this code.
+ *
+ * Every routine restates one function of the reference's FP64 CPU path
+ * (Python/Numba) in plain C with OpenMP over rows.  Reference paths are
+ * relative to 11_Dec_2022/ of hassanfv/SPH_2:
+ *   LIBSX  = GPU_sph/IC/libsx2_2t.py
+ *   LIBSX3 = GPU_sph/work_on_smoothing/libsx2_2t.py
+ *   SHEAR  = GPU_sph/work_on_smoothing/shear.py
+ *   AUGX   = Evrard_collapse_new_h_algorithm/augsphx6.5x.py
+ *   LIBV2  = GPU_sph/Important_Test_Simulations/Evrard_collapse/myCppSPHLibs_v2.h
+ *   LIBV7  = GPU_sph_Type_cooling_II/hfvCLibs_v7.h
+ *   PHOTOPY= GPU_sph/Cooling_v1/photolibs3.py
+ *
+ * Determinism contract shared with the GPU path (compile with
+ * -ffp-contract=off, never -ffast-math):
+ *   - pair distance r2 = (dx*dx + dy*dy) + dz*dz in FP64, dx = x_i - x_j;
+ *   - neighbour tests compare r2 <= s*s (gather radius s = 2h) or
+ *     r2 <= (h_i + h_j)^2 (symmetric pair test, 2*h_ij = h_i + h_j);
+ *   - the smoothing-length iteration and the cooling solve use only
+ *     +, -, *, / and comparisons, so both sides make identical decisions.
+ *
+ * Parity pins (see tests/test_oracle_*.py): closed forms and invariants for
+ * every function; brute-force kNN for the h iteration; an independent numpy
+ * restatement (oracle/npref.py) at small N.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORC_SIGMA (1.0 / 3.14159265358979323846) /* 3-D M4 normalisation 1/pi */
+
+typedef struct {
+    double gamma, alpha, beta, eta, G;
+    int32_t nngb, ndown, nup, max_iter;
+    int32_t av_form; /* 0: Monaghan-Gingold 1983 (AUGX:103-139); 1: Gadget signal velocity (LIBSX:153-169) */
+    int32_t balsara; /* 1: multiply Pi_ij by (f_i+f_j)/2 (LIBV2:400-405) */
+} orc_params;
+
+typedef struct {
+    double unit_u_cgs, unit_rho_cgs, unit_time_s;
+    double XH, mu, gamma, heat, tol;
+    int32_t max_iter, max_bracket, ntab, pad;
+} orc_cool_params;
+
+static inline int64_t row_of(const int64_t *rows, int64_t r) { return rows ? rows[r] : r; }
+
+static inline double pair_r2(const double *pos, int64_t i, int64_t j) {
+    double dx = pos[3 * i + 0] - pos[3 * j + 0];
+    double dy = pos[3 * i + 1] - pos[3 * j + 1];
+    double dz = pos[3 * i + 2] - pos[3 * j + 2];
+    double a = dx * dx;
+    double b = dy * dy;
+    double c = dz * dz;
+    return (a + b) + c;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cube-root ratio used by the smoothing-length update.                      */
+/* cbrt(nngb/k) clamped to [0.5, 2]; Newton iteration with a fixed count so  */
+/* the same doubles come out on any IEEE machine.                            */
+/* ------------------------------------------------------------------------ */
+double orc_h_factor(int32_t nngb, int32_t k) {
+    if (8 * (int64_t)k <= (int64_t)nngb) return 2.0;
+    if ((int64_t)k >= 8 * (int64_t)nngb) return 0.5;
+    double x = (double)nngb / (double)k;
+    double y = 1.0;
+    for (int it = 0; it < 12; ++it) {
+        double y2 = y * y;
+        double y3 = y2 * y;
+        double num = y3 - x;
+        double den = 3.0 * y2;
+        y = y - num / den;
+    }
+    if (y < 0.5) y = 0.5;
+    if (y > 2.0) y = 2.0;
+    return y;
+}
+
+/* Count of j with r2_ij <= s^2, self included (LIBSX:906-915 counting loop). */
+static int64_t count_within(const double *pos, int64_t n, int64_t i, double s) {
+    double s2 = s * s;
+    int64_t k = 0;
+    for (int64_t j = 0; j < n; ++j)
+        if (pair_r2(pos, i, j) <= s2) ++k;
+    return k;
+}
+
+void orc_count_within(int64_t n, const double *pos, const double *radius, const int64_t *rows,
+                      int64_t nrows, int32_t *k_out) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        k_out[r] = (int32_t)count_within(pos, n, i, radius[r]);
+    }
+}
+
+/*
+ * Smoothing-length iteration (canonical form of LIBSX3:946-1010,
+ * LIBV7:255-330 and LIBV2:7-64).
+ *
+ * Search radius s (= 2h).  Start: s = 2h_prev, or with the Gadget-2 eq. 31
+ * predictor s = 2*(0.5*h*(1 + F(k_prev)) + (h*divV)*dt/3) (LIBSX3:960).
+ * Each iteration counts k(s) (self included); done when ndown <= k <= nup.
+ * Otherwise the step is the log-space Newton update s <- s*F(k) with
+ * F(k) = cbrt(nngb/k) clamped to [1/2, 2], kept inside the bracket
+ * (lo: k < ndown, hi: k > nup) by bisection once both ends are known.
+ * Stops without convergence when the bracket has collapsed, when every
+ * particle is already inside s, or after max_iter counts; then the result is
+ * the smallest over-full radius seen (else the last radius).  status bit 1 =
+ * capped.
+ */
+static void smooth_one(const double *pos, int64_t n, int64_t i, double h_prev, int32_t k_prev,
+                       int use_pred, double divv, double dt, int32_t nngb, int32_t ndown,
+                       int32_t nup, int32_t max_iter, double *h_out, int32_t *k_out,
+                       int32_t *it_out, int32_t *st_out) {
+    double s;
+    if (use_pred) {
+        double f = orc_h_factor(nngb, k_prev);
+        double t1 = 0.5 * h_prev;
+        double t2 = 1.0 + f;
+        double t3 = t1 * t2;
+        double t4 = h_prev * divv;
+        double t5 = t4 * dt;
+        double t6 = t5 / 3.0;
+        s = 2.0 * (t3 + t6);
+        if (!(s > 0.0) || !(s < INFINITY)) s = 2.0 * h_prev;
+    } else {
+        s = 2.0 * h_prev;
+    }
+    double lo = 0.0, hi = 0.0;
+    int have_lo = 0, have_hi = 0;
+    int64_t khi = 0, klo = 0;
+    int64_t k = 0;
+    int32_t it = 0;
+    int converged = 0;
+    while (it < max_iter) {
+        k = count_within(pos, n, i, s);
+        ++it;
+        if (k >= ndown && k <= nup) {
+            converged = 1;
+            break;
+        }
+        if (k < ndown) {
+            lo = s; klo = k; have_lo = 1;
+            if (k >= n) break; /* every particle inside: growing cannot help */
+        } else {
+            hi = s; khi = k; have_hi = 1;
+        }
+        double snew = s * orc_h_factor(nngb, (int32_t)(k > 2147483647 ? 2147483647 : k));
+        if (have_lo && have_hi) {
+            if (!(snew > lo && snew < hi)) snew = 0.5 * (lo + hi);
+            if (!(snew > lo && snew < hi)) break; /* bracket collapsed */
+        }
+        s = snew;
+    }
+    int32_t st = 0;
+    if (!converged) {
+        st = 1;
+        if (have_hi) {
+            s = hi; k = khi;
+        } else {
+            s = lo; k = klo;
+        }
+    }
+    *h_out = 0.5 * s;
+    *k_out = (int32_t)k;
+    *it_out = it;
+    *st_out = st;
+}
+
+void orc_smoothing(int64_t n, const double *pos, const double *h_prev, const int32_t *k_prev,
+                   const double *divv, double dt, int32_t nngb, int32_t ndown, int32_t nup,
+                   int32_t max_iter, const int64_t *rows, int64_t nrows, double *h_out,
+                   int32_t *k_out, int32_t *it_out, int32_t *st_out) {
+    int use_pred = (k_prev != NULL && divv != NULL);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        smooth_one(pos, n, i, h_prev[i], use_pred ? k_prev[i] : 0, use_pred,
+                   use_pred ? divv[i] : 0.0, dt, nngb, ndown, nup, max_iter, &h_out[r], &k_out[r],
+                   &it_out[r], &st_out[r]);
+    }
+}
+
+/* Neighbour lists under the symmetric pair test r2 <= (h_i+h_j)^2 (the gate of
+ * LIBV2:89 "rr <= 2*hij").  Two-phase: with idx == NULL only offsets are
+ * written (offsets[r+1] = count for row r); then call again to fill. */
+void orc_neighbors(int64_t n, const double *pos, const double *h, const int64_t *rows, int64_t nrows,
+                   int64_t *offsets, int64_t *idx) {
+    if (!idx) {
+        offsets[0] = 0;
+#pragma omp parallel for schedule(dynamic, 16)
+        for (int64_t r = 0; r < nrows; ++r) {
+            int64_t i = row_of(rows, r);
+            int64_t c = 0;
+            for (int64_t j = 0; j < n; ++j) {
+                double t = h[i] + h[j];
+                if (pair_r2(pos, i, j) <= t * t) ++c;
+            }
+            offsets[r + 1] = c;
+        }
+        for (int64_t r = 0; r < nrows; ++r) offsets[r + 1] += offsets[r];
+        return;
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        int64_t o = offsets[r];
+        for (int64_t j = 0; j < n; ++j) {
+            double t = h[i] + h[j];
+            if (pair_r2(pos, i, j) <= t * t) idx[o++] = j;
+        }
+    }
+}
+
+/* Cubic-spline kernel W(r, h) (LIBSX:36-45). */
+static inline double kernel_w(double q, double hij) {
+    double h3 = hij * hij * hij;
+    if (q <= 1.0) return ORC_SIGMA / h3 * (1.0 - 1.5 * q * q + 0.75 * q * q * q);
+    double w = 2.0 - q;
+    if (w < 0.0) w = 0.0;
+    return ORC_SIGMA / h3 * 0.25 * w * w * w;
+}
+
+/* Gradient factor F with grad_i W = F * (x_i - x_j) (LIBSX:132-150).  The
+ * reference's q + 1e-20 guard only matters at q = 0, which this branch never
+ * sees (q > 1). */
+static inline double kernel_f(double q, double hij) {
+    double h5 = hij * hij * hij * hij * hij;
+    if (q <= 1.0) return ORC_SIGMA / h5 * (-3.0 + 2.25 * q);
+    double w = 2.0 - q;
+    if (w < 0.0) w = 0.0;
+    return -0.75 * ORC_SIGMA / h5 * w * w / q;
+}
+
+/* Density rho_i = sum_j m_j W(r_ij, h_ij), self included (LIBSX:17-51).
+ * Also returns the neighbour count under the symmetric pair test. */
+void orc_density(int64_t n, const double *pos, const double *m, const double *h,
+                 const int64_t *rows, int64_t nrows, double *rho_out, int32_t *nngb_out) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        double s = 0.0;
+        int32_t c = 0;
+        for (int64_t j = 0; j < n; ++j) {
+            double t = h[i] + h[j];
+            double r2 = pair_r2(pos, i, j);
+            if (r2 <= t * t) {
+                double hij = 0.5 * t;
+                double q = sqrt(r2) / hij;
+                s += m[j] * kernel_w(q, hij);
+                ++c;
+            }
+        }
+        rho_out[r] = s;
+        if (nngb_out) nngb_out[r] = c;
+    }
+}
+
+/* Adiabatic EOS: P = (gamma-1) rho u (LIBSX:100-105), c = sqrt(gamma (gamma-1) u) (LIBV2:783-789). */
+void orc_eos(int64_t n, const double *rho, const double *u, double gamma, double *P, double *c) {
+#pragma omp parallel for
+    for (int64_t i = 0; i < n; ++i) {
+        P[i] = (gamma - 1.0) * rho[i] * u[i];
+        c[i] = sqrt(gamma * (gamma - 1.0) * u[i]);
+    }
+}
+
+/* Velocity divergence (signed) and |curl| (SHEAR:8-69). */
+void orc_divcurl(int64_t n, const double *pos, const double *v, const double *rho, const double *m,
+                 const double *h, const int64_t *rows, int64_t nrows, double *div_out,
+                 double *curl_out) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        double sd = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            double t = h[i] + h[j];
+            double r2 = pair_r2(pos, i, j);
+            if (!(r2 <= t * t)) continue;
+            double hij = 0.5 * t;
+            double q = sqrt(r2) / hij;
+            double F = kernel_f(q, hij);
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double gx = F * dx, gy = F * dy, gz = F * dz;
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double w = m[j] / rho[i];
+            sd += w * (-vxij * gx - vyij * gy - vzij * gz);
+            cx += w * (vyij * gz - vzij * gy);
+            cy += w * (vzij * gx - vxij * gz);
+            cz += w * (vxij * gy - vyij * gx);
+        }
+        div_out[r] = sd;
+        curl_out[r] = sqrt(cx * cx + cy * cy + cz * cz);
+    }
+}
+
+/* Artificial viscosity Pi_ij for one pair (returns 0 for receding pairs). */
+static inline double av_pi(const orc_params *p, double vr, double r2, double hij, double ci,
+                           double cj, double rhoi, double rhoj, double fi, double fj) {
+    if (!(vr <= 0.0)) return 0.0;
+    double rhob = 0.5 * (rhoi + rhoj);
+    double pi;
+    if (p->av_form == 0) {
+        /* Monaghan & Gingold 1983 (AUGX:103-139) */
+        double mu = hij * vr / (r2 + hij * hij * p->eta * p->eta);
+        double cb = 0.5 * (ci + cj);
+        pi = (-p->alpha * cb * mu + p->beta * mu * mu) / rhob;
+    } else {
+        /* Gadget signal-velocity form (LIBSX:158-169) */
+        double w = vr / (sqrt(r2) + 1e-5);
+        double vsig = ci + cj - 3.0 * w;
+        pi = -0.5 * p->alpha * vsig * w / rhob;
+    }
+    if (p->balsara) pi *= 0.5 * (fi + fj);
+    return pi;
+}
+
+/* Balsara factor f = |divV| / (|divV| + |curlV| + 1e-4 c/h) (LIBV2:400-401). */
+static inline double balsara_f(double dv, double cv, double c, double h) {
+    double a = fabs(dv);
+    return a / (a + cv + 0.0001 * c / h);
+}
+
+/* Hydro acceleration (LIBSX:110-186, SHEAR:151-235) and du/dt
+ * (LIBSX:363-430, SHEAR:334-406) from the same pair loop and velocity. */
+void orc_hydro(int64_t n, const double *pos, const double *v, const double *rho, const double *P,
+               const double *c, const double *h, const double *m, const double *divv,
+               const double *curlv, const orc_params *p, const int64_t *rows, int64_t nrows,
+               double *acc_out, double *du_out) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        double ax = 0.0, ay = 0.0, az = 0.0, du = 0.0;
+        double Pi = P[i] / (rho[i] * rho[i]);
+        double fi = p->balsara ? balsara_f(divv[i], curlv[i], c[i], h[i]) : 1.0;
+        for (int64_t j = 0; j < n; ++j) {
+            double t = h[i] + h[j];
+            double r2 = pair_r2(pos, i, j);
+            if (!(r2 <= t * t)) continue;
+            double hij = 0.5 * t;
+            double q = sqrt(r2) / hij;
+            double F = kernel_f(q, hij);
+            double dx = pos[3 * i] - pos[3 * j], dy = pos[3 * i + 1] - pos[3 * j + 1],
+                   dz = pos[3 * i + 2] - pos[3 * j + 2];
+            double vxij = v[3 * i] - v[3 * j], vyij = v[3 * i + 1] - v[3 * j + 1],
+                   vzij = v[3 * i + 2] - v[3 * j + 2];
+            double vr = vxij * dx + vyij * dy + vzij * dz;
+            double fj = p->balsara ? balsara_f(divv[j], curlv[j], c[j], h[j]) : 1.0;
+            double pij = av_pi(p, vr, r2, hij, c[i], c[j], rho[i], rho[j], fi, fj);
+            double Pj = P[j] / (rho[j] * rho[j]);
+            double s = m[j] * (Pi + Pj + pij) * F;
+            ax -= s * dx;
+            ay -= s * dy;
+            az -= s * dz;
+            du += m[j] * (Pi + 0.5 * pij) * F * vr;
+        }
+        acc_out[3 * r + 0] = ax;
+        acc_out[3 * r + 1] = ay;
+        acc_out[3 * r + 2] = az;
+        du_out[r] = du;
+    }
+}
+
+/* Spline-softened gravity (LIBSX:633-671 / getAcc_g_smthx LIBSX:720-756)
+ * and, optionally, the per-particle potential phi_i = -G sum_j m_j phi_ij
+ * (the pair term of getPE, LIBSX:593-629). */
+void orc_gravity(int64_t n, const double *pos, const double *m, const double *eps, double G,
+                 const int64_t *rows, int64_t nrows, double *acc_out, double *pot_out) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        double ax = 0.0, ay = 0.0, az = 0.0, ph = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                   dz = pos[3 * j + 2] - pos[3 * i + 2];
+            double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            double e = 0.5 * (eps[i] + eps[j]);
+            double q = rr / e;
+            double fk, pk;
+            if (q <= 1.0) {
+                fk = (1.0 / (e * e * e)) * ((4.0 / 3.0) - (6.0 / 5.0) * q * q + 0.5 * q * q * q);
+                pk = (-2.0 / e) * ((1.0 / 3.0) * q * q - (3.0 / 20.0) * q * q * q * q +
+                                   (1.0 / 20.0) * q * q * q * q * q) +
+                     7.0 / 5.0 / e;
+            } else if (q <= 2.0) {
+                double q2 = q * q, q3 = q2 * q, q4 = q3 * q, q5 = q4 * q, q6 = q5 * q;
+                fk = (1.0 / (rr * rr * rr)) *
+                     ((-1.0 / 15.0) + (8.0 / 3.0) * q3 - 3.0 * q4 + (6.0 / 5.0) * q5 - (1.0 / 6.0) * q6);
+                pk = (-1.0 / 15.0) / rr -
+                     (1.0 / e) * ((4.0 / 3.0) * q2 - q3 + (3.0 / 10.0) * q4 - (1.0 / 30.0) * q5) +
+                     8.0 / 5.0 / e;
+            } else {
+                fk = 1.0 / (rr * rr * rr);
+                pk = 1.0 / rr;
+            }
+            ax += G * fk * dx * m[j];
+            ay += G * fk * dy * m[j];
+            az += G * fk * dz * m[j];
+            ph -= G * m[j] * pk;
+        }
+        acc_out[3 * r + 0] = ax;
+        acc_out[3 * r + 1] = ay;
+        acc_out[3 * r + 2] = az;
+        if (pot_out) pot_out[r] = ph;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tabulated cooling with an implicit FP64 solve (structure of             */
+/* PHOTOPY:489-543 DoCooling_h and PHOTO:613-686).                          */
+/* ------------------------------------------------------------------------ */
+#define ORC_MH 1.6726e-24 /* g (hfv_sph_gpu_v2_evrard.cu:289) */
+#define ORC_KB 1.3807e-16 /* erg/K (hfv_sph_gpu_v2_evrard.cu:290) */
+
+double orc_lambda(double T, const double *Tt, const double *Lt, int32_t nt) {
+    if (T <= Tt[0]) return Lt[0];
+    if (T >= Tt[nt - 1]) return Lt[nt - 1];
+    int32_t lo = 0, hi = nt - 1; /* Tt[lo] <= T < Tt[hi] */
+    while (hi - lo > 1) {
+        int32_t mid = (lo + hi) >> 1;
+        if (Tt[mid] <= T) lo = mid;
+        else hi = mid;
+    }
+    double f = (T - Tt[lo]) / (Tt[hi] - Tt[lo]);
+    return Lt[lo] + (Lt[hi] - Lt[lo]) * f;
+}
+
+/* g(u) = u - u_old - dt * (nH*heat - nH^2 Lambda(T(u))) / rho, all cgs. */
+static inline double cool_g(double u, double u_old, double dt_s, double nH, double rho_cgs,
+                            double tfac, const orc_cool_params *cp, const double *Tt,
+                            const double *Lt) {
+    double T = tfac * u;
+    double lam = orc_lambda(T, Tt, Lt, cp->ntab);
+    double rate = (nH * cp->heat - nH * nH * lam) / rho_cgs;
+    return u - u_old - dt_s * rate;
+}
+
+double orc_cool_one(double rho, double u_code, double dt, const orc_cool_params *cp,
+                    const double *Tt, const double *Lt, int32_t *iters) {
+    double rho_cgs = rho * cp->unit_rho_cgs;
+    double u_old = u_code * cp->unit_u_cgs;
+    double dt_s = dt * cp->unit_time_s;
+    double nH = cp->XH * rho_cgs / ORC_MH;
+    double tfac = (cp->gamma - 1.0) * cp->mu * ORC_MH / ORC_KB;
+    double g0 = cool_g(u_old, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt);
+    int32_t it = 0;
+    if (g0 == 0.0 || !(dt_s > 0.0)) {
+        if (iters) *iters = 0;
+        return u_code;
+    }
+    double lo, hi;
+    if (g0 > 0.0) { /* net cooling: root below u_old */
+        hi = u_old;
+        lo = u_old / 1.1;
+        int32_t nb = 0;
+        while (cool_g(lo, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt) > 0.0 && nb < cp->max_bracket) {
+            hi = lo;
+            lo = lo / 1.1;
+            ++nb;
+        }
+    } else { /* net heating: root above u_old */
+        lo = u_old;
+        hi = u_old * 1.1;
+        int32_t nb = 0;
+        while (cool_g(hi, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt) < 0.0 && nb < cp->max_bracket) {
+            lo = hi;
+            hi = hi * 1.1;
+            ++nb;
+        }
+    }
+    while ((hi - lo) > cp->tol * lo && it < cp->max_iter) {
+        double mid = 0.5 * (lo + hi);
+        if (cool_g(mid, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt) > 0.0) hi = mid;
+        else lo = mid;
+        ++it;
+    }
+    if (iters) *iters = it;
+    return 0.5 * (lo + hi) / cp->unit_u_cgs;
+}
+
+void orc_cool(int64_t n, const double *rho, const double *u, double dt, const orc_cool_params *cp,
+              const double *Tt, const double *Lt, double *u_out, int32_t *it_out) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t it = 0;
+        u_out[i] = orc_cool_one(rho[i], u[i], dt, cp, Tt, Lt, &it);
+        if (it_out) it_out[i] = it;
+    }
+}
+
+/* Kinetic energy sum 0.5 m v^2 (LIBSX:579-588). */
+double orc_ke(int64_t n, const double *v, const double *m) {
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i)
+        s += 0.5 * m[i] * (v[3 * i] * v[3 * i] + v[3 * i + 1] * v[3 * i + 1] + v[3 * i + 2] * v[3 * i + 2]);
+    return s;
+}
+
+/* Potential energy sum_{i<j} -G m_i m_j phi_ij (LIBSX:593-629), from the
+ * per-particle potential: PE = 0.5 * sum_i m_i phi_i. */
+double orc_pe(int64_t n, const double *pos, const double *m, const double *eps, double G) {
+    double *phi = (double *)malloc(sizeof(double) * (size_t)n);
+    double *acc = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    orc_gravity(n, pos, m, eps, G, NULL, n, acc, phi);
+    double s = 0.0;
+    for (int64_t i = 0; i < n; ++i) s += 0.5 * m[i] * phi[i];
+    free(phi);
+    free(acc);
+    return s;
+}

@@ -1,0 +1,180 @@
+"""ctypes binding of libsphx.so (the HIP library) and device-buffer helpers.
+
+The library must have been built (``python -m sph_2_amd.build``); a missing
+library or a missing GPU is an error, never a silent CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsphx.so")
+
+MIXED = 0
+FP64 = 1
+OK = 0
+CAPPED = 1
+
+_lib = None
+_lock = threading.Lock()
+
+
+class SphxError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """sphx_params (include/sphx.h)."""
+    _fields_ = [("gamma", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
+                ("eta", C.c_double), ("G", C.c_double), ("nngb", C.c_int32), ("ndown", C.c_int32),
+                ("nup", C.c_int32), ("max_iter", C.c_int32), ("av_form", C.c_int32),
+                ("balsara", C.c_int32), ("eps_from_h", C.c_int32), ("h_predictor", C.c_int32)]
+
+
+class Cooling(C.Structure):
+    """sphx_cooling (include/sphx.h)."""
+    _fields_ = [("unit_u_cgs", C.c_double), ("unit_rho_cgs", C.c_double),
+                ("unit_time_s", C.c_double), ("XH", C.c_double), ("mu", C.c_double),
+                ("gamma", C.c_double), ("heat", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int32), ("max_bracket", C.c_int32), ("ntab", C.c_int32),
+                ("pad", C.c_int32), ("T", C.POINTER(C.c_double)), ("L", C.POINTER(C.c_double))]
+
+
+class Counters(C.Structure):
+    _fields_ = [("n", C.c_int64), ("steps", C.c_int64), ("h_iters_max", C.c_int64),
+                ("h_iters_sum", C.c_int64), ("h_capped", C.c_int64), ("cool_capped", C.c_int64),
+                ("cand_buckets", C.c_int64), ("buckets", C.c_int64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class Diag(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("ke", "ut", "pe", "px", "py", "pz", "lx", "ly", "lz",
+                                          "mass")]
+
+    def as_dict(self):
+        return {k: float(getattr(self, k)) for k, _ in self._fields_}
+
+
+# Every exported entry point with its argument types (pointers are void*).
+_V = C.c_void_p
+_I64 = C.c_int64
+_I32 = C.c_int32
+_D = C.c_double
+SIGNATURES = {
+    "sphx_last_error": (C.c_char_p, []),
+    "sphx_version": (C.c_char_p, []),
+    "sphx_default_params": (None, [_V]),
+    "sphx_ctx_create": (C.c_int, [C.c_int, _V]),
+    "sphx_ctx_destroy": (C.c_int, [_V]),
+    "sphx_getDensity": (C.c_int, [_V, C.c_int, _I64, _V, _V, _V, _I64, _I64, _V, _V, _V]),
+    "sphx_eos": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V]),
+    "sphx_smoothing": (C.c_int, [_V, C.c_int, _I64, _V, _V, _V, _V, _D, _V, _I64, _I64, _V, _V,
+                                 _V, _V]),
+    "sphx_div_curlVel": (C.c_int, [_V, C.c_int, _I64, _V, _V, _V, _V, _V, _I64, _I64, _V, _V,
+                                   _V]),
+    "sphx_getAcc_sph": (C.c_int, [_V, C.c_int, _I64, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V,
+                                  _I64, _I64, _V, _V, _V]),
+    "sphx_getAcc_g_smth": (C.c_int, [_V, C.c_int, _I64, _V, _V, _V, _D, _I64, _I64, _V, _V,
+                                     _V]),
+    "sphx_neighbors": (C.c_int, [_V, _I64, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V]),
+    "sphx_cool": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V, _V]),
+    "sphx_diagnostics": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _V, _V]),
+    "sphx_morton_sort": (C.c_int, [_V, _I64, _V, _V, _V, _V]),
+    "sphx_plan_create": (C.c_int, [C.c_int, C.c_int, _V, _V, _I64, _V]),
+    "sphx_plan_load": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_plan_init": (C.c_int, [_V, _V]),
+    "sphx_plan_step": (C.c_int, [_V, _I32, _D, _I32, _V]),
+    "sphx_plan_store": (C.c_int, [_V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_plan_query": (C.c_int, [_V, _V]),
+    "sphx_plan_diagnostics": (C.c_int, [_V, _I32, _V, _V]),
+    "sphx_plan_destroy": (C.c_int, [_V]),
+    "sphx_plan_set_gravity_config": (C.c_int, [_V, _I32, _I32, _I32]),
+    "sphx_ctx_set_gravity_config": (C.c_int, [_V, _I32, _I32, _I32]),
+    "sphx_plan_set_timing": (C.c_int, [_V, _I32]),
+    "sphx_plan_timing": (C.c_int, [_V, _V, _V, _I32]),
+}
+
+PHASES = ("sort", "smoothing", "density", "gravity", "divcurl", "hydro", "finish")
+
+
+def load(path: str = LIB_PATH):
+    """Load libsphx.so and declare every entry point; raises if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(path):
+            raise SphxError(f"{path} is missing: build it with `python -m sph_2_amd.build`")
+        L = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+        return L
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc < 0:
+        msg = load().sphx_last_error().decode()
+        raise SphxError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    load().sphx_default_params(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Device buffers (torch-ROCm tensors used only as memory)
+# ---------------------------------------------------------------------------
+def torch():
+    import torch as _t
+    return _t
+
+
+def require_gpu():
+    t = torch()
+    if not t.cuda.is_available():
+        raise SphxError("no HIP device visible: libsphx runs on the GPU only")
+    return t
+
+
+def dev(a, dtype=None):
+    """numpy / torch -> contiguous device tensor."""
+    t = require_gpu()
+    if isinstance(a, t.Tensor):
+        x = a
+        if dtype is not None and x.dtype != dtype:
+            x = x.to(dtype)
+        if not x.is_cuda:
+            x = x.cuda()
+        return x.contiguous()
+    arr = np.ascontiguousarray(a)
+    if not arr.flags.writeable:
+        arr = arr.copy()
+    x = t.from_numpy(arr)
+    if dtype is not None:
+        x = x.to(dtype)
+    return x.cuda().contiguous()
+
+
+def ptr(x) -> int | None:
+    return None if x is None else x.data_ptr()
+
+
+def stream_handle():
+    t = torch()
+    return t.cuda.current_stream().cuda_stream

@@ -1,0 +1,82 @@
+"""Launch-shape selection for the gravity kernel.
+
+choose_gravity_config(n) returns (ipl, tj, js): particles per lane, j-tile
+size and j split.  The defaults below keep >= 4 workgroups per CU on the
+256-CU MI355X; ``tune_gravity`` measures the candidates on the device and
+caches the winner per size class in a JSON file.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+CUS = 256
+CANDIDATES = [(1, 256), (1, 512), (2, 256), (2, 512), (2, 1024), (4, 256), (4, 512)]
+_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gravity_tuning.json")
+
+
+def _size_class(n: int) -> str:
+    k = 1
+    while k < n:
+        k <<= 1
+    return str(k)
+
+
+def _default(n: int):
+    ipl, tj = 2, 512
+    blocks = max(1, (n + 256 * ipl - 1) // (256 * ipl))
+    js = 1
+    while blocks * js < 4 * CUS and js < 32 and (n // tj) // (js * 2) >= 4:
+        js *= 2
+    return ipl, tj, js
+
+
+def choose_gravity_config(n: int):
+    try:
+        with open(_CACHE) as f:
+            tab = json.load(f)
+        v = tab.get(_size_class(n))
+        if v:
+            return tuple(v)
+    except (OSError, ValueError):
+        pass
+    return _default(n)
+
+
+def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
+    """Time every candidate shape on an Evrard sphere of n particles."""
+    import numpy as np
+    import torch
+
+    from . import ic, libsx
+    from ._lib import check, load
+
+    e = ic.evrard(n, seed=seed)
+    ctx = libsx._context(0)
+    res = {}
+    for ipl, tj in CANDIDATES:
+        blocks = (n + 256 * ipl - 1) // (256 * ipl)
+        for js in (1, 2, 4, 8, 16):
+            if js > 1 and blocks * js > 16 * CUS:
+                break
+            check(load().sphx_ctx_set_gravity_config(ctx, ipl, tj, js), "config")
+            libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+            torch.cuda.synchronize()
+            res[(ipl, tj, js)] = (time.perf_counter() - t0) / reps
+    best = min(res, key=res.get)
+    check(load().sphx_ctx_set_gravity_config(ctx, *_default(n)), "config")
+    if write:
+        try:
+            with open(_CACHE) as f:
+                tab = json.load(f)
+        except (OSError, ValueError):
+            tab = {}
+        tab[_size_class(n)] = list(best)
+        with open(_CACHE, "w") as f:
+            json.dump(tab, f, indent=1)
+    return best, {f"{k[0]},{k[1]},{k[2]}": v for k, v in res.items()}

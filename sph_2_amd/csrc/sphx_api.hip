@@ -1,0 +1,955 @@
+// C ABI of libsphx: engine lifecycle, run tables, step orchestration, the
+// stateless drop-in calls and the resident plan.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+
+#include "sphx_internal.h"
+
+int sphx_zero_counters(int64_t *c, hipStream_t st);
+int sphx_combine(Engine &E, hipStream_t st);
+int sphx_neighbor_pass(Engine &E, int64_t r0, int64_t r1, int32_t *counts, const int64_t *offsets,
+                       int32_t *idx, hipStream_t st);
+CoolDev sphx_engine_cool(const Engine &E);
+
+static thread_local std::string g_err;
+void sphx_set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" const char *sphx_last_error(void) { return g_err.c_str(); }
+extern "C" const char *sphx_version(void) { return "sphx 0.1 gfx950"; }
+
+extern "C" void sphx_default_params(sphx_params *p) {
+    p->gamma = 5.0 / 3.0;
+    p->alpha = 1.0;
+    p->beta = 2.0;
+    p->eta = 0.1;
+    p->G = 1.0;
+    p->nngb = 64;
+    p->ndown = 59;
+    p->nup = 69;
+    p->max_iter = 100;
+    p->av_form = 0;
+    p->balsara = 0;
+    p->eps_from_h = 1;
+    p->h_predictor = 1;
+}
+
+namespace {
+
+constexpr int UT = 256;
+inline unsigned gu(int64_t n) { return (unsigned)((n + UT - 1) / UT); }
+
+__global__ void k_iota(int32_t *a, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i < n) a[i] = (int32_t)i;
+}
+__global__ void k_fill_i(int32_t *a, int32_t v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_d(double *a, double v, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void k_copy_i(const int32_t *__restrict__ a, int32_t *__restrict__ b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+// dst[i] = src[id[i]] (bring a caller array into the engine's order)
+__global__ void k_gather_by_id(const double *__restrict__ src, const int32_t *__restrict__ id,
+                               double *__restrict__ dst, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i < n) dst[i] = src[id[i]];
+}
+__global__ void k_gather_by_id_i(const int32_t *__restrict__ src, const int32_t *__restrict__ id,
+                                 int32_t *__restrict__ dst, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i < n) dst[i] = src[id[i]];
+}
+// out[id[i] - r0] = val[i] for r0 <= id[i] < r1
+__global__ void k_scatter_d(const double *__restrict__ val, const int32_t *__restrict__ id,
+                            int64_t r0, int64_t r1, double *__restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i >= n) return;
+    int64_t o = id[i];
+    if (o >= r0 && o < r1) out[o - r0] = val[i];
+}
+__global__ void k_scatter_i(const int32_t *__restrict__ val, const int32_t *__restrict__ id,
+                            int64_t r0, int64_t r1, int32_t *__restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i >= n) return;
+    int64_t o = id[i];
+    if (o >= r0 && o < r1) out[o - r0] = val[i];
+}
+__global__ void k_scatter_aos(const double *__restrict__ a, const double *__restrict__ b,
+                              const double *__restrict__ c, const int32_t *__restrict__ id,
+                              int64_t r0, int64_t r1, double *__restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
+    if (i >= n) return;
+    int64_t o = id[i];
+    if (o >= r0 && o < r1) {
+        int64_t q = 3 * (o - r0);
+        out[q] = a[i];
+        out[q + 1] = b[i];
+        out[q + 2] = c[i];
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Engine lifecycle
+// ---------------------------------------------------------------------------
+Engine::~Engine() {
+    int cur = 0;
+    hipGetDevice(&cur);
+    hipSetDevice(device);
+    for (void *q : owned) hipFree(q);
+    for (hipEvent_t e : tev)
+        if (e) hipEventDestroy(e);
+    if (cool_T) hipFree(cool_T);
+    if (cool_L) hipFree(cool_L);
+    hipSetDevice(cur);
+}
+
+int Engine::alloc(void **ptr, size_t bytes) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes ? bytes : 8);
+    if (e != hipSuccess) {
+        sphx_set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+        return SPHX_ERR_NOMEM;
+    }
+    owned.push_back(q);
+    *ptr = q;
+    return SPHX_OK;
+}
+
+int Engine::reserve(int64_t capacity) {
+    if (capacity <= cap) return SPHX_OK;
+    for (void *q : owned) hipFree(q);
+    owned.clear();
+    grav = GravHost{};
+    tab_i32 = nullptr; tab_i32_cap = 0;
+    tab_d = nullptr; tab_d_cap = 0;
+    bbox = nullptr; bbox_cap = 0;
+    d_run = nullptr;
+    cap = 0;
+    n = 0;
+    R = 0;
+    run_n.clear();
+    size_t D = sizeof(double) * (size_t)capacity, I = sizeof(int32_t) * (size_t)capacity;
+    double **dptr[] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.u, &s.m, &s.h, &s.eps,
+                       &s.ut_prev, &s.ax, &s.ay, &s.az, &s.gx, &s.gy, &s.gz, &s.rho, &s.P, &s.c,
+                       &s.divv, &s.curlv, &s.du, &s.pot};
+    for (double **q : dptr) SPHX_CHECK(alloc((void **)q, D));
+    // tmpd also holds the 256 x 11 diagnostic block partials
+    SPHX_CHECK(alloc((void **)&tmpd, sizeof(double) * (size_t)std::max<int64_t>(capacity, 4096)));
+    int32_t **iptr[] = {&s.k, &s.id, &s.run, &perm, &perm2, &tmpi, &h_iters};
+    for (int32_t **q : iptr) SPHX_CHECK(alloc((void **)q, I));
+    SPHX_CHECK(alloc((void **)&keys, sizeof(uint64_t) * capacity));
+    SPHX_CHECK(alloc((void **)&keys2, sizeof(uint64_t) * capacity));
+    hist_cap = 256 * ((capacity + 4095) / 4096 + 1);
+    SPHX_CHECK(alloc((void **)&hist, sizeof(uint32_t) * hist_cap));
+    SPHX_CHECK(alloc((void **)&counters, sizeof(int64_t) * CNT_SLOTS));
+    SPHX_HIP(hipMemset(counters, 0, sizeof(int64_t) * CNT_SLOTS));
+    cap = capacity;
+    return SPHX_OK;
+}
+
+int Engine::set_runs(const std::vector<int64_t> &sizes) {
+    int64_t total = 0;
+    for (int64_t v : sizes) total += v;
+    if (total > cap) {
+        sphx_set_error("run sizes exceed the engine capacity");
+        return SPHX_ERR_ARG;
+    }
+    if (total > INT32_MAX) {
+        sphx_set_error("more than 2^31-1 particles");
+        return SPHX_ERR_ARG;
+    }
+    hb_beg.clear(); hb_cnt.clear(); hb_run.clear(); hs_bbeg.clear(); hs_bend.clear();
+    hr_sbeg.clear(); hr_send.clear(); hr_pbeg.clear(); hr_pend.clear();
+    int64_t off = 0;
+    for (size_t r = 0; r < sizes.size(); ++r) {
+        int64_t nr = sizes[r];
+        hr_pbeg.push_back((int32_t)off);
+        hr_pend.push_back((int32_t)(off + nr));
+        int b0 = (int)hb_beg.size();
+        for (int64_t a = 0; a < nr; a += SPHX_BUCKET) {
+            hb_beg.push_back((int32_t)(off + a));
+            hb_cnt.push_back((int32_t)std::min<int64_t>(SPHX_BUCKET, nr - a));
+            hb_run.push_back((int32_t)r);
+        }
+        int b1 = (int)hb_beg.size();
+        hr_sbeg.push_back((int32_t)hs_bbeg.size());
+        for (int b = b0; b < b1; b += SPHX_SUPER) {
+            hs_bbeg.push_back(b);
+            hs_bend.push_back(std::min(b1, b + SPHX_SUPER));
+        }
+        hr_send.push_back((int32_t)hs_bbeg.size());
+        off += nr;
+    }
+    int nb = (int)hb_beg.size(), ns = (int)hs_bbeg.size(), nr = (int)sizes.size();
+    int64_t ni = 3 * (int64_t)nb + 2 * (int64_t)ns + 4 * (int64_t)nr;
+    if (ni > tab_i32_cap) {
+        SPHX_CHECK(alloc((void **)&tab_i32, sizeof(int32_t) * (ni + 1024)));
+        tab_i32_cap = ni + 1024;
+    }
+    std::vector<int32_t> all;
+    all.reserve(ni);
+    for (auto *v : {&hb_beg, &hb_cnt, &hb_run, &hs_bbeg, &hs_bend, &hr_sbeg, &hr_send, &hr_pbeg,
+                    &hr_pend})
+        all.insert(all.end(), v->begin(), v->end());
+    if (!all.empty())
+        SPHX_HIP(hipMemcpy(tab_i32, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
+    int64_t nd = 8 * (int64_t)nb + 7 * (int64_t)ns;
+    if (nd > tab_d_cap) {
+        SPHX_CHECK(alloc((void **)&tab_d, sizeof(double) * (nd + 1024)));
+        tab_d_cap = nd + 1024;
+    }
+    int64_t nbx = 6 * (int64_t)ns + 6 * (int64_t)nr;
+    if (nbx > bbox_cap) {
+        SPHX_CHECK(alloc((void **)&bbox, sizeof(double) * (nbx + 1024)));
+        bbox_cap = nbx + 1024;
+    }
+    if ((int)run_params.size() < nr) run_params.resize(nr, RunParams{1e-4, 0, p.eps_from_h});
+    if (R < nr || !d_run) {
+        SPHX_CHECK(alloc((void **)&d_run, sizeof(RunParams) * std::max(nr, 1)));
+    }
+    int32_t *q = tab_i32;
+    t.nb = nb; t.ns = ns; t.R = nr;
+    t.b_beg = q; q += nb;
+    t.b_cnt = q; q += nb;
+    t.b_run = q; q += nb;
+    t.s_bbeg = q; q += ns;
+    t.s_bend = q; q += ns;
+    t.r_sbeg = q; q += nr;
+    t.r_send = q; q += nr;
+    t.r_pbeg = q; q += nr;
+    t.r_pend = q; q += nr;
+    double *d = tab_d;
+    double **bx[] = {&t.bxl, &t.bxh, &t.byl, &t.byh, &t.bzl, &t.bzh, &t.bhmax, &t.bemax};
+    for (double **a : bx) { *a = d; d += nb; }
+    double **sx[] = {&t.sxl, &t.sxh, &t.syl, &t.syh, &t.szl, &t.szh, &t.shmax};
+    for (double **a : sx) { *a = d; d += ns; }
+    run_n = sizes;
+    R = nr;
+    n = total;
+    return SPHX_OK;
+}
+
+int Engine::upload_run_params(hipStream_t st) {
+    if (R == 0) return SPHX_OK;
+    SPHX_HIP(hipMemcpyAsync(d_run, run_params.data(), sizeof(RunParams) * R, hipMemcpyHostToDevice, st));
+    return SPHX_OK;
+}
+
+int Engine::tmark(hipStream_t st, int phase, int end) {
+    if (!timing) return SPHX_OK;
+    hipEvent_t &e = tev[2 * phase + end];
+    if (!e) SPHX_HIP(hipEventCreate(&e));
+    SPHX_HIP(hipEventRecord(e, st));
+    return SPHX_OK;
+}
+
+int Engine::tcollect() {
+    if (!timing) return SPHX_OK;
+    for (int ph = 0; ph < PH_COUNT; ++ph) {
+        hipEvent_t a = tev[2 * ph], b = tev[2 * ph + 1];
+        if (!a || !b) continue;
+        SPHX_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
+            phase_ms[ph] += ms;
+            phase_n[ph] += 1;
+        }
+    }
+    return SPHX_OK;
+}
+
+int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
+    SPHX_CHECK(tmark(st, PH_SMOOTH, 0));
+    SPHX_CHECK(build_boxes(st, false));
+    SPHX_CHECK(smoothing(st, predictor));
+    SPHX_CHECK(tmark(st, PH_SMOOTH, 1));
+    SPHX_CHECK(tmark(st, PH_DENS, 0));
+    SPHX_CHECK(build_boxes(st, true));
+    SPHX_CHECK(density_eos(st));
+    SPHX_CHECK(tmark(st, PH_DENS, 1));
+    SPHX_CHECK(tmark(st, PH_GRAV, 0));
+    SPHX_CHECK(sphx_gravity_run(*this, gcfg, with_pot, st));
+    SPHX_CHECK(tmark(st, PH_GRAV, 1));
+    SPHX_CHECK(tmark(st, PH_DIVCURL, 0));
+    SPHX_CHECK(divcurl(st));
+    SPHX_CHECK(tmark(st, PH_DIVCURL, 1));
+    SPHX_CHECK(tmark(st, PH_HYDRO, 0));
+    SPHX_CHECK(hydro(st));
+    SPHX_CHECK(tmark(st, PH_HYDRO, 1));
+    SPHX_CHECK(sphx_combine(*this, st));
+    return SPHX_OK;
+}
+
+int Engine::plain_forces(hipStream_t st, bool predictor, bool with_pot) {
+    bool pred = predictor && p.h_predictor;
+    double dt = run_params.empty() ? 0.0 : run_params[0].dt;
+    SPHX_CHECK(tmark(st, PH_SMOOTH, 0));
+    SPHX_CHECK(plain_smoothing(s.x, s.y, s.z, s.h, pred ? s.k : nullptr, pred ? s.divv : nullptr,
+                               nullptr, dt, nullptr, p, n, 0, n, s.h, s.k, h_iters, counters, st));
+    SPHX_CHECK(tmark(st, PH_SMOOTH, 1));
+    SPHX_CHECK(tmark(st, PH_DENS, 0));
+    SPHX_CHECK(plain_density(s.x, s.y, s.z, s.m, s.h, n, 0, n, s.rho, tmpi, st));
+    SPHX_CHECK(plain_eos(s.rho, s.u, p.gamma, n, s.P, s.c, st));
+    SPHX_CHECK(tmark(st, PH_DENS, 1));
+    const double *eps = p.eps_from_h ? s.h : s.eps;
+    SPHX_CHECK(tmark(st, PH_GRAV, 0));
+    SPHX_CHECK(plain_gravity(s.x, s.y, s.z, s.m, eps, p.G, n, 0, n, s.gx, s.gy, s.gz,
+                             with_pot ? s.pot : nullptr, st));
+    SPHX_CHECK(tmark(st, PH_GRAV, 1));
+    SPHX_CHECK(tmark(st, PH_DIVCURL, 0));
+    SPHX_CHECK(plain_divcurl(s.x, s.y, s.z, s.vx, s.vy, s.vz, s.rho, s.m, s.h, n, 0, n, s.divv,
+                             s.curlv, st));
+    SPHX_CHECK(tmark(st, PH_DIVCURL, 1));
+    SPHX_CHECK(tmark(st, PH_HYDRO, 0));
+    SPHX_CHECK(plain_acc_sph(s, p, n, 0, n, s.ax, s.ay, s.az, st));
+    SPHX_CHECK(plain_du(s, p, n, 0, n, s.du, st));
+    SPHX_CHECK(tmark(st, PH_HYDRO, 1));
+    SPHX_CHECK(sphx_combine(*this, st));
+    return SPHX_OK;
+}
+
+int Engine::step(hipStream_t st, int nsteps) {
+    for (int k = 0; k < nsteps; ++k) {
+        SPHX_CHECK(sphx_zero_counters(counters, st));
+        SPHX_CHECK(tmark(st, PH_SORT, 0));
+        SPHX_CHECK(kick_drift(st));
+        if (mode == SPHX_MIXED) SPHX_CHECK(sort_and_permute(st, true));
+        SPHX_CHECK(tmark(st, PH_SORT, 1));
+        if (mode == SPHX_MIXED) SPHX_CHECK(forces(st, true, false));
+        else SPHX_CHECK(plain_forces(st, true, false));
+        SPHX_CHECK(tmark(st, PH_FINISH, 0));
+        SPHX_CHECK(finish_step(st));
+        SPHX_CHECK(tmark(st, PH_FINISH, 1));
+        SPHX_CHECK(tcollect());
+        ++steps;
+    }
+    return SPHX_OK;
+}
+
+int Engine::read_counters(sphx_counters *out) {
+    int64_t c[CNT_SLOTS];
+    SPHX_HIP(hipMemcpy(c, counters, sizeof(c), hipMemcpyDeviceToHost));
+    out->n = n;
+    out->steps = steps;
+    out->h_iters_max = c[CNT_H_ITERS_MAX];
+    out->h_iters_sum = c[CNT_H_ITERS_SUM];
+    out->h_capped = c[CNT_H_CAPPED];
+    out->cool_capped = c[CNT_COOL_CAPPED];
+    out->cand_buckets = c[CNT_CAND];
+    out->buckets = t.nb;
+    return SPHX_OK;
+}
+
+int Engine::diagnostics(hipStream_t st, bool with_pe, sphx_diag *out) {
+    if (with_pe) {
+        if (mode == SPHX_MIXED) {
+            SPHX_CHECK(build_boxes(st, true));
+            SPHX_CHECK(sphx_gravity_run(*this, gcfg, true, st));
+        } else {
+            const double *eps = p.eps_from_h ? s.h : s.eps;
+            SPHX_CHECK(plain_gravity(s.x, s.y, s.z, s.m, eps, p.G, n, 0, n, s.gx, s.gy, s.gz,
+                                     s.pot, st));
+        }
+    }
+    return sphx_diag_reduce(s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m, s.u, with_pe ? s.pot : nullptr, n,
+                            tmpd, out, st);
+}
+
+// ---------------------------------------------------------------------------
+// Argument checks
+// ---------------------------------------------------------------------------
+static int check_rows(int64_t n, int64_t r0, int64_t r1) {
+    if (n < 0 || r0 < 0 || r1 < r0 || r1 > n) {
+        sphx_set_error("bad particle count or row range");
+        return SPHX_ERR_ARG;
+    }
+    return SPHX_OK;
+}
+
+static int check_mode(int mode) {
+    if (mode != SPHX_MIXED && mode != SPHX_FP64) {
+        sphx_set_error("mode must be SPHX_MIXED or SPHX_FP64");
+        return SPHX_ERR_ARG;
+    }
+    return SPHX_OK;
+}
+
+static int check_params(const sphx_params *p) {
+    if (!p || p->nngb <= 0 || p->ndown > p->nngb || p->nup < p->nngb || p->max_iter <= 0 ||
+        !(p->gamma > 1.0) || (p->av_form != 0 && p->av_form != 1)) {
+        sphx_set_error("invalid sphx_params");
+        return SPHX_ERR_ARG;
+    }
+    return SPHX_OK;
+}
+
+struct sphx_ctx {
+    Engine E;
+};
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        hipGetDevice(&prev);
+        hipSetDevice(dev);
+    }
+    ~DeviceGuard() { hipSetDevice(prev); }
+};
+
+// Load positions (+ optional SoA extras) into the context engine as a single
+// run and, in mixed mode, put them in Morton order.  Afterwards s.id maps the
+// engine's slots to the caller's indices.
+static int ctx_load(sphx_ctx *ctx, int64_t n, const double *pos, const double *m, const double *h,
+                    const double *v, bool sort, hipStream_t st) {
+    Engine &E = ctx->E;
+    SPHX_CHECK(E.reserve(std::max<int64_t>(n, 64)));
+    SPHX_CHECK(E.set_runs({n}));
+    if (n == 0) return SPHX_OK;
+    SPHX_CHECK(sphx_aos_to_soa(pos, n, E.s.x, E.s.y, E.s.z, st));
+    if (v) SPHX_CHECK(sphx_aos_to_soa(v, n, E.s.vx, E.s.vy, E.s.vz, st));
+    if (m) k_copy_d<<<gu(n), UT, 0, st>>>(m, E.s.m, n);
+    if (h) k_copy_d<<<gu(n), UT, 0, st>>>(h, E.s.h, n);
+    k_iota<<<gu(n), UT, 0, st>>>(E.s.id, n);
+    k_fill_i<<<gu(n), UT, 0, st>>>(E.s.run, 0, n);
+    SPHX_HIP(hipGetLastError());
+    if (sort) SPHX_CHECK(E.sort_and_permute(st, true));
+    return SPHX_OK;
+}
+
+static int gather_in(Engine &E, const double *src, double *dst, hipStream_t st) {
+    if (!src || E.n == 0) return SPHX_OK;
+    k_gather_by_id<<<gu(E.n), UT, 0, st>>>(src, E.s.id, dst, E.n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+static int scatter_out(Engine &E, const double *val, int64_t r0, int64_t r1, double *out,
+                       hipStream_t st) {
+    if (!out || E.n == 0) return SPHX_OK;
+    k_scatter_d<<<gu(E.n), UT, 0, st>>>(val, E.s.id, r0, r1, out, E.n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+static int scatter_out_i(Engine &E, const int32_t *val, int64_t r0, int64_t r1, int32_t *out,
+                         hipStream_t st) {
+    if (!out || E.n == 0) return SPHX_OK;
+    k_scatter_i<<<gu(E.n), UT, 0, st>>>(val, E.s.id, r0, r1, out, E.n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+static int scatter_out_aos(Engine &E, const double *a, const double *b, const double *c, int64_t r0,
+                           int64_t r1, double *out, hipStream_t st) {
+    if (!out || E.n == 0) return SPHX_OK;
+    k_scatter_aos<<<gu(E.n), UT, 0, st>>>(a, b, c, E.s.id, r0, r1, out, E.n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stateless drop-ins
+// ---------------------------------------------------------------------------
+extern "C" int sphx_ctx_create(int device, sphx_ctx **out) {
+    if (!out) {
+        sphx_set_error("null output pointer");
+        return SPHX_ERR_ARG;
+    }
+    int ndev = 0;
+    SPHX_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) {
+        sphx_set_error("no such HIP device");
+        return SPHX_ERR_ARG;
+    }
+    sphx_ctx *c = new sphx_ctx();
+    c->E.device = device;
+    sphx_default_params(&c->E.p);
+    *out = c;
+    return SPHX_OK;
+}
+
+extern "C" int sphx_ctx_destroy(sphx_ctx *ctx) {
+    if (!ctx) return SPHX_OK;
+    delete ctx;
+    return SPHX_OK;
+}
+
+#define CTX_PROLOGUE(ctx)                          \
+    if (!ctx) {                                    \
+        sphx_set_error("null context");            \
+        return SPHX_ERR_ARG;                       \
+    }                                              \
+    DeviceGuard guard_(ctx->E.device);             \
+    hipStream_t st = (hipStream_t)stream;
+
+extern "C" int sphx_getDensity(sphx_ctx *ctx, int mode, int64_t n, const double *pos,
+                               const double *m, const double *h, int64_t row_beg, int64_t row_end,
+                               double *rho, int32_t *nngb, void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_mode(mode));
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    if (n == 0 || row_end == row_beg) return SPHX_OK;
+    if (!pos || !m || !h || !rho) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    if (mode == SPHX_FP64) {
+        SPHX_CHECK(ctx_load(ctx, n, pos, m, h, nullptr, false, st));
+        return plain_density(E.s.x, E.s.y, E.s.z, E.s.m, E.s.h, n, row_beg, row_end, rho, nngb, st);
+    }
+    SPHX_CHECK(ctx_load(ctx, n, pos, m, h, nullptr, true, st));
+    SPHX_CHECK(E.build_boxes(st, true));
+    k_fill_d<<<gu(n), UT, 0, st>>>(E.s.u, 0.0, n);
+    SPHX_CHECK(E.density_eos(st));
+    SPHX_CHECK(scatter_out(E, E.s.rho, row_beg, row_end, rho, st));
+    SPHX_CHECK(scatter_out_i(E, E.tmpi, row_beg, row_end, nngb, st));
+    return SPHX_OK;
+}
+
+extern "C" int sphx_eos(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, double gamma,
+                        double *P, double *c, void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n < 0 || (n > 0 && (!rho || !u || !P || !c))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    return plain_eos(rho, u, gamma, n, P, c, st);
+}
+
+extern "C" int sphx_smoothing(sphx_ctx *ctx, int mode, int64_t n, const double *pos,
+                              const double *h_prev, const int32_t *k_prev, const double *divV,
+                              double dt, const sphx_params *p, int64_t row_beg, int64_t row_end,
+                              double *h_out, int32_t *k_out, int32_t *iters_out, void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_mode(mode));
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    SPHX_CHECK(check_params(p));
+    if (n == 0 || row_end == row_beg) return SPHX_OK;
+    if (!pos || !h_prev || !h_out || !k_out) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    E.p = *p;
+    bool pred = k_prev && divV;
+    SPHX_CHECK(E.reserve(std::max<int64_t>(n, 64)));
+    SPHX_CHECK(sphx_zero_counters(E.counters, st));
+    if (mode == SPHX_FP64) {
+        SPHX_CHECK(ctx_load(ctx, n, pos, nullptr, nullptr, nullptr, false, st));
+        SPHX_CHECK(plain_smoothing(E.s.x, E.s.y, E.s.z, h_prev, pred ? k_prev : nullptr,
+                                   pred ? divV : nullptr, nullptr, dt, nullptr, *p, n, row_beg,
+                                   row_end, h_out, k_out, iters_out, E.counters, st));
+    } else {
+        SPHX_CHECK(ctx_load(ctx, n, pos, nullptr, h_prev, nullptr, true, st));
+        if (pred) {
+            SPHX_CHECK(gather_in(E, divV, E.s.divv, st));
+            k_gather_by_id_i<<<gu(n), UT, 0, st>>>(k_prev, E.s.id, E.s.k, n);
+            SPHX_HIP(hipGetLastError());
+        }
+        E.run_params.assign(1, RunParams{dt, 0, p->eps_from_h});
+        SPHX_CHECK(E.upload_run_params(st));
+        SPHX_CHECK(E.build_boxes(st, false));
+        SPHX_CHECK(E.smoothing(st, pred));
+        SPHX_CHECK(scatter_out(E, E.s.h, row_beg, row_end, h_out, st));
+        SPHX_CHECK(scatter_out_i(E, E.s.k, row_beg, row_end, k_out, st));
+        SPHX_CHECK(scatter_out_i(E, E.h_iters, row_beg, row_end, iters_out, st));
+    }
+    SPHX_HIP(hipStreamSynchronize(st));
+    sphx_counters c{};
+    SPHX_CHECK(E.read_counters(&c));
+    return c.h_capped > 0 ? SPHX_CAPPED : SPHX_OK;
+}
+
+extern "C" int sphx_div_curlVel(sphx_ctx *ctx, int mode, int64_t n, const double *pos,
+                                const double *v, const double *rho, const double *m,
+                                const double *h, int64_t row_beg, int64_t row_end, double *divV,
+                                double *curlV, void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_mode(mode));
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    if (n == 0 || row_end == row_beg) return SPHX_OK;
+    if (!pos || !v || !rho || !m || !h || !divV || !curlV) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    if (mode == SPHX_FP64) {
+        SPHX_CHECK(ctx_load(ctx, n, pos, m, h, v, false, st));
+        return plain_divcurl(E.s.x, E.s.y, E.s.z, E.s.vx, E.s.vy, E.s.vz, rho, E.s.m, E.s.h, n,
+                             row_beg, row_end, divV, curlV, st);
+    }
+    SPHX_CHECK(ctx_load(ctx, n, pos, m, h, v, true, st));
+    SPHX_CHECK(gather_in(E, rho, E.s.rho, st));
+    SPHX_CHECK(E.build_boxes(st, true));
+    SPHX_CHECK(E.divcurl(st));
+    SPHX_CHECK(scatter_out(E, E.s.divv, row_beg, row_end, divV, st));
+    SPHX_CHECK(scatter_out(E, E.s.curlv, row_beg, row_end, curlV, st));
+    return SPHX_OK;
+}
+
+extern "C" int sphx_getAcc_sph(sphx_ctx *ctx, int mode, int64_t n, const double *pos,
+                               const double *v, const double *rho, const double *P,
+                               const double *c, const double *h, const double *m,
+                               const double *divV, const double *curlV, const sphx_params *p,
+                               int64_t row_beg, int64_t row_end, double *acc, double *dudt,
+                               void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_mode(mode));
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    SPHX_CHECK(check_params(p));
+    if (n == 0 || row_end == row_beg) return SPHX_OK;
+    if (!pos || !v || !rho || !P || !c || !h || !m) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    if (p->balsara && (!divV || !curlV)) { sphx_set_error("Balsara switch needs divV and curlV"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    E.p = *p;
+    bool sort = mode == SPHX_MIXED;
+    SPHX_CHECK(ctx_load(ctx, n, pos, m, h, v, sort, st));
+    SPHX_CHECK(gather_in(E, rho, E.s.rho, st));
+    SPHX_CHECK(gather_in(E, P, E.s.P, st));
+    SPHX_CHECK(gather_in(E, c, E.s.c, st));
+    if (p->balsara) {
+        SPHX_CHECK(gather_in(E, divV, E.s.divv, st));
+        SPHX_CHECK(gather_in(E, curlV, E.s.curlv, st));
+    }
+    if (mode == SPHX_FP64) {
+        SPHX_CHECK(plain_acc_sph(E.s, *p, n, 0, n, E.s.ax, E.s.ay, E.s.az, st));
+        SPHX_CHECK(plain_du(E.s, *p, n, 0, n, E.s.du, st));
+    } else {
+        SPHX_CHECK(E.build_boxes(st, true));
+        SPHX_CHECK(E.hydro(st));
+    }
+    SPHX_CHECK(scatter_out_aos(E, E.s.ax, E.s.ay, E.s.az, row_beg, row_end, acc, st));
+    SPHX_CHECK(scatter_out(E, E.s.du, row_beg, row_end, dudt, st));
+    return SPHX_OK;
+}
+
+extern "C" int sphx_getAcc_g_smth(sphx_ctx *ctx, int mode, int64_t n, const double *pos,
+                                  const double *m, const double *eps, double G, int64_t row_beg,
+                                  int64_t row_end, double *acc, double *pot, void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_mode(mode));
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    if (n == 0 || row_end == row_beg) return SPHX_OK;
+    if (!pos || !m || !eps || !acc) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    E.p.G = G;
+    E.p.eps_from_h = 0;
+    if (mode == SPHX_FP64) {
+        SPHX_CHECK(ctx_load(ctx, n, pos, m, nullptr, nullptr, false, st));
+        SPHX_CHECK(plain_gravity(E.s.x, E.s.y, E.s.z, E.s.m, eps, G, n, row_beg, row_end, E.s.gx,
+                                 E.s.gy, E.s.gz, pot, st));
+        return sphx_soa_to_aos(E.s.gx, E.s.gy, E.s.gz, row_end - row_beg, acc, st);
+    }
+    SPHX_CHECK(ctx_load(ctx, n, pos, m, nullptr, nullptr, false, st));
+    k_copy_d<<<gu(n), UT, 0, st>>>(eps, E.s.eps, n);
+    k_copy_d<<<gu(n), UT, 0, st>>>(eps, E.s.h, n);
+    SPHX_HIP(hipGetLastError());
+    SPHX_CHECK(E.sort_and_permute(st, true));
+    SPHX_CHECK(E.build_boxes(st, true));
+    SPHX_CHECK(sphx_gravity_run(E, E.gcfg, pot != nullptr, st));
+    SPHX_CHECK(scatter_out_aos(E, E.s.gx, E.s.gy, E.s.gz, row_beg, row_end, acc, st));
+    if (pot) SPHX_CHECK(scatter_out(E, E.s.pot, row_beg, row_end, pot, st));
+    return SPHX_OK;
+}
+
+extern "C" int sphx_neighbors(sphx_ctx *ctx, int64_t n, const double *pos, const double *h,
+                              int64_t row_beg, int64_t row_end, int64_t *offsets, int32_t *idx,
+                              int64_t capacity, int64_t *total, void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    if (!offsets || !total) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    int64_t rows = row_end - row_beg;
+    *total = 0;
+    if (n == 0 || rows == 0) {
+        int64_t z = 0;
+        SPHX_HIP(hipMemcpy(offsets, &z, sizeof(z), hipMemcpyHostToDevice));
+        return SPHX_OK;
+    }
+    Engine &E = ctx->E;
+    SPHX_CHECK(ctx_load(ctx, n, pos, nullptr, h, nullptr, true, st));
+    SPHX_CHECK(E.build_boxes(st, true));
+    SPHX_CHECK(sphx_neighbor_pass(E, row_beg, row_end, E.tmpi, nullptr, nullptr, st));
+    std::vector<int32_t> cnt(rows);
+    SPHX_HIP(hipMemcpyAsync(cnt.data(), E.tmpi, sizeof(int32_t) * rows, hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    std::vector<int64_t> off(rows + 1, 0);
+    for (int64_t r = 0; r < rows; ++r) off[r + 1] = off[r] + cnt[r];
+    SPHX_HIP(hipMemcpy(offsets, off.data(), sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice));
+    *total = off[rows];
+    if (!idx || off[rows] > capacity) return SPHX_OK;
+    SPHX_CHECK(sphx_neighbor_pass(E, row_beg, row_end, nullptr, offsets, idx, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    return SPHX_OK;
+}
+
+static int copy_cool(Engine &E, const sphx_cooling *cool) {
+    if (!cool) {
+        E.has_cool = false;
+        return SPHX_OK;
+    }
+    if (cool->ntab < 2 || !cool->T || !cool->L || !(cool->unit_u_cgs > 0) ||
+        !(cool->unit_rho_cgs > 0) || !(cool->unit_time_s > 0) || !(cool->tol > 0) ||
+        cool->max_iter <= 0) {
+        sphx_set_error("invalid sphx_cooling");
+        return SPHX_ERR_ARG;
+    }
+    for (int k = 1; k < cool->ntab; ++k)
+        if (!(cool->T[k] > cool->T[k - 1])) {
+            sphx_set_error("cooling table T must increase");
+            return SPHX_ERR_ARG;
+        }
+    if (E.cool_T) hipFree(E.cool_T);
+    if (E.cool_L) hipFree(E.cool_L);
+    E.cool_T = E.cool_L = nullptr;
+    SPHX_HIP(hipMalloc(&E.cool_T, sizeof(double) * cool->ntab));
+    SPHX_HIP(hipMalloc(&E.cool_L, sizeof(double) * cool->ntab));
+    SPHX_HIP(hipMemcpy(E.cool_T, cool->T, sizeof(double) * cool->ntab, hipMemcpyHostToDevice));
+    SPHX_HIP(hipMemcpy(E.cool_L, cool->L, sizeof(double) * cool->ntab, hipMemcpyHostToDevice));
+    E.cool = *cool;
+    E.cool.T = nullptr;
+    E.cool.L = nullptr;
+    E.has_cool = true;
+    return SPHX_OK;
+}
+
+extern "C" int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, double dt,
+                         const sphx_cooling *cool, double *u_out, int32_t *iters, void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n < 0 || (n > 0 && (!rho || !u || !u_out))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    SPHX_CHECK(E.reserve(64));
+    SPHX_CHECK(copy_cool(E, cool));
+    if (!E.has_cool) { sphx_set_error("cooling table required"); return SPHX_ERR_ARG; }
+    if (n == 0) return SPHX_OK;
+    SPHX_CHECK(sphx_zero_counters(E.counters, st));
+    SPHX_CHECK(sphx_cool_launch(sphx_engine_cool(E), rho, u, dt, nullptr, nullptr, nullptr, n,
+                                u_out, iters, E.counters, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    sphx_counters c{};
+    SPHX_CHECK(E.read_counters(&c));
+    return c.cool_capped > 0 ? SPHX_CAPPED : SPHX_OK;
+}
+
+extern "C" int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, const double *v,
+                                const double *m, const double *u, const double *eps, double G,
+                                sphx_diag *out, void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n < 0 || !v || !m || !out) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    SPHX_CHECK(ctx_load(ctx, n, pos ? pos : v, m, nullptr, v, false, st));
+    const double *pot = nullptr;
+    if (pos && eps && n > 0) {
+        SPHX_CHECK(plain_gravity(E.s.x, E.s.y, E.s.z, E.s.m, eps, G, n, 0, n, E.s.gx, E.s.gy,
+                                 E.s.gz, E.s.pot, st));
+        pot = E.s.pot;
+    }
+    return sphx_diag_reduce(pos ? E.s.x : nullptr, E.s.y, E.s.z, E.s.vx, E.s.vy, E.s.vz, E.s.m, u,
+                            pot, n, E.tmpd, out, st);
+}
+
+extern "C" int sphx_morton_sort(sphx_ctx *ctx, int64_t n, const double *pos, uint64_t *keys_out,
+                                int32_t *perm_out, void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n < 0 || (n > 0 && (!pos || !keys_out || !perm_out))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    if (n == 0) return SPHX_OK;
+    Engine &E = ctx->E;
+    SPHX_CHECK(ctx_load(ctx, n, pos, nullptr, nullptr, nullptr, false, st));
+    SPHX_CHECK(E.sort_and_permute(st, false));
+    SPHX_HIP(hipMemcpyAsync(keys_out, E.keys, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
+    SPHX_HIP(hipMemcpyAsync(perm_out, E.perm, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    return SPHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Resident plan
+// ---------------------------------------------------------------------------
+struct sphx_plan {
+    Engine E;
+    bool loaded = false;
+    bool inited = false;
+};
+
+#define PLAN_PROLOGUE(plan)                        \
+    if (!plan) {                                   \
+        sphx_set_error("null plan");               \
+        return SPHX_ERR_ARG;                       \
+    }                                              \
+    DeviceGuard guard_(plan->E.device);            \
+    hipStream_t st = (hipStream_t)stream;
+
+extern "C" int sphx_plan_create(int device, int mode, const sphx_params *p,
+                                const sphx_cooling *cool, int64_t capacity, sphx_plan **out) {
+    if (!out) { sphx_set_error("null output pointer"); return SPHX_ERR_ARG; }
+    SPHX_CHECK(check_mode(mode));
+    SPHX_CHECK(check_params(p));
+    if (capacity <= 0 || capacity > INT32_MAX) { sphx_set_error("bad capacity"); return SPHX_ERR_ARG; }
+    int ndev = 0;
+    SPHX_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) { sphx_set_error("no such HIP device"); return SPHX_ERR_ARG; }
+    DeviceGuard guard_(device);
+    sphx_plan *pl = new sphx_plan();
+    pl->E.device = device;
+    pl->E.mode = mode;
+    pl->E.p = *p;
+    int rc = pl->E.reserve(std::max<int64_t>(capacity, 64));
+    if (rc >= 0) rc = copy_cool(pl->E, cool);
+    if (rc < 0) {
+        delete pl;
+        return rc;
+    }
+    *out = pl;
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, const double *vel,
+                              const double *u, const double *m, const double *h, const double *eps,
+                              void *stream) {
+    PLAN_PROLOGUE(plan);
+    Engine &E = plan->E;
+    if (n <= 0 || n > E.cap) { sphx_set_error("particle count exceeds the plan capacity"); return SPHX_ERR_ARG; }
+    if (!pos || !vel || !u || !m || !h) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    if (!E.p.eps_from_h && !eps) { sphx_set_error("eps required when eps_from_h == 0"); return SPHX_ERR_ARG; }
+    SPHX_CHECK(E.set_runs({n}));
+    SPHX_CHECK(sphx_aos_to_soa(pos, n, E.s.x, E.s.y, E.s.z, st));
+    SPHX_CHECK(sphx_aos_to_soa(vel, n, E.s.vx, E.s.vy, E.s.vz, st));
+    k_copy_d<<<gu(n), UT, 0, st>>>(u, E.s.u, n);
+    k_copy_d<<<gu(n), UT, 0, st>>>(m, E.s.m, n);
+    k_copy_d<<<gu(n), UT, 0, st>>>(h, E.s.h, n);
+    k_copy_d<<<gu(n), UT, 0, st>>>(eps ? eps : h, E.s.eps, n);
+    k_fill_d<<<gu(n), UT, 0, st>>>(E.s.ut_prev, 0.0, n);
+    k_fill_d<<<gu(n), UT, 0, st>>>(E.s.divv, 0.0, n);
+    k_fill_d<<<gu(n), UT, 0, st>>>(E.s.ax, 0.0, n);
+    k_fill_d<<<gu(n), UT, 0, st>>>(E.s.ay, 0.0, n);
+    k_fill_d<<<gu(n), UT, 0, st>>>(E.s.az, 0.0, n);
+    k_fill_i<<<gu(n), UT, 0, st>>>(E.s.k, E.p.nngb, n);
+    k_iota<<<gu(n), UT, 0, st>>>(E.s.id, n);
+    k_fill_i<<<gu(n), UT, 0, st>>>(E.s.run, 0, n);
+    SPHX_HIP(hipGetLastError());
+    E.steps = 0;
+    plan->loaded = true;
+    plan->inited = false;
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_init(sphx_plan *plan, void *stream) {
+    PLAN_PROLOGUE(plan);
+    if (!plan->loaded) { sphx_set_error("plan has no particles loaded"); return SPHX_ERR_STATE; }
+    Engine &E = plan->E;
+    E.run_params.assign(1, RunParams{0.0, 0, E.p.eps_from_h});
+    SPHX_CHECK(E.upload_run_params(st));
+    SPHX_CHECK(sphx_zero_counters(E.counters, st));
+    if (E.mode == SPHX_MIXED) {
+        SPHX_CHECK(E.sort_and_permute(st, true));
+        SPHX_CHECK(E.forces(st, false, false));
+    } else {
+        SPHX_CHECK(E.plain_forces(st, false, false));
+    }
+    SPHX_CHECK(E.bootstrap_finish(st));
+    plan->inited = true;
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32_t use_cooling,
+                              void *stream) {
+    PLAN_PROLOGUE(plan);
+    if (!plan->inited) { sphx_set_error("call sphx_plan_init first"); return SPHX_ERR_STATE; }
+    if (n_steps < 0 || !(dt > 0.0)) { sphx_set_error("bad step count or dt"); return SPHX_ERR_ARG; }
+    Engine &E = plan->E;
+    if (use_cooling && !E.has_cool) { sphx_set_error("plan has no cooling table"); return SPHX_ERR_ARG; }
+    RunParams rp{dt, use_cooling ? 1 : 0, E.p.eps_from_h};
+    if (E.run_params.size() != 1 || E.run_params[0].dt != rp.dt || E.run_params[0].use_cool != rp.use_cool) {
+        E.run_params.assign(1, rp);
+        SPHX_CHECK(E.upload_run_params(st));
+    }
+    bool keep_cool = E.has_cool;
+    if (!use_cooling) E.has_cool = false;
+    int rc = E.step(st, n_steps);
+    E.has_cool = keep_cool;
+    return rc;
+}
+
+extern "C" int sphx_plan_store(sphx_plan *plan, double *pos, double *vel, double *u, double *h,
+                               double *rho, double *acc, double *acc_g, double *dudt, int32_t *k,
+                               void *stream) {
+    PLAN_PROLOGUE(plan);
+    if (!plan->loaded) { sphx_set_error("plan has no particles loaded"); return SPHX_ERR_STATE; }
+    Engine &E = plan->E;
+    int64_t n = E.n;
+    SPHX_CHECK(scatter_out_aos(E, E.s.x, E.s.y, E.s.z, 0, n, pos, st));
+    SPHX_CHECK(scatter_out_aos(E, E.s.vx, E.s.vy, E.s.vz, 0, n, vel, st));
+    SPHX_CHECK(scatter_out(E, E.s.u, 0, n, u, st));
+    SPHX_CHECK(scatter_out(E, E.s.h, 0, n, h, st));
+    SPHX_CHECK(scatter_out(E, E.s.rho, 0, n, rho, st));
+    SPHX_CHECK(scatter_out_aos(E, E.s.ax, E.s.ay, E.s.az, 0, n, acc, st));
+    SPHX_CHECK(scatter_out_aos(E, E.s.gx, E.s.gy, E.s.gz, 0, n, acc_g, st));
+    SPHX_CHECK(scatter_out(E, E.s.du, 0, n, dudt, st));
+    SPHX_CHECK(scatter_out_i(E, E.s.k, 0, n, k, st));
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_query(sphx_plan *plan, sphx_counters *out) {
+    if (!plan || !out) { sphx_set_error("null argument"); return SPHX_ERR_ARG; }
+    DeviceGuard guard_(plan->E.device);
+    SPHX_HIP(hipDeviceSynchronize());
+    return plan->E.read_counters(out);
+}
+
+extern "C" int sphx_plan_diagnostics(sphx_plan *plan, int32_t with_pe, sphx_diag *out,
+                                     void *stream) {
+    PLAN_PROLOGUE(plan);
+    if (!out) { sphx_set_error("null argument"); return SPHX_ERR_ARG; }
+    if (!plan->loaded) { sphx_set_error("plan has no particles loaded"); return SPHX_ERR_STATE; }
+    return plan->E.diagnostics(st, with_pe != 0, out);
+}
+
+extern "C" int sphx_plan_destroy(sphx_plan *plan) {
+    if (!plan) return SPHX_OK;
+    delete plan;
+    return SPHX_OK;
+}
+
+// Gravity launch configuration of a plan (used by the autotuner).
+extern "C" int sphx_plan_set_gravity_config(sphx_plan *plan, int32_t ipl, int32_t tj, int32_t js) {
+    if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
+    if (!(ipl == 1 || ipl == 2 || ipl == 4) || !(tj == 256 || tj == 512 || tj == 1024) || js < 1 ||
+        js > 64 || (ipl == 1 && tj == 1024) || (ipl == 4 && tj == 1024)) {
+        sphx_set_error("unsupported gravity configuration");
+        return SPHX_ERR_ARG;
+    }
+    plan->E.gcfg = GravConfig{ipl, tj, js};
+    return SPHX_OK;
+}
+
+extern "C" int sphx_ctx_set_gravity_config(sphx_ctx *ctx, int32_t ipl, int32_t tj, int32_t js) {
+    if (!ctx) { sphx_set_error("null context"); return SPHX_ERR_ARG; }
+    if (!(ipl == 1 || ipl == 2 || ipl == 4) || !(tj == 256 || tj == 512 || tj == 1024) || js < 1 ||
+        js > 64 || (ipl == 1 && tj == 1024) || (ipl == 4 && tj == 1024)) {
+        sphx_set_error("unsupported gravity configuration");
+        return SPHX_ERR_ARG;
+    }
+    ctx->E.gcfg = GravConfig{ipl, tj, js};
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_set_timing(sphx_plan *plan, int32_t on) {
+    if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
+    plan->E.timing = on != 0;
+    for (int k = 0; k < PH_COUNT; ++k) {
+        plan->E.phase_ms[k] = 0.0;
+        plan->E.phase_n[k] = 0;
+    }
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int32_t nphase) {
+    if (!plan || !ms || !count || nphase < PH_COUNT) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    for (int k = 0; k < PH_COUNT; ++k) {
+        ms[k] = plan->E.phase_ms[k];
+        count[k] = plan->E.phase_n[k];
+    }
+    return SPHX_OK;
+}

@@ -1,0 +1,484 @@
+// Bucketed SPH pair passes for the mixed-precision path: smoothing-length
+// iteration, density + EOS, velocity divergence/curl, hydro acceleration +
+// du/dt.  One wavefront per 64-particle bucket (one particle per lane); the
+// candidate buckets of the same run are found by culling super-bucket and
+// bucket AABBs in FP64, staged through LDS, and tested pair by pair with the
+// oracle's FP64 distance.  Kernel values and pair sums run in FP32 with
+// Kahan-compensated accumulation.
+//
+// Reference kernels replaced (11_Dec_2022/GPU_sph/Important_Test_Simulations/
+// Evrard_collapse/myCppSPHLibs_v2.h): smoothing_h 7-64, getDensity 69-114,
+// div_curlVel 247-322, acc_sph 327-417, get_dU 634-719,
+// getPressure_Adiabatic/getCsound_Adiabatic 766-789.
+#include "sphx_internal.h"
+
+namespace {
+
+// Walk the candidate buckets of bucket `b` (same run).  `reach_s(S)` and
+// `reach_b(J)` give the reach against super S and bucket J; `body(J)` is
+// called once per surviving bucket with the whole wavefront converged.
+template <class ReachS, class ReachB, class Body>
+__device__ __forceinline__ void for_candidates(const Tables &t, int b, int lane, ReachS reach_s,
+                                               ReachB reach_b, Body body) {
+    int r = t.b_run[b];
+    double axl = t.bxl[b], axh = t.bxh[b], ayl = t.byl[b], ayh = t.byh[b], azl = t.bzl[b],
+           azh = t.bzh[b];
+    int s0 = t.r_sbeg[r], s1 = t.r_send[r];
+    for (int sb = s0; sb < s1; sb += 64) {
+        int S = sb + lane;
+        bool sp = false;
+        if (S < s1) {
+            double d2 = sphx_box_d2(axl, axh, ayl, ayh, azl, azh, t.sxl[S], t.sxh[S], t.syl[S],
+                                    t.syh[S], t.szl[S], t.szh[S]);
+            sp = sphx_within(d2, reach_s(S));
+        }
+        uint64_t smask = __ballot(sp);
+        while (smask) {
+            int S2 = sb + __builtin_ctzll(smask);
+            smask &= smask - 1;
+            int j0 = t.s_bbeg[S2], j1 = t.s_bend[S2];
+            int J = j0 + lane;
+            bool bp = false;
+            if (J < j1) {
+                double d2 = sphx_box_d2(axl, axh, ayl, ayh, azl, azh, t.bxl[J], t.bxh[J],
+                                        t.byl[J], t.byh[J], t.bzl[J], t.bzh[J]);
+                bp = sphx_within(d2, reach_b(J));
+            }
+            uint64_t bmask = __ballot(bp);
+            while (bmask) {
+                int J2 = j0 + __builtin_ctzll(bmask);
+                bmask &= bmask - 1;
+                body(J2);
+            }
+        }
+    }
+}
+
+struct SmoothArgs {
+    int32_t nngb, ndown, nup, max_iter;
+    int32_t predictor;
+    const RunParams *rp;
+    int32_t *h_iters;  // per particle, nullable
+    int64_t *counters;
+};
+
+__global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a) {
+    __shared__ double sx[64], sy[64], sz[64];
+    int b = blockIdx.x;
+    int lane = threadIdx.x;
+    int cnt = t.b_cnt[b];
+    int i = t.b_beg[b] + lane;
+    bool ok = lane < cnt;
+    int r = t.b_run[b];
+    int64_t nrun = (int64_t)t.r_pend[r] - t.r_pbeg[r];
+    double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
+    double hprev = ok ? s.h[i] : 0.0;
+    double sr;
+    {
+#pragma clang fp contract(off)
+        sr = 2.0 * hprev;
+        if (a.predictor && ok) {
+            double dt = a.rp[r].dt;
+            double f = sphx_h_factor(a.nngb, s.k[i]);
+            double t1 = 0.5 * hprev;
+            double t2 = 1.0 + f;
+            double t3 = t1 * t2;
+            double t4 = hprev * s.divv[i];
+            double t5 = t4 * dt;
+            double t6 = t5 / 3.0;
+            double sp = 2.0 * (t3 + t6);
+            if (sp > 0.0 && sp < INFINITY) sr = sp;
+        }
+    }
+    double lo = 0.0, hi = 0.0;
+    int klo = 0, khi = 0, kfin = 0;
+    bool have_lo = false, have_hi = false, done = !ok, conv = false;
+    int it = 0;
+    while (true) {
+        bool active = ok && !done;
+        if (!__any(active)) break;
+        double R = wave_max_d(active ? sr : 0.0);
+        double s2 = sphx_sq(sr);
+        int kc = 0;
+        for_candidates(
+            t, b, lane, [&](int) { return R; }, [&](int) { return R; },
+            [&](int J) {
+                int jc = t.b_cnt[J], j0 = t.b_beg[J];
+                __syncthreads();
+                if (lane < jc) {
+                    sx[lane] = s.x[j0 + lane];
+                    sy[lane] = s.y[j0 + lane];
+                    sz[lane] = s.z[j0 + lane];
+                }
+                __syncthreads();
+                for (int q = 0; q < jc; ++q) {
+                    double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
+                    kc += (r2 <= s2) ? 1 : 0;
+                }
+            });
+        if (active) {
+#pragma clang fp contract(off)
+            ++it;
+            if (kc >= a.ndown && kc <= a.nup) {
+                done = true;
+                conv = true;
+                kfin = kc;
+            } else {
+                if (kc < a.ndown) {
+                    lo = sr; klo = kc; have_lo = true;
+                    if ((int64_t)kc >= nrun) done = true;
+                } else {
+                    hi = sr; khi = kc; have_hi = true;
+                }
+                if (!done) {
+                    double snew = sr * sphx_h_factor(a.nngb, kc);
+                    if (have_lo && have_hi) {
+                        if (!(snew > lo && snew < hi)) snew = 0.5 * (lo + hi);
+                        if (!(snew > lo && snew < hi)) done = true;
+                    }
+                    if (!done) {
+                        sr = snew;
+                        if (it >= a.max_iter) done = true;
+                    }
+                }
+            }
+        }
+    }
+    if (ok) {
+        if (!conv) {
+            if (have_hi) { sr = hi; kfin = khi; }
+            else { sr = lo; kfin = klo; }
+            atomicAdd((unsigned long long *)&a.counters[CNT_H_CAPPED], 1ull);
+        }
+        s.h[i] = 0.5 * sr;
+        s.k[i] = kfin;
+        if (a.h_iters) a.h_iters[i] = it;
+    }
+    int itmax = wave_max_i(ok ? it : 0);
+    unsigned long long itsum = 0;
+    {
+        int v = ok ? it : 0;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        itsum = (unsigned long long)v;
+    }
+    if (lane == 0) {
+        atomicMax((unsigned long long *)&a.counters[CNT_H_ITERS_MAX], (unsigned long long)itmax);
+        atomicAdd((unsigned long long *)&a.counters[CNT_H_ITERS_SUM], itsum);
+    }
+}
+
+// Cubic spline W / (sigma / h^3) and grad-W factor F / (sigma / h^5) in FP32.
+__device__ __forceinline__ float w_shape(float q) {
+    if (q <= 1.0f) return 1.0f - 1.5f * q * q + 0.75f * q * q * q;
+    float w = fmaxf(2.0f - q, 0.0f);
+    return 0.25f * w * w * w;
+}
+__device__ __forceinline__ float f_shape(float q) {
+    if (q <= 1.0f) return -3.0f + 2.25f * q;
+    float w = fmaxf(2.0f - q, 0.0f);
+    return -0.75f * w * w * __builtin_amdgcn_rcpf(q);
+}
+
+struct PairGeom {
+    float dx, dy, dz, r2, hinv, q;
+};
+
+__device__ __forceinline__ PairGeom pair_geom(double dx, double dy, double dz, double r2,
+                                              double t) {
+    PairGeom g;
+    g.dx = (float)dx;
+    g.dy = (float)dy;
+    g.dz = (float)dz;
+    g.r2 = (float)r2;
+    float hij = 0.5f * (float)t;
+    g.hinv = __builtin_amdgcn_rcpf(hij);
+    float r = __builtin_amdgcn_sqrtf(g.r2);
+    g.q = r * g.hinv;
+    return g;
+}
+
+struct DensArgs {
+    double gamma;
+    int32_t *nsym;        // symmetric neighbour count, nullable
+    int64_t *counters;
+};
+
+__global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
+    __shared__ double sx[64], sy[64], sz[64], sh[64];
+    __shared__ float sm[64];
+    int b = blockIdx.x;
+    int lane = threadIdx.x;
+    int cnt = t.b_cnt[b];
+    int i = t.b_beg[b] + lane;
+    bool ok = lane < cnt;
+    double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
+    double hi = ok ? s.h[i] : 0.0;
+    double hI = t.bhmax[b];
+    KahanF acc;
+    acc.init();
+    int nn = 0;
+    int ncand = 0;
+    for_candidates(
+        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+        [&](int J) {
+            int jc = t.b_cnt[J], j0 = t.b_beg[J];
+            ++ncand;
+            __syncthreads();
+            if (lane < jc) {
+                sx[lane] = s.x[j0 + lane];
+                sy[lane] = s.y[j0 + lane];
+                sz[lane] = s.z[j0 + lane];
+                sh[lane] = s.h[j0 + lane];
+                sm[lane] = (float)s.m[j0 + lane];
+            }
+            __syncthreads();
+            for (int q = 0; q < jc; ++q) {
+                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
+                double r2 = sphx_r2(dx, dy, dz);
+                double tt = hi + sh[q];
+                if (r2 <= sphx_sq(tt)) {
+                    PairGeom g = pair_geom(dx, dy, dz, r2, tt);
+                    float h3 = g.hinv * g.hinv * g.hinv;
+                    acc.add(sm[q] * (SPHX_SIGMA_F * h3 * w_shape(g.q)));
+                    ++nn;
+                }
+            }
+        });
+    if (ok) {
+        double rho = acc.value();
+        double u = s.u[i];
+        s.rho[i] = rho;
+        s.P[i] = (a.gamma - 1.0) * rho * u;
+        s.c[i] = sqrt(a.gamma * (a.gamma - 1.0) * u);
+        if (a.nsym) a.nsym[i] = nn;
+    }
+    if (lane == 0 && a.counters)
+        atomicAdd((unsigned long long *)&a.counters[CNT_CAND], (unsigned long long)ncand);
+}
+
+__global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
+    __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
+    __shared__ float sm[64];
+    int b = blockIdx.x;
+    int lane = threadIdx.x;
+    int cnt = t.b_cnt[b];
+    int i = t.b_beg[b] + lane;
+    bool ok = lane < cnt;
+    double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
+    double hi = ok ? s.h[i] : 0.0;
+    double vxi = ok ? s.vx[i] : 0.0, vyi = ok ? s.vy[i] : 0.0, vzi = ok ? s.vz[i] : 0.0;
+    double hI = t.bhmax[b];
+    KahanF kd, kx, ky, kz;
+    kd.init(); kx.init(); ky.init(); kz.init();
+    for_candidates(
+        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+        [&](int J) {
+            int jc = t.b_cnt[J], j0 = t.b_beg[J];
+            __syncthreads();
+            if (lane < jc) {
+                int j = j0 + lane;
+                sx[lane] = s.x[j]; sy[lane] = s.y[j]; sz[lane] = s.z[j]; sh[lane] = s.h[j];
+                svx[lane] = s.vx[j]; svy[lane] = s.vy[j]; svz[lane] = s.vz[j];
+                sm[lane] = (float)s.m[j];
+            }
+            __syncthreads();
+            for (int q = 0; q < jc; ++q) {
+                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
+                double r2 = sphx_r2(dx, dy, dz);
+                double tt = hi + sh[q];
+                if (r2 <= sphx_sq(tt)) {
+                    PairGeom g = pair_geom(dx, dy, dz, r2, tt);
+                    float h5 = g.hinv * g.hinv;
+                    h5 = h5 * h5 * g.hinv;
+                    float F = SPHX_SIGMA_F * h5 * f_shape(g.q) * sm[q];
+                    float vx = (float)(vxi - svx[q]), vy = (float)(vyi - svy[q]),
+                          vz = (float)(vzi - svz[q]);
+                    kd.add(F * (vx * g.dx + vy * g.dy + vz * g.dz));
+                    kx.add(F * (vy * g.dz - vz * g.dy));
+                    ky.add(F * (vz * g.dx - vx * g.dz));
+                    kz.add(F * (vx * g.dy - vy * g.dx));
+                }
+            }
+        });
+    if (ok) {
+        double ir = 1.0 / s.rho[i];
+        s.divv[i] = -kd.value() * ir;
+        double cx = kx.value() * ir, cy = ky.value() * ir, cz = kz.value() * ir;
+        s.curlv[i] = sqrt(cx * cx + cy * cy + cz * cz);
+    }
+}
+
+struct HydroArgs {
+    float alpha, beta, eta2;
+    int32_t av_form, balsara;
+};
+
+__device__ __forceinline__ double balsara_f(double dv, double cv, double c, double h) {
+    double a = fabs(dv);
+    return a / (a + cv + 0.0001 * c / h);
+}
+
+__global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
+    __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
+    __shared__ float sm[64], srho[64], spr[64], sc[64], sf[64];
+    int b = blockIdx.x;
+    int lane = threadIdx.x;
+    int cnt = t.b_cnt[b];
+    int i = t.b_beg[b] + lane;
+    bool ok = lane < cnt;
+    double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
+    double hi = ok ? s.h[i] : 0.0;
+    double vxi = ok ? s.vx[i] : 0.0, vyi = ok ? s.vy[i] : 0.0, vzi = ok ? s.vz[i] : 0.0;
+    float rhoi = ok ? (float)s.rho[i] : 1.0f;
+    float pri = ok ? (float)(s.P[i] / (s.rho[i] * s.rho[i])) : 0.0f;
+    float ci = ok ? (float)s.c[i] : 0.0f;
+    float fi = (ok && a.balsara) ? (float)balsara_f(s.divv[i], s.curlv[i], s.c[i], s.h[i]) : 1.0f;
+    double hI = t.bhmax[b];
+    KahanF kx, ky, kz, kdu;
+    kx.init(); ky.init(); kz.init(); kdu.init();
+    for_candidates(
+        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+        [&](int J) {
+            int jc = t.b_cnt[J], j0 = t.b_beg[J];
+            __syncthreads();
+            if (lane < jc) {
+                int j = j0 + lane;
+                sx[lane] = s.x[j]; sy[lane] = s.y[j]; sz[lane] = s.z[j]; sh[lane] = s.h[j];
+                svx[lane] = s.vx[j]; svy[lane] = s.vy[j]; svz[lane] = s.vz[j];
+                double rj = s.rho[j];
+                sm[lane] = (float)s.m[j];
+                srho[lane] = (float)rj;
+                spr[lane] = (float)(s.P[j] / (rj * rj));
+                sc[lane] = (float)s.c[j];
+                sf[lane] = a.balsara ? (float)balsara_f(s.divv[j], s.curlv[j], s.c[j], s.h[j]) : 1.0f;
+            }
+            __syncthreads();
+            for (int q = 0; q < jc; ++q) {
+                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
+                double r2 = sphx_r2(dx, dy, dz);
+                double tt = hi + sh[q];
+                if (r2 <= sphx_sq(tt)) {
+                    PairGeom g = pair_geom(dx, dy, dz, r2, tt);
+                    float h5 = g.hinv * g.hinv;
+                    h5 = h5 * h5 * g.hinv;
+                    float F = SPHX_SIGMA_F * h5 * f_shape(g.q);
+                    float vx = (float)(vxi - svx[q]), vy = (float)(vyi - svy[q]),
+                          vz = (float)(vzi - svz[q]);
+                    float vr = vx * g.dx + vy * g.dy + vz * g.dz;
+                    float pij = 0.0f;
+                    if (vr <= 0.0f) {
+                        float rhob = 0.5f * (rhoi + srho[q]);
+                        float hij = 0.5f * (float)tt;
+                        if (a.av_form == 0) {
+                            float mu = hij * vr / (g.r2 + hij * hij * a.eta2);
+                            pij = (-a.alpha * 0.5f * (ci + sc[q]) * mu + a.beta * mu * mu) / rhob;
+                        } else {
+                            float w = vr / (__builtin_amdgcn_sqrtf(g.r2) + 1e-5f);
+                            pij = -0.5f * a.alpha * (ci + sc[q] - 3.0f * w) * w / rhob;
+                        }
+                        pij *= 0.5f * (fi + sf[q]);
+                    }
+                    float mF = sm[q] * F;
+                    float sc_ = mF * (pri + spr[q] + pij);
+                    kx.add(-sc_ * g.dx);
+                    ky.add(-sc_ * g.dy);
+                    kz.add(-sc_ * g.dz);
+                    kdu.add(mF * (pri + 0.5f * pij) * vr);
+                }
+            }
+        });
+    if (ok) {
+        s.ax[i] = kx.value();
+        s.ay[i] = ky.value();
+        s.az[i] = kz.value();
+        s.du[i] = kdu.value();
+    }
+}
+
+// Neighbour lists under r_ij^2 <= (h_i + h_j)^2 through the bucket culling.
+// counts == nullptr: fill mode, writing the neighbours' caller ids (s.id) at
+// offsets[id_i - r0] for targets with r0 <= id_i < r1.
+__global__ __launch_bounds__(64) void k_nbr(Tables t, Soa s, int64_t r0, int64_t r1,
+                                            int32_t *__restrict__ counts,
+                                            const int64_t *__restrict__ offsets,
+                                            int32_t *__restrict__ idx) {
+    __shared__ double sx[64], sy[64], sz[64], sh[64];
+    __shared__ int32_t sid[64];
+    int b = blockIdx.x;
+    int lane = threadIdx.x;
+    int cnt = t.b_cnt[b];
+    int i = t.b_beg[b] + lane;
+    bool ok = lane < cnt;
+    double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
+    double hi = ok ? s.h[i] : 0.0;
+    int32_t myid = ok ? s.id[i] : -1;
+    bool want = ok && myid >= r0 && myid < r1;
+    int64_t o = (want && !counts) ? offsets[myid - r0] : 0;
+    double hI = t.bhmax[b];
+    int c = 0;
+    for_candidates(
+        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+        [&](int J) {
+            int jc = t.b_cnt[J], j0 = t.b_beg[J];
+            __syncthreads();
+            if (lane < jc) {
+                sx[lane] = s.x[j0 + lane];
+                sy[lane] = s.y[j0 + lane];
+                sz[lane] = s.z[j0 + lane];
+                sh[lane] = s.h[j0 + lane];
+                sid[lane] = s.id[j0 + lane];
+            }
+            __syncthreads();
+            for (int q = 0; q < jc; ++q) {
+                double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
+                double tt = hi + sh[q];
+                if (want && r2 <= sphx_sq(tt)) {
+                    if (!counts) idx[o + c] = sid[q];
+                    ++c;
+                }
+            }
+        });
+    if (want && counts) counts[myid - r0] = c;
+}
+
+}  // namespace
+
+int sphx_neighbor_pass(Engine &E, int64_t r0, int64_t r1, int32_t *counts, const int64_t *offsets,
+                       int32_t *idx, hipStream_t st) {
+    if (E.n == 0) return SPHX_OK;
+    k_nbr<<<E.t.nb, 64, 0, st>>>(E.t, E.s, r0, r1, counts, offsets, idx);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::smoothing(hipStream_t st, bool predictor) {
+    if (n == 0) return SPHX_OK;
+    SmoothArgs a{p.nngb, p.ndown, p.nup, p.max_iter, (predictor && p.h_predictor) ? 1 : 0,
+                 d_run, h_iters, counters};
+    k_smoothing<<<t.nb, 64, 0, st>>>(t, s, a);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::density_eos(hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    DensArgs a{p.gamma, tmpi, counters};
+    k_density<<<t.nb, 64, 0, st>>>(t, s, a);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::divcurl(hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    k_divcurl<<<t.nb, 64, 0, st>>>(t, s);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::hydro(hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    HydroArgs a{(float)p.alpha, (float)p.beta, (float)(p.eta * p.eta), p.av_form, p.balsara};
+    k_hydro<<<t.nb, 64, 0, st>>>(t, s, a);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}

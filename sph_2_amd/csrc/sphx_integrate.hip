@@ -1,0 +1,290 @@
+// FP64 integrator (KDK leapfrog + u trapezoid), the implicit per-particle
+// cooling solve, energy/momentum diagnostics and small utility kernels.
+//
+// Reference: v_evolve / r_evolve / u_updater_main / u_ut_previous_updater
+// (11_Dec_2022/GPU_sph/Important_Test_Simulations/Evrard_collapse/
+// myCppSPHLibs_v2.h:441-486, 724-761) driven as in hfv_sph_gpu_v2_evrard.cu:
+// 407-482; DoCooling_h (GPU_sph/Cooling_v1/photolibs3.py:489-543); getKE /
+// getPE (GPU_sph/IC/libsx2_2t.py:579-629).
+#include "sphx_internal.h"
+
+namespace {
+
+constexpr int IT = 256;
+inline unsigned g1(int64_t n) { return (unsigned)((n + IT - 1) / IT); }
+
+#define MH_CGS 1.6726e-24
+#define KB_CGS 1.3807e-16
+
+__global__ void k_zero_i64(int64_t *p, int n) {
+    int i = threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+__global__ void k_kick_drift(Soa s, const RunParams *__restrict__ rp, int64_t n) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    double dt = rp[s.run[i]].dt;
+    double half = 0.5 * dt;
+    double vx = s.vx[i] + s.ax[i] * half;
+    double vy = s.vy[i] + s.ay[i] * half;
+    double vz = s.vz[i] + s.az[i] * half;
+    s.vx[i] = vx; s.vy[i] = vy; s.vz[i] = vz;
+    s.x[i] = s.x[i] + vx * dt;
+    s.y[i] = s.y[i] + vy * dt;
+    s.z[i] = s.z[i] + vz * dt;
+}
+
+__global__ void k_combine(Soa s, int64_t n) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    s.ax[i] = s.gx[i] + s.ax[i];
+    s.ay[i] = s.gy[i] + s.ay[i];
+    s.az[i] = s.gz[i] + s.az[i];
+}
+
+// Lambda(T): linear in T inside a bin, clamped to the end values outside.
+__device__ double cool_lambda(double T, const double *__restrict__ Tt,
+                              const double *__restrict__ Lt, int nt) {
+#pragma clang fp contract(off)
+    if (T <= Tt[0]) return Lt[0];
+    if (T >= Tt[nt - 1]) return Lt[nt - 1];
+    int lo = 0, hi = nt - 1;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (Tt[mid] <= T) lo = mid;
+        else hi = mid;
+    }
+    double f = (T - Tt[lo]) / (Tt[hi] - Tt[lo]);
+    return Lt[lo] + (Lt[hi] - Lt[lo]) * f;
+}
+
+__device__ double cool_g(double u, double u_old, double dt_s, double nH, double rho_cgs,
+                         double tfac, const CoolDev &cd) {
+#pragma clang fp contract(off)
+    double T = tfac * u;
+    double lam = cool_lambda(T, cd.T, cd.L, cd.ntab);
+    double rate = (nH * cd.heat - nH * nH * lam) / rho_cgs;
+    return u - u_old - dt_s * rate;
+}
+
+// Backward-Euler root of g(u) = 0: bracket by factors of 1.1 from u_old (the
+// search of photolibs3.py:505-521), then bisection to a relative width tol.
+__device__ double cool_one(double rho, double u_code, double dt, const CoolDev &cd, int *iters) {
+#pragma clang fp contract(off)
+    double rho_cgs = rho * cd.unit_rho_cgs;
+    double u_old = u_code * cd.unit_u_cgs;
+    double dt_s = dt * cd.unit_time_s;
+    double nH = cd.XH * rho_cgs / MH_CGS;
+    double tfac = (cd.gamma - 1.0) * cd.mu * MH_CGS / KB_CGS;
+    double g0 = cool_g(u_old, u_old, dt_s, nH, rho_cgs, tfac, cd);
+    *iters = 0;
+    if (g0 == 0.0 || !(dt_s > 0.0)) return u_code;
+    double lo, hi;
+    if (g0 > 0.0) {
+        hi = u_old;
+        lo = u_old / 1.1;
+        int nb = 0;
+        while (cool_g(lo, u_old, dt_s, nH, rho_cgs, tfac, cd) > 0.0 && nb < cd.max_bracket) {
+            hi = lo;
+            lo = lo / 1.1;
+            ++nb;
+        }
+    } else {
+        lo = u_old;
+        hi = u_old * 1.1;
+        int nb = 0;
+        while (cool_g(hi, u_old, dt_s, nH, rho_cgs, tfac, cd) < 0.0 && nb < cd.max_bracket) {
+            lo = hi;
+            hi = hi * 1.1;
+            ++nb;
+        }
+    }
+    int it = 0;
+    while ((hi - lo) > cd.tol * lo && it < cd.max_iter) {
+        double mid = 0.5 * (lo + hi);
+        if (cool_g(mid, u_old, dt_s, nH, rho_cgs, tfac, cd) > 0.0) hi = mid;
+        else lo = mid;
+        ++it;
+    }
+    *iters = it;
+    return 0.5 * (lo + hi) / cd.unit_u_cgs;
+}
+
+__global__ void k_cool(CoolDev cd, const double *__restrict__ rho, const double *__restrict__ u_in,
+                       double dt, const int32_t *__restrict__ run, const RunParams *__restrict__ rp,
+                       int64_t n, double *__restrict__ u_out, int32_t *__restrict__ iters,
+                       int64_t *counters) {
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    double d = dt;
+    if (rp) {
+        const RunParams &r = rp[run[i]];
+        if (!r.use_cool) {
+            u_out[i] = u_in[i];
+            if (iters) iters[i] = 0;
+            return;
+        }
+        d = r.dt;
+    }
+    int it = 0;
+    u_out[i] = cool_one(rho[i], u_in[i], d, cd, &it);
+    if (iters) iters[i] = it;
+    if (it >= cd.max_iter && counters)
+        atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
+}
+
+// Second half kick, u trapezoid (myCppSPHLibs_v2.h:740-747), ut_prev update,
+// then the implicit cooling of the adiabatic u (operator split).
+__global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, CoolDev cd,
+                         int cool_on, int64_t *counters) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    const RunParams &r = rp[s.run[i]];
+    double dt = r.dt;
+    double half = 0.5 * dt;
+    s.vx[i] = s.vx[i] + s.ax[i] * half;
+    s.vy[i] = s.vy[i] + s.ay[i] * half;
+    s.vz[i] = s.vz[i] + s.az[i] * half;
+    double du = s.du[i];
+    double u = s.u[i] + half * (du + s.ut_prev[i]);
+    s.ut_prev[i] = du;
+    if (cool_on && r.use_cool) {
+        int it = 0;
+        u = cool_one(s.rho[i], u, dt, cd, &it);
+        if (it >= cd.max_iter) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
+    }
+    s.u[i] = u;
+}
+
+__global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
+// Per-block partial sums of the 11 diagnostic quantities; fixed tree order.
+constexpr int ND = 11;
+__global__ __launch_bounds__(IT) void k_diag_partial(const double *x, const double *y,
+                                                     const double *z, const double *vx,
+                                                     const double *vy, const double *vz,
+                                                     const double *m, const double *u,
+                                                     const double *pot, int64_t n,
+                                                     double *__restrict__ part) {
+    __shared__ double sh[ND][IT];
+    int tid = threadIdx.x;
+    double v[ND];
+    for (int a = 0; a < ND; ++a) v[a] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * IT + tid; i < n; i += (int64_t)gridDim.x * IT) {
+        double mi = m[i];
+        double a = vx[i], b = vy[i], c = vz[i];
+        v[0] += 0.5 * mi * (a * a + b * b + c * c);
+        v[1] += u ? mi * u[i] : 0.0;
+        v[2] += pot ? 0.5 * mi * pot[i] : 0.0;
+        v[3] += mi * a;
+        v[4] += mi * b;
+        v[5] += mi * c;
+        if (x) {
+            double px = x[i], py = y[i], pz = z[i];
+            v[6] += mi * (py * c - pz * b);
+            v[7] += mi * (pz * a - px * c);
+            v[8] += mi * (px * b - py * a);
+        }
+        v[9] += mi;
+    }
+    for (int a = 0; a < ND; ++a) sh[a][tid] = v[a];
+    __syncthreads();
+    for (int off = IT / 2; off > 0; off >>= 1) {
+        if (tid < off)
+            for (int a = 0; a < ND; ++a) sh[a][tid] += sh[a][tid + off];
+        __syncthreads();
+    }
+    if (tid < ND) part[(int64_t)blockIdx.x * ND + tid] = sh[tid][0];
+}
+
+}  // namespace
+
+int sphx_cool_launch(const CoolDev &cd, const double *rho, const double *u_in, double dt,
+                     const double *, const int32_t *run, const RunParams *rp, int64_t n,
+                     double *u_out, int32_t *iters, int64_t *counters, hipStream_t st) {
+    if (n <= 0) return SPHX_OK;
+    k_cool<<<g1(n), IT, 0, st>>>(cd, rho, u_in, dt, run, rp, n, u_out, iters, counters);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int sphx_diag_reduce(const double *x, const double *y, const double *z, const double *vx,
+                     const double *vy, const double *vz, const double *m, const double *u,
+                     const double *pot, int64_t n, double *scratch, sphx_diag *out,
+                     hipStream_t st) {
+    const int blocks = 256;
+    k_diag_partial<<<blocks, IT, 0, st>>>(x, y, z, vx, vy, vz, m, u, pot, n, scratch);
+    SPHX_HIP(hipGetLastError());
+    std::vector<double> h((size_t)blocks * ND);
+    SPHX_HIP(hipMemcpyAsync(h.data(), scratch, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    double v[ND] = {0};
+    // pairwise combination of the block partials in a fixed order
+    std::vector<double> cur(h);
+    int len = blocks;
+    while (len > 1) {
+        int half = (len + 1) / 2;
+        for (int b = 0; b < len / 2; ++b)
+            for (int a = 0; a < ND; ++a) cur[(size_t)b * ND + a] += cur[(size_t)(b + half) * ND + a];
+        len = half;
+    }
+    for (int a = 0; a < ND; ++a) v[a] = cur[a];
+    out->ke = v[0]; out->ut = v[1]; out->pe = v[2];
+    out->px = v[3]; out->py = v[4]; out->pz = v[5];
+    out->lx = v[6]; out->ly = v[7]; out->lz = v[8];
+    out->mass = v[9];
+    return SPHX_OK;
+}
+
+int Engine::kick_drift(hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    k_kick_drift<<<g1(n), IT, 0, st>>>(s, d_run, n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+static CoolDev make_cool(const Engine &E) {
+    CoolDev cd{};
+    if (E.has_cool) {
+        const sphx_cooling &c = E.cool;
+        cd = CoolDev{c.unit_u_cgs, c.unit_rho_cgs, c.unit_time_s, c.XH, c.mu, c.gamma, c.heat,
+                     c.tol, c.max_iter, c.max_bracket, c.ntab, E.cool_T, E.cool_L};
+    }
+    return cd;
+}
+
+CoolDev sphx_engine_cool(const Engine &E) { return make_cool(E); }
+
+int Engine::finish_step(hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    k_finish<<<g1(n), IT, 0, st>>>(s, d_run, n, make_cool(*this), has_cool ? 1 : 0, counters);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::bootstrap_finish(hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    k_copy_d<<<g1(n), IT, 0, st>>>(s.du, s.ut_prev, n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int sphx_zero_counters(int64_t *c, hipStream_t st) {
+    k_zero_i64<<<1, 64, 0, st>>>(c, CNT_SLOTS);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int sphx_combine(Engine &E, hipStream_t st) {
+    if (E.n == 0) return SPHX_OK;
+    k_combine<<<g1(E.n), IT, 0, st>>>(E.s, E.n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}

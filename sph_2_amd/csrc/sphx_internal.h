@@ -1,0 +1,338 @@
+// Internal declarations of libsphx: device helpers, the Engine that owns the
+// packed particle state and its scratch, and the kernel launchers.
+//
+// Layout (HBM, all SoA, Morton order inside each run):
+//   FP64 state  x y z vx vy vz u m h eps ut_prev ax ay az (total) gx gy gz
+//               (gravity) rho P c divv curlv du pot
+//   int32       k (neighbour count), id (caller's original index), run
+//   buckets     up to 64 consecutive particles of one run (one wavefront);
+//               FP64 AABB + max h + max eps per bucket
+//   supers      up to 64 consecutive buckets of one run, FP64 AABB + max h
+//   candidates  per bucket, the buckets of its run within reach, CSR int32
+// Runs are contiguous segments of the packed arrays; a single simulation is
+// one run.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/sphx.h"
+
+#define SPHX_BUCKET 64
+#define SPHX_SUPER 64
+
+// ---------------------------------------------------------------------------
+// Error plumbing
+// ---------------------------------------------------------------------------
+void sphx_set_error(const std::string &msg);
+
+#define SPHX_HIP(call)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            sphx_set_error(std::string(#call) + ": " + hipGetErrorString(e_));         \
+            return SPHX_ERR_HIP;                                                       \
+        }                                                                              \
+    } while (0)
+
+#define SPHX_CHECK(rc)                 \
+    do {                               \
+        int rc_ = (rc);                \
+        if (rc_ < 0) return rc_;       \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+#define SPHX_SIGMA_D (0.31830988618379067154)  // 1/pi
+#define SPHX_SIGMA_F (0.31830988618379067154f)
+
+// Pair distance^2 exactly as the oracle computes it: separately rounded
+// products, summed left to right.  Neighbour decisions depend on it.
+__device__ __forceinline__ double sphx_r2(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    double a = dx * dx;
+    double b = dy * dy;
+    double c = dz * dz;
+    return (a + b) + c;
+}
+
+__device__ __forceinline__ double sphx_sq(double t) {
+#pragma clang fp contract(off)
+    return t * t;
+}
+
+// cbrt(nngb/k) clamped to [1/2, 2] by a fixed-count Newton iteration with
+// separately rounded operations; identical on any IEEE machine.
+__device__ __forceinline__ double sphx_h_factor(int nngb, int k) {
+#pragma clang fp contract(off)
+    if (8 * (long long)k <= (long long)nngb) return 2.0;
+    if ((long long)k >= 8 * (long long)nngb) return 0.5;
+    double x = (double)nngb / (double)k;
+    double y = 1.0;
+    for (int it = 0; it < 12; ++it) {
+        double y2 = y * y;
+        double y3 = y2 * y;
+        double num = y3 - x;
+        double den = 3.0 * y2;
+        y = y - num / den;
+    }
+    y = y < 0.5 ? 0.5 : y;
+    y = y > 2.0 ? 2.0 : y;
+    return y;
+}
+
+// Squared distance between two axis-aligned boxes.
+__device__ __forceinline__ double sphx_box_d2(double axl, double axh, double ayl, double ayh,
+                                              double azl, double azh, double bxl, double bxh,
+                                              double byl, double byh, double bzl, double bzh) {
+    double dx = fmax(0.0, fmax(bxl - axh, axl - bxh));
+    double dy = fmax(0.0, fmax(byl - ayh, ayl - byh));
+    double dz = fmax(0.0, fmax(bzl - azh, azl - bzh));
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Conservative reach test: true when a pair within `reach` may exist.
+__device__ __forceinline__ bool sphx_within(double d2, double reach) {
+    return d2 <= reach * reach * (1.0 + 1e-9);
+}
+
+// Compensated FP32 accumulation (Kahan): s + c carries the running sum.
+struct KahanF {
+    float s, c;
+    __device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
+    __device__ __forceinline__ void add(float x) {
+#pragma clang fp contract(off)
+        float y = x - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+__device__ __forceinline__ double wave_max_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Device-side tables shared by the kernels (plain struct passed by value)
+// ---------------------------------------------------------------------------
+struct Soa {
+    double *x, *y, *z, *vx, *vy, *vz, *u, *m, *h, *eps, *ut_prev;
+    double *ax, *ay, *az, *gx, *gy, *gz, *rho, *P, *c, *divv, *curlv, *du, *pot;
+    int32_t *k, *id, *run;
+};
+
+struct Tables {
+    int32_t nb, ns, R;
+    const int32_t *b_beg, *b_cnt, *b_run;       // buckets
+    const int32_t *s_bbeg, *s_bend;             // supers: bucket ranges
+    const int32_t *r_sbeg, *r_send;             // run -> super range
+    const int32_t *r_pbeg, *r_pend;             // run -> particle range
+    double *bxl, *bxh, *byl, *byh, *bzl, *bzh, *bhmax, *bemax;  // bucket boxes
+    double *sxl, *sxh, *syl, *syh, *szl, *szh, *shmax;          // super boxes
+};
+
+// Counters accumulated on device (int64 slots).
+enum {
+    CNT_H_ITERS_MAX = 0,
+    CNT_H_ITERS_SUM,
+    CNT_H_CAPPED,
+    CNT_COOL_CAPPED,
+    CNT_CAND,
+    CNT_SLOTS = 8
+};
+
+// Step phases timed with HIP events when Engine::timing is on.
+enum {
+    PH_SORT = 0,   // half kick + drift + Morton sort + gather
+    PH_SMOOTH,     // boxes + smoothing-length iteration
+    PH_DENS,       // boxes with h + density + EOS
+    PH_GRAV,       // tile boxes + gravity
+    PH_DIVCURL,
+    PH_HYDRO,
+    PH_FINISH,     // combine + half kick + u update + cooling
+    PH_COUNT
+};
+
+// Per-run parameters on device.
+struct RunParams {
+    double dt;
+    int32_t use_cool;
+    int32_t eps_from_h;
+};
+
+// ---------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------
+// Gravity launch configuration: particles per lane, j-tile size, j split.
+struct GravConfig {
+    int ipl = 2, tj = 512, js = 1;
+};
+
+// Gravity i-block / j-tile tables, rebuilt when runs or configuration change.
+struct GravHost {
+    int ipl = 0, tj = 0;
+    int nblocks = 0, ntiles = 0;
+    int32_t *dev = nullptr;  // ib_beg | ib_cnt | ib_run | r_tbeg
+    int64_t dev_cap = 0;
+    double *tbox = nullptr;  // 8 doubles per tile
+    int64_t tbox_cap = 0;
+    double *part = nullptr;  // j-split partial sums
+    int64_t part_cap = 0;
+    std::vector<int64_t> sizes;
+};
+
+struct Engine {
+    int device = 0;
+    int mode = SPHX_MIXED;
+    sphx_params p{};
+    bool has_cool = false;
+    sphx_cooling cool{};
+    double *cool_T = nullptr, *cool_L = nullptr;
+
+    int64_t cap = 0, n = 0;
+    int R = 0;
+    std::vector<int64_t> run_n;  // host copy of run sizes (packed order)
+    std::vector<RunParams> run_params;
+
+    Soa s{};
+    // scratch
+    uint64_t *keys = nullptr, *keys2 = nullptr;
+    int32_t *perm = nullptr, *perm2 = nullptr;
+    double *tmpd = nullptr;   // n doubles
+    int32_t *tmpi = nullptr;  // n int32
+    uint32_t *hist = nullptr; // radix histograms
+    int64_t hist_cap = 0;
+    double *bbox = nullptr;   // per-run bounding boxes (6 per run) + chunk partials
+    int64_t bbox_cap = 0;
+    int32_t *h_iters = nullptr;
+    int64_t *counters = nullptr;  // CNT_SLOTS
+    RunParams *d_run = nullptr;
+
+    // tables
+    std::vector<int32_t> hb_beg, hb_cnt, hb_run, hs_bbeg, hs_bend, hr_sbeg, hr_send, hr_pbeg,
+        hr_pend;
+    int32_t *tab_i32 = nullptr;  // packed device copy of the int tables
+    int64_t tab_i32_cap = 0;
+    double *tab_d = nullptr;     // box arrays
+    int64_t tab_d_cap = 0;
+    Tables t{};
+    int32_t *cand_off = nullptr, *cand_cnt = nullptr, *cand = nullptr;
+    int64_t cand_cap = 0;
+    int64_t cand_total = 0;
+
+    std::vector<void *> owned;  // every hipMalloc'd pointer
+
+    GravConfig gcfg;
+    GravHost grav;
+
+    // optional per-phase HIP-event timing of the step
+    bool timing = false;
+    hipEvent_t tev[2 * 8] = {};
+    double phase_ms[8] = {};
+    int64_t phase_n[8] = {};
+    int tmark(hipStream_t st, int phase, int end);
+    int tcollect();
+
+    int steps = 0;
+    bool forces_valid = false;
+
+    ~Engine();
+    int alloc(void **ptr, size_t bytes);
+    int reserve(int64_t capacity);
+    int set_runs(const std::vector<int64_t> &sizes);
+    int upload_run_params(hipStream_t st);
+
+    // pipeline pieces (mixed mode)
+    int sort_and_permute(hipStream_t st, bool gather_state);
+    int build_boxes(hipStream_t st, bool with_h);
+    int smoothing(hipStream_t st, bool predictor);
+    int build_candidates(hipStream_t st);
+    int density_eos(hipStream_t st);
+    int gravity(hipStream_t st, bool with_pot);
+    int divcurl(hipStream_t st);
+    int hydro(hipStream_t st);
+    int forces(hipStream_t st, bool predictor, bool with_pot);
+
+    // plain FP64 baseline pieces
+    int plain_forces(hipStream_t st, bool predictor, bool with_pot);
+
+    // integrator
+    int kick_drift(hipStream_t st);
+    int finish_step(hipStream_t st);
+    int bootstrap_finish(hipStream_t st);
+    int step(hipStream_t st, int nsteps);
+
+    int read_counters(sphx_counters *out);
+    int diagnostics(hipStream_t st, bool with_pe, sphx_diag *out);
+};
+
+// launchers implemented in the .hip files
+int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_t st);
+int sphx_launch_morton(Engine &E, hipStream_t st);
+int sphx_radix_sort(uint64_t *keys, int32_t *vals, uint64_t *keys_tmp, int32_t *vals_tmp,
+                    uint32_t *hist, int64_t n, int key_bits, hipStream_t st);
+int sphx_diag_reduce(const double *x, const double *y, const double *z, const double *vx,
+                     const double *vy, const double *vz, const double *m, const double *u,
+                     const double *pot, int64_t n, double *scratch, sphx_diag *out,
+                     hipStream_t st);
+
+// plain FP64 kernels (SoA inputs, rows [r0, r1))
+int plain_smoothing(const double *x, const double *y, const double *z, const double *h_prev,
+                    const int32_t *k_prev, const double *divv, const double *dt_p, double dt,
+                    const int32_t *run, const sphx_params &p, int64_t n, int64_t r0, int64_t r1,
+                    double *h_out, int32_t *k_out, int32_t *it_out, int64_t *counters,
+                    hipStream_t st);
+int plain_density(const double *x, const double *y, const double *z, const double *m,
+                  const double *h, int64_t n, int64_t r0, int64_t r1, double *rho,
+                  int32_t *nngb, hipStream_t st);
+int plain_eos(const double *rho, const double *u, double gamma, int64_t n, double *P,
+              double *c, hipStream_t st);
+int plain_divcurl(const double *x, const double *y, const double *z, const double *vx,
+                  const double *vy, const double *vz, const double *rho, const double *m,
+                  const double *h, int64_t n, int64_t r0, int64_t r1, double *divv,
+                  double *curlv, hipStream_t st);
+int plain_acc_sph(const Soa &s, const sphx_params &p, int64_t n, int64_t r0, int64_t r1,
+                  double *ax, double *ay, double *az, hipStream_t st);
+int plain_du(const Soa &s, const sphx_params &p, int64_t n, int64_t r0, int64_t r1, double *du,
+             hipStream_t st);
+int plain_gravity(const double *x, const double *y, const double *z, const double *m,
+                  const double *eps, double G, int64_t n, int64_t r0, int64_t r1, double *ax,
+                  double *ay, double *az, double *pot, hipStream_t st);
+int plain_neighbors(const double *x, const double *y, const double *z, const double *h,
+                    int64_t n, int64_t r0, int64_t r1, int64_t *offsets, int32_t *idx,
+                    hipStream_t st);
+
+// cooling (FP64 implicit solve, shared by every mode)
+struct CoolDev {
+    double unit_u_cgs, unit_rho_cgs, unit_time_s, XH, mu, gamma, heat, tol;
+    int32_t max_iter, max_bracket, ntab;
+    const double *T, *L;
+};
+int sphx_cool_launch(const CoolDev &cd, const double *rho, const double *u_in, double dt,
+                     const double *dt_p, const int32_t *run, const RunParams *rp, int64_t n,
+                     double *u_out, int32_t *iters, int64_t *counters, hipStream_t st);
+
+// AoS <-> SoA helpers
+int sphx_aos_to_soa(const double *aos, int64_t n, double *x, double *y, double *z,
+                    hipStream_t st);
+int sphx_soa_to_aos(const double *x, const double *y, const double *z, int64_t n, double *aos,
+                    hipStream_t st);

@@ -1,0 +1,375 @@
+// Morton keys, stable LSD radix sort, permutation gather and bucket boxes.
+//
+// Replaces the reference's neighbour structures (counting-sort cells of
+// Particles_in_BH_Tree_II/Evrard_test/ngb_v3.h:58-117, atomics-filled grid
+// cells of Particles_in_Gridcells/final_revision_uniform_grid/
+// hfvCLibs_v7_grid_v0.h:680-708) with Morton-ordered 64-particle buckets:
+// one wavefront per bucket, bucket and super-bucket AABBs for culling.
+#include "sphx_internal.h"
+
+namespace {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+__device__ __forceinline__ uint64_t spread3(uint64_t v) {
+    // Spread the low 21 bits of v so bit b lands at bit 3b.
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+// Bounding box of each super-bucket straight from the particles (before the
+// sort, in the current order; supers never straddle runs).
+__global__ void k_super_box_pos(Tables t, const double *__restrict__ x,
+                                const double *__restrict__ y, const double *__restrict__ z,
+                                double *__restrict__ out /* 6 per super */) {
+    int sidx = blockIdx.x;
+    int lane = threadIdx.x;
+    int b0 = t.s_bbeg[sidx], b1 = t.s_bend[sidx];
+    int p0 = t.b_beg[b0];
+    int p1 = t.b_beg[b1 - 1] + t.b_cnt[b1 - 1];
+    double xl = INFINITY, xh = -INFINITY, yl = INFINITY, yh = -INFINITY, zl = INFINITY,
+           zh = -INFINITY;
+    for (int i = p0 + lane; i < p1; i += 64) {
+        double a = x[i], b = y[i], c = z[i];
+        xl = fmin(xl, a); xh = fmax(xh, a);
+        yl = fmin(yl, b); yh = fmax(yh, b);
+        zl = fmin(zl, c); zh = fmax(zh, c);
+    }
+    xl = wave_min_d(xl); xh = wave_max_d(xh);
+    yl = wave_min_d(yl); yh = wave_max_d(yh);
+    zl = wave_min_d(zl); zh = wave_max_d(zh);
+    if (lane == 0) {
+        out[6 * sidx + 0] = xl; out[6 * sidx + 1] = xh;
+        out[6 * sidx + 2] = yl; out[6 * sidx + 3] = yh;
+        out[6 * sidx + 4] = zl; out[6 * sidx + 5] = zh;
+    }
+}
+
+// Per-run bounding box from the super boxes; one wavefront per run.
+__global__ void k_run_box(Tables t, const double *__restrict__ sbox, double *__restrict__ rbox) {
+    int r = blockIdx.x;
+    int lane = threadIdx.x;
+    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int s = t.r_sbeg[r] + lane; s < t.r_send[r]; s += 64) {
+        for (int a = 0; a < 6; a += 2) {
+            v[a] = fmin(v[a], sbox[6 * s + a]);
+            v[a + 1] = fmax(v[a + 1], sbox[6 * s + a + 1]);
+        }
+    }
+    for (int a = 0; a < 6; a += 2) {
+        v[a] = wave_min_d(v[a]);
+        v[a + 1] = wave_max_d(v[a + 1]);
+    }
+    if (lane == 0)
+        for (int a = 0; a < 6; ++a) rbox[6 * r + a] = v[a];
+}
+
+// key = run << (3*bits) | interleave(ix, iy, iz); ix = floor((x - lo) * scale)
+// clamped to [0, 2^bits - 1], scale = 2^bits / (extent * (1 + 2^-30)) over the
+// run's cubic box.  All FP64, separately rounded.
+__global__ void k_morton(const double *__restrict__ x, const double *__restrict__ y,
+                         const double *__restrict__ z, const int32_t *__restrict__ run,
+                         const double *__restrict__ rbox, int64_t n, int bits,
+                         uint64_t *__restrict__ keys, int32_t *__restrict__ vals) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int r = run ? run[i] : 0;
+    const double *b = rbox + 6 * r;
+    double ex = b[1] - b[0], ey = b[3] - b[2], ez = b[5] - b[4];
+    double ext = fmax(ex, fmax(ey, ez));
+    double cells = (double)(1ull << bits);
+    double scale = ext > 0.0 ? cells / (ext * (1.0 + 0x1p-30)) : 0.0;
+    uint64_t maxc = (1ull << bits) - 1ull;
+    double fx = (x[i] - b[0]) * scale, fy = (y[i] - b[2]) * scale, fz = (z[i] - b[4]) * scale;
+    uint64_t ix = fx <= 0.0 ? 0ull : (uint64_t)fx;
+    uint64_t iy = fy <= 0.0 ? 0ull : (uint64_t)fy;
+    uint64_t iz = fz <= 0.0 ? 0ull : (uint64_t)fz;
+    ix = ix > maxc ? maxc : ix;
+    iy = iy > maxc ? maxc : iy;
+    iz = iz > maxc ? maxc : iz;
+    uint64_t key = (spread3(ix) << 2) | (spread3(iy) << 1) | spread3(iz);
+    key |= ((uint64_t)r) << (3 * bits);
+    keys[i] = key;
+    vals[i] = (int32_t)i;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restrict__ keys,
+                                                        int64_t n, int shift,
+                                                        uint32_t *__restrict__ hist,
+                                                        int nblocks) {
+    __shared__ uint32_t h[256];
+    int tid = threadIdx.x;
+    h[tid] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        int64_t i = base + r * RS_THREADS + tid;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)tid * nblocks + blockIdx.x] = h[tid];
+}
+
+// Exclusive scan of `total` uint32 entries with one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t *__restrict__ hist, int64_t total) {
+    __shared__ uint32_t part[1024];
+    int tid = threadIdx.x;
+    int64_t per = (total + 1023) / 1024;
+    int64_t b = tid * per, e = min(total, b + per);
+    uint32_t s = 0;
+    for (int64_t i = b; i < e; ++i) s += hist[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    uint32_t run = tid > 0 ? part[tid - 1] : 0u;
+    for (int64_t i = b; i < e; ++i) {
+        uint32_t v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+// Stable scatter: items keep their order inside each digit.  Items of a tile
+// are taken in rounds of 256 (thread order); inside a wavefront, peers with
+// the same digit are found by matching the 8 digit bits with ballots.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(
+    const uint64_t *__restrict__ kin, const int32_t *__restrict__ vin, uint64_t *__restrict__ kout,
+    int32_t *__restrict__ vout, int64_t n, int shift, const uint32_t *__restrict__ hist,
+    int nblocks) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t runc[256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    int tid = threadIdx.x;
+    int lane = tid & 63;
+    int w = tid >> 6;
+    base[tid] = hist[(int64_t)tid * nblocks + blockIdx.x];
+    runc[tid] = 0;
+    uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t tile = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        for (int q = 0; q < RS_THREADS / 64; ++q) wcnt[q][tid] = 0;
+        __syncthreads();
+        int64_t i = tile + r * RS_THREADS + tid;
+        bool valid = i < n;
+        uint64_t key = valid ? kin[i] : 0ull;
+        int32_t val = valid ? vin[i] : 0;
+        uint32_t d = (uint32_t)((key >> shift) & 255u);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            uint64_t m = __ballot(valid && ((d >> b) & 1u));
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = base[d] + runc[d] + rank;
+            for (int q = 0; q < w; ++q) off += wcnt[q][d];
+            kout[off] = key;
+            vout[off] = val;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+        for (int q = 0; q < RS_THREADS / 64; ++q) add += wcnt[q][tid];
+        runc[tid] += add;
+        // next round's clearing of wcnt follows a barrier at its top
+        __syncthreads();
+    }
+}
+
+struct GatherList {
+    const double *src[16];
+    double *dst[16];
+    const int32_t *isrc[4];
+    int32_t *idst[4];
+    int nd, ni;
+};
+
+__global__ void k_gather(GatherList g, const int32_t *__restrict__ perm, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t p = perm[i];
+    for (int a = 0; a < g.nd; ++a) g.dst[a][i] = g.src[a][p];
+    for (int a = 0; a < g.ni; ++a) g.idst[a][i] = g.isrc[a][p];
+}
+
+// Bucket boxes (+ max h, max eps) and super boxes (+ max h); one wavefront
+// per super-bucket walking its buckets.
+__global__ void k_boxes(Tables t, const double *__restrict__ x, const double *__restrict__ y,
+                        const double *__restrict__ z, const double *__restrict__ h,
+                        const double *__restrict__ eps, int with_h) {
+    int sidx = blockIdx.x;
+    int lane = threadIdx.x;
+    double S[7] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
+    for (int b = t.s_bbeg[sidx]; b < t.s_bend[sidx]; ++b) {
+        int cnt = t.b_cnt[b];
+        int i = t.b_beg[b] + lane;
+        bool ok = lane < cnt;
+        double a = ok ? x[i] : INFINITY, bb = ok ? y[i] : INFINITY, c = ok ? z[i] : INFINITY;
+        double xl = wave_min_d(a), yl = wave_min_d(bb), zl = wave_min_d(c);
+        a = ok ? x[i] : -INFINITY; bb = ok ? y[i] : -INFINITY; c = ok ? z[i] : -INFINITY;
+        double xh = wave_max_d(a), yh = wave_max_d(bb), zh = wave_max_d(c);
+        double hm = 0.0, em = 0.0;
+        if (with_h) {
+            hm = wave_max_d(ok ? h[i] : 0.0);
+            em = wave_max_d(ok ? (eps ? eps[i] : h[i]) : 0.0);
+        }
+        if (lane == 0) {
+            t.bxl[b] = xl; t.bxh[b] = xh; t.byl[b] = yl; t.byh[b] = yh;
+            t.bzl[b] = zl; t.bzh[b] = zh;
+            if (with_h) { t.bhmax[b] = hm; t.bemax[b] = em; }
+        }
+        S[0] = fmin(S[0], xl); S[1] = fmax(S[1], xh);
+        S[2] = fmin(S[2], yl); S[3] = fmax(S[3], yh);
+        S[4] = fmin(S[4], zl); S[5] = fmax(S[5], zh);
+        S[6] = fmax(S[6], hm);
+    }
+    if (lane == 0) {
+        t.sxl[sidx] = S[0]; t.sxh[sidx] = S[1]; t.syl[sidx] = S[2]; t.syh[sidx] = S[3];
+        t.szl[sidx] = S[4]; t.szh[sidx] = S[5];
+        if (with_h) t.shmax[sidx] = S[6];
+    }
+}
+
+__global__ void k_aos_to_soa(const double *__restrict__ a, int64_t n, double *__restrict__ x,
+                             double *__restrict__ y, double *__restrict__ z) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = a[3 * i];
+    y[i] = a[3 * i + 1];
+    z[i] = a[3 * i + 2];
+}
+
+__global__ void k_soa_to_aos(const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, int64_t n, double *__restrict__ a) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a[3 * i] = x[i];
+    a[3 * i + 1] = y[i];
+    a[3 * i + 2] = z[i];
+}
+
+inline unsigned grid_for(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+}  // namespace
+
+int sphx_aos_to_soa(const double *aos, int64_t n, double *x, double *y, double *z,
+                    hipStream_t st) {
+    if (n <= 0) return SPHX_OK;
+    k_aos_to_soa<<<grid_for(n, 256), 256, 0, st>>>(aos, n, x, y, z);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int sphx_soa_to_aos(const double *x, const double *y, const double *z, int64_t n, double *aos,
+                    hipStream_t st) {
+    if (n <= 0) return SPHX_OK;
+    k_soa_to_aos<<<grid_for(n, 256), 256, 0, st>>>(x, y, z, n, aos);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int sphx_radix_sort(uint64_t *keys, int32_t *vals, uint64_t *keys_tmp, int32_t *vals_tmp,
+                    uint32_t *hist, int64_t n, int key_bits, hipStream_t st) {
+    if (n <= 1) return SPHX_OK;
+    int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    int passes = (key_bits + 7) / 8;
+    uint64_t *ka = keys, *kb = keys_tmp;
+    int32_t *va = vals, *vb = vals_tmp;
+    for (int p = 0; p < passes; ++p) {
+        int shift = 8 * p;
+        k_rs_hist<<<nblocks, RS_THREADS, 0, st>>>(ka, n, shift, hist, nblocks);
+        k_rs_scan<<<1, 1024, 0, st>>>(hist, (int64_t)256 * nblocks);
+        k_rs_scatter<<<nblocks, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, hist, nblocks);
+        SPHX_HIP(hipGetLastError());
+        uint64_t *tk = ka; ka = kb; kb = tk;
+        int32_t *tv = va; va = vb; vb = tv;
+    }
+    if (ka != keys) {
+        SPHX_HIP(hipMemcpyAsync(keys, ka, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
+        SPHX_HIP(hipMemcpyAsync(vals, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+    }
+    return SPHX_OK;
+}
+
+static int morton_bits(int R) { return R <= 1 ? 21 : 18; }
+
+static int run_key_bits(int R) {
+    int rb = 0;
+    while ((1 << rb) < R) ++rb;
+    return 3 * morton_bits(R) + rb;
+}
+
+int sphx_launch_morton(Engine &E, hipStream_t st) {
+    Tables &t = E.t;
+    double *sbox = E.bbox;                 // 6 per super
+    double *rbox = E.bbox + 6 * (int64_t)t.ns;  // 6 per run
+    k_super_box_pos<<<t.ns, 64, 0, st>>>(t, E.s.x, E.s.y, E.s.z, sbox);
+    k_run_box<<<t.R, 64, 0, st>>>(t, sbox, rbox);
+    k_morton<<<grid_for(E.n, 256), 256, 0, st>>>(E.s.x, E.s.y, E.s.z, E.R > 1 ? E.s.run : nullptr,
+                                                rbox, E.n, morton_bits(E.R), E.keys, E.perm);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::sort_and_permute(hipStream_t st, bool gather_state) {
+    if (n == 0) return SPHX_OK;
+    SPHX_CHECK(sphx_launch_morton(*this, st));
+    SPHX_CHECK(sphx_radix_sort(keys, perm, keys2, perm2, hist, n, run_key_bits(R), st));
+    if (!gather_state) return SPHX_OK;
+    // Gather the persistent state into the new order; each array swaps with
+    // its spare so no copy-back is needed.
+    double **dsrc[] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.u, &s.m,
+                       &s.h, &s.eps, &s.ut_prev, &s.divv};
+    int32_t **isrc[] = {&s.k, &s.id, &s.run};
+    GatherList g{};
+    g.nd = 12;
+    g.ni = 3;
+    // spares live in the derived arrays that are recomputed after the sort
+    double *spares_d[12] = {s.gx, s.gy, s.gz, s.rho, s.P, s.c, s.curlv, s.du, s.pot, s.ax, s.ay, s.az};
+    int32_t *spares_i[3] = {tmpi, perm2, h_iters};
+    for (int a = 0; a < 12; ++a) {
+        g.src[a] = *dsrc[a];
+        g.dst[a] = spares_d[a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        g.isrc[a] = *isrc[a];
+        g.idst[a] = spares_i[a];
+    }
+    k_gather<<<grid_for(n, 256), 256, 0, st>>>(g, perm, n);
+    SPHX_HIP(hipGetLastError());
+    // swap: the gathered arrays become the state, the old state arrays become
+    // the derived/scratch arrays
+    double *old_d[12];
+    for (int a = 0; a < 12; ++a) old_d[a] = *dsrc[a];
+    for (int a = 0; a < 12; ++a) *dsrc[a] = spares_d[a];
+    s.gx = old_d[0]; s.gy = old_d[1]; s.gz = old_d[2]; s.rho = old_d[3]; s.P = old_d[4];
+    s.c = old_d[5]; s.curlv = old_d[6]; s.du = old_d[7]; s.pot = old_d[8]; s.ax = old_d[9];
+    s.ay = old_d[10]; s.az = old_d[11];
+    int32_t *old_i[3] = {s.k, s.id, s.run};
+    s.k = spares_i[0]; s.id = spares_i[1]; s.run = spares_i[2];
+    tmpi = old_i[0]; perm2 = old_i[1]; h_iters = old_i[2];
+    return SPHX_OK;
+}
+
+int Engine::build_boxes(hipStream_t st, bool with_h) {
+    if (n == 0) return SPHX_OK;
+    const double *e = p.eps_from_h ? nullptr : s.eps;
+    k_boxes<<<t.ns, 64, 0, st>>>(t, s.x, s.y, s.z, s.h, e, with_h ? 1 : 0);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}

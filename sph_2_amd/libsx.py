@@ -1,0 +1,380 @@
+"""Drop-in replacements for the reference's FP64 SPH library functions.
+
+Same names, argument order and return shapes as hassanfv/SPH_2
+(11_Dec_2022/GPU_sph/IC/libsx2_2t.py = LIBSX, GPU_sph/work_on_smoothing/
+libsx2_2t.py = LIBSX3, GPU_sph/work_on_smoothing/shear.py = SHEAR,
+GPU_sph/Cooling_v1/photolibs3.py = PHOTOPY): numpy float64 arrays in, fresh
+numpy arrays out.  Each call uploads to the GPU, runs the gfx950 kernels of
+libsphx.so through its C ABI and copies the result back.
+
+Keyword-only extras select the precision (``mode="mixed"`` or ``"fp64"``),
+the artificial-viscosity form and the device.  Where the reference only
+printed (iteration caps), these functions warn.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import warnings
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dev, ptr
+
+_MODES = {"mixed": _lib.MIXED, "fp64": _lib.FP64}
+_ctx = {}
+
+
+def _context(device: int = 0):
+    c = _ctx.get(device)
+    if c is None:
+        _lib.require_gpu()
+        h = C.c_void_p()
+        check(_lib.load().sphx_ctx_create(device, C.byref(h)), "sphx_ctx_create")
+        c = _ctx[device] = h
+    return c
+
+
+def _mode(mode: str) -> int:
+    try:
+        return _MODES[mode]
+    except KeyError:
+        raise ValueError(f"mode must be one of {sorted(_MODES)}") from None
+
+
+def _f64(a, shape_tail=()):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a
+
+
+def _rows(n, nbeg, nend):
+    if nbeg is None:
+        return 0, n
+    if not (0 <= nbeg <= nend <= n):
+        raise ValueError("row range out of bounds")
+    return int(nbeg), int(nend)
+
+
+def _params(gama=5.0 / 3.0, eta=0.1, alpha=1.0, beta=2.0, av_form=1, balsara=0, **kw):
+    return _lib.default_params(gamma=gama, eta=eta, alpha=alpha, beta=beta, av_form=av_form,
+                               balsara=balsara, **kw)
+
+
+def _t():
+    return _lib.torch()
+
+
+# ---------------------------------------------------------------------------
+# Density / EOS
+# ---------------------------------------------------------------------------
+def getDensity(pos, m, h, *, mode="mixed", device=0, return_count=False, nbeg=None, nend=None):
+    """rho_i = sum_j m_j W(r_ij, h_ij), self included (LIBSX:17-51)."""
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    r0, r1 = _rows(n, nbeg, nend)
+    dp, dm, dh = dev(pos), dev(_f64(m)), dev(_f64(h))
+    rho = t.empty(r1 - r0, dtype=t.float64, device="cuda")
+    cnt = t.empty(r1 - r0, dtype=t.int32, device="cuda") if return_count else None
+    check(_lib.load().sphx_getDensity(_context(device), _mode(mode), n, ptr(dp), ptr(dm), ptr(dh),
+                                      r0, r1, ptr(rho), ptr(cnt), _lib.stream_handle()),
+          "getDensity")
+    out = rho.cpu().numpy()
+    return (out, cnt.cpu().numpy()) if return_count else out
+
+
+def getDensity_mpi(nbeg, nend, pos, m, h, **kw):
+    """Rows [nbeg, nend) of getDensity (LIBSX:58-95)."""
+    return getDensity(pos, m, h, nbeg=nbeg, nend=nend, **kw)
+
+
+def getPressure(rho, u, gama):
+    """P = (gama - 1) rho u (LIBSX:100-105)."""
+    return getPressure_csound(rho, u, gama)[0]
+
+
+def getPressure_csound(rho, u, gama, *, device=0):
+    """P and the adiabatic sound speed c = sqrt(gama (gama - 1) u) (LIBV2:766-789)."""
+    t = _t()
+    dr, du = dev(_f64(rho)), dev(_f64(u))
+    n = len(dr)
+    P = t.empty(n, dtype=t.float64, device="cuda")
+    c = t.empty(n, dtype=t.float64, device="cuda")
+    check(_lib.load().sphx_eos(_context(device), n, ptr(dr), ptr(du), float(gama), ptr(P), ptr(c),
+                               _lib.stream_handle()), "eos")
+    return P.cpu().numpy(), c.cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+# Hydro acceleration and du/dt
+# ---------------------------------------------------------------------------
+def _hydro(pos, v, rho, P, c, h, m, divV, curlV, p, mode, device, nbeg, nend, want_acc, want_du):
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    r0, r1 = _rows(n, nbeg, nend)
+    arrs = [dev(_f64(a)) for a in (pos, v, rho, P, c, h, m)]
+    dv = dev(_f64(divV)) if divV is not None else None
+    cv = dev(_f64(curlV)) if curlV is not None else None
+    acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda") if want_acc else None
+    du = t.empty(r1 - r0, dtype=t.float64, device="cuda") if want_du else None
+    check(_lib.load().sphx_getAcc_sph(_context(device), _mode(mode), n, *[ptr(a) for a in arrs],
+                                      ptr(dv), ptr(cv), C.byref(p), r0, r1, ptr(acc), ptr(du),
+                                      _lib.stream_handle()), "getAcc_sph")
+    return (acc.cpu().numpy() if want_acc else None), (du.cpu().numpy() if want_du else None)
+
+
+def getAcc_sph(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, *, av_form=1, mode="mixed",
+               device=0, nbeg=None, nend=None):
+    """Hydro acceleration (LIBSX:110-186).  av_form=1 is the Gadget
+    signal-velocity viscosity of that function; av_form=0 selects the
+    Monaghan-Gingold form of augsphx6.5x.py:103-139."""
+    p = _params(gama, eta, alpha, beta, av_form=av_form)
+    return _hydro(pos, v, rho, P, c, h, m, None, None, p, mode, device, nbeg, nend, True, False)[0]
+
+
+def getAcc_sph_mpi(nbeg, nend, pos, v, rho, P, c, h, m, gama, eta, alpha, beta, **kw):
+    """Rows [nbeg, nend) of getAcc_sph (LIBSX:276-355)."""
+    return getAcc_sph(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, nbeg=nbeg, nend=nend, **kw)
+
+
+def get_dU(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, *, av_form=1, mode="mixed", device=0,
+           nbeg=None, nend=None):
+    """du/dt = sum_j m_j (P_i/rho_i^2 + Pi_ij/2) v_ij . grad_i W (LIBSX:363-430)."""
+    p = _params(gama, eta, alpha, beta, av_form=av_form)
+    return _hydro(pos, v, rho, P, c, h, m, None, None, p, mode, device, nbeg, nend, False, True)[1]
+
+
+def get_dU_mpi(nbeg, nend, pos, v, rho, P, c, h, m, gama, eta, alpha, beta, **kw):
+    return get_dU(pos, v, rho, P, c, h, m, gama, eta, alpha, beta, nbeg=nbeg, nend=nend, **kw)
+
+
+def getAcc_sph_dU(pos, v, rho, P, c, h, m, gama=5.0 / 3.0, eta=0.1, alpha=1.0, beta=2.0, *,
+                  av_form=0, balsara=0, divV=None, curlV=None, mode="mixed", device=0, nbeg=None,
+                  nend=None):
+    """Acceleration and du/dt from one fused pair pass (both at the same v)."""
+    p = _params(gama, eta, alpha, beta, av_form=av_form, balsara=balsara)
+    return _hydro(pos, v, rho, P, c, h, m, divV, curlV, p, mode, device, nbeg, nend, True, True)
+
+
+def getAcc_sph_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, *, mode="mixed", device=0,
+                     nbeg=None, nend=None):
+    """Gadget-form viscosity with the Balsara switch (SHEAR:151-235); f_j uses
+    divV[j] as in myCppSPHLibs_v2.h:400-401."""
+    p = _params(alpha=alpha, av_form=1, balsara=1)
+    return _hydro(pos, v, rho, P, c, h, m, divV, curlV, p, mode, device, nbeg, nend, True, False)[0]
+
+
+def get_dU_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, *, mode="mixed", device=0,
+                 nbeg=None, nend=None):
+    """du/dt with the Balsara-switched viscosity (SHEAR:334-406)."""
+    p = _params(alpha=alpha, av_form=1, balsara=1)
+    return _hydro(pos, v, rho, P, c, h, m, divV, curlV, p, mode, device, nbeg, nend, False, True)[1]
+
+
+# ---------------------------------------------------------------------------
+# Velocity divergence and curl
+# ---------------------------------------------------------------------------
+def div_curlVel(pos, v, rho, m, h, *, mode="mixed", device=0, nbeg=None, nend=None):
+    """Signed div v and |curl v| (SHEAR:8-69)."""
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    r0, r1 = _rows(n, nbeg, nend)
+    arrs = [dev(_f64(a)) for a in (pos, v, rho, m, h)]
+    dv = t.empty(r1 - r0, dtype=t.float64, device="cuda")
+    cv = t.empty(r1 - r0, dtype=t.float64, device="cuda")
+    check(_lib.load().sphx_div_curlVel(_context(device), _mode(mode), n, *[ptr(a) for a in arrs],
+                                       r0, r1, ptr(dv), ptr(cv), _lib.stream_handle()),
+          "div_curlVel")
+    return dv.cpu().numpy(), cv.cpu().numpy()
+
+
+def div_curlVel_mpi(nbeg, nend, pos, v, rho, m, h, **kw):
+    return div_curlVel(pos, v, rho, m, h, nbeg=nbeg, nend=nend, **kw)
+
+
+# ---------------------------------------------------------------------------
+# Gravity and energies
+# ---------------------------------------------------------------------------
+def getAcc_g_smth(pos, mass, G, epsilon, *, mode="mixed", device=0, nbeg=None, nend=None,
+                  return_potential=False):
+    """Spline-softened gravity, eps_ij = (eps_i + eps_j)/2 (LIBSX:633-671)."""
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    r0, r1 = _rows(n, nbeg, nend)
+    eps = np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (n,))
+    dp, dm, de = dev(pos), dev(_f64(mass)), dev(np.ascontiguousarray(eps))
+    acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda")
+    pot = t.empty(r1 - r0, dtype=t.float64, device="cuda") if return_potential else None
+    check(_lib.load().sphx_getAcc_g_smth(_context(device), _mode(mode), n, ptr(dp), ptr(dm),
+                                         ptr(de), float(G), r0, r1, ptr(acc), ptr(pot),
+                                         _lib.stream_handle()), "getAcc_g_smth")
+    a = acc.cpu().numpy()
+    return (a, pot.cpu().numpy()) if return_potential else a
+
+
+def getAcc_g_smth_mpi(nbeg, nend, pos, mass, G, epsilon, **kw):
+    """Rows [nbeg, nend) of the gravity (LIBSX:678-715)."""
+    return getAcc_g_smth(pos, mass, G, epsilon, nbeg=nbeg, nend=nend, **kw)
+
+
+def _diag(pos, v, m, u=None, eps=None, G=1.0, device=0):
+    d = _lib.Diag()
+    dv, dm = dev(_f64(v)), dev(_f64(m))
+    dp = dev(_f64(pos)) if pos is not None else None
+    du = dev(_f64(u)) if u is not None else None
+    de = dev(np.ascontiguousarray(np.broadcast_to(np.asarray(eps, np.float64), (len(dm),)))) \
+        if eps is not None else None
+    check(_lib.load().sphx_diagnostics(_context(device), len(dm), ptr(dp), ptr(dv), ptr(dm),
+                                       ptr(du), ptr(de), float(G), C.byref(d),
+                                       _lib.stream_handle()), "diagnostics")
+    return d.as_dict()
+
+
+def getKE(v, m, *, device=0):
+    """sum 0.5 m v^2 (LIBSX:579-588)."""
+    return _diag(None, v, m, device=device)["ke"]
+
+
+def getPE(pos, m, G, epsilon, *, device=0):
+    """sum_{i<j} -G m_i m_j phi(r_ij, eps_ij) (LIBSX:593-629)."""
+    return _diag(pos, np.zeros_like(np.asarray(pos, np.float64)), m, eps=epsilon, G=G,
+                 device=device)["pe"]
+
+
+def diagnostics(pos, v, m, u=None, epsilon=None, G=1.0, *, device=0):
+    """Kinetic, thermal, potential energy, momentum and angular momentum."""
+    return _diag(pos, v, m, u=u, eps=epsilon, G=G, device=device)
+
+
+# ---------------------------------------------------------------------------
+# Smoothing length
+# ---------------------------------------------------------------------------
+def _smoothing(pos, h, k_prev, divV, dt, nbeg, nend, mode, device, **pkw):
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    r0, r1 = _rows(n, nbeg, nend)
+    p = _lib.default_params(**pkw)
+    dp, dh = dev(pos), dev(_f64(h))
+    dk = dev(np.ascontiguousarray(k_prev, dtype=np.int32)) if k_prev is not None else None
+    ddv = dev(_f64(divV)) if divV is not None else None
+    hout = t.empty(r1 - r0, dtype=t.float64, device="cuda")
+    kout = t.empty(r1 - r0, dtype=t.int32, device="cuda")
+    it = t.empty(r1 - r0, dtype=t.int32, device="cuda")
+    rc = check(_lib.load().sphx_smoothing(_context(device), _mode(mode), n, ptr(dp), ptr(dh),
+                                          ptr(dk), ptr(ddv), float(dt), C.byref(p), r0, r1,
+                                          ptr(hout), ptr(kout), ptr(it), _lib.stream_handle()),
+               "smoothing")
+    if rc == _lib.CAPPED:
+        warnings.warn("smoothing-length iteration capped for some particles", RuntimeWarning)
+    return hout.cpu().numpy(), kout.cpu().numpy(), it.cpu().numpy()
+
+
+def smoothing_fast_new_mpi3(nbeg, nend, pos, h, Nngb_previous, divV, dt, *, mode="mixed",
+                            device=0, **pkw):
+    """Smoothing lengths and neighbour counts for rows [nbeg, nend)
+    (LIBSX3:946-1010): Gadget-2 eq. 31 start, then the count iteration.
+    Returns (h, Nngb) like the reference."""
+    h_out, k, _ = _smoothing(pos, h, np.asarray(Nngb_previous).astype(np.int32), divV, dt, nbeg,
+                             nend, mode, device, **pkw)
+    return h_out, k.astype(np.float64)
+
+
+def smoothing_fast_new_mpi2(nbeg, nend, pos, h, *, mode="mixed", device=0, **pkw):
+    """Rows [nbeg, nend), iteration started at 2h (LIBSX:946-999)."""
+    return _smoothing(pos, h, None, None, 0.0, nbeg, nend, mode, device, **pkw)[0]
+
+
+def smoothing_h(pos, h, *, mode="mixed", device=0, **pkw):
+    """All rows; returns (h, k, iterations)."""
+    return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)
+
+
+def neighbors(pos, h, *, device=0, nbeg=None, nend=None):
+    """Neighbour lists under r_ij <= h_i + h_j (CSR offsets, sorted indices per row)."""
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    r0, r1 = _rows(n, nbeg, nend)
+    dp, dh = dev(pos), dev(_f64(h))
+    off = t.zeros(r1 - r0 + 1, dtype=t.int64, device="cuda")
+    total = C.c_int64()
+    L = _lib.load()
+    check(L.sphx_neighbors(_context(device), n, ptr(dp), ptr(dh), r0, r1, ptr(off), None, 0,
+                           C.byref(total), _lib.stream_handle()), "neighbors")
+    idx = t.empty(max(total.value, 1), dtype=t.int32, device="cuda")
+    check(L.sphx_neighbors(_context(device), n, ptr(dp), ptr(dh), r0, r1, ptr(off), ptr(idx),
+                           total.value, C.byref(total), _lib.stream_handle()), "neighbors")
+    o = off.cpu().numpy()
+    ix = idx.cpu().numpy()[: total.value].astype(np.int64)
+    for r in range(r1 - r0):
+        ix[o[r]:o[r + 1]].sort()
+    return o, ix
+
+
+def morton_sort(pos, *, device=0):
+    """63-bit Morton keys of pos over its bounding box and the stable sort
+    permutation (keys_sorted, perm)."""
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    dp = dev(pos)
+    keys = t.empty(n, dtype=t.int64, device="cuda")
+    perm = t.empty(n, dtype=t.int32, device="cuda")
+    check(_lib.load().sphx_morton_sort(_context(device), n, ptr(dp), ptr(keys), ptr(perm),
+                                       _lib.stream_handle()), "morton_sort")
+    return keys.cpu().numpy().view(np.uint64), perm.cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+# Cooling
+# ---------------------------------------------------------------------------
+class CoolingTable:
+    """Tabulated Lambda(T) plus the unit system of the implicit solve."""
+
+    def __init__(self, T, L, unit_u_cgs=1.0, unit_rho_cgs=1.0, unit_time_s=1.0, XH=0.76, mu=1.22,
+                 gamma=5.0 / 3.0, heat=0.0, tol=1e-10, max_iter=100, max_bracket=200):
+        self.T = np.ascontiguousarray(T, dtype=np.float64)
+        self.L = np.ascontiguousarray(L, dtype=np.float64)
+        self.kw = dict(unit_u_cgs=unit_u_cgs, unit_rho_cgs=unit_rho_cgs, unit_time_s=unit_time_s,
+                       XH=XH, mu=mu, gamma=gamma, heat=heat, tol=tol)
+        self.max_iter = max_iter
+        self.max_bracket = max_bracket
+
+    def c(self) -> _lib.Cooling:
+        dp = C.POINTER(C.c_double)
+        return _lib.Cooling(**self.kw, max_iter=self.max_iter, max_bracket=self.max_bracket,
+                            ntab=len(self.T), pad=0, T=self.T.ctypes.data_as(dp),
+                            L=self.L.ctypes.data_as(dp))
+
+
+def DoCooling(rho, u, dt, table: CoolingTable, *, device=0, return_iters=False):
+    """Implicit per-particle cooling update in code units (PHOTOPY:489-543 structure)."""
+    t = _t()
+    dr, du = dev(_f64(rho)), dev(_f64(u))
+    n = len(dr)
+    out = t.empty(n, dtype=t.float64, device="cuda")
+    it = t.empty(n, dtype=t.int32, device="cuda")
+    c = table.c()
+    rc = check(_lib.load().sphx_cool(_context(device), n, ptr(dr), ptr(du), float(dt), C.byref(c),
+                                     ptr(out), ptr(it), _lib.stream_handle()), "cool")
+    if rc == _lib.CAPPED:
+        warnings.warn("cooling solve hit its iteration cap for some particles", RuntimeWarning)
+    o = out.cpu().numpy()
+    return (o, it.cpu().numpy()) if return_iters else o
+
+
+def DoCooling_h(rho, u_old, dt, XH, table: CoolingTable | None = None):
+    """Scalar cgs form of DoCooling_h(rho, u_old, dt, XH) (PHOTOPY:489): returns
+    u in cgs after an implicit step of dt seconds over the tabulated Lambda(T)."""
+    if table is None:
+        from .ic import cooling_table
+        T, L = cooling_table()
+        table = CoolingTable(T, L, XH=XH)
+    else:
+        table = CoolingTable(table.T, table.L, **{**table.kw, "XH": XH, "unit_u_cgs": 1.0,
+                                                   "unit_rho_cgs": 1.0, "unit_time_s": 1.0})
+    return float(DoCooling(np.array([rho]), np.array([u_old]), dt, table)[0])

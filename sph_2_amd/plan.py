@@ -1,0 +1,101 @@
+"""Resident-state SPH run on one GPU (sphx_plan_* of include/sphx.h).
+
+The whole step stays on the device between calls: half kick, drift, Morton
+sort, smoothing-length iteration, density + EOS, gravity, div/curl, hydro
+acceleration + du/dt, half kick, u trapezoid and optional implicit cooling
+(the step order of 11_Dec_2022/GPU_sph/Important_Test_Simulations/
+Evrard_collapse/hfv_sph_gpu_v2_evrard.cu:407-482).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dev, ptr
+from .libsx import CoolingTable
+
+
+class SphPlan:
+    def __init__(self, capacity: int, *, mode: str = "mixed", params: _lib.Params | None = None,
+                 cooling: CoolingTable | None = None, device: int = 0, **param_kw):
+        _lib.require_gpu()
+        self.mode = {"mixed": _lib.MIXED, "fp64": _lib.FP64}[mode]
+        self.params = params if params is not None else _lib.default_params(**param_kw)
+        self.cooling = cooling
+        self._cool_c = cooling.c() if cooling is not None else None
+        self._h = C.c_void_p()
+        self.device = device
+        self.n = 0
+        check(_lib.load().sphx_plan_create(device, self.mode, C.byref(self.params),
+                                           C.byref(self._cool_c) if self._cool_c else None,
+                                           int(capacity), C.byref(self._h)), "sphx_plan_create")
+
+    def close(self):
+        if self._h:
+            _lib.load().sphx_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_gravity_config(self, ipl: int, tj: int, js: int = 1):
+        check(_lib.load().sphx_plan_set_gravity_config(self._h, ipl, tj, js), "gravity config")
+
+    def load(self, pos, vel, u, m, h, eps=None):
+        t = _lib.torch()
+        self._bufs = [dev(np.asarray(a, np.float64) if not isinstance(a, t.Tensor) else a,
+                          t.float64) for a in (pos, vel, u, m, h)]
+        de = dev(np.asarray(eps, np.float64), t.float64) if eps is not None else None
+        self.n = int(self._bufs[3].shape[0])
+        check(_lib.load().sphx_plan_load(self._h, self.n, *[ptr(b) for b in self._bufs], ptr(de),
+                                         _lib.stream_handle()), "sphx_plan_load")
+        self._eps = de
+
+    def init(self):
+        check(_lib.load().sphx_plan_init(self._h, _lib.stream_handle()), "sphx_plan_init")
+
+    def step(self, n_steps: int, dt: float, cooling: bool = False) -> int:
+        return check(_lib.load().sphx_plan_step(self._h, int(n_steps), float(dt), int(bool(cooling)),
+                                                _lib.stream_handle()), "sphx_plan_step")
+
+    def state(self, fields=("pos", "vel", "u", "h", "rho", "acc", "acc_g", "dudt", "k")) -> dict:
+        """Copy the state out (original particle order) as numpy arrays."""
+        t = _lib.torch()
+        n = self.n
+        shapes = {"pos": (n, 3), "vel": (n, 3), "u": (n,), "h": (n,), "rho": (n,),
+                  "acc": (n, 3), "acc_g": (n, 3), "dudt": (n,), "k": (n,)}
+        bufs = {}
+        for f in ("pos", "vel", "u", "h", "rho", "acc", "acc_g", "dudt", "k"):
+            if f in fields:
+                bufs[f] = t.empty(shapes[f], dtype=t.int32 if f == "k" else t.float64,
+                                  device="cuda")
+        order = ["pos", "vel", "u", "h", "rho", "acc", "acc_g", "dudt", "k"]
+        check(_lib.load().sphx_plan_store(self._h, *[ptr(bufs.get(f)) for f in order],
+                                          _lib.stream_handle()), "sphx_plan_store")
+        return {f: b.cpu().numpy() for f, b in bufs.items()}
+
+    def counters(self) -> dict:
+        c = _lib.Counters()
+        check(_lib.load().sphx_plan_query(self._h, C.byref(c)), "sphx_plan_query")
+        return c.as_dict()
+
+    def set_timing(self, on: bool = True):
+        check(_lib.load().sphx_plan_set_timing(self._h, int(on)), "set_timing")
+
+    def timing(self) -> dict:
+        """Summed milliseconds and step counts per phase since set_timing."""
+        ms = (C.c_double * 8)()
+        cnt = (C.c_int64 * 8)()
+        check(_lib.load().sphx_plan_timing(self._h, ms, cnt, 8), "timing")
+        return {ph: (float(ms[k]), int(cnt[k])) for k, ph in enumerate(_lib.PHASES)}
+
+    def diagnostics(self, with_pe: bool = True) -> dict:
+        d = _lib.Diag()
+        check(_lib.load().sphx_plan_diagnostics(self._h, int(with_pe), C.byref(d),
+                                                _lib.stream_handle()), "diagnostics")
+        return d.as_dict()

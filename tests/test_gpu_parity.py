@@ -1,0 +1,247 @@
+"""GPU parity of the drop-in functions against the FP64 oracle.
+
+Workload: Evrard sphere N = 4096, seed 0 (the reference's one-step CPU
+case), plus a small random cube.  Bars:
+  neighbour counts / sets / smoothing lengths  bit-exact
+  density                                       1e-6 relative (max)
+  hydro and gravity accelerations               1e-5 relative RMS
+  FP64 baseline kernels                         1e-12 relative
+"""
+import numpy as np
+import pytest
+
+import oracle
+from sph_2_amd import ic
+
+pytestmark = pytest.mark.gpu
+
+N = 4096
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def rel_max(a, b):
+    return float(np.max(np.abs(a - b) / np.abs(b)))
+
+
+@pytest.fixture(scope="module")
+def evr():
+    e = ic.evrard(N, seed=0)
+    h, k, it, st = oracle.smoothing(e.pos, e.h)
+    assert st.sum() == 0
+    rho, nn = oracle.density(e.pos, e.m, h)
+    rng = np.random.default_rng(1)
+    v = 0.3 * rng.standard_normal((N, 3))       # non-trivial velocities for AV paths
+    u = e.u * (1.0 + 0.2 * rng.random(N))
+    P, c = oracle.eos(rho, u, 5.0 / 3.0)
+    dv, cv = oracle.divcurl(e.pos, v, rho, e.m, h)
+    return dict(pos=e.pos, m=e.m, h0=e.h, h=h, k=k, rho=rho, nn=nn, v=v, u=u, P=P, c=c, dv=dv,
+                cv=cv)
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_smoothing_bit_exact(gpu, evr, mode):
+    from sph_2_amd import libsx
+
+    h, k, it = libsx.smoothing_h(evr["pos"], evr["h0"], mode=mode)
+    assert np.array_equal(k, evr["k"])
+    assert np.array_equal(h, evr["h"])
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_smoothing_predictor_rows(gpu, evr, mode):
+    from sph_2_amd import libsx
+
+    kprev = np.clip(evr["k"] + np.arange(N) % 7 - 3, 1, None)
+    dt = 1e-3
+    ho, ko, _, _ = oracle.smoothing(evr["pos"], evr["h"], kprev, evr["dv"], dt,
+                                    rows=np.arange(100, 900))
+    h, kk = libsx.smoothing_fast_new_mpi3(100, 900, evr["pos"], evr["h"], kprev, evr["dv"], dt,
+                                          mode=mode)
+    assert np.array_equal(kk.astype(np.int32), ko)
+    assert np.array_equal(h, ho)
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_density(gpu, evr, mode):
+    from sph_2_amd import libsx
+
+    rho, cnt = libsx.getDensity(evr["pos"], evr["m"], evr["h"], mode=mode, return_count=True)
+    assert np.array_equal(cnt, evr["nn"])
+    tol = 1e-6 if mode == "mixed" else 1e-12
+    assert rel_max(rho, evr["rho"]) < tol
+
+
+def test_density_rows(gpu, evr):
+    from sph_2_amd import libsx
+
+    rho = libsx.getDensity_mpi(1000, 1500, evr["pos"], evr["m"], evr["h"])
+    assert rel_max(rho, evr["rho"][1000:1500]) < 1e-6
+
+
+def test_neighbor_sets_bit_exact(gpu, evr):
+    from sph_2_amd import libsx
+
+    rows = (0, 600)
+    off, idx = libsx.neighbors(evr["pos"], evr["h"], nbeg=rows[0], nend=rows[1])
+    ooff, oidx = oracle.neighbors(evr["pos"], evr["h"], rows=np.arange(*rows))
+    assert np.array_equal(off, ooff)
+    assert np.array_equal(idx, oidx)
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_divcurl(gpu, evr, mode):
+    from sph_2_amd import libsx
+
+    dv, cv = libsx.div_curlVel(evr["pos"], evr["v"], evr["rho"], evr["m"], evr["h"], mode=mode)
+    tol = 1e-5 if mode == "mixed" else 1e-12
+    assert rel_rms(dv, evr["dv"]) < tol
+    assert rel_rms(cv, evr["cv"]) < tol
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+@pytest.mark.parametrize("av_form,balsara", [(0, 0), (1, 0), (1, 1), (0, 1)])
+def test_hydro(gpu, evr, mode, av_form, balsara):
+    from sph_2_amd import libsx
+
+    p = oracle.Params(av_form=av_form, balsara=balsara)
+    acc_o, du_o = oracle.hydro(evr["pos"], evr["v"], evr["rho"], evr["P"], evr["c"], evr["h"],
+                               evr["m"], evr["dv"], evr["cv"], p)
+    acc, du = libsx.getAcc_sph_dU(evr["pos"], evr["v"], evr["rho"], evr["P"], evr["c"], evr["h"],
+                                  evr["m"], av_form=av_form, balsara=balsara, divV=evr["dv"],
+                                  curlV=evr["cv"], mode=mode)
+    tol = 1e-5 if mode == "mixed" else 1e-12
+    assert rel_rms(acc, acc_o) < tol
+    assert rel_rms(du, du_o) < tol
+
+
+def test_reference_signatures(gpu, evr):
+    from sph_2_amd import libsx
+
+    g = 5.0 / 3.0
+    a = libsx.getAcc_sph(evr["pos"], evr["v"], evr["rho"], evr["P"], evr["c"], evr["h"], evr["m"],
+                         g, 0.1, 1.0, 2.0)
+    ao, duo = oracle.hydro(evr["pos"], evr["v"], evr["rho"], evr["P"], evr["c"], evr["h"],
+                           evr["m"], p=oracle.Params(av_form=1))
+    assert a.shape == (N, 3)
+    assert rel_rms(a, ao) < 1e-5
+    du = libsx.get_dU(evr["pos"], evr["v"], evr["rho"], evr["P"], evr["c"], evr["h"], evr["m"],
+                      g, 0.1, 1.0, 2.0)
+    assert rel_rms(du, duo) < 1e-5
+    P = libsx.getPressure(evr["rho"], evr["u"], g)
+    assert rel_max(P, evr["P"]) < 1e-15
+    ash = libsx.getAcc_sph_shear(evr["pos"], evr["v"], evr["rho"], evr["P"], evr["c"], evr["h"],
+                                 evr["m"], evr["dv"], evr["cv"], 1.0)
+    ao2, _ = oracle.hydro(evr["pos"], evr["v"], evr["rho"], evr["P"], evr["c"], evr["h"],
+                          evr["m"], evr["dv"], evr["cv"], oracle.Params(av_form=1, balsara=1))
+    assert rel_rms(ash, ao2) < 1e-5
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_gravity(gpu, evr, mode):
+    from sph_2_amd import libsx
+
+    acc_o, pot_o = oracle.gravity(evr["pos"], evr["m"], evr["h"], 1.0, potential=True)
+    acc, pot = libsx.getAcc_g_smth(evr["pos"], evr["m"], 1.0, evr["h"], mode=mode,
+                                   return_potential=True)
+    tol = 1e-5 if mode == "mixed" else 1e-12
+    assert rel_rms(acc, acc_o) < tol
+    assert rel_rms(pot, pot_o) < tol
+
+
+def test_gravity_rows_and_fixed_eps(gpu, evr):
+    from sph_2_amd import libsx
+
+    rows = np.arange(200, 1200)
+    acc_o = oracle.gravity(evr["pos"], evr["m"], 0.02, 1.0, rows=rows)
+    acc = libsx.getAcc_g_smth_mpi(200, 1200, evr["pos"], evr["m"], 1.0, 0.02)
+    assert rel_rms(acc, acc_o) < 1e-5
+    acc64 = libsx.getAcc_g_smth_mpi(200, 1200, evr["pos"], evr["m"], 1.0, 0.02, mode="fp64")
+    assert rel_rms(acc64, acc_o) < 1e-12
+
+
+def test_morton_keys_and_sort(gpu):
+    from sph_2_amd import libsx
+
+    rng = np.random.default_rng(3)
+    n = 50_000
+    pos = rng.random((n, 3))
+    pos[::7] = pos[3]              # ties: equal keys must keep their order
+    keys, perm = libsx.morton_sort(pos)
+    lo = pos.min(0)
+    ext = (pos.max(0) - lo).max()
+    scale = float(2 ** 21) / (ext * (1.0 + 2.0 ** -30))
+    q = np.clip(np.floor((pos - lo) * scale), 0, 2 ** 21 - 1).astype(np.uint64)
+    ref = np.zeros(n, np.uint64)
+    for b in range(21):
+        for a, sh in ((0, 2), (1, 1), (2, 0)):
+            ref |= ((q[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + sh)
+    order = np.argsort(ref, kind="stable")
+    assert np.array_equal(perm, order)
+    assert np.array_equal(keys, ref[order])
+
+
+def test_cooling_bit_exact(gpu):
+    from sph_2_amd import libsx
+
+    T, L = ic.cooling_table(seed=0)
+    units = ic.cloud_units()
+    tab = libsx.CoolingTable(T, L, unit_u_cgs=units["unit_u_cgs"],
+                             unit_rho_cgs=units["unit_rho_cgs"], unit_time_s=units["unit_time_s"],
+                             heat=2e-26)
+    cp = oracle.CoolParams(units["unit_u_cgs"], units["unit_rho_cgs"], units["unit_time_s"],
+                           mu=1.22, heat=2e-26, T=T, L=L)
+    rng = np.random.default_rng(5)
+    n = 20000
+    rho = 10 ** rng.uniform(-3, 3, n)
+    u = 10 ** rng.uniform(-4, 2, n)
+    out = libsx.DoCooling(rho, u, 1e-4, tab)
+    ref, _ = oracle.cool(rho, u, 1e-4, cp)
+    assert np.array_equal(out, ref)
+
+
+def test_plan_one_step(gpu):
+    """Resident plan: bootstrap + one KDK step vs the oracle's composition."""
+    from sph_2_amd.plan import SphPlan
+
+    e = ic.evrard(N, seed=0)
+    p = oracle.Params()
+    s0 = oracle.bootstrap(oracle.State(e.pos, e.vel, e.u, e.m, e.h), p)
+    s1 = oracle.step(s0, 1e-4, p)
+    for mode, tol in (("mixed", 1e-5), ("fp64", 1e-11)):
+        pl = SphPlan(N, mode=mode)
+        pl.load(e.pos, e.vel, e.u, e.m, e.h)
+        pl.init()
+        st0 = pl.state()
+        assert np.array_equal(st0["h"], s0.h)
+        assert rel_max(st0["rho"], s0.rho) < (1e-6 if mode == "mixed" else 1e-12)
+        assert rel_rms(st0["acc"], s0.acc) < tol
+        pl.step(1, 1e-4)
+        st1 = pl.state()
+        # After a step, h starts from the Gadget-2 predictor, which uses div v
+        # from the previous step: bit-exact only with bit-exact inputs (the
+        # stateless smoothing tests), so here h and k carry tolerances.
+        assert np.mean(st1["k"] != s1.k) < 1e-3
+        assert rel_max(st1["h"], s1.h) < 1e-8
+        assert rel_max(st1["rho"], s1.rho) < (1e-6 if mode == "mixed" else 1e-12)
+        assert rel_rms(st1["acc"], s1.acc) < tol
+        assert rel_rms(st1["acc_g"], s1.acc_g) < tol
+        assert rel_rms(st1["dudt"], s1.dudt) < tol
+        assert rel_max(st1["u"], s1.u) < 1e-9
+        assert rel_rms(st1["pos"], s1.pos) < 1e-12
+        pl.close()
+
+
+def test_diagnostics(gpu, evr):
+    from sph_2_amd import libsx
+
+    d = libsx.diagnostics(evr["pos"], evr["v"], evr["m"], evr["u"], evr["h"], 1.0)
+    ke = oracle.ke(evr["v"], evr["m"])
+    pe = oracle.pe(evr["pos"], evr["m"], evr["h"], 1.0)
+    assert abs(d["ke"] - ke) < 1e-13 * ke
+    assert abs(d["pe"] - pe) < 1e-12 * abs(pe)
+    assert abs(d["ut"] - float((evr["m"] * evr["u"]).sum())) < 1e-13
+    assert abs(libsx.getKE(evr["v"], evr["m"]) - ke) < 1e-13 * ke
+    assert abs(libsx.getPE(evr["pos"], evr["m"], 1.0, evr["h"]) - pe) < 1e-12 * abs(pe)
