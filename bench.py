@@ -51,7 +51,7 @@ def main(argv=None) -> int:
     ap.add_argument("--dt", type=float, default=1e-4)
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="timed steps of the plain FP64 GPU step (0: skip)")
-    ap.add_argument("--cpu-rows", type=int, default=2048,
+    ap.add_argument("--cpu-rows", type=int, default=8192,
                     help="rows of the step timed on the FP64 CPU oracle (0: skip)")
     ap.add_argument("--grav", default=None, help="gravity config ipl,tj,js (default: autotuned)")
     a = ap.parse_args(argv)
