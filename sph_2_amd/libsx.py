@@ -172,6 +172,16 @@ def get_dU_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, *, mode="mixed", d
     return _hydro(pos, v, rho, P, c, h, m, divV, curlV, p, mode, device, nbeg, nend, False, True)[1]
 
 
+def getAcc_sph_shear_mpi(nbeg, nend, pos, v, rho, P, c, h, m, divV, curlV, alpha, **kw):
+    """Rows [nbeg, nend) of getAcc_sph_shear (SHEAR:243-330)."""
+    return getAcc_sph_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, nbeg=nbeg, nend=nend, **kw)
+
+
+def get_dU_shear_mpi(nbeg, nend, pos, v, rho, P, c, h, m, divV, curlV, alpha, **kw):
+    """Rows [nbeg, nend) of get_dU_shear (SHEAR:413-485)."""
+    return get_dU_shear(pos, v, rho, P, c, h, m, divV, curlV, alpha, nbeg=nbeg, nend=nend, **kw)
+
+
 # ---------------------------------------------------------------------------
 # Velocity divergence and curl
 # ---------------------------------------------------------------------------
@@ -218,6 +228,20 @@ def getAcc_g_smth(pos, mass, G, epsilon, *, mode="mixed", device=0, nbeg=None, n
 def getAcc_g_smth_mpi(nbeg, nend, pos, mass, G, epsilon, **kw):
     """Rows [nbeg, nend) of the gravity (LIBSX:678-715)."""
     return getAcc_g_smth(pos, mass, G, epsilon, nbeg=nbeg, nend=nend, **kw)
+
+
+def getAcc_g_smth_mimj_mpi(nbeg, nend, pos, mass, G, epsilon, **kw):
+    """(N, 3) partial field of one rank (LIBSX:1377-1414).
+
+    The reference returns the pair terms of rows [nbeg, nend) against j > i,
+    applied to both particles, and the driver sums the fields of all ranks
+    (sph_mpi4_v4.py:154-160).  Here the rank's field holds the complete
+    accelerations of rows [nbeg, nend) and zeros elsewhere, so the sum over a
+    partition of the rows is the same total field."""
+    n = len(pos)
+    out = np.zeros((n, 3))
+    out[nbeg:nend] = getAcc_g_smth(pos, mass, G, epsilon, nbeg=nbeg, nend=nend, **kw)
+    return out
 
 
 def _diag(pos, v, m, u=None, eps=None, G=1.0, device=0):
