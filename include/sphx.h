@@ -41,6 +41,7 @@ extern "C" {
 
 #define SPHX_OK 0
 #define SPHX_CAPPED 1
+#define SPHX_REFUSED 2
 #define SPHX_ERR_ARG (-1)
 #define SPHX_ERR_NOMEM (-2)
 #define SPHX_ERR_HIP (-3)
@@ -213,6 +214,40 @@ int sphx_ctx_set_gravity_config(sphx_ctx *ctx, int32_t ipl, int32_t tj, int32_t 
  * timing returns summed milliseconds and step counts per phase. */
 int sphx_plan_set_timing(sphx_plan *plan, int32_t on);
 int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int32_t nphase);
+
+/* Replay each step from a captured hipGraph (one graph per pointer parity of
+ * the sorted state; recaptured when the run set changes).  Replaces the
+ * launch-and-synchronise sequence of DRV2:407-482.  Timing mode disables it. */
+int sphx_plan_set_graph(sphx_plan *plan, int32_t on);
+
+/* ---------------- served runs: many simulations packed in one engine ------- */
+/* No reference counterpart: the reference runs one simulation per process.
+ * Each admitted run is an independent simulation (its own dt, cooling flag,
+ * Morton order, buckets, gravity blocks) stored as a contiguous segment of
+ * packed arrays; sphx_batch_step advances every live run with one launch per
+ * pass.  A run's results do not depend on which other runs share the batch. */
+int sphx_batch_create(int device, int mode, const sphx_params *p, const sphx_cooling *cool,
+                      int64_t capacity, int32_t max_runs, sphx_batch **out);
+/* Returns SPHX_REFUSED (2) without side effects when the particle capacity or
+ * the run slots are exhausted.  Bootstraps the new run (forces at t = 0). */
+int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, const double *vel,
+                     const double *u, const double *m, const double *h, const double *eps,
+                     double dt, int32_t use_cooling, int64_t *run_id, void *stream);
+/* Copies the run out in its caller order (any pointer may be NULL) and frees
+ * its slot, compacting the packed arrays. */
+int sphx_batch_retire(sphx_batch *b, int64_t run_id, double *pos, double *vel, double *u,
+                      double *h, double *rho, double *acc, int32_t *k, void *stream);
+/* Copies a live run's state out in its caller order without retiring it. */
+int sphx_batch_store(sphx_batch *b, int64_t run_id, double *pos, double *vel, double *u,
+                     double *h, double *rho, double *acc, int32_t *k, void *stream);
+int sphx_batch_step(sphx_batch *b, int32_t n_steps, void *stream);
+int sphx_batch_runs(sphx_batch *b, int64_t *ids, int64_t *sizes, int64_t *steps,
+                    int32_t capacity, int32_t *count);
+int sphx_batch_set_graph(sphx_batch *b, int32_t on);
+int sphx_batch_query(sphx_batch *b, sphx_counters *out);
+/* Device bytes held per particle of capacity (the admission budget unit). */
+int64_t sphx_batch_bytes_per_particle(void);
+int sphx_batch_destroy(sphx_batch *b);
 
 #ifdef __cplusplus
 }

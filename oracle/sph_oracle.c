@@ -448,24 +448,30 @@ static inline double cool_g(double u, double u_old, double dt_s, double nH, doub
 double orc_cool_one(double rho, double u_code, double dt, const orc_cool_params *cp,
                     const double *Tt, const double *Lt, int32_t *iters) {
     double rho_cgs = rho * cp->unit_rho_cgs;
-    double u_old = u_code * cp->unit_u_cgs;
     double dt_s = dt * cp->unit_time_s;
     double nH = cp->XH * rho_cgs / ORC_MH;
     double tfac = (cp->gamma - 1.0) * cp->mu * ORC_MH / ORC_KB;
-    double g0 = cool_g(u_old, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt);
+    /* the gas never cools below the table's first temperature node */
+    double u_floor = Tt[0] / tfac;
+    double u_old = u_code * cp->unit_u_cgs;
+    if (u_old < u_floor) u_old = u_floor;
     int32_t it = 0;
-    if (g0 == 0.0 || !(dt_s > 0.0)) {
-        if (iters) *iters = 0;
-        return u_code;
-    }
+    if (iters) *iters = 0;
+    if (!(dt_s > 0.0)) return u_old / cp->unit_u_cgs;
+    double g0 = cool_g(u_old, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt);
+    if (g0 == 0.0) return u_old / cp->unit_u_cgs;
     double lo, hi;
     if (g0 > 0.0) { /* net cooling: root below u_old */
+        if (cool_g(u_floor, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt) >= 0.0)
+            return u_floor / cp->unit_u_cgs;
         hi = u_old;
         lo = u_old / 1.1;
+        if (lo < u_floor) lo = u_floor;
         int32_t nb = 0;
         while (cool_g(lo, u_old, dt_s, nH, rho_cgs, tfac, cp, Tt, Lt) > 0.0 && nb < cp->max_bracket) {
             hi = lo;
             lo = lo / 1.1;
+            if (lo < u_floor) lo = u_floor;
             ++nb;
         }
     } else { /* net heating: root above u_old */

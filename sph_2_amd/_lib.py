@@ -18,6 +18,7 @@ MIXED = 0
 FP64 = 1
 OK = 0
 CAPPED = 1
+REFUSED = 2
 
 _lib = None
 _lock = threading.Lock()
@@ -98,6 +99,17 @@ SIGNATURES = {
     "sphx_ctx_set_gravity_config": (C.c_int, [_V, _I32, _I32, _I32]),
     "sphx_plan_set_timing": (C.c_int, [_V, _I32]),
     "sphx_plan_timing": (C.c_int, [_V, _V, _V, _I32]),
+    "sphx_plan_set_graph": (C.c_int, [_V, _I32]),
+    "sphx_batch_create": (C.c_int, [C.c_int, C.c_int, _V, _V, _I64, _I32, _V]),
+    "sphx_batch_admit": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _D, _I32, _V, _V]),
+    "sphx_batch_retire": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_batch_store": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_batch_step": (C.c_int, [_V, _I32, _V]),
+    "sphx_batch_runs": (C.c_int, [_V, _V, _V, _V, _I32, _V]),
+    "sphx_batch_set_graph": (C.c_int, [_V, _I32]),
+    "sphx_batch_query": (C.c_int, [_V, _V]),
+    "sphx_batch_bytes_per_particle": (_I64, []),
+    "sphx_batch_destroy": (C.c_int, [_V]),
 }
 
 PHASES = ("sort", "smoothing", "density", "gravity", "divcurl", "hydro", "finish")

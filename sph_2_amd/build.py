@@ -27,6 +27,7 @@ SOURCES = [
     "sphx_gravity.hip",
     "sphx_plain.hip",
     "sphx_integrate.hip",
+    "sphx_batch.hip",
 ]
 
 ARCH = os.environ.get("SPHX_OFFLOAD_ARCH", "gfx950")

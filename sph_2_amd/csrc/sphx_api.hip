@@ -112,6 +112,10 @@ Engine::~Engine() {
     for (void *q : owned) hipFree(q);
     for (hipEvent_t e : tev)
         if (e) hipEventDestroy(e);
+    drop_graph();
+    if (gev_in) hipEventDestroy(gev_in);
+    if (gev_out) hipEventDestroy(gev_out);
+    if (gstream) hipStreamDestroy(gstream);
     if (cool_T) hipFree(cool_T);
     if (cool_L) hipFree(cool_L);
     hipSetDevice(cur);
@@ -221,8 +225,10 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
     if (R < nr || !d_run) {
         SPHX_CHECK(alloc((void **)&d_run, sizeof(RunParams) * std::max(nr, 1)));
     }
+    drop_graph();
     int32_t *q = tab_i32;
     t.nb = nb; t.ns = ns; t.R = nr;
+    t.boff = 0;
     t.b_beg = q; q += nb;
     t.b_cnt = q; q += nb;
     t.b_run = q; q += nb;
@@ -322,20 +328,114 @@ int Engine::plain_forces(hipStream_t st, bool predictor, bool with_pot) {
     return SPHX_OK;
 }
 
+int Engine::step_body(hipStream_t st) {
+    SPHX_CHECK(sphx_zero_counters(counters, st));
+    SPHX_CHECK(tmark(st, PH_SORT, 0));
+    SPHX_CHECK(kick_drift(st));
+    if (mode == SPHX_MIXED) SPHX_CHECK(sort_and_permute(st, true));
+    SPHX_CHECK(tmark(st, PH_SORT, 1));
+    if (mode == SPHX_MIXED) SPHX_CHECK(forces(st, true, false));
+    else SPHX_CHECK(plain_forces(st, true, false));
+    SPHX_CHECK(tmark(st, PH_FINISH, 0));
+    SPHX_CHECK(finish_step(st));
+    SPHX_CHECK(tmark(st, PH_FINISH, 1));
+    return SPHX_OK;
+}
+
 int Engine::step(hipStream_t st, int nsteps) {
     for (int k = 0; k < nsteps; ++k) {
-        SPHX_CHECK(sphx_zero_counters(counters, st));
-        SPHX_CHECK(tmark(st, PH_SORT, 0));
-        SPHX_CHECK(kick_drift(st));
-        if (mode == SPHX_MIXED) SPHX_CHECK(sort_and_permute(st, true));
-        SPHX_CHECK(tmark(st, PH_SORT, 1));
-        if (mode == SPHX_MIXED) SPHX_CHECK(forces(st, true, false));
-        else SPHX_CHECK(plain_forces(st, true, false));
-        SPHX_CHECK(tmark(st, PH_FINISH, 0));
-        SPHX_CHECK(finish_step(st));
-        SPHX_CHECK(tmark(st, PH_FINISH, 1));
+        SPHX_CHECK(step_body(st));
         SPHX_CHECK(tcollect());
         ++steps;
+    }
+    return SPHX_OK;
+}
+
+void Engine::drop_graph() {
+    for (int p = 0; p < 2; ++p) {
+        if (graph_exec[p]) hipGraphExecDestroy(graph_exec[p]);
+        if (graph[p]) hipGraphDestroy(graph[p]);
+        graph_exec[p] = nullptr;
+        graph[p] = nullptr;
+        graph_valid[p] = false;
+    }
+}
+
+// One step through the captured graph of the current pointer parity.  The
+// first call per parity captures the step (the host-side pointer swap of the
+// sort happens during capture); later calls replay it and mirror the swap.
+int Engine::step_graph(hipStream_t st) {
+    if (mode != SPHX_MIXED) return step(st, 1);
+    int p = gpar;
+    if (!graph_valid[p]) {
+        if (graph_exec[p]) hipGraphExecDestroy(graph_exec[p]);
+        if (graph[p]) hipGraphDestroy(graph[p]);
+        graph_exec[p] = nullptr;
+        graph[p] = nullptr;
+        SPHX_CHECK(sphx_gravity_prepare(*this, gcfg, st));
+        SPHX_HIP(hipStreamSynchronize(st));
+        bool t_on = timing;
+        timing = false;
+        SPHX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = step_body(st);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        timing = t_on;
+        if (rc < 0 || e != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            if (rc >= 0) {
+                sphx_set_error(std::string("stream capture: ") + hipGetErrorString(e));
+                rc = SPHX_ERR_HIP;
+            }
+            return rc;
+        }
+        hipError_t ei = hipGraphInstantiate(&graph_exec[p], g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            hipGraphDestroy(g);
+            sphx_set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+            return SPHX_ERR_HIP;
+        }
+        graph[p] = g;
+        graph_valid[p] = true;
+        // capture ran the host side of the step (gpar is now p ^ 1); execute it
+        SPHX_HIP(hipGraphLaunch(graph_exec[p], st));
+    } else {
+        SPHX_HIP(hipGraphLaunch(graph_exec[p], st));
+        swap_sorted();
+    }
+    ++steps;
+    return SPHX_OK;
+}
+
+int Engine::step_graphs(hipStream_t caller, int nsteps) {
+    if (!gstream) SPHX_HIP(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
+    if (!gev_in) SPHX_HIP(hipEventCreateWithFlags(&gev_in, hipEventDisableTiming));
+    if (!gev_out) SPHX_HIP(hipEventCreateWithFlags(&gev_out, hipEventDisableTiming));
+    SPHX_HIP(hipEventRecord(gev_in, caller));
+    SPHX_HIP(hipStreamWaitEvent(gstream, gev_in, 0));
+    for (int k = 0; k < nsteps; ++k) SPHX_CHECK(step_graph(gstream));
+    SPHX_HIP(hipEventRecord(gev_out, gstream));
+    SPHX_HIP(hipStreamWaitEvent(caller, gev_out, 0));
+    return SPHX_OK;
+}
+
+int Engine::restrict_to(int run) {
+    if (run < 0) {
+        rrun = -1;
+        return SPHX_OK;
+    }
+    if (run >= R) {
+        sphx_set_error("run index out of range");
+        return SPHX_ERR_ARG;
+    }
+    rrun = run;
+    rp0 = hr_pbeg[run];
+    rp1 = hr_pend[run];
+    if (hr_send[run] > hr_sbeg[run]) {
+        rb0 = hs_bbeg[hr_sbeg[run]];
+        rb1 = hs_bend[hr_send[run] - 1];
+    } else {
+        rb0 = rb1 = 0;
     }
     return SPHX_OK;
 }
@@ -685,7 +785,7 @@ extern "C" int sphx_neighbors(sphx_ctx *ctx, int64_t n, const double *pos, const
     return SPHX_OK;
 }
 
-static int copy_cool(Engine &E, const sphx_cooling *cool) {
+int sphx_copy_cool(Engine &E, const sphx_cooling *cool) {
     if (!cool) {
         E.has_cool = false;
         return SPHX_OK;
@@ -721,7 +821,7 @@ extern "C" int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const doub
     if (n < 0 || (n > 0 && (!rho || !u || !u_out))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
     Engine &E = ctx->E;
     SPHX_CHECK(E.reserve(64));
-    SPHX_CHECK(copy_cool(E, cool));
+    SPHX_CHECK(sphx_copy_cool(E, cool));
     if (!E.has_cool) { sphx_set_error("cooling table required"); return SPHX_ERR_ARG; }
     if (n == 0) return SPHX_OK;
     SPHX_CHECK(sphx_zero_counters(E.counters, st));
@@ -796,7 +896,7 @@ extern "C" int sphx_plan_create(int device, int mode, const sphx_params *p,
     pl->E.mode = mode;
     pl->E.p = *p;
     int rc = pl->E.reserve(std::max<int64_t>(capacity, 64));
-    if (rc >= 0) rc = copy_cool(pl->E, cool);
+    if (rc >= 0) rc = sphx_copy_cool(pl->E, cool);
     if (rc < 0) {
         delete pl;
         return rc;
@@ -865,11 +965,9 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
         E.run_params.assign(1, rp);
         SPHX_CHECK(E.upload_run_params(st));
     }
-    bool keep_cool = E.has_cool;
-    if (!use_cooling) E.has_cool = false;
-    int rc = E.step(st, n_steps);
-    E.has_cool = keep_cool;
-    return rc;
+    // cooling is gated by the run's use_cool flag (device memory), so a
+    // captured step graph stays valid when the flag changes
+    return (E.use_graph && !E.timing) ? E.step_graphs(st, n_steps) : E.step(st, n_steps);
 }
 
 extern "C" int sphx_plan_store(sphx_plan *plan, double *pos, double *vel, double *u, double *h,
@@ -951,5 +1049,12 @@ extern "C" int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int
         ms[k] = plan->E.phase_ms[k];
         count[k] = plan->E.phase_n[k];
     }
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_set_graph(sphx_plan *plan, int32_t on) {
+    if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
+    plan->E.use_graph = on != 0;
+    plan->E.drop_graph();
     return SPHX_OK;
 }

@@ -276,10 +276,10 @@ __global__ __launch_bounds__(256) void k_grav(GravTabs gt, GravIO io) {
     }
 }
 
-__global__ void k_grav_reduce(const double *__restrict__ part, int js, int64_t n, double *ax,
-                              double *ay, double *az, double *pot) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_grav_reduce(const double *__restrict__ part, int js, int64_t n, int64_t p0,
+                              int64_t p1, double *ax, double *ay, double *az, double *pot) {
+    int64_t i = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p1) return;
     double s[4] = {0, 0, 0, 0};
     for (int k = 0; k < js; ++k) {
         const double *pp = part + ((int64_t)k * n + i) * 4;
@@ -300,17 +300,19 @@ int launch_grav(const GravTabs &gt, const GravIO &io, int nblocks, bool pot, hip
 
 }  // namespace
 
-int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_t st) {
+int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     if (E.n == 0) return SPHX_OK;
     GravHost &H = E.grav;
     int ib = 256 * cfg.ipl;
     bool rebuild = H.ipl != cfg.ipl || H.tj != cfg.tj || H.sizes != E.run_n;
     if (rebuild) {
         std::vector<int32_t> beg, cnt, run, tbeg;
+        H.run_ib0.clear();
         int64_t off = 0;
         int tiles = 0;
         for (int r = 0; r < E.R; ++r) {
             int64_t nr = E.run_n[r];
+            H.run_ib0.push_back((int32_t)beg.size());
             for (int64_t a = 0; a < nr; a += ib) {
                 beg.push_back((int32_t)(off + a));
                 cnt.push_back((int32_t)std::min<int64_t>(ib, nr - a));
@@ -320,22 +322,24 @@ int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_
             tiles += (int)((nr + cfg.tj - 1) / cfg.tj);
             off += nr;
         }
+        H.run_ib0.push_back((int32_t)beg.size());
         int nb = (int)beg.size();
         int64_t need = 3 * (int64_t)nb + E.R;
         if (need > H.dev_cap) {
-            SPHX_CHECK(E.alloc((void **)&H.dev, sizeof(int32_t) * need));
-            H.dev_cap = need;
+            SPHX_CHECK(E.alloc((void **)&H.dev, sizeof(int32_t) * (need + 1024)));
+            H.dev_cap = need + 1024;
         }
         std::vector<int32_t> all;
         all.insert(all.end(), beg.begin(), beg.end());
         all.insert(all.end(), cnt.begin(), cnt.end());
         all.insert(all.end(), run.begin(), run.end());
         all.insert(all.end(), tbeg.begin(), tbeg.end());
-        SPHX_HIP(hipMemcpyAsync(H.dev, all.data(), sizeof(int32_t) * all.size(),
-                                hipMemcpyHostToDevice, st));
+        if (!all.empty())
+            SPHX_HIP(hipMemcpyAsync(H.dev, all.data(), sizeof(int32_t) * all.size(),
+                                    hipMemcpyHostToDevice, st));
         if (8 * (int64_t)tiles > H.tbox_cap) {
-            SPHX_CHECK(E.alloc((void **)&H.tbox, sizeof(double) * 8 * (int64_t)tiles));
-            H.tbox_cap = 8 * (int64_t)tiles;
+            SPHX_CHECK(E.alloc((void **)&H.tbox, sizeof(double) * (8 * (int64_t)tiles + 1024)));
+            H.tbox_cap = 8 * (int64_t)tiles + 1024;
         }
         H.ipl = cfg.ipl;
         H.tj = cfg.tj;
@@ -351,30 +355,53 @@ int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_
             H.part_cap = need;
         }
     }
+    return SPHX_OK;
+}
+
+int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_t st) {
+    if (E.n == 0) return SPHX_OK;
+    SPHX_CHECK(sphx_gravity_prepare(E, cfg, st));
+    GravHost &H = E.grav;
     int32_t *d = H.dev;
-    GravTabs gt{d, d + H.nblocks, d + 2 * H.nblocks, E.t.r_pbeg, E.t.r_pend, d + 3 * H.nblocks,
-                H.tbox, cfg.js};
-    k_tile_boxes<<<H.ntiles, 64, 0, st>>>(E.t, d + 3 * H.nblocks, H.ntiles, cfg.tj, H.tbox);
-    SPHX_HIP(hipGetLastError());
+    // a run-restricted launch covers only that run's i-blocks (its j range is
+    // its own run anyway)
+    int ib0 = 0, ib1 = H.nblocks;
+    if (E.rrun >= 0) {
+        ib0 = H.run_ib0[E.rrun];
+        ib1 = H.run_ib0[E.rrun + 1];
+    }
+    if (ib1 <= ib0) return SPHX_OK;
+    GravTabs gt{d + ib0, d + H.nblocks + ib0, d + 2 * H.nblocks + ib0, E.t.r_pbeg, E.t.r_pend,
+                d + 3 * H.nblocks, H.tbox, cfg.js};
+    if (H.ntiles > 0) {
+        k_tile_boxes<<<H.ntiles, 64, 0, st>>>(E.t, d + 3 * H.nblocks, H.ntiles, cfg.tj, H.tbox);
+        SPHX_HIP(hipGetLastError());
+    }
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
     GravIO io{E.s.x, E.s.y, E.s.z, E.s.m, eps, E.p.G, E.s.gx, E.s.gy, E.s.gz, E.s.pot, H.part, E.n};
+    int nblk = ib1 - ib0;
     int rc;
     switch (cfg.ipl * 10000 + cfg.tj) {
-        case 10256: rc = launch_grav<1, 256>(gt, io, H.nblocks, with_pot, st); break;
-        case 10512: rc = launch_grav<1, 512>(gt, io, H.nblocks, with_pot, st); break;
-        case 20256: rc = launch_grav<2, 256>(gt, io, H.nblocks, with_pot, st); break;
-        case 20512: rc = launch_grav<2, 512>(gt, io, H.nblocks, with_pot, st); break;
-        case 21024: rc = launch_grav<2, 1024>(gt, io, H.nblocks, with_pot, st); break;
-        case 40256: rc = launch_grav<4, 256>(gt, io, H.nblocks, with_pot, st); break;
-        case 40512: rc = launch_grav<4, 512>(gt, io, H.nblocks, with_pot, st); break;
+        case 10256: rc = launch_grav<1, 256>(gt, io, nblk, with_pot, st); break;
+        case 10512: rc = launch_grav<1, 512>(gt, io, nblk, with_pot, st); break;
+        case 20256: rc = launch_grav<2, 256>(gt, io, nblk, with_pot, st); break;
+        case 20512: rc = launch_grav<2, 512>(gt, io, nblk, with_pot, st); break;
+        case 21024: rc = launch_grav<2, 1024>(gt, io, nblk, with_pot, st); break;
+        case 40256: rc = launch_grav<4, 256>(gt, io, nblk, with_pot, st); break;
+        case 40512: rc = launch_grav<4, 512>(gt, io, nblk, with_pot, st); break;
         default:
             sphx_set_error("unsupported gravity configuration");
             return SPHX_ERR_ARG;
     }
     SPHX_CHECK(rc);
     if (cfg.js > 1) {
-        k_grav_reduce<<<(unsigned)((E.n + 255) / 256), 256, 0, st>>>(
-            H.part, cfg.js, E.n, E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr);
+        int64_t p0 = 0, p1 = E.n;
+        if (E.rrun >= 0) {
+            p0 = E.rp0;
+            p1 = E.rp1;
+        }
+        k_grav_reduce<<<(unsigned)((p1 - p0 + 255) / 256), 256, 0, st>>>(
+            H.part, cfg.js, E.n, p0, p1, E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr);
         SPHX_HIP(hipGetLastError());
     }
     return SPHX_OK;

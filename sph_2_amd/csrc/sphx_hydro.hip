@@ -64,7 +64,7 @@ struct SmoothArgs {
 
 __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a) {
     __shared__ double sx[64], sy[64], sz[64];
-    int b = blockIdx.x;
+    int b = t.boff + blockIdx.x;
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -206,7 +206,7 @@ struct DensArgs {
 __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     __shared__ double sx[64], sy[64], sz[64], sh[64];
     __shared__ float sm[64];
-    int b = blockIdx.x;
+    int b = t.boff + blockIdx.x;
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
 __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
     __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
     __shared__ float sm[64];
-    int b = blockIdx.x;
+    int b = t.boff + blockIdx.x;
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -321,7 +321,7 @@ __device__ __forceinline__ double balsara_f(double dv, double cv, double c, doub
 __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
     __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
     __shared__ float sm[64], srho[64], spr[64], sc[64], sf[64];
-    int b = blockIdx.x;
+    int b = t.boff + blockIdx.x;
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -404,7 +404,7 @@ __global__ __launch_bounds__(64) void k_nbr(Tables t, Soa s, int64_t r0, int64_t
                                             int32_t *__restrict__ idx) {
     __shared__ double sx[64], sy[64], sz[64], sh[64];
     __shared__ int32_t sid[64];
-    int b = blockIdx.x;
+    int b = t.boff + blockIdx.x;
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -452,33 +452,33 @@ int sphx_neighbor_pass(Engine &E, int64_t r0, int64_t r1, int32_t *counts, const
 }
 
 int Engine::smoothing(hipStream_t st, bool predictor) {
-    if (n == 0) return SPHX_OK;
+    if (n == 0 || nb_launch() == 0) return SPHX_OK;
     SmoothArgs a{p.nngb, p.ndown, p.nup, p.max_iter, (predictor && p.h_predictor) ? 1 : 0,
                  d_run, h_iters, counters};
-    k_smoothing<<<t.nb, 64, 0, st>>>(t, s, a);
+    k_smoothing<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
 
 int Engine::density_eos(hipStream_t st) {
-    if (n == 0) return SPHX_OK;
+    if (n == 0 || nb_launch() == 0) return SPHX_OK;
     DensArgs a{p.gamma, tmpi, counters};
-    k_density<<<t.nb, 64, 0, st>>>(t, s, a);
+    k_density<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
 
 int Engine::divcurl(hipStream_t st) {
-    if (n == 0) return SPHX_OK;
-    k_divcurl<<<t.nb, 64, 0, st>>>(t, s);
+    if (n == 0 || nb_launch() == 0) return SPHX_OK;
+    k_divcurl<<<nb_launch(), 64, 0, st>>>(tview(), s);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
 
 int Engine::hydro(hipStream_t st) {
-    if (n == 0) return SPHX_OK;
+    if (n == 0 || nb_launch() == 0) return SPHX_OK;
     HydroArgs a{(float)p.alpha, (float)p.beta, (float)(p.eta * p.eta), p.av_form, p.balsara};
-    k_hydro<<<t.nb, 64, 0, st>>>(t, s, a);
+    k_hydro<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -36,10 +36,10 @@ __global__ void k_kick_drift(Soa s, const RunParams *__restrict__ rp, int64_t n)
     s.z[i] = s.z[i] + vz * dt;
 }
 
-__global__ void k_combine(Soa s, int64_t n) {
+__global__ void k_combine(Soa s, int64_t p0, int64_t p1) {
 #pragma clang fp contract(off)
-    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
-    if (i >= n) return;
+    int64_t i = p0 + (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= p1) return;
     s.ax[i] = s.gx[i] + s.ax[i];
     s.ay[i] = s.gy[i] + s.ay[i];
     s.az[i] = s.gz[i] + s.az[i];
@@ -72,24 +72,33 @@ __device__ double cool_g(double u, double u_old, double dt_s, double nH, double 
 
 // Backward-Euler root of g(u) = 0: bracket by factors of 1.1 from u_old (the
 // search of photolibs3.py:505-521), then bisection to a relative width tol.
+// The gas never cools below the table's first temperature node: the input is
+// floored there, the bracket stops there, and a net cooling rate at the floor
+// returns the floor.
 __device__ double cool_one(double rho, double u_code, double dt, const CoolDev &cd, int *iters) {
 #pragma clang fp contract(off)
     double rho_cgs = rho * cd.unit_rho_cgs;
-    double u_old = u_code * cd.unit_u_cgs;
     double dt_s = dt * cd.unit_time_s;
     double nH = cd.XH * rho_cgs / MH_CGS;
     double tfac = (cd.gamma - 1.0) * cd.mu * MH_CGS / KB_CGS;
-    double g0 = cool_g(u_old, u_old, dt_s, nH, rho_cgs, tfac, cd);
+    double u_floor = cd.T[0] / tfac;
+    double u_old = u_code * cd.unit_u_cgs;
+    if (u_old < u_floor) u_old = u_floor;
     *iters = 0;
-    if (g0 == 0.0 || !(dt_s > 0.0)) return u_code;
+    if (!(dt_s > 0.0)) return u_old / cd.unit_u_cgs;
+    double g0 = cool_g(u_old, u_old, dt_s, nH, rho_cgs, tfac, cd);
+    if (g0 == 0.0) return u_old / cd.unit_u_cgs;
     double lo, hi;
     if (g0 > 0.0) {
+        if (cool_g(u_floor, u_old, dt_s, nH, rho_cgs, tfac, cd) >= 0.0) return u_floor / cd.unit_u_cgs;
         hi = u_old;
         lo = u_old / 1.1;
+        if (lo < u_floor) lo = u_floor;
         int nb = 0;
         while (cool_g(lo, u_old, dt_s, nH, rho_cgs, tfac, cd) > 0.0 && nb < cd.max_bracket) {
             hi = lo;
             lo = lo / 1.1;
+            if (lo < u_floor) lo = u_floor;
             ++nb;
         }
     } else {
@@ -160,9 +169,10 @@ __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, Coo
     s.u[i] = u;
 }
 
-__global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
-    if (i < n) b[i] = a[i];
+__global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t p0,
+                         int64_t p1) {
+    int64_t i = p0 + (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i < p1) b[i] = a[i];
 }
 
 // Per-block partial sums of the 11 diagnostic quantities; fixed tree order.
@@ -271,7 +281,9 @@ int Engine::finish_step(hipStream_t st) {
 
 int Engine::bootstrap_finish(hipStream_t st) {
     if (n == 0) return SPHX_OK;
-    k_copy_d<<<g1(n), IT, 0, st>>>(s.du, s.ut_prev, n);
+    int64_t p0 = rrun >= 0 ? rp0 : 0, p1 = rrun >= 0 ? rp1 : n;
+    if (p1 <= p0) return SPHX_OK;
+    k_copy_d<<<g1(p1 - p0), IT, 0, st>>>(s.du, s.ut_prev, p0, p1);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
@@ -284,7 +296,9 @@ int sphx_zero_counters(int64_t *c, hipStream_t st) {
 
 int sphx_combine(Engine &E, hipStream_t st) {
     if (E.n == 0) return SPHX_OK;
-    k_combine<<<g1(E.n), IT, 0, st>>>(E.s, E.n);
+    int64_t p0 = E.rrun >= 0 ? E.rp0 : 0, p1 = E.rrun >= 0 ? E.rp1 : E.n;
+    if (p1 <= p0) return SPHX_OK;
+    k_combine<<<g1(p1 - p0), IT, 0, st>>>(E.s, p0, p1);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

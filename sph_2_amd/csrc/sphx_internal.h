@@ -142,6 +142,7 @@ struct Soa {
 
 struct Tables {
     int32_t nb, ns, R;
+    int32_t boff;                               // first bucket of a restricted launch
     const int32_t *b_beg, *b_cnt, *b_run;       // buckets
     const int32_t *s_bbeg, *s_bend;             // supers: bucket ranges
     const int32_t *r_sbeg, *r_send;             // run -> super range
@@ -192,6 +193,7 @@ struct GravHost {
     int ipl = 0, tj = 0;
     int nblocks = 0, ntiles = 0;
     int32_t *dev = nullptr;  // ib_beg | ib_cnt | ib_run | r_tbeg
+    std::vector<int32_t> run_ib0;  // first i-block of each run (+ total at the end)
     int64_t dev_cap = 0;
     double *tbox = nullptr;  // 8 doubles per tile
     int64_t tbox_cap = 0;
@@ -244,6 +246,37 @@ struct Engine {
     GravConfig gcfg;
     GravHost grav;
 
+    // Launch restriction to one run (bootstrap of a newly admitted run):
+    // bucket range [rb0, rb1), particle range [rp0, rp1), run index rrun.
+    int32_t rrun = -1, rb0 = 0, rb1 = 0, rp0 = 0, rp1 = 0;
+    int restrict_to(int run);  // run < 0: whole set
+    Tables tview() const {
+        Tables v = t;
+        v.boff = rrun >= 0 ? rb0 : 0;
+        return v;
+    }
+    unsigned nb_launch() const { return (unsigned)(rrun >= 0 ? rb1 - rb0 : t.nb); }
+    int morton_bits = 21;      // bits per axis of the Morton key
+    int max_runs = 1;          // run id bits are reserved for this many runs
+
+    // Step graphs: one captured hipGraph of one step per pointer parity (the
+    // sort swaps state and spare arrays, so consecutive steps alternate
+    // between two pointer assignments).  Valid for the current run set.
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    bool graph_valid[2] = {false, false};
+    int gpar = 0;
+    bool use_graph = false;
+    void drop_graph();
+    void swap_sorted();
+    int step_body(hipStream_t st);
+    int step_graph(hipStream_t st);
+    // n graph steps on a private non-blocking stream, ordered after and
+    // before the caller's stream with events (capture needs a real stream)
+    hipStream_t gstream = nullptr;
+    hipEvent_t gev_in = nullptr, gev_out = nullptr;
+    int step_graphs(hipStream_t caller, int n);
+
     // optional per-phase HIP-event timing of the step
     bool timing = false;
     hipEvent_t tev[2 * 8] = {};
@@ -287,6 +320,7 @@ struct Engine {
 
 // launchers implemented in the .hip files
 int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_t st);
+int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st);
 int sphx_launch_morton(Engine &E, hipStream_t st);
 int sphx_radix_sort(uint64_t *keys, int32_t *vals, uint64_t *keys_tmp, int32_t *vals_tmp,
                     uint32_t *hist, int64_t n, int key_bits, hipStream_t st);

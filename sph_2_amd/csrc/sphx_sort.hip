@@ -5,6 +5,8 @@
 // cells of Particles_in_Gridcells/final_revision_uniform_grid/
 // hfvCLibs_v7_grid_v0.h:680-708) with Morton-ordered 64-particle buckets:
 // one wavefront per bucket, bucket and super-bucket AABBs for culling.
+#include <utility>
+
 #include "sphx_internal.h"
 
 namespace {
@@ -306,13 +308,21 @@ int sphx_radix_sort(uint64_t *keys, int32_t *vals, uint64_t *keys_tmp, int32_t *
     return SPHX_OK;
 }
 
-static int morton_bits(int R) { return R <= 1 ? 21 : 18; }
-
-static int run_key_bits(int R) {
+static int run_bits(int max_runs) {
     int rb = 0;
-    while ((1 << rb) < R) ++rb;
-    return 3 * morton_bits(R) + rb;
+    while ((1 << rb) < max_runs) ++rb;
+    return rb;
 }
+
+// Bits per axis: 21 for a lone run; fewer when run ids share the 63-bit key.
+// Depends on the engine's run capacity only, so a run sorts the same way
+// whether it is alone or packed with others.
+int sphx_morton_bits(int max_runs) {
+    int b = (63 - run_bits(max_runs)) / 3;
+    return b > 21 ? 21 : b;
+}
+
+static int run_key_bits(int max_runs) { return 3 * sphx_morton_bits(max_runs) + run_bits(max_runs); }
 
 int sphx_launch_morton(Engine &E, hipStream_t st) {
     Tables &t = E.t;
@@ -320,49 +330,60 @@ int sphx_launch_morton(Engine &E, hipStream_t st) {
     double *rbox = E.bbox + 6 * (int64_t)t.ns;  // 6 per run
     k_super_box_pos<<<t.ns, 64, 0, st>>>(t, E.s.x, E.s.y, E.s.z, sbox);
     k_run_box<<<t.R, 64, 0, st>>>(t, sbox, rbox);
-    k_morton<<<grid_for(E.n, 256), 256, 0, st>>>(E.s.x, E.s.y, E.s.z, E.R > 1 ? E.s.run : nullptr,
-                                                rbox, E.n, morton_bits(E.R), E.keys, E.perm);
+    k_morton<<<grid_for(E.n, 256), 256, 0, st>>>(E.s.x, E.s.y, E.s.z,
+                                                E.max_runs > 1 ? E.s.run : nullptr, rbox, E.n,
+                                                sphx_morton_bits(E.max_runs), E.keys, E.perm);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
+}
+
+// Persistent arrays gathered by the sort, and the derived arrays that serve
+// as their destinations (recomputed after every sort).  After the gather the
+// roles swap; applying the swap twice restores the original assignment.
+static void sort_roles(Engine &E, double **dst[12], double **spare[12], int32_t **idst[3],
+                       int32_t **ispare[3]) {
+    Soa &s = E.s;
+    double **d[12] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.u, &s.m,
+                      &s.h, &s.eps, &s.ut_prev, &s.divv};
+    double **sp[12] = {&s.gx, &s.gy, &s.gz, &s.rho, &s.P, &s.c, &s.curlv, &s.du,
+                       &s.pot, &s.ax, &s.ay, &s.az};
+    int32_t **id[3] = {&s.k, &s.id, &s.run};
+    int32_t **isp[3] = {&E.tmpi, &E.perm2, &E.h_iters};
+    for (int a = 0; a < 12; ++a) { dst[a] = d[a]; spare[a] = sp[a]; }
+    for (int a = 0; a < 3; ++a) { idst[a] = id[a]; ispare[a] = isp[a]; }
+}
+
+void Engine::swap_sorted() {
+    double **d[12], **sp[12];
+    int32_t **id[3], **isp[3];
+    sort_roles(*this, d, sp, id, isp);
+    for (int a = 0; a < 12; ++a) std::swap(*d[a], *sp[a]);
+    for (int a = 0; a < 3; ++a) std::swap(*id[a], *isp[a]);
+    gpar ^= 1;
 }
 
 int Engine::sort_and_permute(hipStream_t st, bool gather_state) {
     if (n == 0) return SPHX_OK;
     SPHX_CHECK(sphx_launch_morton(*this, st));
-    SPHX_CHECK(sphx_radix_sort(keys, perm, keys2, perm2, hist, n, run_key_bits(R), st));
+    SPHX_CHECK(sphx_radix_sort(keys, perm, keys2, perm2, hist, n, run_key_bits(max_runs), st));
     if (!gather_state) return SPHX_OK;
-    // Gather the persistent state into the new order; each array swaps with
-    // its spare so no copy-back is needed.
-    double **dsrc[] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.u, &s.m,
-                       &s.h, &s.eps, &s.ut_prev, &s.divv};
-    int32_t **isrc[] = {&s.k, &s.id, &s.run};
+    double **d[12], **sp[12];
+    int32_t **id[3], **isp[3];
+    sort_roles(*this, d, sp, id, isp);
     GatherList g{};
     g.nd = 12;
     g.ni = 3;
-    // spares live in the derived arrays that are recomputed after the sort
-    double *spares_d[12] = {s.gx, s.gy, s.gz, s.rho, s.P, s.c, s.curlv, s.du, s.pot, s.ax, s.ay, s.az};
-    int32_t *spares_i[3] = {tmpi, perm2, h_iters};
     for (int a = 0; a < 12; ++a) {
-        g.src[a] = *dsrc[a];
-        g.dst[a] = spares_d[a];
+        g.src[a] = *d[a];
+        g.dst[a] = *sp[a];
     }
     for (int a = 0; a < 3; ++a) {
-        g.isrc[a] = *isrc[a];
-        g.idst[a] = spares_i[a];
+        g.isrc[a] = *id[a];
+        g.idst[a] = *isp[a];
     }
     k_gather<<<grid_for(n, 256), 256, 0, st>>>(g, perm, n);
     SPHX_HIP(hipGetLastError());
-    // swap: the gathered arrays become the state, the old state arrays become
-    // the derived/scratch arrays
-    double *old_d[12];
-    for (int a = 0; a < 12; ++a) old_d[a] = *dsrc[a];
-    for (int a = 0; a < 12; ++a) *dsrc[a] = spares_d[a];
-    s.gx = old_d[0]; s.gy = old_d[1]; s.gz = old_d[2]; s.rho = old_d[3]; s.P = old_d[4];
-    s.c = old_d[5]; s.curlv = old_d[6]; s.du = old_d[7]; s.pot = old_d[8]; s.ax = old_d[9];
-    s.ay = old_d[10]; s.az = old_d[11];
-    int32_t *old_i[3] = {s.k, s.id, s.run};
-    s.k = spares_i[0]; s.id = spares_i[1]; s.run = spares_i[2];
-    tmpi = old_i[0]; perm2 = old_i[1]; h_iters = old_i[2];
+    swap_sorted();
     return SPHX_OK;
 }
 

@@ -84,6 +84,10 @@ class SphPlan:
         check(_lib.load().sphx_plan_query(self._h, C.byref(c)), "sphx_plan_query")
         return c.as_dict()
 
+    def set_graph(self, on: bool = True):
+        """Replay steps from captured hipGraphs (one per pointer parity)."""
+        check(_lib.load().sphx_plan_set_graph(self._h, int(on)), "set_graph")
+
     def set_timing(self, on: bool = True):
         check(_lib.load().sphx_plan_set_timing(self._h, int(on)), "set_timing")
 

@@ -392,3 +392,15 @@ def test_step_energy_budget():
     for _ in range(5):
         s = oracle.step(s, 1e-3, p)
     assert abs(energy(s) - e0) < 2e-4 * abs(e0)
+
+
+def test_cooling_floor_at_table_minimum():
+    """Net cooling that would overshoot the lowest table temperature stops at
+    u(T_min); inputs below the floor are raised to it."""
+    cp = _cool(1e-20)
+    tfac = (5.0 / 3.0 - 1.0) * cp.mu * 1.6726e-24 / 1.3807e-16
+    u_floor = 10.0 / tfac
+    u, _ = oracle.cool(np.array([1e-20]), np.array([1e12]), 1e12, cp)
+    assert u[0] == u_floor
+    u, _ = oracle.cool(np.array([1e-30]), np.array([-5.0]), 1.0, _cool(0.0))
+    assert u[0] == u_floor
