@@ -41,12 +41,21 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_rms(a, b) -> float:
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="evrard", choices=["evrard", "c2", "c3", "bh", "serve"])
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=1_048_576)
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--dt", type=float, default=1e-4)
     ap.add_argument("--fp64-steps", type=int, default=1,
@@ -54,6 +63,13 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-rows", type=int, default=8192,
                     help="rows of the step timed on the FP64 CPU oracle (0: skip)")
     ap.add_argument("--grav", default=None, help="gravity config ipl,tj,js (default: autotuned)")
+    ap.add_argument("--graph", type=int, default=0, help="replay steps from hipGraphs")
+    ap.add_argument("--runs", type=int, default=512, help="serve: requests in the stream")
+    ap.add_argument("--nmin", type=int, default=2048)
+    ap.add_argument("--nmax", type=int, default=262144)
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: arrivals per second")
+    ap.add_argument("--run-steps", type=int, default=200, help="serve: steps per request")
+    ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve: HBM budget")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -63,12 +79,21 @@ def main(argv=None) -> int:
 
     import torch
 
-    from sph_2_amd import ic
-    from sph_2_amd.plan import SphPlan
-
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         return 1
+    if a.workload != "evrard":
+        return {"c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve}[a.workload](a)
+    if a.n is None:
+        a.n = 1_048_576
+    return bench_evrard(a)
+
+
+def bench_evrard(a) -> int:
+    import torch
+
+    from sph_2_amd import ic
+    from sph_2_amd.plan import SphPlan
 
     n = a.n
     e = ic.evrard(n, seed=a.seed)
@@ -87,7 +112,7 @@ def main(argv=None) -> int:
     pl.init()
     pl.step(a.warmup, a.dt)
     torch.cuda.synchronize()
-    print("[bench] mixed warmup done", file=sys.stderr, flush=True)
+    log("mixed warmup done")
     pl.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -195,6 +220,192 @@ def main(argv=None) -> int:
         "parity_vs_fp64_oracle": parity,
     }
     print(json.dumps(out))
+    return 0
+
+
+def _plan_run(n_steps, e, mode, dt, cooling=None, record=(1,), graph=False):
+    """Run a plan; returns ({step: state}, ms per step over the steps)."""
+    import torch
+
+    from sph_2_amd.plan import SphPlan
+
+    pl = SphPlan(e.n, mode=mode, cooling=cooling)
+    if graph:
+        pl.set_graph(True)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h)
+    pl.init()
+    torch.cuda.synchronize()
+    states = {}
+    t = 0.0
+    done = 0
+    for target in sorted(set(record) | {n_steps}):
+        t0 = time.perf_counter()
+        pl.step(target - done, dt, cooling=cooling is not None)
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+        done = target
+        if target in record:
+            states[target] = pl.state()
+    diag = pl.diagnostics(True)
+    cnt = pl.counters()
+    pl.close()
+    return states, 1e3 * t / n_steps, diag, cnt
+
+
+def bench_c2(a) -> int:
+    """Evrard N=131072, 100 steps at dt=1e-4, mixed vs plain FP64 on the GPU,
+    errors at steps 1 and 100; the FP64 GPU forces at step 1 checked against
+    the CPU oracle on sampled rows."""
+    import oracle
+    from sph_2_amd import ic
+
+    n = a.n or 131072
+    steps = 100
+    e = ic.evrard(n, seed=a.seed)
+    log(f"c2 Evrard N={n}")
+    mx, ms_mx, d_mx, c_mx = _plan_run(steps, e, "mixed", a.dt, record=(1, steps), graph=bool(a.graph))
+    log(f"mixed {ms_mx:.2f} ms/step")
+    fp, ms_fp, d_fp, c_fp = _plan_run(steps, e, "fp64", a.dt, record=(1, steps))
+    log(f"fp64 {ms_fp:.2f} ms/step")
+    errs = {}
+    for s in (1, steps):
+        A, B = mx[s], fp[s]
+        errs[str(s)] = {
+            "density_rel_max": float(np.max(np.abs(A["rho"] / B["rho"] - 1))),
+            "acc_rel_rms": rel_rms(A["acc"], B["acc"]),
+            "acc_grav_rel_rms": rel_rms(A["acc_g"], B["acc_g"]),
+            "h_rel_max": float(np.max(np.abs(A["h"] / B["h"] - 1))),
+            "k_mismatch_frac": float(np.mean(A["k"] != B["k"])),
+            "pos_rel_rms": rel_rms(A["pos"], B["pos"]),
+            "u_rel_max": float(np.max(np.abs(A["u"] / B["u"] - 1))),
+        }
+    # chain of trust: FP64 GPU forces at step 1 vs the CPU oracle on rows.
+    # Density and gravity depend only on the stored positions and h; the
+    # hydro terms used the half-kicked velocity and the pre-update u, which the
+    # stored state no longer holds (their parity is the C1 stateless tests').
+    B = fp[1]
+    rows = np.arange(0, n, max(1, n // 2048))
+    t0 = time.perf_counter()
+    rho_o, _ = oracle.density(B["pos"], e.m, B["h"], rows=rows)
+    g_o = oracle.gravity(B["pos"], e.m, B["h"], 1.0, rows=rows)
+    tcpu = time.perf_counter() - t0
+    chain = {"rows": int(len(rows)),
+             "density_rel_max": float(np.max(np.abs(B["rho"][rows] / rho_o - 1))),
+             "gravity_rel_rms": rel_rms(B["acc_g"][rows], g_o),
+             "cpu_s": tcpu}
+    out = {"workload": f"c2: Evrard N={n}, {steps} KDK steps at dt={a.dt}",
+           "mixed_ms_per_step": ms_mx, "fp64_ms_per_step": ms_fp,
+           "mixed_particle_steps_per_s": n * 1e3 / ms_mx,
+           "fp64_particle_steps_per_s": n * 1e3 / ms_fp,
+           "speedup": ms_fp / ms_mx,
+           "mixed_vs_fp64_gpu": errs,
+           "fp64_gpu_vs_cpu_oracle_step1": chain,
+           "energy_mixed": d_mx, "energy_fp64": d_fp,
+           "counters_mixed": c_mx}
+    print(json.dumps(out))
+    return 0
+
+
+def bench_c3(a) -> int:
+    """Turbulent cooling cloud N=1,048,576, 50 mixed-precision steps with the
+    implicit FP64 cooling solve."""
+    import torch
+
+    from sph_2_amd import ic
+    from sph_2_amd.plan import SphPlan
+    from sph_2_amd.serve import default_cooling
+
+    n = a.n or 1_048_576
+    steps = 50
+    t0 = time.perf_counter()
+    e = ic.turbulent_cloud(n, seed=a.seed)
+    log(f"c3 cloud N={n} IC in {time.perf_counter() - t0:.1f}s")
+    cool = default_cooling()
+    pl = SphPlan(n, cooling=cool)
+    if a.graph:
+        pl.set_graph(True)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h)
+    pl.init()
+    d0 = pl.diagnostics(True)
+    st0 = pl.state(("u",))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pl.step(steps, a.dt, cooling=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    d1 = pl.diagnostics(True)
+    st1 = pl.state(("u", "rho"))
+    cnt = pl.counters()
+    pl.close()
+    units = e.units
+    tfac = (5.0 / 3.0 - 1.0) * cool.kw["mu"] * ic.MH / ic.KB * units["unit_u_cgs"]
+    out = {"workload": f"c3: turbulent cooling cloud N={n}, rms Mach 2, T0=1e4 K, {steps} steps "
+                       f"at dt={a.dt} (code units: 1e4 Msun, 1 pc), 200-node Lambda(T) table",
+           "ms_per_step": 1e3 * wall / steps, "particle_steps_per_s": n * steps / wall,
+           "T_median_K_start": float(np.median(st0["u"]) * tfac),
+           "T_median_K_end": float(np.median(st1["u"]) * tfac),
+           "T_min_K_end": float(st1["u"].min() * tfac),
+           "energy_start": d0, "energy_end": d1, "counters": cnt,
+           "finite": bool(np.isfinite(st1["u"]).all() and np.isfinite(st1["rho"]).all())}
+    print(json.dumps(out))
+    return 0
+
+
+def bench_bh(a) -> int:
+    """Barnes-Hut on a Plummer sphere N=4,194,304, eps=0.01, theta 0.3/0.5/0.7;
+    force RMS error against FP64 direct summation on 65,536 seeded rows."""
+    import torch
+
+    from sph_2_amd import ic, libsx
+
+    n = a.n or 4_194_304
+    e = ic.plummer(n, seed=a.seed, eps=0.01)
+    nsub = min(65536, n)
+    log(f"bh Plummer N={n}")
+    t0 = time.perf_counter()
+    ref = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.eps, mode="fp64", nbeg=0, nend=nsub)
+    torch.cuda.synchronize()
+    t_ref = time.perf_counter() - t0
+    log(f"fp64 direct rows {t_ref:.2f}s")
+    res = {}
+    for theta in (0.3, 0.5, 0.7):
+        libsx.getAcc_g_bh(e.pos, e.m, 1.0, e.eps, theta)   # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        acc, st = libsx.getAcc_g_bh(e.pos, e.m, 1.0, e.eps, theta, return_stats=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        err = acc[:nsub] - ref
+        rel = np.linalg.norm(err, axis=1) / np.linalg.norm(ref, axis=1)
+        res[str(theta)] = {"s_per_eval_incl_transfer": t, "rel_rms": rel_rms(acc[:nsub], ref),
+                           "rel_median": float(np.median(rel)), "rel_p99": float(np.percentile(rel, 99)),
+                           "interactions_per_particle": st["interactions"] / n, **st}
+        log(f"theta {theta}: {t:.3f}s rel_rms {res[str(theta)]['rel_rms']:.2e}")
+    out = {"workload": f"c4: Barnes-Hut, Plummer N={n}, a=1, eps=0.01, FP32; error vs FP64 "
+                       f"direct sum on rows 0..{nsub} (random order IC)",
+           "fp64_direct_rows_s": t_ref, "theta": res}
+    print(json.dumps(out))
+    return 0
+
+
+def bench_serve(a) -> int:
+    """Served stream: Poisson arrivals, N log-uniform, 200 steps each, admitted
+    by HBM budget, every live run advanced by one launch per pass."""
+    from sph_2_amd.serve import Server, make_stream
+
+    reqs = make_stream(a.runs, n_min=a.nmin, n_max=a.nmax, rate=a.rate, steps=a.run_steps,
+                       seed=a.seed)
+    t0 = time.perf_counter()
+    from sph_2_amd.serve import build_payload
+    for r in reqs:
+        r.payload = build_payload(r)
+    log(f"serve: {len(reqs)} requests built in {time.perf_counter() - t0:.1f}s")
+    srv = Server(int(a.hbm_gb * 2 ** 30), max_runs=512, graph=bool(a.graph))
+    rep = srv.run(reqs, log=None)
+    rep["workload"] = (f"c5: {a.runs} runs, N log-uniform in [{a.nmin}, {a.nmax}], "
+                       f"{a.run_steps} steps each, Poisson arrivals at {a.rate}/s, "
+                       f"HBM budget {a.hbm_gb} GB, graph={bool(a.graph)}")
+    print(json.dumps(rep))
     return 0
 
 

@@ -146,6 +146,17 @@ int sphx_getAcc_g_smth(sphx_ctx *ctx, int mode, int64_t n, const double *pos, co
                        const double *eps, double G, int64_t row_beg, int64_t row_end,
                        double *acc, double *pot, void *stream);
 
+/* Barnes-Hut gravity with the spline softening of getAcc_g_smth: Morton-built
+ * octree (cells = key ranges), monopole moments, acceptance when a cell's
+ * distance from the 64-particle Morton group's box exceeds s/theta + |com -
+ * centre|.  Replaces the octree of Particles_in_BH_Tree_II/Evrard_test/
+ * bh_tree_iteration_v2.h:29-588.  stats (nullable, 5 int64): cells, depth,
+ * pair interactions, frontier overflows, largest interaction-list fill.
+ * Synchronous; SPHX_ERR_STATE if a group's frontier overflowed. */
+int sphx_bh_gravity(sphx_ctx *ctx, int64_t n, const double *pos, const double *m,
+                    const double *eps, double G, double theta, int64_t row_beg, int64_t row_end,
+                    double *acc, double *pot, int64_t *stats, void *stream);
+
 /* Neighbour lists under r_ij <= h_i + h_j for rows [row_beg,row_end), ascending
  * j per row (the gate of LIBV2:89).  offsets (rows+1, int64) always written;
  * idx (int32) written when capacity suffices; *total receives the pair count.

@@ -83,6 +83,7 @@ SIGNATURES = {
                                   _I64, _I64, _V, _V, _V]),
     "sphx_getAcc_g_smth": (C.c_int, [_V, C.c_int, _I64, _V, _V, _V, _D, _I64, _I64, _V, _V,
                                      _V]),
+    "sphx_bh_gravity": (C.c_int, [_V, _I64, _V, _V, _V, _D, _D, _I64, _I64, _V, _V, _V, _V]),
     "sphx_neighbors": (C.c_int, [_V, _I64, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V]),
     "sphx_cool": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V, _V]),
     "sphx_diagnostics": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _V, _V]),

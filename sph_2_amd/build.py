@@ -28,6 +28,7 @@ SOURCES = [
     "sphx_plain.hip",
     "sphx_integrate.hip",
     "sphx_batch.hip",
+    "sphx_bh.hip",
 ]
 
 ARCH = os.environ.get("SPHX_OFFLOAD_ARCH", "gfx950")

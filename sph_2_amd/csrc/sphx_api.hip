@@ -499,6 +499,10 @@ static int check_params(const sphx_params *p) {
 
 struct sphx_ctx {
     Engine E;
+    BhHost *bh = nullptr;
+    ~sphx_ctx() {
+        if (bh) sphx_bh_host_free(bh);
+    }
 };
 
 struct DeviceGuard {
@@ -1056,5 +1060,43 @@ extern "C" int sphx_plan_set_graph(sphx_plan *plan, int32_t on) {
     if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
     plan->E.use_graph = on != 0;
     plan->E.drop_graph();
+    return SPHX_OK;
+}
+
+extern "C" int sphx_bh_gravity(sphx_ctx *ctx, int64_t n, const double *pos, const double *m,
+                               const double *eps, double G, double theta, int64_t row_beg,
+                               int64_t row_end, double *acc, double *pot, int64_t *stats,
+                               void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    if (n == 0 || row_end == row_beg) return SPHX_OK;
+    if (!pos || !m || !eps || !acc) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    if (!(theta > 0.0) || theta > 2.0) { sphx_set_error("theta must be in (0, 2]"); return SPHX_ERR_ARG; }
+    Engine &E = ctx->E;
+    E.p.G = G;
+    E.p.eps_from_h = 0;
+    E.max_runs = 1;
+    SPHX_CHECK(ctx_load(ctx, n, pos, m, nullptr, nullptr, false, st));
+    k_copy_d<<<gu(n), UT, 0, st>>>(eps, E.s.eps, n);
+    k_copy_d<<<gu(n), UT, 0, st>>>(eps, E.s.h, n);
+    SPHX_HIP(hipGetLastError());
+    SPHX_CHECK(E.sort_and_permute(st, true));
+    if (!ctx->bh) ctx->bh = sphx_bh_host_new();
+    BhStats bs{};
+    SPHX_CHECK(sphx_bh_run(E, *ctx->bh, theta, pot != nullptr, st, &bs));
+    SPHX_CHECK(scatter_out_aos(E, E.s.gx, E.s.gy, E.s.gz, row_beg, row_end, acc, st));
+    if (pot) SPHX_CHECK(scatter_out(E, E.s.pot, row_beg, row_end, pot, st));
+    if (stats) {
+        stats[0] = bs.nodes;
+        stats[1] = bs.depth;
+        stats[2] = bs.interactions;
+        stats[3] = bs.overflow;
+        stats[4] = bs.max_fill;
+    }
+    SPHX_HIP(hipStreamSynchronize(st));
+    if (bs.overflow > 0) {
+        sphx_set_error("Barnes-Hut frontier overflow: increase theta");
+        return SPHX_ERR_STATE;
+    }
     return SPHX_OK;
 }

@@ -1,0 +1,459 @@
+// Barnes-Hut gravity: Morton-built octree, monopole moments, group walk.
+//
+// Replaces the reference's iterative octree (Particles_in_BH_Tree_II/
+// Evrard_test/bh_tree_iteration_v2.h: single-block atomic insertion build
+// 149-367, per-warp stack walk 454-588, theta = 0.5) with a build that
+// follows the sorted 63-bit Morton keys: every octree cell is a contiguous
+// key range, and a cell's eight children are found by binary search on the
+// next three key bits, one level per pass.  Moments (mass, centre of mass,
+// max eps) are summed bottom-up in FP64.  The walk is per 64-particle Morton
+// group (one wavefront): a cell is accepted for the whole group when its
+// distance from the group's bounding box exceeds s/theta + |com - centre|
+// (cell side s); accepted cells and the particles of opened leaves go to an
+// LDS interaction list that is flushed through the FP32 spline-softened pair
+// formula with Kahan-compensated sums.
+#include <algorithm>
+
+#include "sphx_internal.h"
+
+namespace {
+
+constexpr int BH_LEAF = 8;        // particles per leaf at most
+constexpr int BH_FCAP = 1536;     // frontier capacity per group (cells)
+constexpr int BH_ICAP = 768;      // interaction list capacity per group
+constexpr int BHT = 256;
+
+struct BhTree {
+    int32_t *beg, *cnt, *child, *nchild, *level;
+    double *mass, *cx, *cy, *cz, *gx, *gy, *gz, *size, *delta, *emax;
+};
+
+__device__ __forceinline__ int digit_at(uint64_t key, int shift) { return (int)((key >> shift) & 7ull); }
+
+// For each cell of the current level: the nine child boundaries (first index
+// of each child digit) and the number of non-empty children.
+__global__ void k_bh_split(BhTree T, int32_t f0, int32_t fc, const uint64_t *__restrict__ keys,
+                           int level, int32_t *__restrict__ nk, int32_t *__restrict__ bounds) {
+    int t = blockIdx.x * BHT + threadIdx.x;
+    if (t >= fc) return;
+    int i = f0 + t;
+    int b = T.beg[i], c = T.cnt[i];
+    int32_t *bd = bounds + 9 * t;
+    if (c <= BH_LEAF || level >= 21) {
+        nk[t] = 0;
+        bd[0] = -1;
+        return;
+    }
+    int shift = 3 * (20 - level);
+    bd[0] = b;
+    bd[8] = b + c;
+    for (int d = 1; d < 8; ++d) {
+        int lo = b, hi = b + c;  // first index with digit >= d
+        while (lo < hi) {
+            int mid = (lo + hi) >> 1;
+            if (digit_at(keys[mid], shift) < d) lo = mid + 1;
+            else hi = mid;
+        }
+        bd[d] = lo;
+    }
+    int k = 0;
+    for (int d = 0; d < 8; ++d) k += bd[d + 1] > bd[d] ? 1 : 0;
+    nk[t] = k;
+}
+
+// Exclusive scan of fc counts (one workgroup), total written to *total.
+__global__ __launch_bounds__(1024) void k_bh_scan(const int32_t *__restrict__ in,
+                                                  int32_t *__restrict__ out, int32_t fc,
+                                                  int32_t *__restrict__ total) {
+    __shared__ int32_t part[1024];
+    int tid = threadIdx.x;
+    int per = (fc + 1023) / 1024;
+    int b = tid * per, e = min(fc, b + per);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += in[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = tid > 0 ? part[tid - 1] : 0;
+    for (int i = b; i < e; ++i) {
+        out[i] = run;
+        run += in[i];
+    }
+    if (tid == 1023) *total = part[1023];
+}
+
+__global__ void k_bh_emit(BhTree T, int32_t f0, int32_t fc, const int32_t *__restrict__ off,
+                          const int32_t *__restrict__ bounds, int32_t base, int level) {
+    int t = blockIdx.x * BHT + threadIdx.x;
+    if (t >= fc) return;
+    int i = f0 + t;
+    const int32_t *bd = bounds + 9 * t;
+    if (bd[0] < 0) {
+        T.child[i] = -1;
+        T.nchild[i] = 0;
+        return;
+    }
+    int o = base + off[t];
+    int k = 0;
+    for (int d = 0; d < 8; ++d) {
+        if (bd[d + 1] > bd[d]) {
+            T.beg[o + k] = bd[d];
+            T.cnt[o + k] = bd[d + 1] - bd[d];
+            T.level[o + k] = level + 1;
+            ++k;
+        }
+    }
+    T.child[i] = o;
+    T.nchild[i] = k;
+}
+
+__device__ __forceinline__ uint32_t compact3(uint64_t v) {
+    // gather every third bit of v (bit 3b -> bit b)
+    v &= 0x1249249249249249ull;
+    v = (v | (v >> 2)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v >> 4)) & 0x100f00f00f00f00full;
+    v = (v | (v >> 8)) & 0x1f0000ff0000ffull;
+    v = (v | (v >> 16)) & 0x1f00000000ffffull;
+    v = (v | (v >> 32)) & 0x1fffffull;
+    return (uint32_t)v;
+}
+
+// Moments of the cells of one level (children are one level deeper and done).
+__global__ void k_bh_moments(BhTree T, int32_t l0, int32_t l1, const uint64_t *__restrict__ keys,
+                             const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, const double *__restrict__ m,
+                             const double *__restrict__ eps, const double *__restrict__ rbox,
+                             double cell0) {
+    int i = l0 + blockIdx.x * BHT + threadIdx.x;
+    if (i >= l1) return;
+    double M = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, em = 0.0;
+    if (T.child[i] < 0) {
+        for (int q = T.beg[i]; q < T.beg[i] + T.cnt[i]; ++q) {
+            double mq = m[q];
+            M += mq;
+            sx += mq * x[q]; sy += mq * y[q]; sz += mq * z[q];
+            em = fmax(em, eps[q]);
+        }
+    } else {
+        for (int c = T.child[i]; c < T.child[i] + T.nchild[i]; ++c) {
+            double mc = T.mass[c];
+            M += mc;
+            sx += mc * T.cx[c]; sy += mc * T.cy[c]; sz += mc * T.cz[c];
+            em = fmax(em, T.emax[c]);
+        }
+    }
+    double cx = M > 0.0 ? sx / M : 0.0, cy = M > 0.0 ? sy / M : 0.0, cz = M > 0.0 ? sz / M : 0.0;
+    int l = T.level[i];
+    double side = cell0 * (double)(1ull << (21 - l));   // cell0 = side of a level-21 cell
+    uint64_t pre = l > 0 ? keys[T.beg[i]] >> (3 * (21 - l)) : 0ull;
+    double gx = rbox[0] + (compact3(pre >> 2) + 0.5) * side;
+    double gy = rbox[2] + (compact3(pre >> 1) + 0.5) * side;
+    double gz = rbox[4] + (compact3(pre) + 0.5) * side;
+    T.mass[i] = M;
+    T.cx[i] = cx; T.cy[i] = cy; T.cz[i] = cz;
+    T.gx[i] = gx; T.gy[i] = gy; T.gz[i] = gz;
+    T.size[i] = side;
+    T.delta[i] = sqrt((cx - gx) * (cx - gx) + (cy - gy) * (cy - gy) + (cz - gz) * (cz - gz));
+    T.emax[i] = em;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total) {
+    int incl = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    *total = __shfl(incl, 63, 64);
+    return incl - v;
+}
+
+struct BhArgs {
+    const double *x, *y, *z, *m, *eps;
+    int64_t n;
+    double G, theta;
+    double *ax, *ay, *az, *pot;
+    int64_t *stats;  // [0] interactions, [1] frontier overflow, [2] max list fill
+};
+
+__device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, float4 j, float ej,
+                                        float &ax, float &ay, float &az, float &ph) {
+    float dx = j.x - xi, dy = j.y - yi, dz = j.z - zi;
+    float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    float e = 0.5f * (ei + ej);
+    float f, p;
+    if (r2 >= 4.0f * e * e) {
+        float ri = __builtin_amdgcn_rsqf(r2);
+        f = ri * ri * ri;
+        p = ri;
+    } else {
+        float r = r2 > 0.0f ? __builtin_amdgcn_sqrtf(r2) : 0.0f;
+        float ie = __builtin_amdgcn_rcpf(e);
+        float q = r * ie;
+        if (q <= 1.0f) {
+            float q2 = q * q;
+            f = ie * ie * ie * (4.0f / 3.0f - 1.2f * q2 + 0.5f * q2 * q);
+            p = r2 > 0.0f ? ie * (1.4f - 2.0f * q2 * (1.0f / 3.0f - 0.15f * q2 + 0.05f * q2 * q)) : 0.0f;
+        } else {
+            float w = 2.0f - q;
+            float w2 = w * w;
+            float ri = __builtin_amdgcn_rcpf(r);
+            f = ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
+            float q2 = q * q;
+            p = -ri * (1.0f / 15.0f) -
+                ie * q2 * (4.0f / 3.0f - q + 0.3f * q2 - q2 * q * (1.0f / 30.0f)) + 1.6f * ie;
+        }
+    }
+    float s = j.w * f;
+    ax = __builtin_fmaf(s, dx, ax);
+    ay = __builtin_fmaf(s, dy, ay);
+    az = __builtin_fmaf(s, dz, az);
+    ph = __builtin_fmaf(j.w, p, ph);
+}
+
+__global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
+    __shared__ int32_t fr[2][BH_FCAP];
+    __shared__ float4 il[BH_ICAP];
+    __shared__ float ie[BH_ICAP];
+    const int lane = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+    const bool ok = i < a.n;
+    double xd = ok ? a.x[i] : 0.0, yd = ok ? a.y[i] : 0.0, zd = ok ? a.z[i] : 0.0;
+    double lx = wave_min_d(ok ? xd : INFINITY), hx = wave_max_d(ok ? xd : -INFINITY);
+    double ly = wave_min_d(ok ? yd : INFINITY), hy = wave_max_d(ok ? yd : -INFINITY);
+    double lz = wave_min_d(ok ? zd : INFINITY), hz = wave_max_d(ok ? zd : -INFINITY);
+    const double ox = 0.5 * (lx + hx), oy = 0.5 * (ly + hy), oz = 0.5 * (lz + hz);
+    const float xi = (float)(xd - ox), yi = (float)(yd - oy), zi = (float)(zd - oz);
+    const float ei = ok ? (float)a.eps[i] : 1.0f;
+    float ax = 0.f, ay = 0.f, az = 0.f, ph = 0.f;
+    KahanF kx, ky, kz, kp;
+    kx.init(); ky.init(); kz.init(); kp.init();
+    int icnt = 0;
+    int64_t ninter = 0;
+    int overflow = 0, maxfill = 0;
+    if (lane == 0) fr[0][0] = 0;
+    int fcnt = 1, cur = 0;
+    __syncthreads();
+
+    auto flush = [&]() {
+        __syncthreads();
+        for (int q = 0; q < icnt; ++q) bh_pair(xi, yi, zi, ei, il[q], ie[q], ax, ay, az, ph);
+        kx.add(ax); ky.add(ay); kz.add(az); kp.add(ph);
+        ax = ay = az = ph = 0.f;
+        ninter += icnt;
+        icnt = 0;
+        __syncthreads();
+    };
+
+    while (fcnt > 0) {
+        int ncnt = 0;
+        for (int base = 0; base < fcnt; base += 64) {
+            int node = base + lane < fcnt ? fr[cur][base + lane] : -1;
+            int act = 0, nent = 0, nfr = 0;
+            if (node >= 0) {
+                double cxn = T.cx[node], cyn = T.cy[node], czn = T.cz[node];
+                double dx = fmax(0.0, fmax(lx - cxn, cxn - hx));
+                double dy = fmax(0.0, fmax(ly - cyn, cyn - hy));
+                double dz = fmax(0.0, fmax(lz - czn, czn - hz));
+                double d = sqrt(dx * dx + dy * dy + dz * dz);
+                if (d * a.theta > T.size[node] + a.theta * T.delta[node]) {
+                    act = 1;
+                    nent = 1;
+                } else if (T.child[node] < 0) {
+                    act = 2;
+                    nent = T.cnt[node];
+                } else {
+                    act = 3;
+                    nfr = T.nchild[node];
+                }
+            }
+            // accepted cells: at most one entry per lane
+            int tot;
+            int pos = wave_excl_scan(act == 1 ? 1 : 0, lane, &tot);
+            if (icnt + tot > BH_ICAP) flush();
+            if (act == 1) {
+                il[icnt + pos] = make_float4((float)(T.cx[node] - ox), (float)(T.cy[node] - oy),
+                                             (float)(T.cz[node] - oz), (float)T.mass[node]);
+                ie[icnt + pos] = (float)T.emax[node];
+            }
+            icnt += tot;
+            // particles of opened leaves: rounds of at most BH_LEAF per lane, so
+            // one round never exceeds 64 * BH_LEAF <= BH_ICAP entries
+            int rem = act == 2 ? nent : 0;
+            int pb = act == 2 ? T.beg[node] : 0;
+            while (__any(rem > 0)) {
+                int take = rem < BH_LEAF ? rem : BH_LEAF;
+                int lpos = wave_excl_scan(take, lane, &tot);
+                if (icnt + tot > BH_ICAP) flush();
+                for (int q = 0; q < take; ++q) {
+                    int p = pb + q;
+                    il[icnt + lpos + q] = make_float4((float)(a.x[p] - ox), (float)(a.y[p] - oy),
+                                                      (float)(a.z[p] - oz), (float)a.m[p]);
+                    ie[icnt + lpos + q] = (float)a.eps[p];
+                }
+                icnt += tot;
+                rem -= take;
+                pb += take;
+            }
+            maxfill = max(maxfill, icnt);
+            int ftot;
+            int fpos = ncnt + wave_excl_scan(nfr, lane, &ftot);
+            if (act == 3) {
+                int c0 = T.child[node];
+                for (int q = 0; q < nfr; ++q)
+                    if (fpos + q < BH_FCAP) fr[cur ^ 1][fpos + q] = c0 + q;
+            }
+            if (ncnt + ftot > BH_FCAP) overflow = 1;
+            ncnt = min(ncnt + ftot, BH_FCAP);
+        }
+        __syncthreads();
+        cur ^= 1;
+        fcnt = ncnt;
+    }
+    flush();
+    if (ok) {
+        a.ax[i] = a.G * kx.value();
+        a.ay[i] = a.G * ky.value();
+        a.az[i] = a.G * kz.value();
+        if (a.pot) a.pot[i] = -a.G * kp.value();
+    }
+    if (lane == 0) {
+        atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)(ninter * (a.n - (int64_t)blockIdx.x * 64 < 64 ? a.n - (int64_t)blockIdx.x * 64 : 64)));
+        if (overflow) atomicAdd((unsigned long long *)&a.stats[1], 1ull);
+        atomicMax((unsigned long long *)&a.stats[2], (unsigned long long)maxfill);
+    }
+}
+
+}  // namespace
+
+// Host side: tree storage (owned here, not by the engine) and the build.
+struct BhHost {
+    BhTree T{};
+    int64_t cap = 0;
+    int32_t *nk = nullptr, *off = nullptr, *bounds = nullptr, *total = nullptr;
+    int64_t scratch_cap = 0;
+    int64_t *stats = nullptr;
+    std::vector<void *> owned;
+    ~BhHost() {
+        for (void *q : owned) hipFree(q);
+    }
+    int alloc(void **ptr, size_t bytes) {
+        void *q = nullptr;
+        if (hipMalloc(&q, bytes ? bytes : 8) != hipSuccess) {
+            sphx_set_error("hipMalloc failed for the Barnes-Hut tree");
+            return SPHX_ERR_NOMEM;
+        }
+        owned.push_back(q);
+        *ptr = q;
+        return SPHX_OK;
+    }
+};
+
+static int bh_alloc_nodes(BhHost &H, int64_t need) {
+    if (need <= H.cap) return SPHX_OK;
+    int64_t cap = std::max<int64_t>(need, 2 * H.cap);
+    BhTree T{};
+    int32_t **ip[] = {&T.beg, &T.cnt, &T.child, &T.nchild, &T.level};
+    double **dp[] = {&T.mass, &T.cx, &T.cy, &T.cz, &T.gx, &T.gy, &T.gz, &T.size, &T.delta, &T.emax};
+    for (int32_t **q : ip) SPHX_CHECK(H.alloc((void **)q, sizeof(int32_t) * cap));
+    for (double **q : dp) SPHX_CHECK(H.alloc((void **)q, sizeof(double) * cap));
+    if (H.cap > 0) {
+        int32_t *oi[] = {H.T.beg, H.T.cnt, H.T.child, H.T.nchild, H.T.level};
+        int32_t *ni[] = {T.beg, T.cnt, T.child, T.nchild, T.level};
+        for (int k = 0; k < 5; ++k)
+            SPHX_HIP(hipMemcpy(ni[k], oi[k], sizeof(int32_t) * H.cap, hipMemcpyDeviceToDevice));
+    }
+    H.T = T;
+    H.cap = cap;
+    return SPHX_OK;
+}
+
+// Build the octree and walk it for every particle of the engine (single run,
+// Morton-sorted, eps in E.s.eps); accelerations into E.s.gx..gz, potential
+// into E.s.pot when with_pot.
+int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t st, BhStats *out) {
+    int64_t n = E.n;
+    if (n == 0) return SPHX_OK;
+    SPHX_CHECK(bh_alloc_nodes(H, std::max<int64_t>(1024, n / 2)));
+    if (!H.total) {
+        SPHX_CHECK(H.alloc((void **)&H.total, sizeof(int32_t) * 4));
+        SPHX_CHECK(H.alloc((void **)&H.stats, sizeof(int64_t) * 4));
+    }
+    // root
+    int32_t root[5] = {0, (int32_t)n, -1, 0, 0};
+    SPHX_HIP(hipMemcpyAsync(H.T.beg, &root[0], 4, hipMemcpyHostToDevice, st));
+    SPHX_HIP(hipMemcpyAsync(H.T.cnt, &root[1], 4, hipMemcpyHostToDevice, st));
+    SPHX_HIP(hipMemcpyAsync(H.T.level, &root[4], 4, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> lev{0, 1};
+    int32_t nodes = 1;
+    for (int level = 0; level < 21; ++level) {
+        int32_t f0 = lev[level], fc = lev[level + 1] - lev[level];
+        if (fc <= 0) break;
+        if (fc > H.scratch_cap) {
+            int64_t c = std::max<int64_t>(fc, 2 * H.scratch_cap);
+            SPHX_CHECK(H.alloc((void **)&H.nk, sizeof(int32_t) * c));
+            SPHX_CHECK(H.alloc((void **)&H.off, sizeof(int32_t) * c));
+            SPHX_CHECK(H.alloc((void **)&H.bounds, sizeof(int32_t) * 9 * c));
+            H.scratch_cap = c;
+        }
+        unsigned g = (unsigned)((fc + BHT - 1) / BHT);
+        k_bh_split<<<g, BHT, 0, st>>>(H.T, f0, fc, E.keys, level, H.nk, H.bounds);
+        k_bh_scan<<<1, 1024, 0, st>>>(H.nk, H.off, fc, H.total);
+        int32_t tot = 0;
+        SPHX_HIP(hipMemcpyAsync(&tot, H.total, 4, hipMemcpyDeviceToHost, st));
+        SPHX_HIP(hipStreamSynchronize(st));
+        if ((int64_t)nodes + tot > H.cap) SPHX_CHECK(bh_alloc_nodes(H, (int64_t)nodes + tot));
+        k_bh_emit<<<g, BHT, 0, st>>>(H.T, f0, fc, H.off, H.bounds, nodes, level);
+        SPHX_HIP(hipGetLastError());
+        nodes += tot;
+        lev.push_back(nodes);
+        if (tot == 0) break;
+    }
+    // leaves of the deepest level (never split) get child = -1
+    {
+        int32_t f0 = lev[lev.size() - 2], fc = lev.back() - f0;
+        if (fc > 0 && (int)lev.size() - 2 >= 21) {
+            std::vector<int32_t> neg(fc, -1), zero(fc, 0);
+            SPHX_HIP(hipMemcpyAsync(H.T.child + f0, neg.data(), 4 * fc, hipMemcpyHostToDevice, st));
+            SPHX_HIP(hipMemcpyAsync(H.T.nchild + f0, zero.data(), 4 * fc, hipMemcpyHostToDevice, st));
+            SPHX_HIP(hipStreamSynchronize(st));
+        }
+    }
+    int depth = (int)lev.size() - 2;
+    // moments bottom-up; the run box gives the cell geometry
+    const double *rbox = E.bbox + 6 * (int64_t)E.t.ns;
+    double hb[6];
+    SPHX_HIP(hipMemcpyAsync(hb, rbox, sizeof(hb), hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    double ext = std::max(hb[1] - hb[0], std::max(hb[3] - hb[2], hb[5] - hb[4]));
+    double cell0 = ext > 0.0 ? ext * (1.0 + 0x1p-30) / (double)(1ull << 21) : 1.0;
+    for (int l = depth; l >= 0; --l) {
+        int32_t l0 = lev[l], l1 = lev[l + 1];
+        if (l1 <= l0) continue;
+        k_bh_moments<<<(unsigned)((l1 - l0 + BHT - 1) / BHT), BHT, 0, st>>>(
+            H.T, l0, l1, E.keys, E.s.x, E.s.y, E.s.z, E.s.m, E.s.eps, rbox, cell0);
+    }
+    SPHX_HIP(hipMemsetAsync(H.stats, 0, sizeof(int64_t) * 4, st));
+    BhArgs a{E.s.x, E.s.y, E.s.z, E.s.m, E.s.eps, n, E.p.G, theta,
+             E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr, H.stats};
+    k_bh_walk<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(H.T, a);
+    SPHX_HIP(hipGetLastError());
+    if (out) {
+        int64_t sv[4] = {0, 0, 0, 0};
+        SPHX_HIP(hipMemcpyAsync(sv, H.stats, sizeof(sv), hipMemcpyDeviceToHost, st));
+        SPHX_HIP(hipStreamSynchronize(st));
+        out->nodes = nodes;
+        out->depth = depth;
+        out->interactions = sv[0];
+        out->overflow = sv[1];
+        out->max_fill = sv[2];
+    }
+    return SPHX_OK;
+}
+
+BhHost *sphx_bh_host_new() { return new BhHost(); }
+void sphx_bh_host_free(BhHost *h) { delete h; }

@@ -318,6 +318,15 @@ struct Engine {
     int diagnostics(hipStream_t st, bool with_pe, sphx_diag *out);
 };
 
+// Barnes-Hut tree (sphx_bh.hip)
+struct BhHost;
+struct BhStats {
+    int64_t nodes, depth, interactions, overflow, max_fill;
+};
+BhHost *sphx_bh_host_new();
+void sphx_bh_host_free(BhHost *h);
+int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t st, BhStats *out);
+
 // launchers implemented in the .hip files
 int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_t st);
 int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st);

@@ -244,6 +244,31 @@ def getAcc_g_smth_mimj_mpi(nbeg, nend, pos, mass, G, epsilon, **kw):
     return out
 
 
+def getAcc_g_bh(pos, mass, G, epsilon, theta=0.5, *, device=0, nbeg=None, nend=None,
+                return_potential=False, return_stats=False):
+    """Barnes-Hut spline-softened gravity (Morton-built octree, monopoles,
+    opening angle theta); the direct-sum counterpart is getAcc_g_smth."""
+    t = _t()
+    pos = _f64(pos)
+    n = len(pos)
+    r0, r1 = _rows(n, nbeg, nend)
+    eps = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (n,)))
+    dp, dm, de = dev(pos), dev(_f64(mass)), dev(eps)
+    acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda")
+    pot = t.empty(r1 - r0, dtype=t.float64, device="cuda") if return_potential else None
+    st = (C.c_int64 * 5)()
+    check(_lib.load().sphx_bh_gravity(_context(device), n, ptr(dp), ptr(dm), ptr(de), float(G),
+                                      float(theta), r0, r1, ptr(acc), ptr(pot), st,
+                                      _lib.stream_handle()), "bh_gravity")
+    out = [acc.cpu().numpy()]
+    if return_potential:
+        out.append(pot.cpu().numpy())
+    if return_stats:
+        out.append(dict(zip(("cells", "depth", "interactions", "overflow", "max_fill"),
+                            [int(v) for v in st])))
+    return out[0] if len(out) == 1 else tuple(out)
+
+
 def _diag(pos, v, m, u=None, eps=None, G=1.0, device=0):
     d = _lib.Diag()
     dv, dm = dev(_f64(v)), dev(_f64(m))
