@@ -97,12 +97,13 @@ struct sphx_batch {
     int64_t next_id = 1;
 };
 
-// Arrays that travel with a particle between steps: the persistent state and
-// the total acceleration (needed by the next half kick).
+// Arrays that travel with a particle between steps: the persistent state, the
+// total acceleration (needed by the next half kick) and the density (returned
+// when the run retires).
 static std::vector<double *> moving_d(Engine &E) {
     Soa &s = E.s;
     return {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.u, s.m, s.h, s.eps, s.ut_prev, s.divv,
-            s.ax, s.ay, s.az};
+            s.ax, s.ay, s.az, s.rho};
 }
 static std::vector<int32_t *> moving_i(Engine &E) { return {E.s.k, E.s.id, E.s.run}; }
 

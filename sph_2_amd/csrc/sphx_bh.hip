@@ -19,7 +19,8 @@
 namespace {
 
 constexpr int BH_LEAF = 8;        // particles per leaf at most
-constexpr int BH_FCAP = 1536;     // frontier capacity per group (cells)
+constexpr int BH_SCAP = 3072;     // cell stack capacity per group
+constexpr int BH_SLACK = 160;     // > 7 x 21 levels of pure depth-first growth
 constexpr int BH_ICAP = 768;      // interaction list capacity per group
 constexpr int BHT = 256;
 
@@ -216,7 +217,7 @@ __device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, 
 }
 
 __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
-    __shared__ int32_t fr[2][BH_FCAP];
+    __shared__ int32_t stk[BH_SCAP];
     __shared__ float4 il[BH_ICAP];
     __shared__ float ie[BH_ICAP];
     const int lane = threadIdx.x;
@@ -235,8 +236,8 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
     int icnt = 0;
     int64_t ninter = 0;
     int overflow = 0, maxfill = 0;
-    if (lane == 0) fr[0][0] = 0;
-    int fcnt = 1, cur = 0;
+    if (lane == 0) stk[0] = 0;
+    int sp = 1;
     __syncthreads();
 
     auto flush = [&]() {
@@ -249,70 +250,78 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         __syncthreads();
     };
 
-    while (fcnt > 0) {
-        int ncnt = 0;
-        for (int base = 0; base < fcnt; base += 64) {
-            int node = base + lane < fcnt ? fr[cur][base + lane] : -1;
-            int act = 0, nent = 0, nfr = 0;
-            if (node >= 0) {
-                double cxn = T.cx[node], cyn = T.cy[node], czn = T.cz[node];
-                double dx = fmax(0.0, fmax(lx - cxn, cxn - hx));
-                double dy = fmax(0.0, fmax(ly - cyn, cyn - hy));
-                double dz = fmax(0.0, fmax(lz - czn, czn - hz));
-                double d = sqrt(dx * dx + dy * dy + dz * dz);
-                if (d * a.theta > T.size[node] + a.theta * T.delta[node]) {
-                    act = 1;
-                    nent = 1;
-                } else if (T.child[node] < 0) {
-                    act = 2;
-                    nent = T.cnt[node];
-                } else {
-                    act = 3;
-                    nfr = T.nchild[node];
-                }
+    // Depth-first in batches: pop up to 64 cells from the top of the stack,
+    // classify them, push the children of the opened ones.  A popped cell adds
+    // at most 7 net entries, so popping at most (cap - slack - sp) / 7 keeps the
+    // stack inside cap - slack; past that the walk is pure depth-first (one
+    // cell per batch), whose growth is bounded by 7 x the tree depth (<= 147),
+    // inside the slack.
+    while (sp > 0) {
+        int room = (BH_SCAP - BH_SLACK - sp) / 7;
+        int take = room < 1 ? 1 : (room < 64 ? room : 64);
+        take = sp < take ? sp : take;
+        int base = sp - take;
+        int node = lane < take ? stk[base + lane] : -1;
+        sp = base;
+        __syncthreads();
+        int act = 0, nent = 0, nfr = 0;
+        if (node >= 0) {
+            double cxn = T.cx[node], cyn = T.cy[node], czn = T.cz[node];
+            double dx = fmax(0.0, fmax(lx - cxn, cxn - hx));
+            double dy = fmax(0.0, fmax(ly - cyn, cyn - hy));
+            double dz = fmax(0.0, fmax(lz - czn, czn - hz));
+            double d = sqrt(dx * dx + dy * dy + dz * dz);
+            if (d * a.theta > T.size[node] + a.theta * T.delta[node]) {
+                act = 1;
+                nent = 1;
+            } else if (T.child[node] < 0) {
+                act = 2;
+                nent = T.cnt[node];
+            } else {
+                act = 3;
+                nfr = T.nchild[node];
             }
-            // accepted cells: at most one entry per lane
-            int tot;
-            int pos = wave_excl_scan(act == 1 ? 1 : 0, lane, &tot);
+        }
+        // accepted cells: at most one entry per lane
+        int tot;
+        int pos = wave_excl_scan(act == 1 ? 1 : 0, lane, &tot);
+        if (icnt + tot > BH_ICAP) flush();
+        if (act == 1) {
+            il[icnt + pos] = make_float4((float)(T.cx[node] - ox), (float)(T.cy[node] - oy),
+                                         (float)(T.cz[node] - oz), (float)T.mass[node]);
+            ie[icnt + pos] = (float)T.emax[node];
+        }
+        icnt += tot;
+        // particles of opened leaves: rounds of at most BH_LEAF per lane, so
+        // one round never exceeds 64 * BH_LEAF <= BH_ICAP entries
+        int rem = act == 2 ? nent : 0;
+        int pb = act == 2 ? T.beg[node] : 0;
+        while (__any(rem > 0)) {
+            int tk = rem < BH_LEAF ? rem : BH_LEAF;
+            int lpos = wave_excl_scan(tk, lane, &tot);
             if (icnt + tot > BH_ICAP) flush();
-            if (act == 1) {
-                il[icnt + pos] = make_float4((float)(T.cx[node] - ox), (float)(T.cy[node] - oy),
-                                             (float)(T.cz[node] - oz), (float)T.mass[node]);
-                ie[icnt + pos] = (float)T.emax[node];
+            for (int q = 0; q < tk; ++q) {
+                int p = pb + q;
+                il[icnt + lpos + q] = make_float4((float)(a.x[p] - ox), (float)(a.y[p] - oy),
+                                                  (float)(a.z[p] - oz), (float)a.m[p]);
+                ie[icnt + lpos + q] = (float)a.eps[p];
             }
             icnt += tot;
-            // particles of opened leaves: rounds of at most BH_LEAF per lane, so
-            // one round never exceeds 64 * BH_LEAF <= BH_ICAP entries
-            int rem = act == 2 ? nent : 0;
-            int pb = act == 2 ? T.beg[node] : 0;
-            while (__any(rem > 0)) {
-                int take = rem < BH_LEAF ? rem : BH_LEAF;
-                int lpos = wave_excl_scan(take, lane, &tot);
-                if (icnt + tot > BH_ICAP) flush();
-                for (int q = 0; q < take; ++q) {
-                    int p = pb + q;
-                    il[icnt + lpos + q] = make_float4((float)(a.x[p] - ox), (float)(a.y[p] - oy),
-                                                      (float)(a.z[p] - oz), (float)a.m[p]);
-                    ie[icnt + lpos + q] = (float)a.eps[p];
-                }
-                icnt += tot;
-                rem -= take;
-                pb += take;
-            }
-            maxfill = max(maxfill, icnt);
-            int ftot;
-            int fpos = ncnt + wave_excl_scan(nfr, lane, &ftot);
-            if (act == 3) {
-                int c0 = T.child[node];
-                for (int q = 0; q < nfr; ++q)
-                    if (fpos + q < BH_FCAP) fr[cur ^ 1][fpos + q] = c0 + q;
-            }
-            if (ncnt + ftot > BH_FCAP) overflow = 1;
-            ncnt = min(ncnt + ftot, BH_FCAP);
+            rem -= tk;
+            pb += tk;
         }
+        maxfill = max(maxfill, icnt);
+        // children of opened cells onto the stack
+        int ftot;
+        int fpos = sp + wave_excl_scan(nfr, lane, &ftot);
+        if (act == 3) {
+            int c0 = T.child[node];
+            for (int q = 0; q < nfr; ++q)
+                if (fpos + q < BH_SCAP) stk[fpos + q] = c0 + q;
+        }
+        if (sp + ftot > BH_SCAP) overflow = 1;
+        sp = min(sp + ftot, BH_SCAP);
         __syncthreads();
-        cur ^= 1;
-        fcnt = ncnt;
     }
     flush();
     if (ok) {
