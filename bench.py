@@ -60,7 +60,7 @@ def main(argv=None) -> int:
     ap.add_argument("--dt", type=float, default=1e-4)
     ap.add_argument("--fp64-steps", type=int, default=1,
                     help="timed steps of the plain FP64 GPU step (0: skip)")
-    ap.add_argument("--cpu-rows", type=int, default=8192,
+    ap.add_argument("--cpu-rows", type=int, default=32768,
                     help="rows of the step timed on the FP64 CPU oracle (0: skip)")
     ap.add_argument("--grav", default=None, help="gravity config ipl,tj,js (default: autotuned)")
     ap.add_argument("--graph", type=int, default=0, help="replay steps from hipGraphs")
@@ -380,7 +380,8 @@ def bench_bh(a) -> int:
         res[str(theta)] = {"s_per_eval_incl_transfer": t, "rel_rms": rel_rms(acc[:nsub], ref),
                            "rel_median": float(np.median(rel)), "rel_p99": float(np.percentile(rel, 99)),
                            "interactions_per_particle": st["interactions"] / n, **st}
-        log(f"theta {theta}: {t:.3f}s rel_rms {res[str(theta)]['rel_rms']:.2e}")
+        log(f"theta {theta}: {t:.3f}s (device build {st['build_us'] / 1e3:.1f} ms, walk "
+            f"{st['walk_us'] / 1e3:.1f} ms) rel_rms {res[str(theta)]['rel_rms']:.2e}")
     out = {"workload": f"c4: Barnes-Hut, Plummer N={n}, a=1, eps=0.01, FP32; error vs FP64 "
                        f"direct sum on rows 0..{nsub} (random order IC)",
            "fp64_direct_rows_s": t_ref, "theta": res}

@@ -85,6 +85,7 @@ typedef struct sphx_counters {
     int64_t cool_capped;    /* particles whose cooling solve hit its cap */
     int64_t cand_buckets;   /* candidate bucket pairs in the last neighbour build */
     int64_t buckets;        /* number of 64-particle buckets */
+    int64_t bh_overflow;    /* Barnes-Hut walk groups whose cell stack overflowed */
 } sphx_counters;
 
 typedef struct sphx_diag {
@@ -150,8 +151,9 @@ int sphx_getAcc_g_smth(sphx_ctx *ctx, int mode, int64_t n, const double *pos, co
  * octree (cells = key ranges), monopole moments, acceptance when a cell's
  * distance from the 64-particle Morton group's box exceeds s/theta + |com -
  * centre|.  Replaces the octree of Particles_in_BH_Tree_II/Evrard_test/
- * bh_tree_iteration_v2.h:29-588.  stats (nullable, 5 int64): cells, depth,
- * pair interactions, frontier overflows, largest interaction-list fill.
+ * bh_tree_iteration_v2.h:29-588.  stats (nullable, 8 int64): cells, depth,
+ * pair interactions, stack overflows, largest interaction-list fill, device
+ * microseconds of the tree build and of the walk, walk groups.
  * Synchronous; SPHX_ERR_STATE if a group's frontier overflowed. */
 int sphx_bh_gravity(sphx_ctx *ctx, int64_t n, const double *pos, const double *m,
                     const double *eps, double G, double theta, int64_t row_beg, int64_t row_end,
@@ -176,6 +178,16 @@ int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, doub
 int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, const double *v,
                      const double *m, const double *u, const double *eps, double G,
                      sphx_diag *out, void *stream);
+
+/* Global adaptive time step: dt_array (LIBV2:491-522) plus the driver's host
+ * reduction (DRV2:517-561), done on the device without the O(N^2) v_sig loop
+ * or the six host copies.  acc = total, acc_g = gravitational acceleration
+ * (n, 3); divV signed or absolute.  out (device, 5 doubles) = dt, dt_f,
+ * dt_kin, dt_cfl, dt_dens with dt = 0.25 min(...) clamped to [dt_min, dt_max]
+ * (the driver: C_CFL = 0.25, [1e-5, 1e-3]). */
+int sphx_global_dt(sphx_ctx *ctx, int64_t n, const double *h, const double *c,
+                   const double *acc, const double *acc_g, const double *divV, double c_cfl,
+                   double dt_min, double dt_max, double *out, void *stream);
 
 /* Morton keys (21 bits per axis, 63-bit key) over the bounding box of pos,
  * then a stable LSD radix sort; keys_out[i] and perm_out[i] = original index of
@@ -226,6 +238,25 @@ int sphx_ctx_set_gravity_config(sphx_ctx *ctx, int32_t ipl, int32_t tj, int32_t 
 int sphx_plan_set_timing(sphx_plan *plan, int32_t on);
 int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int32_t nphase);
 
+/* Adaptive global time step inside the step loop (DRV2:517-561): after each
+ * step's forces the device evaluates the criterion of sphx_global_dt and
+ * uses it for the next step, with no host round trip.  The dt argument of
+ * the first sphx_plan_step after enabling seeds the first step (the driver
+ * starts from dt = 2e-4, DRV2:303); later dt arguments are ignored.
+ * sphx_plan_time returns (host scalars) the simulated time -- the sum of the
+ * dt each step used --, the next step's dt and (nullable, 4 doubles) the
+ * last dt_f, dt_kin, dt_cfl, dt_dens.  Synchronous. */
+/* Gravity of the step from the Barnes-Hut tree of sphx_bh_gravity instead of
+ * the direct sum (theta > 0; 0 restores the direct sum).  Mixed precision,
+ * single-run plans; replaces the BH gravity call of the reference's tree
+ * drivers (Particles_in_BH_Tree_II/Evrard_test/bh_tree_iteration_v2.h:29-588
+ * with theta = 0.5).  Steps with the tree are not replayed from graphs. */
+int sphx_plan_set_gravity_bh(sphx_plan *plan, double theta);
+
+int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl, double dt_min,
+                              double dt_max);
+int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts, void *stream);
+
 /* Replay each step from a captured hipGraph (one graph per pointer parity of
  * the sorted state; recaptured when the run set changes).  Replaces the
  * launch-and-synchronise sequence of DRV2:407-482.  Timing mode disables it. */
@@ -254,6 +285,13 @@ int sphx_batch_store(sphx_batch *b, int64_t run_id, double *pos, double *vel, do
 int sphx_batch_step(sphx_batch *b, int32_t n_steps, void *stream);
 int sphx_batch_runs(sphx_batch *b, int64_t *ids, int64_t *sizes, int64_t *steps,
                     int32_t capacity, int32_t *count);
+/* Per-run adaptive time step (as sphx_plan_set_adaptive_dt; the admit dt
+ * seeds the first step) and the run's simulated time / next dt.  No
+ * counterpart in the reference, whose driver runs one simulation. */
+int sphx_batch_set_adaptive_dt(sphx_batch *b, int64_t run_id, int32_t on, double c_cfl,
+                               double dt_min, double dt_max, void *stream);
+int sphx_batch_time(sphx_batch *b, int64_t run_id, double *t, double *dt, double *dt_parts,
+                    void *stream);
 int sphx_batch_set_graph(sphx_batch *b, int32_t on);
 int sphx_batch_query(sphx_batch *b, sphx_counters *out);
 /* Device bytes held per particle of capacity (the admission budget unit). */

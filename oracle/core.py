@@ -58,6 +58,8 @@ def lib():
         L.orc_ke.argtypes = [i64, dp, dp]
         L.orc_pe.restype = C.c_double
         L.orc_pe.argtypes = [i64, dp, dp, dp, C.c_double]
+        L.orc_global_dt.restype = C.c_double
+        L.orc_global_dt.argtypes = [i64, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         _LIB = L
     return _LIB
 
@@ -300,6 +302,21 @@ def pe(pos, m, eps, G=1.0):
     return lib().orc_pe(n, pp, mp, ep, G)
 
 
+def global_dt(h, c, acc, acc_g, divV, c_cfl=0.25, dt_min=1e-5, dt_max=1e-3):
+    """Adaptive global dt of the GPU driver (myCppSPHLibs_v2.h:491-522 dt_array +
+    hfv_sph_gpu_v2_evrard.cu:517-561).  Returns (dt, (dt_f, dt_kin, dt_cfl, dt_dens))."""
+    n = len(h)
+    h, hp = _d(h)
+    c, cp = _d(c)
+    acc, ap = _d(acc)
+    acc_g, gp = _d(acc_g)
+    dv, dvp = _d(divV)
+    parts = np.zeros(4)
+    parts, pp = _d(parts)
+    dt = lib().orc_global_dt(n, hp, cp, ap, gp, dvp, c_cfl, dt_min, dt_max, pp)
+    return dt, tuple(float(v) for v in parts)
+
+
 # ---------------------------------------------------------------------------
 # Full step, composed from the functions above in the order of the GPU driver
 # (hfv_sph_gpu_v2_evrard.cu:407-482), with du/dt evaluated at the same
@@ -326,9 +343,10 @@ class State:
     ut_prev: np.ndarray | None = None
     h_iters: np.ndarray | None = None
     h_status: np.ndarray | None = None
+    t: float = 0.0
 
     def copy(self):
-        return State(**{k: (None if v is None else np.array(v, copy=True))
+        return State(**{k: (v if v is None or isinstance(v, float) else np.array(v, copy=True))
                         for k, v in self.__dict__.items()})
 
 
@@ -366,4 +384,10 @@ def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None) -> St
     s.ut_prev = s.dudt.copy()
     if cool_p is not None:
         s.u, _ = cool(s.rho, s.u, dt, cool_p)
+    s.t = s.t + dt
     return s
+
+
+def next_dt(s: State, c_cfl=0.25, dt_min=1e-5, dt_max=1e-3) -> float:
+    """dt of the next step from this step's forces (hfv_sph_gpu_v2_evrard.cu:517-561)."""
+    return global_dt(s.h, s.c, s.acc, s.acc_g, s.divV, c_cfl, dt_min, dt_max)[0]

@@ -524,3 +524,46 @@ double orc_pe(int64_t n, const double *pos, const double *m, const double *eps, 
     free(acc);
     return s;
 }
+
+/* Global adaptive time step of the GPU driver: dt_array (LIBV2:491-522) and
+ * the host reduction after it (DRV2:517-561).  The reference's O(N^2) v_sig
+ * loop is max_j max(c_i, c_j) = max over all c.  With hmin = min h:
+ *   dt_f    = sqrt(hmin / max |a_grav|)
+ *   dt_kin  = sqrt(hmin / max |a_total|)
+ *   dt_cfl  = c_cfl * hmin / max c
+ *   dt_dens = c_cfl * min_i h_i / |h_i divV_i / 3|
+ *   dt      = 0.25 * min(dt_f, dt_kin, dt_cfl, dt_dens), clamped to
+ *             [dt_min, dt_max] (1e-5 and 1e-3 in the driver).
+ * parts (nullable) receives dt_f, dt_kin, dt_cfl, dt_dens. */
+double orc_global_dt(int64_t n, const double *h, const double *c, const double *acc,
+                     const double *acc_g, const double *divv, double c_cfl, double dt_min,
+                     double dt_max, double *parts) {
+    double vmax = 0.0, hmin = INFINITY, agmax = 0.0, amax = 0.0, ddmin = INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        const double *a = acc + 3 * i, *g = acc_g + 3 * i;
+        double ag = sqrt((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
+        double at = sqrt((a[0] * a[0] + a[1] * a[1]) + a[2] * a[2]);
+        double dd = h[i] / fabs((1.0 / 3.0) * h[i] * divv[i]);
+        vmax = c[i] > vmax ? c[i] : vmax;
+        hmin = h[i] < hmin ? h[i] : hmin;
+        agmax = ag > agmax ? ag : agmax;
+        amax = at > amax ? at : amax;
+        ddmin = dd < ddmin ? dd : ddmin;
+    }
+    double dt_f = sqrt(hmin / agmax), dt_kin = sqrt(hmin / amax);
+    double dt_cfl = c_cfl * hmin / vmax, dt_dens = c_cfl * ddmin;
+    double m = dt_f;
+    m = dt_kin < m ? dt_kin : m;
+    m = dt_cfl < m ? dt_cfl : m;
+    m = dt_dens < m ? dt_dens : m;
+    double dt = 0.25 * m;
+    if (dt > dt_max) dt = dt_max;
+    if (dt < dt_min) dt = dt_min;
+    if (parts) {
+        parts[0] = dt_f;
+        parts[1] = dt_kin;
+        parts[2] = dt_cfl;
+        parts[3] = dt_dens;
+    }
+    return dt;
+}

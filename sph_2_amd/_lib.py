@@ -48,7 +48,7 @@ class Cooling(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("n", C.c_int64), ("steps", C.c_int64), ("h_iters_max", C.c_int64),
                 ("h_iters_sum", C.c_int64), ("h_capped", C.c_int64), ("cool_capped", C.c_int64),
-                ("cand_buckets", C.c_int64), ("buckets", C.c_int64)]
+                ("cand_buckets", C.c_int64), ("buckets", C.c_int64), ("bh_overflow", C.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -87,6 +87,7 @@ SIGNATURES = {
     "sphx_neighbors": (C.c_int, [_V, _I64, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V]),
     "sphx_cool": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V, _V]),
     "sphx_diagnostics": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _V, _V]),
+    "sphx_global_dt": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _D, _D, _V, _V]),
     "sphx_morton_sort": (C.c_int, [_V, _I64, _V, _V, _V, _V]),
     "sphx_plan_create": (C.c_int, [C.c_int, C.c_int, _V, _V, _I64, _V]),
     "sphx_plan_load": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _V]),
@@ -99,6 +100,9 @@ SIGNATURES = {
     "sphx_plan_set_gravity_config": (C.c_int, [_V, _I32, _I32, _I32]),
     "sphx_ctx_set_gravity_config": (C.c_int, [_V, _I32, _I32, _I32]),
     "sphx_plan_set_timing": (C.c_int, [_V, _I32]),
+    "sphx_plan_set_adaptive_dt": (C.c_int, [_V, _I32, _D, _D, _D]),
+    "sphx_plan_set_gravity_bh": (C.c_int, [_V, _D]),
+    "sphx_plan_time": (C.c_int, [_V, _V, _V, _V, _V]),
     "sphx_plan_timing": (C.c_int, [_V, _V, _V, _I32]),
     "sphx_plan_set_graph": (C.c_int, [_V, _I32]),
     "sphx_batch_create": (C.c_int, [C.c_int, C.c_int, _V, _V, _I64, _I32, _V]),
@@ -107,6 +111,8 @@ SIGNATURES = {
     "sphx_batch_store": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_batch_step": (C.c_int, [_V, _I32, _V]),
     "sphx_batch_runs": (C.c_int, [_V, _V, _V, _V, _I32, _V]),
+    "sphx_batch_set_adaptive_dt": (C.c_int, [_V, _I64, _I32, _D, _D, _D, _V]),
+    "sphx_batch_time": (C.c_int, [_V, _I64, _V, _V, _V, _V]),
     "sphx_batch_set_graph": (C.c_int, [_V, _I32]),
     "sphx_batch_query": (C.c_int, [_V, _V]),
     "sphx_batch_bytes_per_particle": (_I64, []),

@@ -109,6 +109,7 @@ Engine::~Engine() {
     int cur = 0;
     hipGetDevice(&cur);
     hipSetDevice(device);
+    if (bh) sphx_bh_host_free(bh);
     for (void *q : owned) hipFree(q);
     for (hipEvent_t e : tev)
         if (e) hipEventDestroy(e);
@@ -142,6 +143,7 @@ int Engine::reserve(int64_t capacity) {
     tab_d = nullptr; tab_d_cap = 0;
     bbox = nullptr; bbox_cap = 0;
     d_run = nullptr;
+    dtpart = nullptr; dtpart_cap = 0;
     cap = 0;
     n = 0;
     R = 0;
@@ -222,6 +224,10 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
         bbox_cap = nbx + 1024;
     }
     if ((int)run_params.size() < nr) run_params.resize(nr, RunParams{1e-4, 0, p.eps_from_h});
+    if (5 * (int64_t)nb > dtpart_cap) {
+        SPHX_CHECK(alloc((void **)&dtpart, sizeof(double) * (5 * (int64_t)nb + 1024)));
+        dtpart_cap = 5 * (int64_t)nb + 1024;
+    }
     if (R < nr || !d_run) {
         SPHX_CHECK(alloc((void **)&d_run, sizeof(RunParams) * std::max(nr, 1)));
     }
@@ -252,6 +258,14 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
 int Engine::upload_run_params(hipStream_t st) {
     if (R == 0) return SPHX_OK;
     SPHX_HIP(hipMemcpyAsync(d_run, run_params.data(), sizeof(RunParams) * R, hipMemcpyHostToDevice, st));
+    return SPHX_OK;
+}
+
+int Engine::pull_run_params(hipStream_t st) {
+    if (R == 0 || !d_run) return SPHX_OK;
+    if ((int)run_params.size() < R) run_params.resize(R);
+    SPHX_HIP(hipMemcpyAsync(run_params.data(), d_run, sizeof(RunParams) * R, hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipStreamSynchronize(st));
     return SPHX_OK;
 }
 
@@ -288,7 +302,12 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
     SPHX_CHECK(density_eos(st));
     SPHX_CHECK(tmark(st, PH_DENS, 1));
     SPHX_CHECK(tmark(st, PH_GRAV, 0));
-    SPHX_CHECK(sphx_gravity_run(*this, gcfg, with_pot, st));
+    if (bh_theta > 0.0) {
+        if (!bh) bh = sphx_bh_host_new();
+        SPHX_CHECK(sphx_bh_run(*this, *bh, bh_theta, with_pot, st, nullptr));
+    } else {
+        SPHX_CHECK(sphx_gravity_run(*this, gcfg, with_pot, st));
+    }
     SPHX_CHECK(tmark(st, PH_GRAV, 1));
     SPHX_CHECK(tmark(st, PH_DIVCURL, 0));
     SPHX_CHECK(divcurl(st));
@@ -302,10 +321,13 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
 
 int Engine::plain_forces(hipStream_t st, bool predictor, bool with_pot) {
     bool pred = predictor && p.h_predictor;
+    // the step's dt is read on the device (run 0's parameters), where the
+    // adaptive time step leaves it
+    const double *dt_p = d_run ? &d_run[0].dt : nullptr;
     double dt = run_params.empty() ? 0.0 : run_params[0].dt;
     SPHX_CHECK(tmark(st, PH_SMOOTH, 0));
     SPHX_CHECK(plain_smoothing(s.x, s.y, s.z, s.h, pred ? s.k : nullptr, pred ? s.divv : nullptr,
-                               nullptr, dt, nullptr, p, n, 0, n, s.h, s.k, h_iters, counters, st));
+                               dt_p, dt, nullptr, p, n, 0, n, s.h, s.k, h_iters, counters, st));
     SPHX_CHECK(tmark(st, PH_SMOOTH, 1));
     SPHX_CHECK(tmark(st, PH_DENS, 0));
     SPHX_CHECK(plain_density(s.x, s.y, s.z, s.m, s.h, n, 0, n, s.rho, tmpi, st));
@@ -338,6 +360,7 @@ int Engine::step_body(hipStream_t st) {
     else SPHX_CHECK(plain_forces(st, true, false));
     SPHX_CHECK(tmark(st, PH_FINISH, 0));
     SPHX_CHECK(finish_step(st));
+    SPHX_CHECK(global_dt(st));
     SPHX_CHECK(tmark(st, PH_FINISH, 1));
     return SPHX_OK;
 }
@@ -451,6 +474,7 @@ int Engine::read_counters(sphx_counters *out) {
     out->cool_capped = c[CNT_COOL_CAPPED];
     out->cand_buckets = c[CNT_CAND];
     out->buckets = t.nb;
+    out->bh_overflow = c[CNT_BH_OVERFLOW];
     return SPHX_OK;
 }
 
@@ -875,6 +899,11 @@ struct sphx_plan {
     Engine E;
     bool loaded = false;
     bool inited = false;
+    // adaptive time step: configuration, and whether the next sphx_plan_step's
+    // dt argument seeds the first adaptive step
+    int32_t adaptive = 0;
+    double c_cfl = 0.25, dt_min = 1e-5, dt_max = 1e-3;
+    bool seed_pending = false;
 };
 
 #define PLAN_PROLOGUE(plan)                        \
@@ -943,7 +972,13 @@ extern "C" int sphx_plan_init(sphx_plan *plan, void *stream) {
     PLAN_PROLOGUE(plan);
     if (!plan->loaded) { sphx_set_error("plan has no particles loaded"); return SPHX_ERR_STATE; }
     Engine &E = plan->E;
-    E.run_params.assign(1, RunParams{0.0, 0, E.p.eps_from_h});
+    RunParams rp0{0.0, 0, E.p.eps_from_h};
+    rp0.adaptive = plan->adaptive;
+    rp0.c_cfl = plan->c_cfl;
+    rp0.dt_min = plan->dt_min;
+    rp0.dt_max = plan->dt_max;
+    plan->seed_pending = plan->adaptive != 0;
+    E.run_params.assign(1, rp0);
     SPHX_CHECK(E.upload_run_params(st));
     SPHX_CHECK(sphx_zero_counters(E.counters, st));
     if (E.mode == SPHX_MIXED) {
@@ -964,14 +999,97 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
     if (n_steps < 0 || !(dt > 0.0)) { sphx_set_error("bad step count or dt"); return SPHX_ERR_ARG; }
     Engine &E = plan->E;
     if (use_cooling && !E.has_cool) { sphx_set_error("plan has no cooling table"); return SPHX_ERR_ARG; }
-    RunParams rp{dt, use_cooling ? 1 : 0, E.p.eps_from_h};
-    if (E.run_params.size() != 1 || E.run_params[0].dt != rp.dt || E.run_params[0].use_cool != rp.use_cool) {
-        E.run_params.assign(1, rp);
+    // The host copy of the run parameters is current except for t and, with
+    // the adaptive step, dt: pull the device copy before changing anything.
+    RunParams &hp = E.run_params[0];
+    int32_t uc = use_cooling ? 1 : 0;
+    bool set_dt = plan->adaptive ? plan->seed_pending : hp.dt != dt;
+    if (set_dt || hp.use_cool != uc) {
+        SPHX_CHECK(E.pull_run_params(st));
+        if (set_dt) E.run_params[0].dt = dt;
+        E.run_params[0].use_cool = uc;
+        plan->seed_pending = false;
         SPHX_CHECK(E.upload_run_params(st));
     }
     // cooling is gated by the run's use_cool flag (device memory), so a
     // captured step graph stays valid when the flag changes
-    return (E.use_graph && !E.timing) ? E.step_graphs(st, n_steps) : E.step(st, n_steps);
+    // the Barnes-Hut build sizes its levels on the host, so those steps run eagerly
+    return (E.use_graph && !E.timing && E.bh_theta == 0.0) ? E.step_graphs(st, n_steps)
+                                                            : E.step(st, n_steps);
+}
+
+extern "C" int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl,
+                                         double dt_min, double dt_max) {
+    void *stream = nullptr;
+    PLAN_PROLOGUE(plan);
+    if (on && !(c_cfl > 0.0 && dt_min > 0.0 && dt_max >= dt_min)) {
+        sphx_set_error("adaptive dt needs c_cfl > 0 and 0 < dt_min <= dt_max");
+        return SPHX_ERR_ARG;
+    }
+    plan->adaptive = on ? 1 : 0;
+    if (on) {
+        plan->c_cfl = c_cfl;
+        plan->dt_min = dt_min;
+        plan->dt_max = dt_max;
+    }
+    plan->seed_pending = plan->adaptive != 0;
+    if (plan->inited) {
+        Engine &E = plan->E;
+        SPHX_CHECK(E.pull_run_params(st));
+        RunParams &q = E.run_params[0];
+        q.adaptive = plan->adaptive;
+        q.c_cfl = plan->c_cfl;
+        q.dt_min = plan->dt_min;
+        q.dt_max = plan->dt_max;
+        SPHX_CHECK(E.upload_run_params(st));
+        SPHX_HIP(hipStreamSynchronize(st));
+    }
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_set_gravity_bh(sphx_plan *plan, double theta) {
+    if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
+    if (theta < 0.0 || theta > 2.0) { sphx_set_error("theta must be 0 (direct sum) or in (0, 2]"); return SPHX_ERR_ARG; }
+    if (theta > 0.0 && plan->E.mode != SPHX_MIXED) {
+        sphx_set_error("Barnes-Hut gravity is a mixed-precision option");
+        return SPHX_ERR_ARG;
+    }
+    plan->E.bh_theta = theta;
+    plan->E.drop_graph();
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts,
+                              void *stream) {
+    PLAN_PROLOGUE(plan);
+    if (!plan->inited) { sphx_set_error("call sphx_plan_init first"); return SPHX_ERR_STATE; }
+    Engine &E = plan->E;
+    SPHX_CHECK(E.pull_run_params(st));
+    const RunParams &q = E.run_params[0];
+    if (t) *t = q.t;
+    if (dt) *dt = q.dt;
+    if (dt_parts)
+        for (int k = 0; k < 4; ++k) dt_parts[k] = q.dt_parts[k];
+    return SPHX_OK;
+}
+
+extern "C" int sphx_global_dt(sphx_ctx *ctx, int64_t n, const double *h, const double *c,
+                              const double *acc, const double *acc_g, const double *divV,
+                              double c_cfl, double dt_min, double dt_max, double *out,
+                              void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n <= 0 || !h || !c || !acc || !acc_g || !divV || !out) {
+        sphx_set_error("bad arguments");
+        return SPHX_ERR_ARG;
+    }
+    if (!(c_cfl > 0.0 && dt_min > 0.0 && dt_max >= dt_min)) {
+        sphx_set_error("global dt needs c_cfl > 0 and 0 < dt_min <= dt_max");
+        return SPHX_ERR_ARG;
+    }
+    Engine &E = ctx->E;
+    SPHX_CHECK(E.reserve(std::max<int64_t>(n, 4096)));
+    return sphx_global_dt_launch(h, c, acc, 3, acc_g, 3, divV, n, c_cfl, dt_min, dt_max, E.tmpd,
+                                 E.cap, out, st);
 }
 
 extern "C" int sphx_plan_store(sphx_plan *plan, double *pos, double *vel, double *u, double *h,
@@ -1092,6 +1210,9 @@ extern "C" int sphx_bh_gravity(sphx_ctx *ctx, int64_t n, const double *pos, cons
         stats[2] = bs.interactions;
         stats[3] = bs.overflow;
         stats[4] = bs.max_fill;
+        stats[5] = (int64_t)(bs.build_ms * 1000.0);
+        stats[6] = (int64_t)(bs.walk_ms * 1000.0);
+        stats[7] = bs.groups;
     }
     SPHX_HIP(hipStreamSynchronize(st));
     if (bs.overflow > 0) {

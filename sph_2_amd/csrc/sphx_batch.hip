@@ -217,6 +217,7 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
     kb_iota<<<gb(n), BT, 0, st>>>(E.s.id + p0, n);
     kb_fill_i<<<gb(n), BT, 0, st>>>(E.s.run + p0, r, n);
     SPHX_HIP(hipGetLastError());
+    SPHX_CHECK(E.pull_run_params(st));   // device dt (adaptive runs) and t
     std::vector<int64_t> sizes = E.run_n;
     sizes.push_back(n);
     SPHX_CHECK(E.set_runs(sizes));
@@ -293,6 +294,7 @@ extern "C" int sphx_batch_retire(sphx_batch *b, int64_t run_id, double *pos, dou
         kb_add_i<<<gb(tail), BT, 0, st>>>(E.s.run + p0, -1, tail);
         SPHX_HIP(hipGetLastError());
     }
+    SPHX_CHECK(E.pull_run_params(st));
     std::vector<int64_t> sizes = E.run_n;
     sizes.erase(sizes.begin() + r);
     E.run_params.erase(E.run_params.begin() + r);
@@ -312,6 +314,45 @@ extern "C" int sphx_batch_step(sphx_batch *b, int32_t n_steps, void *stream) {
     if (E.use_graph) SPHX_CHECK(E.step_graphs(st, n_steps));
     else SPHX_CHECK(E.step(st, n_steps));
     for (auto &rr : b->runs) rr.steps += n_steps;
+    return SPHX_OK;
+}
+
+extern "C" int sphx_batch_set_adaptive_dt(sphx_batch *b, int64_t run_id, int32_t on,
+                                          double c_cfl, double dt_min, double dt_max,
+                                          void *stream) {
+    BATCH_PROLOGUE(b);
+    Engine &E = b->E;
+    int r = find_run(b, run_id);
+    if (r < 0) { sphx_set_error("no such run"); return SPHX_ERR_ARG; }
+    if (on && !(c_cfl > 0.0 && dt_min > 0.0 && dt_max >= dt_min)) {
+        sphx_set_error("adaptive dt needs c_cfl > 0 and 0 < dt_min <= dt_max");
+        return SPHX_ERR_ARG;
+    }
+    SPHX_CHECK(E.pull_run_params(st));
+    RunParams &q = E.run_params[r];
+    q.adaptive = on ? 1 : 0;
+    if (on) {
+        q.c_cfl = c_cfl;
+        q.dt_min = dt_min;
+        q.dt_max = dt_max;
+    }
+    SPHX_CHECK(E.upload_run_params(st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    return SPHX_OK;
+}
+
+extern "C" int sphx_batch_time(sphx_batch *b, int64_t run_id, double *t, double *dt,
+                               double *dt_parts, void *stream) {
+    BATCH_PROLOGUE(b);
+    Engine &E = b->E;
+    int r = find_run(b, run_id);
+    if (r < 0) { sphx_set_error("no such run"); return SPHX_ERR_ARG; }
+    SPHX_CHECK(E.pull_run_params(st));
+    const RunParams &q = E.run_params[r];
+    if (t) *t = q.t;
+    if (dt) *dt = q.dt;
+    if (dt_parts)
+        for (int k = 0; k < 4; ++k) dt_parts[k] = q.dt_parts[k];
     return SPHX_OK;
 }
 

@@ -6,8 +6,12 @@
 // follows the sorted 63-bit Morton keys: every octree cell is a contiguous
 // key range, and a cell's eight children are found by binary search on the
 // next three key bits, one level per pass.  Moments (mass, centre of mass,
-// max eps) are summed bottom-up in FP64.  The walk is per 64-particle Morton
-// group (one wavefront): a cell is accepted for the whole group when its
+// max eps) are summed bottom-up in FP64.  The walk is per group (one
+// wavefront): the groups are the largest cells holding at most 64 particles
+// (unsplittable leaves of coincident particles are cut into runs of 64), so a
+// group's bounding box is always inside one cell's cube -- unlike 64
+// consecutive Morton indices, which can straddle a jump of the curve and then
+// open most of the tree.  A cell is accepted for the whole group when its
 // distance from the group's bounding box exceeds s/theta + |com - centre|
 // (cell side s); accepted cells and the particles of opened leaves go to an
 // LDS interaction list that is flushed through the FP32 spline-softened pair
@@ -24,9 +28,22 @@ constexpr int BH_SLACK = 160;     // > 7 x 21 levels of pure depth-first growth
 constexpr int BH_ICAP = 768;      // interaction list capacity per group
 constexpr int BHT = 256;
 
+// What the walk reads of a cell, packed so that one cell costs two 32-byte
+// loads: the monopole (FP64 centre of mass) and the opening/link record.
+struct __align__(32) BhMono {
+    double cx, cy, cz;
+    float mass, emax;
+};
+struct __align__(32) BhLink {
+    float size, delta;
+    int32_t child, nchild, beg, cnt, pad0, pad1;
+};
+
 struct BhTree {
     int32_t *beg, *cnt, *child, *nchild, *level;
     double *mass, *cx, *cy, *cz, *gx, *gy, *gz, *size, *delta, *emax;
+    BhMono *mono;
+    BhLink *link;
 };
 
 __device__ __forceinline__ int digit_at(uint64_t key, int shift) { return (int)((key >> shift) & 7ull); }
@@ -159,8 +176,57 @@ __global__ void k_bh_moments(BhTree T, int32_t l0, int32_t l1, const uint64_t *_
     T.cx[i] = cx; T.cy[i] = cy; T.cz[i] = cz;
     T.gx[i] = gx; T.gy[i] = gy; T.gz[i] = gz;
     T.size[i] = side;
-    T.delta[i] = sqrt((cx - gx) * (cx - gx) + (cy - gy) * (cy - gy) + (cz - gz) * (cz - gz));
+    double dl = sqrt((cx - gx) * (cx - gx) + (cy - gy) * (cy - gy) + (cz - gz) * (cz - gz));
+    T.delta[i] = dl;
     T.emax[i] = em;
+    BhMono mo;
+    mo.cx = cx; mo.cy = cy; mo.cz = cz;
+    mo.mass = (float)M;
+    mo.emax = (float)em;
+    T.mono[i] = mo;
+    BhLink lk;
+    // rounded up so that the FP32 opening test is never looser than the FP64 one
+    lk.size = __double2float_ru(side);
+    lk.delta = __double2float_ru(dl);
+    lk.child = T.child[i];
+    lk.nchild = T.nchild[i];
+    lk.beg = T.beg[i];
+    lk.cnt = T.cnt[i];
+    lk.pad0 = lk.pad1 = 0;
+    T.link[i] = lk;
+}
+
+// Walk groups: every child (cnt <= 64) of a cell with more than 64
+// particles, the root when it has at most 64, and runs of 64 inside an
+// unsplittable leaf with more.  Each particle lands in exactly one group; the
+// list order (atomic append) does not affect any particle's result.
+__global__ void k_bh_groups(BhTree T, int32_t ncell, int32_t *__restrict__ gbeg,
+                            int32_t *__restrict__ gcnt, int32_t *__restrict__ counter) {
+    int i = blockIdx.x * BHT + threadIdx.x;
+    if (i >= ncell) return;
+    int c = T.cnt[i];
+    if (i == 0 && c <= 64) {
+        int g = atomicAdd(counter, 1);
+        gbeg[g] = T.beg[0];
+        gcnt[g] = c;
+    }
+    if (c <= 64) return;
+    if (T.child[i] >= 0) {
+        for (int q = T.child[i]; q < T.child[i] + T.nchild[i]; ++q) {
+            int cq = T.cnt[q];
+            if (cq <= 64) {
+                int g = atomicAdd(counter, 1);
+                gbeg[g] = T.beg[q];
+                gcnt[g] = cq;
+            }
+        }
+    } else {
+        for (int o = 0; o < c; o += 64) {
+            int g = atomicAdd(counter, 1);
+            gbeg[g] = T.beg[i] + o;
+            gcnt[g] = min(64, c - o);
+        }
+    }
 }
 
 __device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total) {
@@ -175,12 +241,15 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total) {
 
 struct BhArgs {
     const double *x, *y, *z, *m, *eps;
+    const int32_t *gbeg, *gcnt;   // walk groups
     int64_t n;
     double G, theta;
     double *ax, *ay, *az, *pot;
     int64_t *stats;  // [0] interactions, [1] frontier overflow, [2] max list fill
+    int64_t *ovf;    // engine counter of overflowing groups (nullable)
 };
 
+template <bool POT>
 __device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, float4 j, float ej,
                                         float &ax, float &ay, float &az, float &ph) {
     float dx = j.x - xi, dy = j.y - yi, dz = j.z - zi;
@@ -198,31 +267,35 @@ __device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, 
         if (q <= 1.0f) {
             float q2 = q * q;
             f = ie * ie * ie * (4.0f / 3.0f - 1.2f * q2 + 0.5f * q2 * q);
-            p = r2 > 0.0f ? ie * (1.4f - 2.0f * q2 * (1.0f / 3.0f - 0.15f * q2 + 0.05f * q2 * q)) : 0.0f;
+            p = (POT && r2 > 0.0f) ? ie * (1.4f - 2.0f * q2 * (1.0f / 3.0f - 0.15f * q2 + 0.05f * q2 * q))
+                                   : 0.0f;
         } else {
             float w = 2.0f - q;
             float w2 = w * w;
             float ri = __builtin_amdgcn_rcpf(r);
             f = ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
             float q2 = q * q;
-            p = -ri * (1.0f / 15.0f) -
-                ie * q2 * (4.0f / 3.0f - q + 0.3f * q2 - q2 * q * (1.0f / 30.0f)) + 1.6f * ie;
+            p = POT ? -ri * (1.0f / 15.0f) -
+                          ie * q2 * (4.0f / 3.0f - q + 0.3f * q2 - q2 * q * (1.0f / 30.0f)) + 1.6f * ie
+                    : 0.0f;
         }
     }
     float s = j.w * f;
     ax = __builtin_fmaf(s, dx, ax);
     ay = __builtin_fmaf(s, dy, ay);
     az = __builtin_fmaf(s, dz, az);
-    ph = __builtin_fmaf(j.w, p, ph);
+    if (POT) ph = __builtin_fmaf(j.w, p, ph);
 }
 
+template <bool POT>
 __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
     __shared__ int32_t stk[BH_SCAP];
     __shared__ float4 il[BH_ICAP];
     __shared__ float ie[BH_ICAP];
     const int lane = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * 64 + lane;
-    const bool ok = i < a.n;
+    const int gn = a.gcnt[blockIdx.x];
+    const int64_t i = (int64_t)a.gbeg[blockIdx.x] + lane;
+    const bool ok = lane < gn;
     double xd = ok ? a.x[i] : 0.0, yd = ok ? a.y[i] : 0.0, zd = ok ? a.z[i] : 0.0;
     double lx = wave_min_d(ok ? xd : INFINITY), hx = wave_max_d(ok ? xd : -INFINITY);
     double ly = wave_min_d(ok ? yd : INFINITY), hy = wave_max_d(ok ? yd : -INFINITY);
@@ -242,7 +315,7 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
 
     auto flush = [&]() {
         __syncthreads();
-        for (int q = 0; q < icnt; ++q) bh_pair(xi, yi, zi, ei, il[q], ie[q], ax, ay, az, ph);
+        for (int q = 0; q < icnt; ++q) bh_pair<POT>(xi, yi, zi, ei, il[q], ie[q], ax, ay, az, ph);
         kx.add(ax); ky.add(ay); kz.add(az); kp.add(ph);
         ax = ay = az = ph = 0.f;
         ninter += icnt;
@@ -264,22 +337,25 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         int node = lane < take ? stk[base + lane] : -1;
         sp = base;
         __syncthreads();
-        int act = 0, nent = 0, nfr = 0;
+        int act = 0, nent = 0, nfr = 0, lnk = 0;
+        BhMono mo{};
         if (node >= 0) {
-            double cxn = T.cx[node], cyn = T.cy[node], czn = T.cz[node];
-            double dx = fmax(0.0, fmax(lx - cxn, cxn - hx));
-            double dy = fmax(0.0, fmax(ly - cyn, cyn - hy));
-            double dz = fmax(0.0, fmax(lz - czn, czn - hz));
+            mo = T.mono[node];
+            const BhLink lk = T.link[node];
+            double dx = fmax(0.0, fmax(lx - mo.cx, mo.cx - hx));
+            double dy = fmax(0.0, fmax(ly - mo.cy, mo.cy - hy));
+            double dz = fmax(0.0, fmax(lz - mo.cz, mo.cz - hz));
             double d = sqrt(dx * dx + dy * dy + dz * dz);
-            if (d * a.theta > T.size[node] + a.theta * T.delta[node]) {
+            if (d * a.theta > (double)lk.size + a.theta * (double)lk.delta) {
                 act = 1;
-                nent = 1;
-            } else if (T.child[node] < 0) {
+            } else if (lk.child < 0) {
                 act = 2;
-                nent = T.cnt[node];
+                nent = lk.cnt;
+                lnk = lk.beg;
             } else {
                 act = 3;
-                nfr = T.nchild[node];
+                nfr = lk.nchild;
+                lnk = lk.child;
             }
         }
         // accepted cells: at most one entry per lane
@@ -287,15 +363,15 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         int pos = wave_excl_scan(act == 1 ? 1 : 0, lane, &tot);
         if (icnt + tot > BH_ICAP) flush();
         if (act == 1) {
-            il[icnt + pos] = make_float4((float)(T.cx[node] - ox), (float)(T.cy[node] - oy),
-                                         (float)(T.cz[node] - oz), (float)T.mass[node]);
-            ie[icnt + pos] = (float)T.emax[node];
+            il[icnt + pos] = make_float4((float)(mo.cx - ox), (float)(mo.cy - oy),
+                                         (float)(mo.cz - oz), mo.mass);
+            ie[icnt + pos] = mo.emax;
         }
         icnt += tot;
         // particles of opened leaves: rounds of at most BH_LEAF per lane, so
         // one round never exceeds 64 * BH_LEAF <= BH_ICAP entries
         int rem = act == 2 ? nent : 0;
-        int pb = act == 2 ? T.beg[node] : 0;
+        int pb = act == 2 ? lnk : 0;
         while (__any(rem > 0)) {
             int tk = rem < BH_LEAF ? rem : BH_LEAF;
             int lpos = wave_excl_scan(tk, lane, &tot);
@@ -315,7 +391,7 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         int ftot;
         int fpos = sp + wave_excl_scan(nfr, lane, &ftot);
         if (act == 3) {
-            int c0 = T.child[node];
+            int c0 = lnk;
             for (int q = 0; q < nfr; ++q)
                 if (fpos + q < BH_SCAP) stk[fpos + q] = c0 + q;
         }
@@ -328,11 +404,14 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         a.ax[i] = a.G * kx.value();
         a.ay[i] = a.G * ky.value();
         a.az[i] = a.G * kz.value();
-        if (a.pot) a.pot[i] = -a.G * kp.value();
+        if (POT) a.pot[i] = -a.G * kp.value();
     }
     if (lane == 0) {
-        atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)(ninter * (a.n - (int64_t)blockIdx.x * 64 < 64 ? a.n - (int64_t)blockIdx.x * 64 : 64)));
-        if (overflow) atomicAdd((unsigned long long *)&a.stats[1], 1ull);
+        atomicAdd((unsigned long long *)&a.stats[0], (unsigned long long)(ninter * gn));
+        if (overflow) {
+            atomicAdd((unsigned long long *)&a.stats[1], 1ull);
+            if (a.ovf) atomicAdd((unsigned long long *)a.ovf, 1ull);
+        }
         atomicMax((unsigned long long *)&a.stats[2], (unsigned long long)maxfill);
     }
 }
@@ -343,6 +422,8 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
 struct BhHost {
     BhTree T{};
     int64_t cap = 0;
+    int32_t *gbeg = nullptr, *gcnt = nullptr;
+    int64_t gcap = 0;
     int32_t *nk = nullptr, *off = nullptr, *bounds = nullptr, *total = nullptr;
     int64_t scratch_cap = 0;
     int64_t *stats = nullptr;
@@ -370,6 +451,8 @@ static int bh_alloc_nodes(BhHost &H, int64_t need) {
     double **dp[] = {&T.mass, &T.cx, &T.cy, &T.cz, &T.gx, &T.gy, &T.gz, &T.size, &T.delta, &T.emax};
     for (int32_t **q : ip) SPHX_CHECK(H.alloc((void **)q, sizeof(int32_t) * cap));
     for (double **q : dp) SPHX_CHECK(H.alloc((void **)q, sizeof(double) * cap));
+    SPHX_CHECK(H.alloc((void **)&T.mono, sizeof(BhMono) * cap));
+    SPHX_CHECK(H.alloc((void **)&T.link, sizeof(BhLink) * cap));
     if (H.cap > 0) {
         int32_t *oi[] = {H.T.beg, H.T.cnt, H.T.child, H.T.nchild, H.T.level};
         int32_t *ni[] = {T.beg, T.cnt, T.child, T.nchild, T.level};
@@ -387,6 +470,20 @@ static int bh_alloc_nodes(BhHost &H, int64_t need) {
 int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t st, BhStats *out) {
     int64_t n = E.n;
     if (n == 0) return SPHX_OK;
+    if (E.R != 1 || E.max_runs != 1 || E.rrun >= 0) {
+        sphx_set_error("Barnes-Hut gravity needs a single-run engine");
+        return SPHX_ERR_STATE;
+    }
+    const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
+    hipEvent_t ev[3];
+    for (auto &e : ev) SPHX_HIP(hipEventCreate(&e));
+    struct EvFree {
+        hipEvent_t *e;
+        ~EvFree() {
+            for (int k = 0; k < 3; ++k) hipEventDestroy(e[k]);
+        }
+    } evfree{ev};
+    SPHX_HIP(hipEventRecord(ev[0], st));
     SPHX_CHECK(bh_alloc_nodes(H, std::max<int64_t>(1024, n / 2)));
     if (!H.total) {
         SPHX_CHECK(H.alloc((void **)&H.total, sizeof(int32_t) * 4));
@@ -444,13 +541,36 @@ int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t s
         int32_t l0 = lev[l], l1 = lev[l + 1];
         if (l1 <= l0) continue;
         k_bh_moments<<<(unsigned)((l1 - l0 + BHT - 1) / BHT), BHT, 0, st>>>(
-            H.T, l0, l1, E.keys, E.s.x, E.s.y, E.s.z, E.s.m, E.s.eps, rbox, cell0);
+            H.T, l0, l1, E.keys, E.s.x, E.s.y, E.s.z, E.s.m, eps, rbox, cell0);
+    }
+    // walk groups
+    int64_t gneed = (int64_t)nodes + n / 64 + 1;
+    if (gneed > H.gcap) {
+        int64_t c = std::max<int64_t>(gneed, 2 * H.gcap);
+        SPHX_CHECK(H.alloc((void **)&H.gbeg, sizeof(int32_t) * c));
+        SPHX_CHECK(H.alloc((void **)&H.gcnt, sizeof(int32_t) * c));
+        H.gcap = c;
+    }
+    SPHX_HIP(hipMemsetAsync(H.total + 1, 0, sizeof(int32_t), st));
+    k_bh_groups<<<(unsigned)((nodes + BHT - 1) / BHT), BHT, 0, st>>>(H.T, nodes, H.gbeg, H.gcnt,
+                                                                      H.total + 1);
+    SPHX_HIP(hipGetLastError());
+    int32_t ngroups = 0;
+    SPHX_HIP(hipMemcpyAsync(&ngroups, H.total + 1, 4, hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    if (ngroups <= 0 || ngroups > gneed) {
+        sphx_set_error("Barnes-Hut group list inconsistent with the tree");
+        return SPHX_ERR_STATE;
     }
     SPHX_HIP(hipMemsetAsync(H.stats, 0, sizeof(int64_t) * 4, st));
-    BhArgs a{E.s.x, E.s.y, E.s.z, E.s.m, E.s.eps, n, E.p.G, theta,
-             E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr, H.stats};
-    k_bh_walk<<<(unsigned)((n + 63) / 64), 64, 0, st>>>(H.T, a);
+    SPHX_HIP(hipEventRecord(ev[1], st));
+    BhArgs a{E.s.x, E.s.y, E.s.z, E.s.m, eps, H.gbeg, H.gcnt, n, E.p.G, theta,
+             E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr, H.stats,
+             E.counters + CNT_BH_OVERFLOW};
+    if (with_pot) k_bh_walk<true><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    else k_bh_walk<false><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
     SPHX_HIP(hipGetLastError());
+    SPHX_HIP(hipEventRecord(ev[2], st));
     if (out) {
         int64_t sv[4] = {0, 0, 0, 0};
         SPHX_HIP(hipMemcpyAsync(sv, H.stats, sizeof(sv), hipMemcpyDeviceToHost, st));
@@ -460,6 +580,12 @@ int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t s
         out->interactions = sv[0];
         out->overflow = sv[1];
         out->max_fill = sv[2];
+        out->groups = ngroups;
+        float b = 0.f, w = 0.f;
+        SPHX_HIP(hipEventElapsedTime(&b, ev[0], ev[1]));
+        SPHX_HIP(hipEventElapsedTime(&w, ev[1], ev[2]));
+        out->build_ms = b;
+        out->walk_ms = w;
     }
     return SPHX_OK;
 }

@@ -302,3 +302,174 @@ int sphx_combine(Engine &E, hipStream_t st) {
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Global time-step criterion (myCppSPHLibs_v2.h:491-522 dt_array and the
+// host reduction of hfv_sph_gpu_v2_evrard.cu:517-561), reduced on the device:
+// per 64-particle bucket the five extrema (max c, min h, max |a_grav|,
+// max |a_total|, min h/|h divV/3|), then per run the four limits
+//   dt_f = sqrt(hmin/max|a_g|), dt_kin = sqrt(hmin/max|a|),
+//   dt_cfl = C_CFL hmin / max c, dt_dens = C_CFL min h/|h divV/3|,
+// dt = 0.25 min(...) clamped to [dt_min, dt_max].  The reference's O(N^2)
+// v_sig loop (max_j max(c_i, c_j)) is the global max of c.  min/max are
+// order-independent, so the result is bit-identical to a serial FP64 pass.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct DtExt {
+    double v[5];  // vmax, hmin, agmax, amax, ddmin
+};
+
+__device__ __forceinline__ void dt_ident(DtExt &e) {
+    e.v[0] = 0.0; e.v[1] = INFINITY; e.v[2] = 0.0; e.v[3] = 0.0; e.v[4] = INFINITY;
+}
+
+__device__ __forceinline__ void dt_merge(DtExt &e, const DtExt &o) {
+    e.v[0] = fmax(e.v[0], o.v[0]);
+    e.v[1] = fmin(e.v[1], o.v[1]);
+    e.v[2] = fmax(e.v[2], o.v[2]);
+    e.v[3] = fmax(e.v[3], o.v[3]);
+    e.v[4] = fmin(e.v[4], o.v[4]);
+}
+
+__device__ __forceinline__ void dt_terms(double h, double c, double ax, double ay, double az,
+                                         double gx, double gy, double gz, double dv, DtExt &e) {
+#pragma clang fp contract(off)
+    e.v[0] = c;
+    e.v[1] = h;
+    e.v[2] = sqrt((gx * gx + gy * gy) + gz * gz);
+    e.v[3] = sqrt((ax * ax + ay * ay) + az * az);
+    e.v[4] = h / fabs((1.0 / 3.0) * h * dv);
+}
+
+__device__ __forceinline__ void dt_wave(DtExt &e) {
+    for (int o = 32; o > 0; o >>= 1) {
+        DtExt t;
+        for (int k = 0; k < 5; ++k) t.v[k] = __shfl_xor(e.v[k], o, 64);
+        dt_merge(e, t);
+    }
+}
+
+// out: dt, dt_f, dt_kin, dt_cfl, dt_dens
+__device__ __forceinline__ void dt_limits(const DtExt &e, double c_cfl, double dt_min,
+                                          double dt_max, double *out) {
+#pragma clang fp contract(off)
+    double dt_f = sqrt(e.v[1] / e.v[2]), dt_kin = sqrt(e.v[1] / e.v[3]);
+    double dt_cfl = c_cfl * e.v[1] / e.v[0], dt_dens = c_cfl * e.v[4];
+    double m = dt_f;
+    m = dt_kin < m ? dt_kin : m;
+    m = dt_cfl < m ? dt_cfl : m;
+    m = dt_dens < m ? dt_dens : m;
+    double dt = 0.25 * m;
+    if (dt > dt_max) dt = dt_max;
+    if (dt < dt_min) dt = dt_min;
+    out[0] = dt; out[1] = dt_f; out[2] = dt_kin; out[3] = dt_cfl; out[4] = dt_dens;
+}
+
+// One wavefront per bucket (b_beg == nullptr: bucket b is [64 b, 64 b + 64)).
+__global__ __launch_bounds__(64) void k_dt_part(const double *__restrict__ h,
+                                                const double *__restrict__ c,
+                                                const double *__restrict__ ax,
+                                                const double *__restrict__ ay,
+                                                const double *__restrict__ az, int64_t as,
+                                                const double *__restrict__ gx,
+                                                const double *__restrict__ gy,
+                                                const double *__restrict__ gz, int64_t gs,
+                                                const double *__restrict__ dv,
+                                                const int32_t *__restrict__ b_beg,
+                                                const int32_t *__restrict__ b_cnt, int64_t n,
+                                                double *__restrict__ part) {
+    const int64_t b = blockIdx.x;
+    int64_t beg = b_beg ? (int64_t)b_beg[b] : 64 * b;
+    int64_t cnt = b_beg ? (int64_t)b_cnt[b] : (n - beg < 64 ? n - beg : 64);
+    DtExt e;
+    dt_ident(e);
+    if ((int64_t)threadIdx.x < cnt) {
+        int64_t i = beg + threadIdx.x;
+        dt_terms(h[i], c[i], ax[as * i], ay[as * i], az[as * i], gx[gs * i], gy[gs * i], gz[gs * i],
+                 dv[i], e);
+    }
+    dt_wave(e);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) part[5 * b + k] = e.v[k];
+}
+
+__device__ void dt_block(const double *__restrict__ part, int64_t b0, int64_t b1, DtExt &e) {
+    __shared__ double red[4][5];
+    dt_ident(e);
+    for (int64_t b = b0 + threadIdx.x; b < b1; b += blockDim.x) {
+        DtExt o;
+        for (int k = 0; k < 5; ++k) o.v[k] = part[5 * b + k];
+        dt_merge(e, o);
+    }
+    dt_wave(e);
+    int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; ++k) red[w][k] = e.v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            DtExt o;
+            for (int k = 0; k < 5; ++k) o.v[k] = red[q][k];
+            dt_merge(e, o);
+        }
+}
+
+// One workgroup per run: t += dt; adaptive runs take the new dt.
+__global__ __launch_bounds__(256) void k_dt_runs(const double *__restrict__ part, Tables t,
+                                                 RunParams *__restrict__ rp) {
+    const int r = blockIdx.x;
+    int64_t b0 = 0, b1 = 0;
+    if (t.r_send[r] > t.r_sbeg[r]) {
+        b0 = t.s_bbeg[t.r_sbeg[r]];
+        b1 = t.s_bend[t.r_send[r] - 1];
+    }
+    DtExt e;
+    dt_block(part, b0, b1, e);
+    if (threadIdx.x == 0) {
+        RunParams q = rp[r];
+        q.t = q.t + q.dt;
+        if (b1 > b0) {
+            double o[5];
+            dt_limits(e, q.c_cfl, q.dt_min, q.dt_max, o);
+            for (int k = 0; k < 4; ++k) q.dt_parts[k] = o[k + 1];
+            if (q.adaptive) q.dt = o[0];
+        }
+        rp[r] = q;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_dt_one(const double *__restrict__ part, int64_t nb,
+                                                double c_cfl, double dt_min, double dt_max,
+                                                double *__restrict__ out) {
+    DtExt e;
+    dt_block(part, 0, nb, e);
+    if (threadIdx.x == 0) dt_limits(e, c_cfl, dt_min, dt_max, out);
+}
+
+}  // namespace
+
+int Engine::global_dt(hipStream_t st) {
+    if (n == 0 || R == 0) return SPHX_OK;
+    k_dt_part<<<(unsigned)t.nb, 64, 0, st>>>(s.h, s.c, s.ax, s.ay, s.az, 1, s.gx, s.gy, s.gz, 1,
+                                             s.divv, t.b_beg, t.b_cnt, n, dtpart);
+    k_dt_runs<<<(unsigned)R, 256, 0, st>>>(dtpart, t, d_run);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int sphx_global_dt_launch(const double *h, const double *c, const double *acc, int64_t as,
+                          const double *acc_g, int64_t gs, const double *divv, int64_t n,
+                          double c_cfl, double dt_min, double dt_max, double *scratch,
+                          int64_t scratch_len, double *out, hipStream_t st) {
+    int64_t nb = (n + 63) / 64;
+    if (n <= 0 || 5 * nb > scratch_len) {
+        sphx_set_error("global dt: bad size or scratch too small");
+        return SPHX_ERR_ARG;
+    }
+    k_dt_part<<<(unsigned)nb, 64, 0, st>>>(h, c, acc, acc + 1, acc + 2, as, acc_g, acc_g + 1,
+                                           acc_g + 2, gs, divv, nullptr, nullptr, n, scratch);
+    k_dt_one<<<1, 256, 0, st>>>(scratch, nb, c_cfl, dt_min, dt_max, out);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}

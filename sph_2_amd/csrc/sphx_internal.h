@@ -158,6 +158,7 @@ enum {
     CNT_H_CAPPED,
     CNT_COOL_CAPPED,
     CNT_CAND,
+    CNT_BH_OVERFLOW,
     CNT_SLOTS = 8
 };
 
@@ -173,11 +174,20 @@ enum {
     PH_COUNT
 };
 
-// Per-run parameters on device.
+// Per-run parameters on device.  dt is the step size of the next step; with
+// `adaptive` set, the end of every step replaces it by the global time-step
+// criterion on that step's forces (dt_parts keeps the four limits).  t is the
+// simulated time, advanced by the dt each step used.  The device copy is the
+// authority for dt and t once steps run (Engine::pull_run_params).
 struct RunParams {
     double dt;
     int32_t use_cool;
     int32_t eps_from_h;
+    int32_t adaptive;
+    int32_t pad;
+    double t;
+    double c_cfl, dt_min, dt_max;
+    double dt_parts[4];  // dt_f, dt_kin, dt_cfl, dt_dens of the last evaluation
 };
 
 // ---------------------------------------------------------------------------
@@ -228,6 +238,8 @@ struct Engine {
     int32_t *h_iters = nullptr;
     int64_t *counters = nullptr;  // CNT_SLOTS
     RunParams *d_run = nullptr;
+    double *dtpart = nullptr;     // 5 extrema per bucket for the time-step criterion
+    int64_t dtpart_cap = 0;
 
     // tables
     std::vector<int32_t> hb_beg, hb_cnt, hb_run, hs_bbeg, hs_bend, hr_sbeg, hr_send, hr_pbeg,
@@ -245,6 +257,9 @@ struct Engine {
 
     GravConfig gcfg;
     GravHost grav;
+    // Barnes-Hut gravity in the step (theta > 0; single-run mixed plans)
+    struct BhHost *bh = nullptr;
+    double bh_theta = 0.0;
 
     // Launch restriction to one run (bootstrap of a newly admitted run):
     // bucket range [rb0, rb1), particle range [rp0, rp1), run index rrun.
@@ -293,6 +308,7 @@ struct Engine {
     int reserve(int64_t capacity);
     int set_runs(const std::vector<int64_t> &sizes);
     int upload_run_params(hipStream_t st);
+    int pull_run_params(hipStream_t st);  // device dt, t, dt_parts -> host copy
 
     // pipeline pieces (mixed mode)
     int sort_and_permute(hipStream_t st, bool gather_state);
@@ -312,6 +328,7 @@ struct Engine {
     int kick_drift(hipStream_t st);
     int finish_step(hipStream_t st);
     int bootstrap_finish(hipStream_t st);
+    int global_dt(hipStream_t st);  // t += dt; adaptive runs: dt <- criterion
     int step(hipStream_t st, int nsteps);
 
     int read_counters(sphx_counters *out);
@@ -321,7 +338,8 @@ struct Engine {
 // Barnes-Hut tree (sphx_bh.hip)
 struct BhHost;
 struct BhStats {
-    int64_t nodes, depth, interactions, overflow, max_fill;
+    int64_t nodes, depth, interactions, overflow, max_fill, groups;
+    double build_ms, walk_ms;  // device time (HIP events) of the build and the walk
 };
 BhHost *sphx_bh_host_new();
 void sphx_bh_host_free(BhHost *h);
@@ -333,6 +351,12 @@ int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st);
 int sphx_launch_morton(Engine &E, hipStream_t st);
 int sphx_radix_sort(uint64_t *keys, int32_t *vals, uint64_t *keys_tmp, int32_t *vals_tmp,
                     uint32_t *hist, int64_t n, int key_bits, hipStream_t st);
+// Time-step criterion over n particles (acc and acc_g with element strides
+// as/gs); out[0..4] = dt, dt_f, dt_kin, dt_cfl, dt_dens.
+int sphx_global_dt_launch(const double *h, const double *c, const double *acc, int64_t as,
+                          const double *acc_g, int64_t gs, const double *divv, int64_t n,
+                          double c_cfl, double dt_min, double dt_max, double *scratch,
+                          int64_t scratch_len, double *out, hipStream_t st);
 int sphx_diag_reduce(const double *x, const double *y, const double *z, const double *vx,
                      const double *vy, const double *vz, const double *m, const double *u,
                      const double *pot, int64_t n, double *scratch, sphx_diag *out,

@@ -18,12 +18,14 @@ inline unsigned grid_rows(int64_t r0, int64_t r1) { return (unsigned)((r1 - r0 +
 __global__ __launch_bounds__(PT) void kp_smoothing(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     const double *__restrict__ h_prev, const int32_t *__restrict__ k_prev,
-    const double *__restrict__ divv, double dt, int32_t nngb, int32_t ndown, int32_t nup,
-    int32_t max_iter, int64_t n, int64_t r0, int64_t r1, double *__restrict__ h_out,
-    int32_t *__restrict__ k_out, int32_t *__restrict__ it_out, int64_t *counters) {
+    const double *__restrict__ divv, const double *__restrict__ dt_p, double dt, int32_t nngb,
+    int32_t ndown, int32_t nup, int32_t max_iter, int64_t n, int64_t r0, int64_t r1,
+    double *__restrict__ h_out, int32_t *__restrict__ k_out, int32_t *__restrict__ it_out,
+    int64_t *counters) {
 #pragma clang fp contract(off)
     int64_t i = r0 + (int64_t)blockIdx.x * PT + threadIdx.x;
     if (i >= r1) return;
+    if (dt_p) dt = *dt_p;   // the step's dt in device memory (adaptive time step)
     double xi = x[i], yi = y[i], zi = z[i];
     double hp = h_prev[i];
     double s = 2.0 * hp;
@@ -318,12 +320,12 @@ __global__ __launch_bounds__(PT) void kp_neighbors(const double *__restrict__ x,
 }  // namespace
 
 int plain_smoothing(const double *x, const double *y, const double *z, const double *h_prev,
-                    const int32_t *k_prev, const double *divv, const double *, double dt,
+                    const int32_t *k_prev, const double *divv, const double *dt_p, double dt,
                     const int32_t *, const sphx_params &p, int64_t n, int64_t r0, int64_t r1,
                     double *h_out, int32_t *k_out, int32_t *it_out, int64_t *counters,
                     hipStream_t st) {
     if (r1 <= r0) return SPHX_OK;
-    kp_smoothing<<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt, p.nngb,
+    kp_smoothing<<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt, p.nngb,
                                                     p.ndown, p.nup, p.max_iter, n, r0, r1, h_out,
                                                     k_out, it_out, counters);
     SPHX_HIP(hipGetLastError());

@@ -256,7 +256,7 @@ def getAcc_g_bh(pos, mass, G, epsilon, theta=0.5, *, device=0, nbeg=None, nend=N
     dp, dm, de = dev(pos), dev(_f64(mass)), dev(eps)
     acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda")
     pot = t.empty(r1 - r0, dtype=t.float64, device="cuda") if return_potential else None
-    st = (C.c_int64 * 5)()
+    st = (C.c_int64 * 8)()
     check(_lib.load().sphx_bh_gravity(_context(device), n, ptr(dp), ptr(dm), ptr(de), float(G),
                                       float(theta), r0, r1, ptr(acc), ptr(pot), st,
                                       _lib.stream_handle()), "bh_gravity")
@@ -264,8 +264,8 @@ def getAcc_g_bh(pos, mass, G, epsilon, theta=0.5, *, device=0, nbeg=None, nend=N
     if return_potential:
         out.append(pot.cpu().numpy())
     if return_stats:
-        out.append(dict(zip(("cells", "depth", "interactions", "overflow", "max_fill"),
-                            [int(v) for v in st])))
+        out.append(dict(zip(("cells", "depth", "interactions", "overflow", "max_fill",
+                             "build_us", "walk_us", "groups"), [int(v) for v in st])))
     return out[0] if len(out) == 1 else tuple(out)
 
 
@@ -291,6 +291,23 @@ def getPE(pos, m, G, epsilon, *, device=0):
     """sum_{i<j} -G m_i m_j phi(r_ij, eps_ij) (LIBSX:593-629)."""
     return _diag(pos, np.zeros_like(np.asarray(pos, np.float64)), m, eps=epsilon, G=G,
                  device=device)["pe"]
+
+
+def global_dt(h, csnd, acc_tot, acc_g, divV, C_CFL=0.25, dt_min=1e-5, dt_max=1e-3, *,
+              device=0):
+    """Adaptive global dt of the GPU driver: dt_array (myCppSPHLibs_v2.h:491-522)
+    and the host reduction after it (hfv_sph_gpu_v2_evrard.cu:517-561), on the
+    device.  Returns (dt, (dt_f, dt_kin, dt_cfl, dt_dens))."""
+    t = _t()
+    dh = dev(_f64(h))
+    n = len(dh)
+    bufs = [dev(_f64(a)) for a in (csnd, acc_tot, acc_g, divV)]
+    out = t.empty(5, dtype=t.float64, device="cuda")
+    check(_lib.load().sphx_global_dt(_context(device), n, ptr(dh), *[ptr(b) for b in bufs],
+                                     float(C_CFL), float(dt_min), float(dt_max), ptr(out),
+                                     _lib.stream_handle()), "global_dt")
+    o = out.cpu().numpy()
+    return float(o[0]), tuple(float(v) for v in o[1:])
 
 
 def diagnostics(pos, v, m, u=None, epsilon=None, G=1.0, *, device=0):

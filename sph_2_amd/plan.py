@@ -45,9 +45,13 @@ class SphPlan:
 
     def set_gravity_config(self, ipl: int, tj: int, js: int = 1):
         check(_lib.load().sphx_plan_set_gravity_config(self._h, ipl, tj, js), "gravity config")
+        self._gcfg_set = True
 
     def load(self, pos, vel, u, m, h, eps=None):
         t = _lib.torch()
+        if self.mode == _lib.MIXED and not getattr(self, "_gcfg_set", False):
+            from .autotune import choose_gravity_config
+            self.set_gravity_config(*choose_gravity_config(len(m)))
         self._bufs = [dev(np.asarray(a, np.float64) if not isinstance(a, t.Tensor) else a,
                           t.float64) for a in (pos, vel, u, m, h)]
         de = dev(np.asarray(eps, np.float64), t.float64) if eps is not None else None
@@ -87,6 +91,26 @@ class SphPlan:
     def set_graph(self, on: bool = True):
         """Replay steps from captured hipGraphs (one per pointer parity)."""
         check(_lib.load().sphx_plan_set_graph(self._h, int(on)), "set_graph")
+
+    def set_gravity_bh(self, theta: float):
+        """Barnes-Hut gravity in the step (theta > 0); 0 restores the direct sum."""
+        check(_lib.load().sphx_plan_set_gravity_bh(self._h, float(theta)), "set_gravity_bh")
+
+    def set_adaptive_dt(self, on: bool = True, c_cfl: float = 0.25, dt_min: float = 1e-5,
+                        dt_max: float = 1e-3):
+        """Global adaptive dt (hfv_sph_gpu_v2_evrard.cu:517-561) evaluated on the
+        device after every step; the next ``step``'s dt seeds the first step."""
+        check(_lib.load().sphx_plan_set_adaptive_dt(self._h, int(on), c_cfl, dt_min, dt_max),
+              "set_adaptive_dt")
+
+    def time(self) -> tuple[float, float, tuple[float, float, float, float]]:
+        """(simulated time, next dt, (dt_f, dt_kin, dt_cfl, dt_dens))."""
+        t = C.c_double()
+        dt = C.c_double()
+        parts = (C.c_double * 4)()
+        check(_lib.load().sphx_plan_time(self._h, C.byref(t), C.byref(dt), parts,
+                                         _lib.stream_handle()), "time")
+        return t.value, dt.value, tuple(parts)
 
     def set_timing(self, on: bool = True):
         check(_lib.load().sphx_plan_set_timing(self._h, int(on)), "set_timing")

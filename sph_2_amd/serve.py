@@ -78,6 +78,20 @@ class Batch:
         self.sizes[rid.value] = ic_.n
         return rid.value
 
+    def set_adaptive_dt(self, run_id: int, on: bool = True, c_cfl: float = 0.25,
+                        dt_min: float = 1e-5, dt_max: float = 1e-3):
+        check(_lib.load().sphx_batch_set_adaptive_dt(self._h, int(run_id), int(on), c_cfl, dt_min,
+                                                     dt_max, _lib.stream_handle()),
+              "set_adaptive_dt")
+
+    def time(self, run_id: int) -> tuple[float, float, tuple[float, float, float, float]]:
+        t = C.c_double()
+        dt = C.c_double()
+        parts = (C.c_double * 4)()
+        check(_lib.load().sphx_batch_time(self._h, int(run_id), C.byref(t), C.byref(dt), parts,
+                                          _lib.stream_handle()), "time")
+        return t.value, dt.value, tuple(parts)
+
     def step(self, n_steps: int = 1):
         check(_lib.load().sphx_batch_step(self._h, int(n_steps), _lib.stream_handle()),
               "sphx_batch_step")

@@ -56,3 +56,44 @@ def test_bh_rows_and_duplicates(gpu):
     ref = oracle.gravity(pos, p.m, p.eps, 1.0, rows=np.arange(50, 300))
     assert acc.shape == (250, 3)
     assert rel_rms(acc, ref) < 5e-3
+
+
+def test_plan_step_with_tree_gravity(gpu):
+    """The step with Barnes-Hut gravity: theta -> 0 reproduces the direct-sum
+    step; theta = 0.5 stays within the tree's force error; the SPH part of
+    the bootstrap is identical (same positions, same neighbour decisions)."""
+    from sph_2_amd.plan import SphPlan
+
+    e = ic.evrard(8192, seed=21)
+    p = oracle.Params()
+    s0 = oracle.bootstrap(oracle.State(e.pos, e.vel, e.u, e.m, e.h), p)
+    out = {}
+    for theta in (0.0, 1e-6, 0.5):
+        pl = SphPlan(8192)
+        pl.set_gravity_bh(theta)
+        pl.load(e.pos, e.vel, e.u, e.m, e.h)
+        pl.init()
+        st0 = pl.state()
+        pl.step(2, 1e-4)
+        out[theta] = (st0, pl.state(), pl.counters())
+        pl.close()
+    d0, d1, _ = out[0.0]
+    for theta in (1e-6, 0.5):
+        b0, b1, cnt = out[theta]
+        assert cnt["bh_overflow"] == 0
+        assert np.array_equal(b0["h"], d0["h"]) and np.array_equal(b0["rho"], d0["rho"])
+        tol = 1e-5 if theta < 1e-3 else 3e-3
+        assert rel_rms(b0["acc_g"], s0.acc_g) < tol
+        assert rel_rms(b1["acc_g"], d1["acc_g"]) < tol
+        assert rel_rms(b1["pos"], d1["pos"]) < 1e-9
+    assert rel_rms(out[1e-6][0]["acc_g"], d0["acc_g"]) < 1e-5
+    assert rel_rms(d0["acc_g"], s0.acc_g) < 1e-5
+
+
+def test_plan_tree_gravity_rejected_in_fp64_mode(gpu):
+    from sph_2_amd.plan import SphPlan
+
+    pl = SphPlan(1024, mode="fp64")
+    with pytest.raises(Exception):
+        pl.set_gravity_bh(0.5)
+    pl.close()

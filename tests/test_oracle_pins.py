@@ -404,3 +404,60 @@ def test_cooling_floor_at_table_minimum():
     assert u[0] == u_floor
     u, _ = oracle.cool(np.array([1e-30]), np.array([-5.0]), 1.0, _cool(0.0))
     assert u[0] == u_floor
+
+
+# ---------------------------------------------------------------------------
+# Global adaptive time step (myCppSPHLibs_v2.h:491-522 + hfv_sph_gpu_v2_evrard.cu:517-561)
+# ---------------------------------------------------------------------------
+def _dt_case():
+    h = np.array([0.1, 0.2, 0.05, 0.3, 0.2])
+    c = np.array([1.0, 2.0, 0.5, 0.1, 0.3])
+    acc = np.zeros((5, 3))
+    acc[2] = [3.0, 4.0, 0.0]           # |a| = 5
+    acc_g = np.zeros((5, 3))
+    acc_g[1] = [0.0, 0.0, 2.0]         # |a_g| = 2
+    divV = np.array([0.0, 1.0, -3.0, 0.5, 0.0])
+    return h, c, acc, acc_g, divV
+
+
+def test_global_dt_hand_values():
+    h, c, acc, acc_g, divV = _dt_case()
+    dt, parts = oracle.global_dt(h, c, acc, acc_g, divV, 0.25, 1e-9, 1.0)
+    # hmin = 0.05; dt_f = sqrt(0.05/2); dt_kin = sqrt(0.05/5) = 0.1;
+    # dt_cfl = 0.25 * 0.05 / 2; h/|h divV/3| = 3/|divV| -> min 1 (divV = -3)
+    assert parts == (math.sqrt(0.025), 0.1, 0.00625, 0.25)
+    assert dt == 0.25 * 0.00625
+    # clamps of the driver
+    assert oracle.global_dt(h, c, acc, acc_g, divV)[0] == 1e-3
+    assert oracle.global_dt(h * 1e-6, c, acc, acc_g, divV)[0] == 1e-5
+
+
+def test_global_dt_each_limit_can_bind():
+    h, c, acc, acc_g, divV = _dt_case()
+    big = 1e9
+    c_slow = c * 1e-6
+    # gravity binds: huge |a_g| (|a_total| kept small)
+    g = acc_g.copy(); g[1] = [0, 0, 4e6]
+    dt, p = oracle.global_dt(h, c_slow, acc, g, divV * 0, 0.25, 1e-30, big)
+    assert dt == 0.25 * math.sqrt(0.05 / 4e6) and p[0] < min(p[1:])
+    # total acceleration binds
+    a = acc.copy(); a[2] = [0, 0, 9e6]
+    dt, p = oracle.global_dt(h, c_slow, a, acc_g, divV * 0, 0.25, 1e-30, big)
+    assert dt == 0.25 * math.sqrt(0.05 / 9e6) and p[1] < min(p[0], p[2], p[3])
+    # density change binds; divV = 0 everywhere gives an infinite dt_dens
+    dt, p = oracle.global_dt(h, c_slow, acc * 0, acc_g * 0, divV * 1e7, 0.25, 1e-30, big)
+    assert p[0] == math.inf and p[1] == math.inf
+    assert dt == pytest.approx(0.25 * 0.25 * (3.0 / 3e7), rel=1e-15)
+    assert oracle.global_dt(h, c, acc, acc_g, divV * 0, 0.25, 1e-30, big)[1][3] == math.inf
+
+
+def test_global_dt_vsig_is_the_reference_double_loop():
+    """dt_array's v_sig[i] = max_j max(c_i, c_j) (myCppSPHLibs_v2.h:506-517),
+    reduced with max_finder: the global maximum of c."""
+    rng = np.random.default_rng(5)
+    c = rng.random(40)
+    vsig = max(max(max(ci, cj) for cj in c) for ci in c)
+    h = rng.random(40) + 0.5
+    acc = np.zeros((40, 3))
+    dt, p = oracle.global_dt(h, c, acc, acc, np.zeros(40), 0.25, 1e-30, 1e30)
+    assert p[2] == 0.25 * h.min() / vsig
