@@ -95,6 +95,14 @@ typedef struct sphx_diag {
     double mass;
 } sphx_diag;
 
+/* The positional params.GPU file of the reference's GPU drivers
+ * (GPU_sph/Important_Test_Simulations/Evrard_collapse/
+ * hfv_sph_gpu_v2_evrard.cu:38-60). */
+typedef struct sphx_params_gpu {
+    int64_t N;
+    double c_0, gamma, Rcld_in_pc, Rcld_in_cm, Mcld_in_g, mu, Mach, grav_const_in_cgs, G;
+} sphx_params_gpu;
+
 typedef struct sphx_ctx sphx_ctx;
 typedef struct sphx_plan sphx_plan;
 typedef struct sphx_batch sphx_batch;
@@ -297,6 +305,58 @@ int sphx_batch_query(sphx_batch *b, sphx_counters *out);
 /* Device bytes held per particle of capacity (the admission budget unit). */
 int64_t sphx_batch_bytes_per_particle(void);
 int sphx_batch_destroy(sphx_batch *b);
+
+/* ---------------- files (HOST arrays; nothing here touches a GPU) ---------- *
+ * The reference's formats, as its drivers and IC scripts read and write them
+ * (CSV values here carry 17 significant digits instead of 6, which the
+ * reference's stof still reads).  pos/vel are (n, 3) float64 host arrays. */
+
+/* params.GPU (hfv_sph_gpu_v2_evrard.cu:38-60; IC/3.py:66-73). */
+int sphx_read_params_gpu(const char *path, sphx_params_gpu *out);
+int sphx_write_params_gpu(const char *path, const sphx_params_gpu *p);
+
+/* IC / restart CSV rows x,y,z,vx,vy,vz,m,h,eps,u[,dudt]
+ * (hfv_sph_gpu_v2_evrard.cu:75-131; Check_Restart_version/hfv_sph_gpu_v5.cu:
+ * 348-396 read, 745-753 written).  dudt nullable (needs 11 columns);
+ * rows_read (nullable) = data rows in the file. */
+int sphx_read_ic_csv(const char *path, int64_t n, double *pos, double *vel, double *m, double *h,
+                     double *eps, double *u, double *dudt, int64_t *rows_read);
+int sphx_write_restart_csv(const char *path, int64_t n, const double *pos, const double *vel,
+                           const double *m, const double *h, const double *eps, const double *u,
+                           const double *dudt);
+/* Snapshot CSV "x,y,z,vx,vy,vz,h,rho,u" (hfv_sph_gpu_v2_evrard.cu:495-514). */
+int sphx_write_snapshot_csv(const char *path, int64_t n, const double *pos, const double *vel,
+                            const double *h, const double *rho, const double *u);
+
+/* Binary IC: int32 Typ[n], float32 x,y,z,vx,vy,vz,m,h,eps,u (IC_evrard_binary.py:
+ * 60-82; read by GPU_sph_Type/Evrard_collapse/hfvCppLibs.h:8-63).  typ nullable
+ * (written as 0). */
+int sphx_write_ic_bin(const char *path, int64_t n, const int32_t *typ, const double *pos,
+                      const double *vel, const double *m, const double *h, const double *eps,
+                      const double *u);
+int sphx_read_ic_bin(const char *path, int64_t n, int32_t *typ, double *pos, double *vel,
+                     double *m, double *h, double *eps, double *u);
+/* Binary snapshot: int32 N, int32 Typ[N], float32 x,y,z,vx,vy,vz,rho,h,u,mass
+ * (hfvCppLibs.h:68-96).  Read with n_cap = 0 first to learn N. */
+int sphx_write_snapshot_bin(const char *path, int64_t n, const int32_t *typ, const double *pos,
+                            const double *vel, const double *rho, const double *h,
+                            const double *u, const double *m);
+int sphx_read_snapshot_bin(const char *path, int64_t n_cap, int64_t *n_out, int32_t *typ,
+                           double *pos, double *vel, double *rho, double *h, double *u,
+                           double *m);
+
+/* Exact checkpoints (no counterpart in the reference, whose restart CSV makes
+ * the driver recompute forces from rounded values): every per-particle array
+ * of a resident run in float64, in its internal Morton order, with its
+ * parameters, step count, simulated time and dt.  A restored run continues
+ * bit-identically to one that was never stopped.  The cooling table is not
+ * stored: restore into a plan/batch created with the same table.  Restoring
+ * into a batch needs the batch's sphx_params to equal the run's; it returns
+ * SPHX_REFUSED when the capacity or run slots are exhausted. */
+int sphx_plan_save(sphx_plan *plan, const char *path, void *stream);
+int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream);
+int sphx_batch_save_run(sphx_batch *b, int64_t run_id, const char *path, void *stream);
+int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *run_id, void *stream);
 
 #ifdef __cplusplus
 }

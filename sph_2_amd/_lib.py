@@ -54,6 +54,17 @@ class Counters(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class ParamsGPU(C.Structure):
+    """The positional params.GPU file of the reference's GPU drivers."""
+    _fields_ = [("N", C.c_int64)] + [(k, C.c_double) for k in (
+        "c_0", "gamma", "Rcld_in_pc", "Rcld_in_cm", "Mcld_in_g", "mu", "Mach",
+        "grav_const_in_cgs", "G")]
+
+    def as_dict(self):
+        return {k: (int(getattr(self, k)) if k == "N" else float(getattr(self, k)))
+                for k, _ in self._fields_}
+
+
 class Diag(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("ke", "ut", "pe", "px", "py", "pz", "lx", "ly", "lz",
                                           "mass")]
@@ -64,6 +75,7 @@ class Diag(C.Structure):
 
 # Every exported entry point with its argument types (pointers are void*).
 _V = C.c_void_p
+_S = C.c_char_p
 _I64 = C.c_int64
 _I32 = C.c_int32
 _D = C.c_double
@@ -112,6 +124,19 @@ SIGNATURES = {
     "sphx_batch_step": (C.c_int, [_V, _I32, _V]),
     "sphx_batch_runs": (C.c_int, [_V, _V, _V, _V, _I32, _V]),
     "sphx_batch_set_adaptive_dt": (C.c_int, [_V, _I64, _I32, _D, _D, _D, _V]),
+    "sphx_read_params_gpu": (C.c_int, [_S, _V]),
+    "sphx_write_params_gpu": (C.c_int, [_S, _V]),
+    "sphx_read_ic_csv": (C.c_int, [_S, _I64, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_write_restart_csv": (C.c_int, [_S, _I64, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_write_snapshot_csv": (C.c_int, [_S, _I64, _V, _V, _V, _V, _V]),
+    "sphx_write_ic_bin": (C.c_int, [_S, _I64, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_read_ic_bin": (C.c_int, [_S, _I64, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_write_snapshot_bin": (C.c_int, [_S, _I64, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_read_snapshot_bin": (C.c_int, [_S, _I64, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_plan_save": (C.c_int, [_V, _S, _V]),
+    "sphx_plan_restore": (C.c_int, [_V, _S, _V]),
+    "sphx_batch_save_run": (C.c_int, [_V, _I64, _S, _V]),
+    "sphx_batch_restore_run": (C.c_int, [_V, _S, _V, _V]),
     "sphx_batch_time": (C.c_int, [_V, _I64, _V, _V, _V, _V]),
     "sphx_batch_set_graph": (C.c_int, [_V, _I32]),
     "sphx_batch_query": (C.c_int, [_V, _V]),

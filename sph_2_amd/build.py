@@ -29,6 +29,7 @@ SOURCES = [
     "sphx_integrate.hip",
     "sphx_batch.hip",
     "sphx_bh.hip",
+    "sphx_io.hip",
 ]
 
 ARCH = os.environ.get("SPHX_OFFLOAD_ARCH", "gfx950")

@@ -1059,6 +1059,63 @@ extern "C" int sphx_plan_set_gravity_bh(sphx_plan *plan, double theta) {
     return SPHX_OK;
 }
 
+static void ckpt_header(CkptHeader &h, const Engine &E) {
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, SPHX_CKPT_MAGIC, 8);
+    h.version = SPHX_CKPT_VERSION;
+    h.mode = E.mode;
+    h.p = E.p;
+    h.g = E.gcfg;
+    h.bh_theta = E.bh_theta;
+    h.morton_bits = E.morton_bits;
+}
+
+extern "C" int sphx_plan_save(sphx_plan *plan, const char *path, void *stream) {
+    PLAN_PROLOGUE(plan);
+    if (!plan->inited) { sphx_set_error("nothing to save: call sphx_plan_init first"); return SPHX_ERR_STATE; }
+    Engine &E = plan->E;
+    SPHX_CHECK(E.pull_run_params(st));
+    CkptHeader h;
+    ckpt_header(h, E);
+    h.n = E.n;
+    h.steps = E.steps;
+    h.rp = E.run_params[0];
+    h.seed_pending = plan->seed_pending ? 1 : 0;
+    return sphx_ckpt_write(path, h, E, 0, st);
+}
+
+extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream) {
+    PLAN_PROLOGUE(plan);
+    Engine &E = plan->E;
+    CkptHeader h;
+    SPHX_CHECK(sphx_ckpt_read_header(path, &h));
+    if (h.mode != E.mode) { sphx_set_error("checkpoint precision mode differs from the plan's"); return SPHX_ERR_ARG; }
+    if (h.n <= 0 || h.n > E.cap) { sphx_set_error("checkpoint particle count exceeds the plan capacity"); return SPHX_ERR_ARG; }
+    if (h.rp.use_cool && !E.has_cool) { sphx_set_error("checkpoint run uses cooling; the plan has no table"); return SPHX_ERR_ARG; }
+    if (h.morton_bits != E.morton_bits) { sphx_set_error("checkpoint from an engine with another key layout"); return SPHX_ERR_ARG; }
+    E.p = h.p;
+    SPHX_CHECK(E.set_runs({h.n}));
+    SPHX_CHECK(sphx_ckpt_upload(path, E, 0, h.n, st));
+    k_fill_i<<<gu(h.n), UT, 0, st>>>(E.s.run, 0, h.n);
+    SPHX_HIP(hipGetLastError());
+    E.run_params.assign(1, h.rp);
+    SPHX_CHECK(E.upload_run_params(st));
+    E.steps = (int)h.steps;
+    E.gcfg = h.g;
+    E.bh_theta = h.bh_theta;
+    plan->adaptive = h.rp.adaptive;
+    if (h.rp.adaptive) {
+        plan->c_cfl = h.rp.c_cfl;
+        plan->dt_min = h.rp.dt_min;
+        plan->dt_max = h.rp.dt_max;
+    }
+    plan->seed_pending = h.seed_pending != 0;
+    plan->loaded = plan->inited = true;
+    E.drop_graph();
+    SPHX_HIP(hipStreamSynchronize(st));
+    return SPHX_OK;
+}
+
 extern "C" int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts,
                               void *stream) {
     PLAN_PROLOGUE(plan);

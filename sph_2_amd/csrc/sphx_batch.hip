@@ -9,6 +9,8 @@
 // retirement copies the run out in its caller order and compacts the arrays.
 #include <algorithm>
 
+#include <cstring>
+
 #include "sphx_internal.h"
 
 int sphx_zero_counters(int64_t *c, hipStream_t st);
@@ -353,6 +355,70 @@ extern "C" int sphx_batch_time(sphx_batch *b, int64_t run_id, double *t, double 
     if (dt) *dt = q.dt;
     if (dt_parts)
         for (int k = 0; k < 4; ++k) dt_parts[k] = q.dt_parts[k];
+    return SPHX_OK;
+}
+
+extern "C" int sphx_batch_save_run(sphx_batch *b, int64_t run_id, const char *path,
+                                   void *stream) {
+    BATCH_PROLOGUE(b);
+    Engine &E = b->E;
+    int r = find_run(b, run_id);
+    if (r < 0) { sphx_set_error("no such run"); return SPHX_ERR_ARG; }
+    SPHX_CHECK(E.pull_run_params(st));
+    CkptHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, SPHX_CKPT_MAGIC, 8);
+    h.version = SPHX_CKPT_VERSION;
+    h.mode = E.mode;
+    h.p = E.p;
+    h.g = E.gcfg;
+    h.morton_bits = E.morton_bits;
+    h.n = E.hr_pend[r] - E.hr_pbeg[r];
+    h.steps = b->runs[r].steps;
+    h.rp = E.run_params[r];
+    return sphx_ckpt_write(path, h, E, E.hr_pbeg[r], st);
+}
+
+// Re-admit a saved run: its particles are appended in their saved internal
+// order with their saved state, so no sort and no bootstrap are needed.
+extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *run_id,
+                                      void *stream) {
+    BATCH_PROLOGUE(b);
+    Engine &E = b->E;
+    if (!run_id) { sphx_set_error("null argument"); return SPHX_ERR_ARG; }
+    CkptHeader h;
+    SPHX_CHECK(sphx_ckpt_read_header(path, &h));
+    if (h.mode != E.mode || std::memcmp(&h.p, &E.p, sizeof(sphx_params)) != 0) {
+        sphx_set_error("checkpoint physics parameters differ from the batch's");
+        return SPHX_ERR_ARG;
+    }
+    if (h.morton_bits != E.morton_bits) {
+        sphx_set_error("checkpoint from an engine with another key layout");
+        return SPHX_ERR_ARG;
+    }
+    if (h.rp.use_cool && !E.has_cool) { sphx_set_error("batch has no cooling table"); return SPHX_ERR_ARG; }
+    if (h.n <= 0) { sphx_set_error("empty checkpoint"); return SPHX_ERR_ARG; }
+    if (E.n + h.n > E.cap || (int)b->runs.size() >= E.max_runs) {
+        sphx_set_error("admission refused: particle capacity or run slots exhausted");
+        return SPHX_REFUSED;
+    }
+    int64_t p0 = E.n, n = h.n;
+    int r = (int)b->runs.size();
+    SPHX_CHECK(sphx_ckpt_upload(path, E, p0, n, st));
+    kb_fill_i<<<gb(n), BT, 0, st>>>(E.s.run + p0, r, n);
+    SPHX_HIP(hipGetLastError());
+    SPHX_CHECK(E.pull_run_params(st));
+    std::vector<int64_t> sizes = E.run_n;
+    sizes.push_back(n);
+    SPHX_CHECK(E.set_runs(sizes));
+    E.run_params.resize(E.R);
+    E.run_params[r] = h.rp;
+    SPHX_CHECK(E.upload_run_params(st));
+    E.drop_graph();
+    BatchRun br{b->next_id++, n, h.steps};
+    b->runs.push_back(br);
+    *run_id = br.id;
+    SPHX_HIP(hipStreamSynchronize(st));
     return SPHX_OK;
 }
 

@@ -335,6 +335,27 @@ struct Engine {
     int diagnostics(hipStream_t st, bool with_pe, sphx_diag *out);
 };
 
+// Exact checkpoint of one resident run (sphx_io.hip)
+#define SPHX_CKPT_MAGIC "SPHXRUN1"
+#define SPHX_CKPT_VERSION 1
+#define SPHX_CKPT_ND 24  // float64 arrays: every double array of Soa
+#define SPHX_CKPT_NI 2   // int32 arrays: k, id
+struct CkptHeader {
+    char magic[8];
+    int32_t version, mode;
+    int64_t n, steps;
+    sphx_params p;
+    RunParams rp;
+    GravConfig g;
+    int32_t seed_pending;  // the next plan step's dt seeds the adaptive step
+    double bh_theta;
+    int32_t morton_bits, reserved[7];
+};
+int sphx_ckpt_write(const char *path, const CkptHeader &hdr, Engine &E, int64_t p0,
+                    hipStream_t st);
+int sphx_ckpt_read_header(const char *path, CkptHeader *hdr);
+int sphx_ckpt_upload(const char *path, Engine &E, int64_t p0, int64_t n, hipStream_t st);
+
 // Barnes-Hut tree (sphx_bh.hip)
 struct BhHost;
 struct BhStats {

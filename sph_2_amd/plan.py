@@ -9,12 +9,26 @@ Evrard_collapse/hfv_sph_gpu_v2_evrard.cu:407-482).
 from __future__ import annotations
 
 import ctypes as C
+import os
+import struct
 
 import numpy as np
 
 from . import _lib
 from ._lib import check, dev, ptr
 from .libsx import CoolingTable
+
+
+def checkpoint_info(path) -> dict:
+    """Header fields of a checkpoint written by SphPlan.save / Batch.save:
+    magic "SPHXRUN1", int32 version, int32 mode, int64 n, int64 steps."""
+    with open(path, "rb") as f:
+        head = f.read(32)
+    if len(head) < 32 or head[:8] != b"SPHXRUN1":
+        raise ValueError(f"{path}: not an sphx checkpoint")
+    version, mode, n, steps = struct.unpack("<iiqq", head[8:32])
+    return {"version": version, "mode": "mixed" if mode == _lib.MIXED else "fp64", "n": n,
+            "steps": steps}
 
 
 class SphPlan:
@@ -62,6 +76,18 @@ class SphPlan:
 
     def init(self):
         check(_lib.load().sphx_plan_init(self._h, _lib.stream_handle()), "sphx_plan_init")
+
+    def save(self, path):
+        """Exact checkpoint of the resident run (restored runs continue bit-identically)."""
+        check(_lib.load().sphx_plan_save(self._h, os.fsencode(path), _lib.stream_handle()),
+              "sphx_plan_save")
+
+    def restore(self, path):
+        """Replace the plan's run by a checkpointed one (same mode, capacity >= N)."""
+        check(_lib.load().sphx_plan_restore(self._h, os.fsencode(path), _lib.stream_handle()),
+              "sphx_plan_restore")
+        self.n = checkpoint_info(path)["n"]
+        self._gcfg_set = True
 
     def step(self, n_steps: int, dt: float, cooling: bool = False) -> int:
         return check(_lib.load().sphx_plan_step(self._h, int(n_steps), float(dt), int(bool(cooling)),

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import collections
 import ctypes as C
+import os
 import time
 from dataclasses import dataclass, field
 
@@ -91,6 +92,24 @@ class Batch:
         check(_lib.load().sphx_batch_time(self._h, int(run_id), C.byref(t), C.byref(dt), parts,
                                           _lib.stream_handle()), "time")
         return t.value, dt.value, tuple(parts)
+
+    def save(self, run_id: int, path):
+        """Exact checkpoint of one live run (it stays live)."""
+        check(_lib.load().sphx_batch_save_run(self._h, int(run_id), os.fsencode(path),
+                                              _lib.stream_handle()), "sphx_batch_save_run")
+
+    def restore(self, path) -> int | None:
+        """Re-admit a checkpointed run; None when the budget or run slots are full."""
+        from .plan import checkpoint_info
+
+        rid = C.c_int64()
+        rc = check(_lib.load().sphx_batch_restore_run(self._h, os.fsencode(path), C.byref(rid),
+                                                      _lib.stream_handle()),
+                   "sphx_batch_restore_run")
+        if rc == _lib.REFUSED:
+            return None
+        self.sizes[rid.value] = checkpoint_info(path)["n"]
+        return rid.value
 
     def step(self, n_steps: int = 1):
         check(_lib.load().sphx_batch_step(self._h, int(n_steps), _lib.stream_handle()),

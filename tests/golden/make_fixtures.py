@@ -18,9 +18,15 @@ Outputs (tests/golden/):
                           N = 1472, m = 1/N, u = 0.05, dt = 1e-3)
   gadget4_profiles.npz    r, rho / P / vr arrays of Evrard_collapse/
                           {rho,P,vr}Gadget.pkl (Gadget-4 run, t ~ 0.8)
+  params_GPU_evrard.txt   the ten numbers of GPU_sph/Important_Test_Simulations/
+                          Evrard_collapse/IC/params.GPU (a data file, copied)
+  params_GPU_restart_cloud.txt
+                          the same for GPU_sph/With_metal_cooling/
+                          Check_Restart_version/params.GPU
 """
 import os
 import pickletools
+import shutil
 import sys
 
 import numpy as np
@@ -65,6 +71,12 @@ def main(ref_root):
     np.savez(os.path.join(here, "gadget4_profiles.npz"), **prof)
     for f, d in (("ic", ic), ("energy", en), ("profiles", prof)):
         print(f, {k: (v.dtype.str, v.shape) for k, v in d.items()})
+    gpu = os.path.join(ref_root, "11_Dec_2022", "GPU_sph")
+    for src, dst in (("Important_Test_Simulations/Evrard_collapse/IC/params.GPU",
+                      "params_GPU_evrard.txt"),
+                     ("With_metal_cooling/Check_Restart_version/params.GPU",
+                      "params_GPU_restart_cloud.txt")):
+        shutil.copyfile(os.path.join(gpu, src), os.path.join(here, dst))
 
 
 if __name__ == "__main__":
