@@ -76,6 +76,19 @@ typedef struct sphx_cooling {
     const double *L;
 } sphx_cooling;
 
+/* 2-D (u, rho) cooling table (the applyCooling method of
+ * GPU_sph/With_cooling/Model_A/myCppSPHLibs_v3.h:794-859): DEVICE arrays of
+ * strictly increasing cgs nodes and the cgs energy lost over dt_ref_s at each
+ * node, du[j * nrho + i] for (u_nodes[j], rho_nodes[i]); code -> cgs factors. */
+typedef struct sphx_cool_grid {
+    int64_t nu, nrho;
+    const double *u_nodes;
+    const double *rho_nodes;
+    double *du;
+    double dt_ref_s;
+    double unit_u_cgs, unit_rho_cgs, unit_time_s;
+} sphx_cool_grid;
+
 typedef struct sphx_counters {
     int64_t n;              /* particles resident */
     int64_t steps;          /* steps taken since load */
@@ -181,6 +194,18 @@ int sphx_neighbors(sphx_ctx *ctx, int64_t n, const double *pos, const double *h,
 int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, double dt,
               const sphx_cooling *cool, double *u_out, int32_t *iters, void *stream);
 
+/* applyCooling (LIBV3 myCppSPHLibs_v3.h:794-859) on device arrays: u, rho to
+ * cgs, clamped inside the table, bin = first j with g[j] <= x <= g[j+1] (the
+ * reference's linear scans, found here by binary search), u_new = u_nodes[j]
+ * - (dt_s/dt_ref_s) du[j, i], floored at u_nodes[0] (the reference has no
+ * floor).  u_out may alias u_in. */
+int sphx_cool_grid_apply(sphx_ctx *ctx, int64_t n, const double *rho, const double *u_in,
+                         double dt, const sphx_cool_grid *grid, double *u_out, void *stream);
+/* Fill grid->du from the implicit solve of sphx_cool over dt_ref_s at every
+ * node (the role of GPU_sph/Cooling_v1/createCoolingGridX2.cu:9-35). */
+int sphx_cool_grid_build(sphx_ctx *ctx, const sphx_cooling *cool, const sphx_cool_grid *grid,
+                         void *stream);
+
 /* getKE (LIBSX:579-588), thermal energy sum m u, momentum and angular
  * momentum; pe only when pos/eps given (getPE, LIBSX:593-629).  Synchronous. */
 int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, const double *v,
@@ -216,8 +241,9 @@ int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, const double *
                    void *stream);
 /* Forces at t = 0 (DRV2:311-392 without its Euler u update). */
 int sphx_plan_init(sphx_plan *plan, void *stream);
-/* n_steps KDK steps at fixed dt (cooling on when the plan has a table and
- * use_cooling != 0). */
+/* n_steps KDK steps at fixed dt.  use_cooling: 0 none, 1 the implicit solve
+ * on the plan's Lambda(T) table, 2 the 2-D (u, rho) table of
+ * sphx_plan_set_cooling_grid. */
 int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32_t use_cooling,
                    void *stream);
 /* Copy state out in the original particle order; any pointer may be NULL.
@@ -261,6 +287,11 @@ int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int32_t nphase
  * with theta = 0.5).  Steps with the tree are not replayed from graphs. */
 int sphx_plan_set_gravity_bh(sphx_plan *plan, double theta);
 
+/* Attach a 2-D cooling table (copied); runs stepped with use_cooling = 2
+ * cool by table lookup instead of the implicit solve (DRV3: hfv_sph_gpu_v3.cu:
+ * 496-508, 600-609 apply it after the u update). */
+int sphx_plan_set_cooling_grid(sphx_plan *plan, const sphx_cool_grid *grid);
+
 int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl, double dt_min,
                               double dt_max);
 int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts, void *stream);
@@ -279,7 +310,8 @@ int sphx_plan_set_graph(sphx_plan *plan, int32_t on);
 int sphx_batch_create(int device, int mode, const sphx_params *p, const sphx_cooling *cool,
                       int64_t capacity, int32_t max_runs, sphx_batch **out);
 /* Returns SPHX_REFUSED (2) without side effects when the particle capacity or
- * the run slots are exhausted.  Bootstraps the new run (forces at t = 0). */
+ * the run slots are exhausted.  Bootstraps the new run (forces at t = 0).
+ * use_cooling as in sphx_plan_step (0 none, 1 solve, 2 2-D table). */
 int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, const double *vel,
                      const double *u, const double *m, const double *h, const double *eps,
                      double dt, int32_t use_cooling, int64_t *run_id, void *stream);
@@ -296,6 +328,8 @@ int sphx_batch_runs(sphx_batch *b, int64_t *ids, int64_t *sizes, int64_t *steps,
 /* Per-run adaptive time step (as sphx_plan_set_adaptive_dt; the admit dt
  * seeds the first step) and the run's simulated time / next dt.  No
  * counterpart in the reference, whose driver runs one simulation. */
+/* As sphx_plan_set_cooling_grid, for runs admitted with use_cooling = 2. */
+int sphx_batch_set_cooling_grid(sphx_batch *b, const sphx_cool_grid *grid);
 int sphx_batch_set_adaptive_dt(sphx_batch *b, int64_t run_id, int32_t on, double c_cfl,
                                double dt_min, double dt_max, void *stream);
 int sphx_batch_time(sphx_batch *b, int64_t run_id, double *t, double *dt, double *dt_parts,

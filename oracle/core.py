@@ -58,6 +58,9 @@ def lib():
         L.orc_ke.argtypes = [i64, dp, dp]
         L.orc_pe.restype = C.c_double
         L.orc_pe.argtypes = [i64, dp, dp, dp, C.c_double]
+        L.orc_cool_grid_apply.argtypes = [i64, dp, dp, C.c_double, i64, dp, i64, dp, dp,
+                                          C.c_double, C.c_double, C.c_double, C.c_double, dp]
+        L.orc_cool_grid_build.argtypes = [i64, dp, i64, dp, C.c_double, C.c_void_p, dp, dp, dp]
         L.orc_global_dt.restype = C.c_double
         L.orc_global_dt.argtypes = [i64, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         _LIB = L
@@ -302,6 +305,36 @@ def pe(pos, m, eps, G=1.0):
     return lib().orc_pe(n, pp, mp, ep, G)
 
 
+def cool_grid_apply(rho, u, dt, u_nodes, rho_nodes, du, dt_ref_s, unit_u_cgs, unit_rho_cgs,
+                    unit_time_s):
+    """applyCooling of the 2-D (u, rho) table (myCppSPHLibs_v3.h:794-859)."""
+    n = len(u)
+    rho, rp = _d(rho)
+    u, up = _d(u)
+    un, unp = _d(u_nodes)
+    rn, rnp = _d(rho_nodes)
+    du, dup = _d(du)
+    out = np.zeros(n)
+    out, op = _d(out)
+    lib().orc_cool_grid_apply(n, rp, up, dt, len(un), unp, len(rn), rnp, dup, dt_ref_s,
+                              unit_u_cgs, unit_rho_cgs, unit_time_s, op)
+    return out
+
+
+def cool_grid_build(u_nodes, rho_nodes, dt_ref_s, cp: CoolParams):
+    """The table over the grid nodes from the oracle's implicit solve
+    (createCoolingGridX2.cu:9-35): du[j, i] = u_j - u_new(u_j, rho_i), cgs."""
+    un, unp = _d(u_nodes)
+    rn, rnp = _d(rho_nodes)
+    du = np.zeros((len(un), len(rn)))
+    du, dup = _d(du)
+    c = cp.c()
+    Tt, Tp = _d(cp.T)
+    Lt, Lp = _d(cp.L)
+    lib().orc_cool_grid_build(len(un), unp, len(rn), rnp, dt_ref_s, C.byref(c), Tp, Lp, dup)
+    return du
+
+
 def global_dt(h, c, acc, acc_g, divV, c_cfl=0.25, dt_min=1e-5, dt_max=1e-3):
     """Adaptive global dt of the GPU driver (myCppSPHLibs_v2.h:491-522 dt_array +
     hfv_sph_gpu_v2_evrard.cu:517-561).  Returns (dt, (dt_f, dt_kin, dt_cfl, dt_dens))."""
@@ -373,8 +406,11 @@ def bootstrap(s: State, p: Params) -> State:
     return s
 
 
-def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None) -> State:
-    """One KDK step (hfv_sph_gpu_v2_evrard.cu:410-482)."""
+def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None,
+         grid: dict | None = None) -> State:
+    """One KDK step (hfv_sph_gpu_v2_evrard.cu:410-482).  grid (dict of
+    cool_grid_apply's table arguments) cools by the 2-D table after the u
+    update, as hfv_sph_gpu_v3.cu:600-609 does."""
     s = s.copy()
     s.vel = s.vel + s.acc * (0.5 * dt)
     s.pos = s.pos + s.vel * dt
@@ -384,6 +420,8 @@ def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None) -> St
     s.ut_prev = s.dudt.copy()
     if cool_p is not None:
         s.u, _ = cool(s.rho, s.u, dt, cool_p)
+    elif grid is not None:
+        s.u = cool_grid_apply(s.rho, s.u, dt, **grid)
     s.t = s.t + dt
     return s
 

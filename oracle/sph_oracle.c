@@ -567,3 +567,58 @@ double orc_global_dt(int64_t n, const double *h, const double *c, const double *
     }
     return dt;
 }
+
+/* 2-D (u, rho) cooling table of the reference's v3 driver: applyCooling
+ * (GPU_sph/With_cooling/Model_A/myCppSPHLibs_v3.h:794-859; driver
+ * hfv_sph_gpu_v3.cu:115-204, 496-508).  u_nodes[nu], rho_nodes[nrho] are the
+ * increasing cgs grid nodes, du[j * nrho + i] the cgs energy change of a gas
+ * at (u_nodes[j], rho_nodes[i]) over dt_ref_s.  Per particle:
+ *   u, rho -> cgs; clamp below the first node to node + 0.001 node and above
+ *   the last to node - 0.001 node; j = first bin with g[j] <= x <= g[j+1]
+ *   (the reference's linear scans); u_new = u_nodes[j] - (dt_s / dt_ref_s) du
+ *   back to code units.  The reference sets no floor; here u_new is floored at
+ *   the first u node, so the gas never leaves the table. */
+static int64_t orc_grid_bin(const double *g, int64_t ng, double x) {
+    for (int64_t j = 0; j < ng - 1; ++j)
+        if (x >= g[j] && x <= g[j + 1]) return j;
+    return ng - 2;
+}
+
+void orc_cool_grid_apply(int64_t n, const double *rho, const double *u, double dt, int64_t nu,
+                         const double *u_nodes, int64_t nrho, const double *rho_nodes,
+                         const double *du, double dt_ref_s, double unit_u_cgs,
+                         double unit_rho_cgs, double unit_time_s, double *u_out) {
+    double coeff = (dt * unit_time_s) / dt_ref_s;
+    double umin = u_nodes[0], umax = u_nodes[nu - 1];
+    double rmin = rho_nodes[0], rmax = rho_nodes[nrho - 1];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double uc = u[i] * unit_u_cgs, rc = rho[i] * unit_rho_cgs;
+        if (uc < umin) uc = umin + 0.001 * umin;
+        if (uc > umax) uc = umax - 0.001 * umax;
+        if (rc < rmin) rc = rmin + 0.001 * rmin;
+        if (rc > rmax) rc = rmax - 0.001 * rmax;
+        int64_t ju = orc_grid_bin(u_nodes, nu, uc), jr = orc_grid_bin(rho_nodes, nrho, rc);
+        double un = u_nodes[ju] - coeff * du[ju * nrho + jr];
+        if (un < umin) un = umin;
+        u_out[i] = un / unit_u_cgs;
+    }
+}
+
+/* The table itself (GPU_sph/Cooling_v1/createCoolingGridX2.cu:9-35): the
+ * energy change of the implicit solve over dt_ref_s at every node, from this
+ * oracle's own solve (orc_cool_one): du = u - u_new, in cgs. */
+void orc_cool_grid_build(int64_t nu, const double *u_nodes, int64_t nrho, const double *rho_nodes,
+                         double dt_ref_s, const orc_cool_params *cp, const double *Tt,
+                         const double *Lt, double *du) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t j = 0; j < nu; ++j)
+        for (int64_t i = 0; i < nrho; ++i) {
+            double u_code = u_nodes[j] / cp->unit_u_cgs;
+            double rho_code = rho_nodes[i] / cp->unit_rho_cgs;
+            double dt_code = dt_ref_s / cp->unit_time_s;
+            int32_t it = 0;
+            double un = orc_cool_one(rho_code, u_code, dt_code, cp, Tt, Lt, &it);
+            du[j * nrho + i] = (u_code - un) * cp->unit_u_cgs;
+        }
+}

@@ -65,6 +65,13 @@ class ParamsGPU(C.Structure):
                 for k, _ in self._fields_}
 
 
+class CoolGrid(C.Structure):
+    _fields_ = [("nu", C.c_int64), ("nrho", C.c_int64), ("u_nodes", C.c_void_p),
+                ("rho_nodes", C.c_void_p), ("du", C.c_void_p), ("dt_ref_s", C.c_double),
+                ("unit_u_cgs", C.c_double), ("unit_rho_cgs", C.c_double),
+                ("unit_time_s", C.c_double)]
+
+
 class Diag(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("ke", "ut", "pe", "px", "py", "pz", "lx", "ly", "lz",
                                           "mass")]
@@ -114,6 +121,10 @@ SIGNATURES = {
     "sphx_plan_set_timing": (C.c_int, [_V, _I32]),
     "sphx_plan_set_adaptive_dt": (C.c_int, [_V, _I32, _D, _D, _D]),
     "sphx_plan_set_gravity_bh": (C.c_int, [_V, _D]),
+    "sphx_plan_set_cooling_grid": (C.c_int, [_V, _V]),
+    "sphx_batch_set_cooling_grid": (C.c_int, [_V, _V]),
+    "sphx_cool_grid_apply": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V]),
+    "sphx_cool_grid_build": (C.c_int, [_V, _V, _V, _V]),
     "sphx_plan_time": (C.c_int, [_V, _V, _V, _V, _V]),
     "sphx_plan_timing": (C.c_int, [_V, _V, _V, _I32]),
     "sphx_plan_set_graph": (C.c_int, [_V, _I32]),
@@ -169,6 +180,17 @@ def check(rc: int, what: str = "") -> int:
         msg = load().sphx_last_error().decode()
         raise SphxError(f"{what} failed ({rc}): {msg}")
     return rc
+
+
+def cooling_mode(cooling) -> int:
+    """False/0: none; True/1/"solve": implicit solve; 2/"table": 2-D table."""
+    if cooling in ("table", 2):
+        return 2
+    if cooling in ("solve", True, 1):
+        return 1
+    if cooling in (False, None, 0):
+        return 0
+    raise ValueError(f"unknown cooling mode {cooling!r}")
 
 
 def default_params(**kw) -> Params:

@@ -62,7 +62,10 @@ def _compile(src: str) -> str:
     cmd = [hipcc(), *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed on {src}:\n{r.stderr[-6000:]}")
+        lines = r.stderr.splitlines()
+        errs = [k for k, ln in enumerate(lines) if "error:" in ln]
+        shown = "\n".join("\n".join(lines[k:k + 4]) for k in errs[:8]) or r.stderr[-6000:]
+        raise RuntimeError(f"hipcc failed on {src}:\n{shown}")
     return obj
 
 

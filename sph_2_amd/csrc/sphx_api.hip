@@ -119,6 +119,7 @@ Engine::~Engine() {
     if (gstream) hipStreamDestroy(gstream);
     if (cool_T) hipFree(cool_T);
     if (cool_L) hipFree(cool_L);
+    if (grid_buf) hipFree(grid_buf);
     hipSetDevice(cur);
 }
 
@@ -813,6 +814,74 @@ extern "C" int sphx_neighbors(sphx_ctx *ctx, int64_t n, const double *pos, const
     return SPHX_OK;
 }
 
+// Validate a cooling table (device arrays) on the host: sizes, units and
+// strictly increasing positive nodes.
+static int check_grid(const sphx_cool_grid *g, std::vector<double> *un, std::vector<double> *rn,
+                      hipStream_t st) {
+    if (!g || g->nu < 2 || g->nrho < 2 || !g->u_nodes || !g->rho_nodes || !g->du) {
+        sphx_set_error("cooling table needs >= 2 nodes per axis and device arrays");
+        return SPHX_ERR_ARG;
+    }
+    if (!(g->dt_ref_s > 0.0 && g->unit_u_cgs > 0.0 && g->unit_rho_cgs > 0.0 && g->unit_time_s > 0.0)) {
+        sphx_set_error("cooling table needs positive dt_ref_s and unit factors");
+        return SPHX_ERR_ARG;
+    }
+    un->resize(g->nu);
+    rn->resize(g->nrho);
+    SPHX_HIP(hipMemcpyAsync(un->data(), g->u_nodes, sizeof(double) * g->nu, hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipMemcpyAsync(rn->data(), g->rho_nodes, sizeof(double) * g->nrho, hipMemcpyDeviceToHost, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    for (const std::vector<double> *v : {un, rn}) {
+        if (!((*v)[0] > 0.0)) { sphx_set_error("cooling table nodes must be positive"); return SPHX_ERR_ARG; }
+        for (size_t k = 1; k < v->size(); ++k)
+            if (!((*v)[k] > (*v)[k - 1])) {
+                sphx_set_error("cooling table nodes must increase strictly");
+                return SPHX_ERR_ARG;
+            }
+    }
+    return SPHX_OK;
+}
+
+static GridDev grid_view(const sphx_cool_grid *g) {
+    GridDev d;
+    d.nu = g->nu;
+    d.nrho = g->nrho;
+    d.u_nodes = g->u_nodes;
+    d.rho_nodes = g->rho_nodes;
+    d.du = g->du;
+    d.dt_ref_s = g->dt_ref_s;
+    d.unit_u_cgs = g->unit_u_cgs;
+    d.unit_rho_cgs = g->unit_rho_cgs;
+    d.unit_time_s = g->unit_time_s;
+    return d;
+}
+
+// Copy a cooling table into engine-owned memory (kept until the engine dies).
+int Engine::set_cool_grid(const sphx_cool_grid *g, hipStream_t st) {
+    std::vector<double> un, rn;
+    SPHX_CHECK(check_grid(g, &un, &rn, st));
+    int64_t need = g->nu + g->nrho + g->nu * g->nrho;
+    if (need > grid_cap) {
+        if (grid_buf) hipFree(grid_buf);
+        grid_buf = nullptr;
+        grid_cap = 0;
+        SPHX_HIP(hipMalloc(&grid_buf, sizeof(double) * need));
+        grid_cap = need;
+    }
+    SPHX_HIP(hipMemcpyAsync(grid_buf, g->u_nodes, sizeof(double) * g->nu, hipMemcpyDeviceToDevice, st));
+    SPHX_HIP(hipMemcpyAsync(grid_buf + g->nu, g->rho_nodes, sizeof(double) * g->nrho,
+                            hipMemcpyDeviceToDevice, st));
+    SPHX_HIP(hipMemcpyAsync(grid_buf + g->nu + g->nrho, g->du, sizeof(double) * g->nu * g->nrho,
+                            hipMemcpyDeviceToDevice, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    grid = grid_view(g);
+    grid.u_nodes = grid_buf;
+    grid.rho_nodes = grid_buf + g->nu;
+    grid.du = grid_buf + g->nu + g->nrho;
+    drop_graph();
+    return SPHX_OK;
+}
+
 int sphx_copy_cool(Engine &E, const sphx_cooling *cool) {
     if (!cool) {
         E.has_cool = false;
@@ -859,6 +928,30 @@ extern "C" int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const doub
     sphx_counters c{};
     SPHX_CHECK(E.read_counters(&c));
     return c.cool_capped > 0 ? SPHX_CAPPED : SPHX_OK;
+}
+
+extern "C" int sphx_cool_grid_apply(sphx_ctx *ctx, int64_t n, const double *rho,
+                                    const double *u_in, double dt, const sphx_cool_grid *grid,
+                                    double *u_out, void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n < 0 || (n > 0 && (!rho || !u_in || !u_out))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    std::vector<double> un, rn;
+    SPHX_CHECK(check_grid(grid, &un, &rn, st));
+    return sphx_grid_apply_launch(grid_view(grid), rho, u_in, dt, n, u_out, st);
+}
+
+extern "C" int sphx_cool_grid_build(sphx_ctx *ctx, const sphx_cooling *cool,
+                                    const sphx_cool_grid *grid, void *stream) {
+    CTX_PROLOGUE(ctx);
+    Engine &E = ctx->E;
+    std::vector<double> un, rn;
+    SPHX_CHECK(check_grid(grid, &un, &rn, st));
+    SPHX_CHECK(E.reserve(64));
+    SPHX_CHECK(sphx_copy_cool(E, cool));
+    if (!E.has_cool) { sphx_set_error("cooling table required"); return SPHX_ERR_ARG; }
+    SPHX_CHECK(sphx_grid_build_launch(grid_view(grid), sphx_engine_cool(E), grid->du, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    return SPHX_OK;
 }
 
 extern "C" int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, const double *v,
@@ -998,11 +1091,13 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
     if (!plan->inited) { sphx_set_error("call sphx_plan_init first"); return SPHX_ERR_STATE; }
     if (n_steps < 0 || !(dt > 0.0)) { sphx_set_error("bad step count or dt"); return SPHX_ERR_ARG; }
     Engine &E = plan->E;
-    if (use_cooling && !E.has_cool) { sphx_set_error("plan has no cooling table"); return SPHX_ERR_ARG; }
+    if (use_cooling < 0 || use_cooling > 2) { sphx_set_error("use_cooling is 0, 1 (solve) or 2 (table)"); return SPHX_ERR_ARG; }
+    if (use_cooling == 1 && !E.has_cool) { sphx_set_error("plan has no cooling table"); return SPHX_ERR_ARG; }
+    if (use_cooling == 2 && !E.grid.du) { sphx_set_error("plan has no 2-D cooling table"); return SPHX_ERR_ARG; }
     // The host copy of the run parameters is current except for t and, with
     // the adaptive step, dt: pull the device copy before changing anything.
     RunParams &hp = E.run_params[0];
-    int32_t uc = use_cooling ? 1 : 0;
+    int32_t uc = use_cooling;
     bool set_dt = plan->adaptive ? plan->seed_pending : hp.dt != dt;
     if (set_dt || hp.use_cool != uc) {
         SPHX_CHECK(E.pull_run_params(st));
@@ -1091,7 +1186,10 @@ extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream
     SPHX_CHECK(sphx_ckpt_read_header(path, &h));
     if (h.mode != E.mode) { sphx_set_error("checkpoint precision mode differs from the plan's"); return SPHX_ERR_ARG; }
     if (h.n <= 0 || h.n > E.cap) { sphx_set_error("checkpoint particle count exceeds the plan capacity"); return SPHX_ERR_ARG; }
-    if (h.rp.use_cool && !E.has_cool) { sphx_set_error("checkpoint run uses cooling; the plan has no table"); return SPHX_ERR_ARG; }
+    if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du)) {
+        sphx_set_error("checkpoint run cools with a table the plan does not have");
+        return SPHX_ERR_ARG;
+    }
     if (h.morton_bits != E.morton_bits) { sphx_set_error("checkpoint from an engine with another key layout"); return SPHX_ERR_ARG; }
     E.p = h.p;
     SPHX_CHECK(E.set_runs({h.n}));
@@ -1114,6 +1212,12 @@ extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream
     E.drop_graph();
     SPHX_HIP(hipStreamSynchronize(st));
     return SPHX_OK;
+}
+
+extern "C" int sphx_plan_set_cooling_grid(sphx_plan *plan, const sphx_cool_grid *grid) {
+    void *stream = nullptr;
+    PLAN_PROLOGUE(plan);
+    return plan->E.set_cool_grid(grid, st);
 }
 
 extern "C" int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts,

@@ -200,7 +200,9 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
         return SPHX_ERR_ARG;
     }
     if (!E.p.eps_from_h && !eps) { sphx_set_error("eps required when eps_from_h == 0"); return SPHX_ERR_ARG; }
-    if (use_cooling && !E.has_cool) { sphx_set_error("batch has no cooling table"); return SPHX_ERR_ARG; }
+    if (use_cooling < 0 || use_cooling > 2) { sphx_set_error("use_cooling is 0, 1 (solve) or 2 (table)"); return SPHX_ERR_ARG; }
+    if (use_cooling == 1 && !E.has_cool) { sphx_set_error("batch has no cooling table"); return SPHX_ERR_ARG; }
+    if (use_cooling == 2 && !E.grid.du) { sphx_set_error("batch has no 2-D cooling table"); return SPHX_ERR_ARG; }
     if (E.n + n > E.cap || (int)b->runs.size() >= E.max_runs) {
         sphx_set_error("admission refused: particle capacity or run slots exhausted");
         return SPHX_REFUSED;
@@ -224,7 +226,7 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
     sizes.push_back(n);
     SPHX_CHECK(E.set_runs(sizes));
     E.run_params.resize(E.R);
-    E.run_params[r] = RunParams{dt, use_cooling ? 1 : 0, E.p.eps_from_h};
+    E.run_params[r] = RunParams{dt, use_cooling, E.p.eps_from_h};
     SPHX_CHECK(E.upload_run_params(st));
     E.drop_graph();
     // bootstrap only the new run: sort its segment, forces at t = 0, du_prev
@@ -319,6 +321,12 @@ extern "C" int sphx_batch_step(sphx_batch *b, int32_t n_steps, void *stream) {
     return SPHX_OK;
 }
 
+extern "C" int sphx_batch_set_cooling_grid(sphx_batch *b, const sphx_cool_grid *grid) {
+    void *stream = nullptr;
+    BATCH_PROLOGUE(b);
+    return b->E.set_cool_grid(grid, st);
+}
+
 extern "C" int sphx_batch_set_adaptive_dt(sphx_batch *b, int64_t run_id, int32_t on,
                                           double c_cfl, double dt_min, double dt_max,
                                           void *stream) {
@@ -396,7 +404,10 @@ extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *
         sphx_set_error("checkpoint from an engine with another key layout");
         return SPHX_ERR_ARG;
     }
-    if (h.rp.use_cool && !E.has_cool) { sphx_set_error("batch has no cooling table"); return SPHX_ERR_ARG; }
+    if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du)) {
+        sphx_set_error("checkpoint run cools with a table the batch does not have");
+        return SPHX_ERR_ARG;
+    }
     if (h.n <= 0) { sphx_set_error("empty checkpoint"); return SPHX_ERR_ARG; }
     if (E.n + h.n > E.cap || (int)b->runs.size() >= E.max_runs) {
         sphx_set_error("admission refused: particle capacity or run slots exhausted");

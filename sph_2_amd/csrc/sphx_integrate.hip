@@ -147,8 +147,60 @@ __global__ void k_cool(CoolDev cd, const double *__restrict__ rho, const double 
 
 // Second half kick, u trapezoid (myCppSPHLibs_v2.h:740-747), ut_prev update,
 // then the implicit cooling of the adiabatic u (operator split).
+// First bin j of an increasing grid with g[j] <= x <= g[j+1] for g[0] <= x:
+// the reference's linear scan (myCppSPHLibs_v3.h:831-849) as a binary search
+// for the first k >= 1 with g[k] >= x; past the last node -> ng - 2.
+__device__ __forceinline__ int64_t grid_bin(const double *__restrict__ g, int64_t ng, double x) {
+    int64_t lo = 1, hi = ng;  // first k in [1, ng) with g[k] >= x, else ng
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (g[mid] >= x) hi = mid;
+        else lo = mid + 1;
+    }
+    int64_t j = lo - 1;
+    return j > ng - 2 ? ng - 2 : j;
+}
+
+// applyCooling of the 2-D table (myCppSPHLibs_v3.h:794-859), FP64, floored at
+// the first u node.
+__device__ double grid_apply(double rho, double u_code, double dt, const GridDev &g) {
+#pragma clang fp contract(off)
+    double coeff = (dt * g.unit_time_s) / g.dt_ref_s;
+    double umin = g.u_nodes[0], umax = g.u_nodes[g.nu - 1];
+    double rmin = g.rho_nodes[0], rmax = g.rho_nodes[g.nrho - 1];
+    double uc = u_code * g.unit_u_cgs, rc = rho * g.unit_rho_cgs;
+    if (uc < umin) uc = umin + 0.001 * umin;
+    if (uc > umax) uc = umax - 0.001 * umax;
+    if (rc < rmin) rc = rmin + 0.001 * rmin;
+    if (rc > rmax) rc = rmax - 0.001 * rmax;
+    int64_t ju = grid_bin(g.u_nodes, g.nu, uc), jr = grid_bin(g.rho_nodes, g.nrho, rc);
+    double un = g.u_nodes[ju] - coeff * g.du[ju * g.nrho + jr];
+    if (un < umin) un = umin;
+    return un / g.unit_u_cgs;
+}
+
+__global__ void k_grid_apply(GridDev g, const double *__restrict__ rho,
+                             const double *u_in, double dt, int64_t n, double *u_out) {
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i < n) u_out[i] = grid_apply(rho[i], u_in[i], dt, g);
+}
+
+// du at every node from the implicit solve over dt_ref_s (cgs in and out).
+__global__ void k_grid_build(GridDev g, CoolDev cd, double *__restrict__ du) {
+#pragma clang fp contract(off)
+    int64_t t = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (t >= g.nu * g.nrho) return;
+    int64_t j = t / g.nrho, i = t - j * g.nrho;
+    double u_code = g.u_nodes[j] / cd.unit_u_cgs;
+    double rho_code = g.rho_nodes[i] / cd.unit_rho_cgs;
+    double dt_code = g.dt_ref_s / cd.unit_time_s;
+    int it = 0;
+    double un = cool_one(rho_code, u_code, dt_code, cd, &it);
+    du[t] = (u_code - un) * cd.unit_u_cgs;
+}
+
 __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, CoolDev cd,
-                         int cool_on, int64_t *counters) {
+                         int cool_on, GridDev gd, int64_t *counters) {
 #pragma clang fp contract(off)
     int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
     if (i >= n) return;
@@ -161,10 +213,12 @@ __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, Coo
     double du = s.du[i];
     double u = s.u[i] + half * (du + s.ut_prev[i]);
     s.ut_prev[i] = du;
-    if (cool_on && r.use_cool) {
+    if (cool_on && r.use_cool == 1) {
         int it = 0;
         u = cool_one(s.rho[i], u, dt, cd, &it);
         if (it >= cd.max_iter) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
+    } else if (r.use_cool == 2 && gd.du) {
+        u = grid_apply(s.rho[i], u, dt, gd);
     }
     s.u[i] = u;
 }
@@ -225,6 +279,22 @@ int sphx_cool_launch(const CoolDev &cd, const double *rho, const double *u_in, d
     return SPHX_OK;
 }
 
+int sphx_grid_apply_launch(const GridDev &g, const double *rho, const double *u_in, double dt,
+                           int64_t n, double *u_out, hipStream_t st) {
+    if (n <= 0) return SPHX_OK;
+    k_grid_apply<<<g1(n), IT, 0, st>>>(g, rho, u_in, dt, n, u_out);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int sphx_grid_build_launch(const GridDev &g, const CoolDev &cd, double *du, hipStream_t st) {
+    int64_t nn = g.nu * g.nrho;
+    if (nn <= 0) return SPHX_OK;
+    k_grid_build<<<g1(nn), IT, 0, st>>>(g, cd, du);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
 int sphx_diag_reduce(const double *x, const double *y, const double *z, const double *vx,
                      const double *vy, const double *vz, const double *m, const double *u,
                      const double *pot, int64_t n, double *scratch, sphx_diag *out,
@@ -274,7 +344,8 @@ CoolDev sphx_engine_cool(const Engine &E) { return make_cool(E); }
 
 int Engine::finish_step(hipStream_t st) {
     if (n == 0) return SPHX_OK;
-    k_finish<<<g1(n), IT, 0, st>>>(s, d_run, n, make_cool(*this), has_cool ? 1 : 0, counters);
+    k_finish<<<g1(n), IT, 0, st>>>(s, d_run, n, make_cool(*this), has_cool ? 1 : 0, grid,
+                                   counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

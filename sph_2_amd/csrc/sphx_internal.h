@@ -190,6 +190,13 @@ struct RunParams {
     double dt_parts[4];  // dt_f, dt_kin, dt_cfl, dt_dens of the last evaluation
 };
 
+// 2-D (u, rho) cooling table on the device
+struct GridDev {
+    int64_t nu = 0, nrho = 0;
+    const double *u_nodes = nullptr, *rho_nodes = nullptr, *du = nullptr;
+    double dt_ref_s = 0, unit_u_cgs = 0, unit_rho_cgs = 0, unit_time_s = 0;
+};
+
 // ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
@@ -219,6 +226,10 @@ struct Engine {
     bool has_cool = false;
     sphx_cooling cool{};
     double *cool_T = nullptr, *cool_L = nullptr;
+    GridDev grid{};            // optional 2-D cooling table (runs with use_cool == 2)
+    double *grid_buf = nullptr;
+    int64_t grid_cap = 0;
+    int set_cool_grid(const sphx_cool_grid *g, hipStream_t st);
 
     int64_t cap = 0, n = 0;
     int R = 0;
@@ -415,6 +426,9 @@ struct CoolDev {
     int32_t max_iter, max_bracket, ntab;
     const double *T, *L;
 };
+int sphx_grid_apply_launch(const GridDev &g, const double *rho, const double *u_in, double dt,
+                           int64_t n, double *u_out, hipStream_t st);
+int sphx_grid_build_launch(const GridDev &g, const CoolDev &cd, double *du, hipStream_t st);
 int sphx_cool_launch(const CoolDev &cd, const double *rho, const double *u_in, double dt,
                      const double *dt_p, const int32_t *run, const RunParams *rp, int64_t n,
                      double *u_out, int32_t *iters, int64_t *counters, hipStream_t st);

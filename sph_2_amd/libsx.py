@@ -433,6 +433,72 @@ def DoCooling(rho, u, dt, table: CoolingTable, *, device=0, return_iters=False):
     return (o, it.cpu().numpy()) if return_iters else o
 
 
+MH_CGS = 1.6726e-24
+KB_CGS = 1.380649e-16
+YEAR_S = 365.24 * 24.0 * 3600.0
+
+
+class CoolingGrid:
+    """2-D (u, rho) cooling table of the reference's v3 driver
+    (GPU_sph/With_cooling/Model_A/hfv_sph_gpu_v3.cu:115-204): strictly
+    increasing cgs nodes u_nodes[nu], rho_nodes[nrho] and du[nu, nrho], the cgs
+    energy lost over dt_ref_s at each node.  Arrays live on the device."""
+
+    def __init__(self, u_nodes, rho_nodes, du, dt_ref_s, unit_u_cgs, unit_rho_cgs, unit_time_s):
+        t = _t()
+        self.u_nodes = dev(_f64(u_nodes))
+        self.rho_nodes = dev(_f64(rho_nodes))
+        du = _f64(du).reshape(len(self.u_nodes), len(self.rho_nodes))
+        self.du = dev(du.ravel())
+        self.dt_ref_s = float(dt_ref_s)
+        self.units = (float(unit_u_cgs), float(unit_rho_cgs), float(unit_time_s))
+        self._t = t
+
+    def c(self) -> _lib.CoolGrid:
+        return _lib.CoolGrid(len(self.u_nodes), len(self.rho_nodes), ptr(self.u_nodes),
+                             ptr(self.rho_nodes), ptr(self.du), self.dt_ref_s, *self.units)
+
+    @staticmethod
+    def nodes(xmin, xmax, n):
+        """The reference's log-uniform grid: 10^(log10 xmin + i (log10 xmax - log10 xmin)/n),
+        i < n (hfv_sph_gpu_v3.cu:158-166, 183-191)."""
+        step = (np.log10(xmax) - np.log10(xmin)) / n
+        return 10.0 ** (np.log10(xmin) + np.arange(n) * step)
+
+    @classmethod
+    def from_solve(cls, table: CoolingTable, T_min=1e4, T_max=1e6, n_u=1000, nH_min=1e-4,
+                   nH_max=1e2, n_rho=1000, dt_ref_s=200.0 * YEAR_S, *, device=0):
+        """Fill the table from the implicit solve over dt_ref_s (the role of
+        GPU_sph/Cooling_v1/createCoolingGridX2.cu); u nodes from T nodes with the
+        solve's T(u) = (gamma-1) mu m_H u / k_B, rho nodes = n_H m_H."""
+        kw = table.kw
+        tfac = (kw["gamma"] - 1.0) * kw["mu"] * MH_CGS / KB_CGS
+        un = cls.nodes(T_min, T_max, n_u) / tfac
+        rn = cls.nodes(nH_min * MH_CGS, nH_max * MH_CGS, n_rho)
+        g = cls(un, rn, np.zeros((n_u, n_rho)), dt_ref_s, kw["unit_u_cgs"], kw["unit_rho_cgs"],
+                kw["unit_time_s"])
+        c = table.c()
+        gc = g.c()
+        check(_lib.load().sphx_cool_grid_build(_context(device), C.byref(c), C.byref(gc),
+                                               _lib.stream_handle()), "cool_grid_build")
+        return g
+
+    def du_host(self):
+        return self.du.cpu().numpy().reshape(len(self.u_nodes), len(self.rho_nodes))
+
+
+def applyCooling(rho, u, dt, grid: CoolingGrid, *, device=0):
+    """applyCooling (myCppSPHLibs_v3.h:794-859) in code units: returns the cooled u."""
+    t = _t()
+    dr, du = dev(_f64(rho)), dev(_f64(u))
+    out = t.empty(len(dr), dtype=t.float64, device="cuda")
+    g = grid.c()
+    check(_lib.load().sphx_cool_grid_apply(_context(device), len(dr), ptr(dr), ptr(du), float(dt),
+                                           C.byref(g), ptr(out), _lib.stream_handle()),
+          "applyCooling")
+    return out.cpu().numpy()
+
+
 def DoCooling_h(rho, u_old, dt, XH, table: CoolingTable | None = None):
     """Scalar cgs form of DoCooling_h(rho, u_old, dt, XH) (PHOTOPY:489): returns
     u in cgs after an implicit step of dt seconds over the tabulated Lambda(T)."""

@@ -89,9 +89,16 @@ class SphPlan:
         self.n = checkpoint_info(path)["n"]
         self._gcfg_set = True
 
-    def step(self, n_steps: int, dt: float, cooling: bool = False) -> int:
-        return check(_lib.load().sphx_plan_step(self._h, int(n_steps), float(dt), int(bool(cooling)),
-                                                _lib.stream_handle()), "sphx_plan_step")
+    def set_cooling_grid(self, grid):
+        """Attach a libsx.CoolingGrid; step(..., cooling="table") uses it."""
+        g = grid.c()
+        check(_lib.load().sphx_plan_set_cooling_grid(self._h, C.byref(g)), "set_cooling_grid")
+
+    def step(self, n_steps: int, dt: float, cooling=False) -> int:
+        """cooling: False, True / "solve" (implicit Lambda(T) solve) or "table"."""
+        return check(_lib.load().sphx_plan_step(self._h, int(n_steps), float(dt),
+                                                _lib.cooling_mode(cooling), _lib.stream_handle()),
+                     "sphx_plan_step")
 
     def state(self, fields=("pos", "vel", "u", "h", "rho", "acc", "acc_g", "dudt", "k")) -> dict:
         """Copy the state out (original particle order) as numpy arrays."""

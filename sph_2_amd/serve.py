@@ -64,7 +64,12 @@ class Batch:
     def live_particles(self) -> int:
         return sum(self.sizes.values())
 
-    def admit(self, ic_: ic.IC, dt: float, cooling: bool = False) -> int | None:
+    def set_cooling_grid(self, grid):
+        """Attach a libsx.CoolingGrid for runs admitted with cooling="table"."""
+        g = grid.c()
+        check(_lib.load().sphx_batch_set_cooling_grid(self._h, C.byref(g)), "set_cooling_grid")
+
+    def admit(self, ic_: ic.IC, dt: float, cooling=False) -> int | None:
         """Admit one run; None when the budget or the run slots are full."""
         t = _lib.torch()
         bufs = [dev(np.asarray(a, np.float64), t.float64) for a in
@@ -72,7 +77,7 @@ class Batch:
         de = dev(np.asarray(ic_.eps, np.float64), t.float64) if ic_.eps is not None else None
         rid = C.c_int64()
         rc = check(_lib.load().sphx_batch_admit(self._h, ic_.n, *[ptr(b) for b in bufs], ptr(de),
-                                                float(dt), int(bool(cooling)), C.byref(rid),
+                                                float(dt), _lib.cooling_mode(cooling), C.byref(rid),
                                                 _lib.stream_handle()), "sphx_batch_admit")
         if rc == _lib.REFUSED:
             return None

@@ -461,3 +461,81 @@ def test_global_dt_vsig_is_the_reference_double_loop():
     acc = np.zeros((40, 3))
     dt, p = oracle.global_dt(h, c, acc, acc, np.zeros(40), 0.25, 1e-30, 1e30)
     assert p[2] == 0.25 * h.min() / vsig
+
+
+# ---------------------------------------------------------------------------
+# 2-D (u, rho) cooling table (myCppSPHLibs_v3.h:794-859)
+# ---------------------------------------------------------------------------
+def _grid_units():
+    return dict(unit_u_cgs=2.0e10, unit_rho_cgs=3.0e-23, unit_time_s=5.0e13)
+
+
+def _grid_nodes():
+    un = 10.0 ** (11 + np.arange(40) * (2.0 / 40))
+    rn = 10.0 ** (-28 + np.arange(30) * (6.0 / 30))
+    return un, rn
+
+
+def test_cool_grid_bin_is_the_reference_scan():
+    """With du = 0 the result is the node of the chosen bin: the first j with
+    g[j] <= x <= g[j+1], restated here with numpy's searchsorted."""
+    un, rn = _grid_nodes()
+    U = _grid_units()
+    du = np.zeros((len(un), len(rn)))
+    rng = np.random.default_rng(9)
+    xu = np.concatenate([10.0 ** rng.uniform(11, 13, 500), un[1:-1], un[1:-1] * (1 + 1e-14)])
+    rho = np.full(len(xu), rn[3] * 1.5)
+    got = oracle.cool_grid_apply(rho / U["unit_rho_cgs"], xu / U["unit_u_cgs"], 1.0, un, rn, du,
+                                 1.0, **U)
+    uc = (xu / U["unit_u_cgs"]) * U["unit_u_cgs"]
+    j = np.minimum(np.searchsorted(un[1:], uc, side="left"), len(un) - 2)
+    assert np.array_equal(got, un[j] / U["unit_u_cgs"])
+
+
+def test_cool_grid_clamps_scales_and_floors():
+    un, rn = _grid_nodes()
+    U = _grid_units()
+    rng = np.random.default_rng(2)
+    du = rng.random((len(un), len(rn))) * un[:, None] * 0.1
+    cu, cr = U["unit_u_cgs"], U["unit_rho_cgs"]
+    # below / above the table: bins 0 and n-2
+    u = np.array([un[0] * 0.5, un[-1] * 3.0]) / cu
+    r = np.array([rn[0] * 0.1, rn[-1] * 10.0]) / cr
+    dt_ref = 7.0e10
+    out = oracle.cool_grid_apply(r, u, dt_ref / U["unit_time_s"], un, rn, du, dt_ref, **U)
+    assert out[0] * cu == pytest.approx(max(un[0] - du[0, 0], un[0]), rel=1e-15)  # floored
+    assert out[1] * cu == pytest.approx(un[-2] - du[-2, -2], rel=1e-15)
+    # linear in dt / dt_ref
+    x = np.array([un[7] * 1.001]) / cu
+    y = np.array([rn[4] * 1.001]) / cr
+    d1 = un[7] - oracle.cool_grid_apply(y, x, dt_ref / U["unit_time_s"], un, rn, du, dt_ref, **U)[0] * cu
+    d2 = un[7] - oracle.cool_grid_apply(y, x, 2 * dt_ref / U["unit_time_s"], un, rn, du, dt_ref, **U)[0] * cu
+    assert d1 == pytest.approx(du[7, 4], rel=1e-9) and d2 == pytest.approx(2 * du[7, 4], rel=1e-9)
+    # floor at the first u node
+    big = np.full_like(du, 1e30)
+    assert oracle.cool_grid_apply(y, x, 1.0, un, rn, big, 1.0, **U)[0] * cu == pytest.approx(un[0])
+
+
+def test_cool_grid_reproduces_the_solve_at_its_nodes():
+    """Table built from the implicit solve over dt_ref: at a node and dt = dt_ref
+    the lookup equals the solve."""
+    units = ic.cloud_units()
+    T, L = ic.cooling_table(seed=0)
+    cp = oracle.CoolParams(units["unit_u_cgs"], units["unit_rho_cgs"], units["unit_time_s"],
+                           T=T, L=L, heat=2e-26)
+    tfac = (cp.gamma - 1.0) * cp.mu * 1.6726e-24 / 1.380649e-16
+    un = 10.0 ** (4 + np.arange(30) * (2.0 / 30)) / tfac
+    rn = 10.0 ** (-4 + np.arange(20) * (6.0 / 20)) * 1.6726e-24
+    dt_ref = 200.0 * 365.24 * 24 * 3600
+    du = oracle.cool_grid_build(un, rn, dt_ref, cp)
+    assert np.all(np.isfinite(du))
+    j, i = np.meshgrid(np.arange(1, 29), np.arange(1, 19), indexing="ij")
+    j, i = j.ravel(), i.ravel()
+    u = un[j] * (1 + 1e-12) / units["unit_u_cgs"]
+    rho = rn[i] * (1 + 1e-12) / units["unit_rho_cgs"]
+    dt = dt_ref / units["unit_time_s"]
+    got = oracle.cool_grid_apply(rho, u, dt, un, rn, du, dt_ref, units["unit_u_cgs"],
+                                 units["unit_rho_cgs"], units["unit_time_s"])
+    want, _ = oracle.cool(rn[i] / units["unit_rho_cgs"], un[j] / units["unit_u_cgs"], dt, cp)
+    want = np.maximum(want, un[0] / units["unit_u_cgs"])
+    assert np.max(np.abs(got / want - 1)) < 1e-9
