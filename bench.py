@@ -52,7 +52,8 @@ def rel_rms(a, b) -> float:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="evrard", choices=["evrard", "c2", "c3", "bh", "serve"])
+    ap.add_argument("--workload", default="evrard",
+                    choices=["evrard", "c2", "c3", "bh", "serve", "kernels"])
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None)
@@ -83,7 +84,8 @@ def main(argv=None) -> int:
         print(json.dumps({"error": "no HIP device"}))
         return 1
     if a.workload != "evrard":
-        return {"c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve}[a.workload](a)
+        return {"c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve,
+                "kernels": bench_kernels}[a.workload](a)
     if a.n is None:
         a.n = 1_048_576
     return bench_evrard(a)
@@ -407,6 +409,39 @@ def bench_serve(a) -> int:
                        f"{a.run_steps} steps each, Poisson arrivals at {a.rate}/s, "
                        f"HBM budget {a.hbm_gb} GB, graph={bool(a.graph)}")
     print(json.dumps(rep))
+    return 0
+
+
+COPY_BYTES = 256 * 2 ** 20
+
+
+def bench_kernels(a) -> int:
+    """Driver for hardware-counter runs (rocprofv3 --pmc): first a device copy
+    of COPY_BYTES (a kernel of known traffic, to calibrate FETCH_SIZE and
+    WRITE_SIZE), then the Evrard plan of N particles (default 1,048,576):
+    bootstrap plus --steps mixed steps.  Prints what ran."""
+    import torch
+
+    from sph_2_amd import ic
+    from sph_2_amd.plan import SphPlan
+
+    n = a.n or 1_048_576
+    x = torch.ones(COPY_BYTES // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
+    e = ic.evrard(n, seed=a.seed)
+    pl = SphPlan(n)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h)
+    pl.init()
+    pl.step(a.steps, a.dt)
+    torch.cuda.synchronize()
+    pl.close()
+    print(json.dumps({"workload": f"counter driver: {COPY_BYTES}-byte device copy, then Evrard "
+                                  f"N={n} bootstrap + {a.steps} mixed steps", "n": n,
+                      "steps": a.steps, "copy_bytes": COPY_BYTES}))
     return 0
 
 

@@ -54,6 +54,28 @@ __device__ __forceinline__ void for_candidates(const Tables &t, int b, int lane,
     }
 }
 
+// Candidate filter: lane q holds particle q of a staged candidate bucket and
+// keeps it when it may lie within `reach` of the target bucket's box (the same
+// conservative FP64 test as the bucket culling).  A dropped particle is out of
+// reach of every target particle, so the kept ones, compacted in their
+// original order, give exactly the sums of the unfiltered loop.
+struct Kept {
+    int n, slot;
+};
+__device__ __forceinline__ Kept keep_mask(bool keep, int lane) {
+    uint64_t m = __ballot(keep);
+    Kept k;
+    k.n = __popcll(m);
+    k.slot = __popcll(m & ((1ull << lane) - 1ull));
+    return k;
+}
+
+__device__ __forceinline__ bool near_box(const double *bx, double x, double y, double z,
+                                         double reach) {
+    return sphx_within(sphx_box_d2(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], x, x, y, y, z, z),
+                       reach);
+}
+
 struct SmoothArgs {
     int32_t nngb, ndown, nup, max_iter;
     int32_t predictor;
@@ -64,7 +86,7 @@ struct SmoothArgs {
 
 __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a) {
     __shared__ double sx[64], sy[64], sz[64];
-    int b = t.boff + blockIdx.x;
+    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -73,6 +95,7 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     int64_t nrun = (int64_t)t.r_pend[r] - t.r_pbeg[r];
     double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
     double hprev = ok ? s.h[i] : 0.0;
+    const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     double sr;
     {
 #pragma clang fp contract(off)
@@ -104,14 +127,24 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
             t, b, lane, [&](int) { return R; }, [&](int) { return R; },
             [&](int J) {
                 int jc = t.b_cnt[J], j0 = t.b_beg[J];
-                __syncthreads();
+                double xj = 0.0, yj = 0.0, zj = 0.0;
+                bool keep = false;
                 if (lane < jc) {
-                    sx[lane] = s.x[j0 + lane];
-                    sy[lane] = s.y[j0 + lane];
-                    sz[lane] = s.z[j0 + lane];
+                    xj = s.x[j0 + lane];
+                    yj = s.y[j0 + lane];
+                    zj = s.z[j0 + lane];
+                    keep = near_box(ib, xj, yj, zj, R);
+                }
+                Kept k = keep_mask(keep, lane);
+                if (k.n == 0) return;
+                __syncthreads();
+                if (keep) {
+                    sx[k.slot] = xj;
+                    sy[k.slot] = yj;
+                    sz[k.slot] = zj;
                 }
                 __syncthreads();
-                for (int q = 0; q < jc; ++q) {
+                for (int q = 0; q < k.n; ++q) {
                     double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
                     kc += (r2 <= s2) ? 1 : 0;
                 }
@@ -206,7 +239,7 @@ struct DensArgs {
 __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     __shared__ double sx[64], sy[64], sz[64], sh[64];
     __shared__ float sm[64];
-    int b = t.boff + blockIdx.x;
+    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -214,6 +247,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
     double hi = ok ? s.h[i] : 0.0;
     double hI = t.bhmax[b];
+    const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     KahanF acc;
     acc.init();
     int nn = 0;
@@ -223,16 +257,27 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
         [&](int J) {
             int jc = t.b_cnt[J], j0 = t.b_beg[J];
             ++ncand;
-            __syncthreads();
+            double xj = 0.0, yj = 0.0, zj = 0.0, hj = 0.0;
+            bool keep = false;
             if (lane < jc) {
-                sx[lane] = s.x[j0 + lane];
-                sy[lane] = s.y[j0 + lane];
-                sz[lane] = s.z[j0 + lane];
-                sh[lane] = s.h[j0 + lane];
-                sm[lane] = (float)s.m[j0 + lane];
+                xj = s.x[j0 + lane];
+                yj = s.y[j0 + lane];
+                zj = s.z[j0 + lane];
+                hj = s.h[j0 + lane];
+                keep = near_box(ib, xj, yj, zj, hI + hj);
+            }
+            Kept k = keep_mask(keep, lane);
+            if (k.n == 0) return;
+            __syncthreads();
+            if (keep) {
+                sx[k.slot] = xj;
+                sy[k.slot] = yj;
+                sz[k.slot] = zj;
+                sh[k.slot] = hj;
+                sm[k.slot] = (float)s.m[j0 + lane];
             }
             __syncthreads();
-            for (int q = 0; q < jc; ++q) {
+            for (int q = 0; q < k.n; ++q) {
                 double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
                 double r2 = sphx_r2(dx, dy, dz);
                 double tt = hi + sh[q];
@@ -259,7 +304,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
 __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
     __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
     __shared__ float sm[64];
-    int b = t.boff + blockIdx.x;
+    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -268,21 +313,31 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
     double hi = ok ? s.h[i] : 0.0;
     double vxi = ok ? s.vx[i] : 0.0, vyi = ok ? s.vy[i] : 0.0, vzi = ok ? s.vz[i] : 0.0;
     double hI = t.bhmax[b];
+    const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     KahanF kd, kx, ky, kz;
     kd.init(); kx.init(); ky.init(); kz.init();
     for_candidates(
         t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
         [&](int J) {
             int jc = t.b_cnt[J], j0 = t.b_beg[J];
-            __syncthreads();
+            int j = j0 + lane;
+            double xj = 0.0, yj = 0.0, zj = 0.0, hj = 0.0;
+            bool keep = false;
             if (lane < jc) {
-                int j = j0 + lane;
-                sx[lane] = s.x[j]; sy[lane] = s.y[j]; sz[lane] = s.z[j]; sh[lane] = s.h[j];
-                svx[lane] = s.vx[j]; svy[lane] = s.vy[j]; svz[lane] = s.vz[j];
-                sm[lane] = (float)s.m[j];
+                xj = s.x[j]; yj = s.y[j]; zj = s.z[j]; hj = s.h[j];
+                keep = near_box(ib, xj, yj, zj, hI + hj);
+            }
+            Kept k = keep_mask(keep, lane);
+            if (k.n == 0) return;
+            __syncthreads();
+            if (keep) {
+                int q = k.slot;
+                sx[q] = xj; sy[q] = yj; sz[q] = zj; sh[q] = hj;
+                svx[q] = s.vx[j]; svy[q] = s.vy[j]; svz[q] = s.vz[j];
+                sm[q] = (float)s.m[j];
             }
             __syncthreads();
-            for (int q = 0; q < jc; ++q) {
+            for (int q = 0; q < k.n; ++q) {
                 double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
                 double r2 = sphx_r2(dx, dy, dz);
                 double tt = hi + sh[q];
@@ -321,7 +376,7 @@ __device__ __forceinline__ double balsara_f(double dv, double cv, double c, doub
 __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
     __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
     __shared__ float sm[64], srho[64], spr[64], sc[64], sf[64];
-    int b = t.boff + blockIdx.x;
+    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -334,26 +389,36 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
     float ci = ok ? (float)s.c[i] : 0.0f;
     float fi = (ok && a.balsara) ? (float)balsara_f(s.divv[i], s.curlv[i], s.c[i], s.h[i]) : 1.0f;
     double hI = t.bhmax[b];
+    const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     KahanF kx, ky, kz, kdu;
     kx.init(); ky.init(); kz.init(); kdu.init();
     for_candidates(
         t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
         [&](int J) {
             int jc = t.b_cnt[J], j0 = t.b_beg[J];
-            __syncthreads();
+            int j = j0 + lane;
+            double xj = 0.0, yj = 0.0, zj = 0.0, hj = 0.0;
+            bool keep = false;
             if (lane < jc) {
-                int j = j0 + lane;
-                sx[lane] = s.x[j]; sy[lane] = s.y[j]; sz[lane] = s.z[j]; sh[lane] = s.h[j];
-                svx[lane] = s.vx[j]; svy[lane] = s.vy[j]; svz[lane] = s.vz[j];
+                xj = s.x[j]; yj = s.y[j]; zj = s.z[j]; hj = s.h[j];
+                keep = near_box(ib, xj, yj, zj, hI + hj);
+            }
+            Kept k = keep_mask(keep, lane);
+            if (k.n == 0) return;
+            __syncthreads();
+            if (keep) {
+                int q = k.slot;
+                sx[q] = xj; sy[q] = yj; sz[q] = zj; sh[q] = hj;
+                svx[q] = s.vx[j]; svy[q] = s.vy[j]; svz[q] = s.vz[j];
                 double rj = s.rho[j];
-                sm[lane] = (float)s.m[j];
-                srho[lane] = (float)rj;
-                spr[lane] = (float)(s.P[j] / (rj * rj));
-                sc[lane] = (float)s.c[j];
-                sf[lane] = a.balsara ? (float)balsara_f(s.divv[j], s.curlv[j], s.c[j], s.h[j]) : 1.0f;
+                sm[q] = (float)s.m[j];
+                srho[q] = (float)rj;
+                spr[q] = (float)(s.P[j] / (rj * rj));
+                sc[q] = (float)s.c[j];
+                sf[q] = a.balsara ? (float)balsara_f(s.divv[j], s.curlv[j], s.c[j], s.h[j]) : 1.0f;
             }
             __syncthreads();
-            for (int q = 0; q < jc; ++q) {
+            for (int q = 0; q < k.n; ++q) {
                 double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
                 double r2 = sphx_r2(dx, dy, dz);
                 double tt = hi + sh[q];
@@ -404,7 +469,7 @@ __global__ __launch_bounds__(64) void k_nbr(Tables t, Soa s, int64_t r0, int64_t
                                             int32_t *__restrict__ idx) {
     __shared__ double sx[64], sy[64], sz[64], sh[64];
     __shared__ int32_t sid[64];
-    int b = t.boff + blockIdx.x;
+    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -415,6 +480,7 @@ __global__ __launch_bounds__(64) void k_nbr(Tables t, Soa s, int64_t r0, int64_t
     bool want = ok && myid >= r0 && myid < r1;
     int64_t o = (want && !counts) ? offsets[myid - r0] : 0;
     double hI = t.bhmax[b];
+    const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     int c = 0;
     for_candidates(
         t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },

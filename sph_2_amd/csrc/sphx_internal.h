@@ -190,6 +190,19 @@ struct RunParams {
     double dt_parts[4];  // dt_f, dt_kin, dt_cfl, dt_dens of the last evaluation
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup w runs on
+// XCD w % 8).  Spatially adjacent buckets share neighbour candidates, so give
+// each XCD (and its L2) a contiguous range of buckets instead of every 8th one:
+// XCD x handles [x q + min(x, r), ...) with q = nblk / 8, r = nblk % 8.  A
+// bijection on [0, nblk); it changes which workgroup computes a bucket, never
+// the bucket's arithmetic.
+__device__ __forceinline__ int xcd_block(int bid, int nblk) {
+    constexpr int X = 8;
+    int q = nblk / X, r = nblk % X;
+    int x = bid % X, k = bid / X;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // 2-D (u, rho) cooling table on the device
 struct GridDev {
     int64_t nu = 0, nrho = 0;
