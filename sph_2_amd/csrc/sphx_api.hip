@@ -180,12 +180,21 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
         return SPHX_ERR_ARG;
     }
     hb_beg.clear(); hb_cnt.clear(); hb_run.clear(); hs_bbeg.clear(); hs_bend.clear();
-    hr_sbeg.clear(); hr_send.clear(); hr_pbeg.clear(); hr_pend.clear();
+    hr_sbeg.clear(); hr_send.clear(); hr_pbeg.clear(); hr_pend.clear(); hr_lcut.clear();
     int64_t off = 0;
     for (size_t r = 0; r < sizes.size(); ++r) {
         int64_t nr = sizes[r];
         hr_pbeg.push_back((int32_t)off);
         hr_pend.push_back((int32_t)(off + nr));
+        // A bucket is cut where consecutive particles lie in different cells
+        // of this Morton level: cells of about 8 x 64 particles or more, so
+        // the cut costs few extra segments but a bucket never straddles a big
+        // jump of the curve.  floor(log8(n / 64)) - 1, at least 1.
+        {
+            int l = 0;
+            while (((int64_t)64 << (3 * (l + 1))) <= nr) ++l;
+            hr_lcut.push_back(std::max(1, l - 1));
+        }
         int b0 = (int)hb_beg.size();
         for (int64_t a = 0; a < nr; a += SPHX_BUCKET) {
             hb_beg.push_back((int32_t)(off + a));
@@ -202,7 +211,7 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
         off += nr;
     }
     int nb = (int)hb_beg.size(), ns = (int)hs_bbeg.size(), nr = (int)sizes.size();
-    int64_t ni = 3 * (int64_t)nb + 2 * (int64_t)ns + 4 * (int64_t)nr;
+    int64_t ni = 3 * (int64_t)nb + 2 * (int64_t)ns + 5 * (int64_t)nr;
     if (ni > tab_i32_cap) {
         SPHX_CHECK(alloc((void **)&tab_i32, sizeof(int32_t) * (ni + 1024)));
         tab_i32_cap = ni + 1024;
@@ -210,7 +219,7 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
     std::vector<int32_t> all;
     all.reserve(ni);
     for (auto *v : {&hb_beg, &hb_cnt, &hb_run, &hs_bbeg, &hs_bend, &hr_sbeg, &hr_send, &hr_pbeg,
-                    &hr_pend})
+                    &hr_pend, &hr_lcut})
         all.insert(all.end(), v->begin(), v->end());
     if (!all.empty())
         SPHX_HIP(hipMemcpy(tab_i32, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
@@ -245,6 +254,7 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
     t.r_send = q; q += nr;
     t.r_pbeg = q; q += nr;
     t.r_pend = q; q += nr;
+    t.r_lcut = q; q += nr;
     double *d = tab_d;
     double **bx[] = {&t.bxl, &t.bxh, &t.byl, &t.byh, &t.bzl, &t.bzh, &t.bhmax, &t.bemax};
     for (double **a : bx) { *a = d; d += nb; }

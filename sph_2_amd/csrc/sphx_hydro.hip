@@ -14,15 +14,14 @@
 
 namespace {
 
-// Walk the candidate buckets of bucket `b` (same run).  `reach_s(S)` and
+// Walk the candidate buckets, in increasing order, of a group of particles of
+// run `r` with bounding box ib (xl, xh, yl, yh, zl, zh).  `reach_s(S)` and
 // `reach_b(J)` give the reach against super S and bucket J; `body(J)` is
 // called once per surviving bucket with the whole wavefront converged.
 template <class ReachS, class ReachB, class Body>
-__device__ __forceinline__ void for_candidates(const Tables &t, int b, int lane, ReachS reach_s,
-                                               ReachB reach_b, Body body) {
-    int r = t.b_run[b];
-    double axl = t.bxl[b], axh = t.bxh[b], ayl = t.byl[b], ayh = t.byh[b], azl = t.bzl[b],
-           azh = t.bzh[b];
+__device__ __forceinline__ void for_candidates(const Tables &t, int r, const double *ib, int lane,
+                                               ReachS reach_s, ReachB reach_b, Body body) {
+    double axl = ib[0], axh = ib[1], ayl = ib[2], ayh = ib[3], azl = ib[4], azh = ib[5];
     int s0 = t.r_sbeg[r], s1 = t.r_send[r];
     for (int sb = s0; sb < s1; sb += 64) {
         int S = sb + lane;
@@ -54,8 +53,48 @@ __device__ __forceinline__ void for_candidates(const Tables &t, int b, int lane,
     }
 }
 
+// Segments of a bucket.  64 consecutive Morton-sorted particles usually fill
+// one compact cell, but where the curve jumps between distant cells their
+// bounding box spans much of the run and would make every bucket of that
+// region a candidate.  A bucket is therefore cut wherever two consecutive
+// particles lie in different cells of Morton level t.r_lcut[run]; each
+// segment walks the candidates of its own box and only its lanes accumulate.
+// Candidates are still met in increasing order, so every sum is unchanged.
+// Returns the ballot of lanes that start a segment (lane 0 never set).
+__device__ __forceinline__ uint64_t bucket_cuts(const Tables &t, int r, int i, bool ok, int lane) {
+    bool cut = false;
+    if (t.keys && ok && lane > 0) {
+        uint64_t x = t.keys[i] ^ t.keys[i - 1];
+        if (x) {
+            int hb = 63 - __builtin_clzll(x);
+            int lev = t.mbits - 1 - hb / 3;  // Morton level of the cell boundary crossed
+            cut = lev <= t.r_lcut[r];
+        }
+    }
+    return __ballot(cut);
+}
+
+// End (exclusive) of the segment that starts at lane s0.
+__device__ __forceinline__ int seg_end(uint64_t cuts, int s0, int cnt) {
+    uint64_t rest = s0 >= 63 ? 0ull : cuts & (~0ull << (s0 + 1));
+    int s1 = rest ? __builtin_ctzll(rest) : cnt;
+    return s1 < cnt ? s1 : cnt;
+}
+
+// Bounding box and max h of the lanes with `mine` set.
+__device__ __forceinline__ void seg_box(bool mine, double x, double y, double z, double h,
+                                        double *box, double *hmax) {
+    box[0] = wave_min_d(mine ? x : INFINITY);
+    box[1] = wave_max_d(mine ? x : -INFINITY);
+    box[2] = wave_min_d(mine ? y : INFINITY);
+    box[3] = wave_max_d(mine ? y : -INFINITY);
+    box[4] = wave_min_d(mine ? z : INFINITY);
+    box[5] = wave_max_d(mine ? z : -INFINITY);
+    *hmax = wave_max_d(mine ? h : 0.0);
+}
+
 // Candidate filter: lane q holds particle q of a staged candidate bucket and
-// keeps it when it may lie within `reach` of the target bucket's box (the same
+// keeps it when it may lie within `reach` of the target group's box (the same
 // conservative FP64 test as the bucket culling).  A dropped particle is out of
 // reach of every target particle, so the kept ones, compacted in their
 // original order, give exactly the sums of the unfiltered loop.
@@ -96,6 +135,7 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
     double hprev = ok ? s.h[i] : 0.0;
     const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
+    const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
     double sr;
     {
 #pragma clang fp contract(off)
@@ -120,35 +160,48 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     while (true) {
         bool active = ok && !done;
         if (!__any(active)) break;
-        double R = wave_max_d(active ? sr : 0.0);
         double s2 = sphx_sq(sr);
         int kc = 0;
-        for_candidates(
-            t, b, lane, [&](int) { return R; }, [&](int) { return R; },
-            [&](int J) {
-                int jc = t.b_cnt[J], j0 = t.b_beg[J];
-                double xj = 0.0, yj = 0.0, zj = 0.0;
-                bool keep = false;
-                if (lane < jc) {
-                    xj = s.x[j0 + lane];
-                    yj = s.y[j0 + lane];
-                    zj = s.z[j0 + lane];
-                    keep = near_box(ib, xj, yj, zj, R);
-                }
-                Kept k = keep_mask(keep, lane);
-                if (k.n == 0) return;
-                __syncthreads();
-                if (keep) {
-                    sx[k.slot] = xj;
-                    sy[k.slot] = yj;
-                    sz[k.slot] = zj;
-                }
-                __syncthreads();
-                for (int q = 0; q < k.n; ++q) {
-                    double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
-                    kc += (r2 <= s2) ? 1 : 0;
-                }
-            });
+        for (int g0 = 0; g0 < cnt;) {
+            int g1 = seg_end(cuts, g0, cnt);
+            bool mine = lane >= g0 && lane < g1;
+            double R = wave_max_d((active && mine) ? sr : 0.0);
+            double gb[6];
+            if (cuts) {
+                double hdummy;
+                seg_box(ok && mine, xi, yi, zi, 0.0, gb, &hdummy);
+            } else {
+                for (int k = 0; k < 6; ++k) gb[k] = ib[k];
+            }
+            g0 = g1;
+            if (!(R > 0.0)) continue;  // no active lane in this segment
+            for_candidates(
+                t, r, gb, lane, [&](int) { return R; }, [&](int) { return R; },
+                [&](int J) {
+                    int jc = t.b_cnt[J], j0 = t.b_beg[J];
+                    double xj = 0.0, yj = 0.0, zj = 0.0;
+                    bool keep = false;
+                    if (lane < jc) {
+                        xj = s.x[j0 + lane];
+                        yj = s.y[j0 + lane];
+                        zj = s.z[j0 + lane];
+                        keep = near_box(gb, xj, yj, zj, R);
+                    }
+                    Kept k = keep_mask(keep, lane);
+                    if (k.n == 0) return;
+                    __syncthreads();
+                    if (keep) {
+                        sx[k.slot] = xj;
+                        sy[k.slot] = yj;
+                        sz[k.slot] = zj;
+                    }
+                    __syncthreads();
+                    for (int q = 0; q < k.n; ++q) {
+                        double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
+                        kc += (mine && r2 <= s2) ? 1 : 0;
+                    }
+                });
+        }
         if (active) {
 #pragma clang fp contract(off)
             ++it;
@@ -252,8 +305,21 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     acc.init();
     int nn = 0;
     int ncand = 0;
-    for_candidates(
-        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+    const int r = t.b_run[b];
+    const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
+    for (int g0 = 0; g0 < cnt;) {
+      const int g1 = seg_end(cuts, g0, cnt);
+      const bool mine = lane >= g0 && lane < g1;
+      double gb[6], hS;
+      if (cuts) {
+          seg_box(ok && mine, xi, yi, zi, hi, gb, &hS);
+      } else {
+          for (int k = 0; k < 6; ++k) gb[k] = ib[k];
+          hS = hI;
+      }
+      g0 = g1;
+      for_candidates(
+        t, r, gb, lane, [&](int S) { return hS + t.shmax[S]; }, [&](int J) { return hS + t.bhmax[J]; },
         [&](int J) {
             int jc = t.b_cnt[J], j0 = t.b_beg[J];
             ++ncand;
@@ -264,7 +330,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                 yj = s.y[j0 + lane];
                 zj = s.z[j0 + lane];
                 hj = s.h[j0 + lane];
-                keep = near_box(ib, xj, yj, zj, hI + hj);
+                keep = near_box(gb, xj, yj, zj, hS + hj);
             }
             Kept k = keep_mask(keep, lane);
             if (k.n == 0) return;
@@ -281,7 +347,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                 double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
                 double r2 = sphx_r2(dx, dy, dz);
                 double tt = hi + sh[q];
-                if (r2 <= sphx_sq(tt)) {
+                if (mine && r2 <= sphx_sq(tt)) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h3 = g.hinv * g.hinv * g.hinv;
                     acc.add(sm[q] * (SPHX_SIGMA_F * h3 * w_shape(g.q)));
@@ -289,6 +355,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                 }
             }
         });
+    }
     if (ok) {
         double rho = acc.value();
         double u = s.u[i];
@@ -316,8 +383,21 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
     const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     KahanF kd, kx, ky, kz;
     kd.init(); kx.init(); ky.init(); kz.init();
-    for_candidates(
-        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+    const int r = t.b_run[b];
+    const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
+    for (int g0 = 0; g0 < cnt;) {
+      const int g1 = seg_end(cuts, g0, cnt);
+      const bool mine = lane >= g0 && lane < g1;
+      double gb[6], hS;
+      if (cuts) {
+          seg_box(ok && mine, xi, yi, zi, hi, gb, &hS);
+      } else {
+          for (int k = 0; k < 6; ++k) gb[k] = ib[k];
+          hS = hI;
+      }
+      g0 = g1;
+      for_candidates(
+        t, r, gb, lane, [&](int S) { return hS + t.shmax[S]; }, [&](int J) { return hS + t.bhmax[J]; },
         [&](int J) {
             int jc = t.b_cnt[J], j0 = t.b_beg[J];
             int j = j0 + lane;
@@ -325,7 +405,7 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
             bool keep = false;
             if (lane < jc) {
                 xj = s.x[j]; yj = s.y[j]; zj = s.z[j]; hj = s.h[j];
-                keep = near_box(ib, xj, yj, zj, hI + hj);
+                keep = near_box(gb, xj, yj, zj, hS + hj);
             }
             Kept k = keep_mask(keep, lane);
             if (k.n == 0) return;
@@ -341,7 +421,7 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
                 double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
                 double r2 = sphx_r2(dx, dy, dz);
                 double tt = hi + sh[q];
-                if (r2 <= sphx_sq(tt)) {
+                if (mine && r2 <= sphx_sq(tt)) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h5 = g.hinv * g.hinv;
                     h5 = h5 * h5 * g.hinv;
@@ -355,6 +435,7 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
                 }
             }
         });
+    }
     if (ok) {
         double ir = 1.0 / s.rho[i];
         s.divv[i] = -kd.value() * ir;
@@ -392,8 +473,21 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
     const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     KahanF kx, ky, kz, kdu;
     kx.init(); ky.init(); kz.init(); kdu.init();
-    for_candidates(
-        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+    const int r = t.b_run[b];
+    const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
+    for (int g0 = 0; g0 < cnt;) {
+      const int g1 = seg_end(cuts, g0, cnt);
+      const bool mine = lane >= g0 && lane < g1;
+      double gb[6], hS;
+      if (cuts) {
+          seg_box(ok && mine, xi, yi, zi, hi, gb, &hS);
+      } else {
+          for (int k = 0; k < 6; ++k) gb[k] = ib[k];
+          hS = hI;
+      }
+      g0 = g1;
+      for_candidates(
+        t, r, gb, lane, [&](int S) { return hS + t.shmax[S]; }, [&](int J) { return hS + t.bhmax[J]; },
         [&](int J) {
             int jc = t.b_cnt[J], j0 = t.b_beg[J];
             int j = j0 + lane;
@@ -401,7 +495,7 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
             bool keep = false;
             if (lane < jc) {
                 xj = s.x[j]; yj = s.y[j]; zj = s.z[j]; hj = s.h[j];
-                keep = near_box(ib, xj, yj, zj, hI + hj);
+                keep = near_box(gb, xj, yj, zj, hS + hj);
             }
             Kept k = keep_mask(keep, lane);
             if (k.n == 0) return;
@@ -422,7 +516,7 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
                 double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
                 double r2 = sphx_r2(dx, dy, dz);
                 double tt = hi + sh[q];
-                if (r2 <= sphx_sq(tt)) {
+                if (mine && r2 <= sphx_sq(tt)) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h5 = g.hinv * g.hinv;
                     h5 = h5 * h5 * g.hinv;
@@ -452,6 +546,7 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
                 }
             }
         });
+    }
     if (ok) {
         s.ax[i] = kx.value();
         s.ay[i] = ky.value();
@@ -483,7 +578,8 @@ __global__ __launch_bounds__(64) void k_nbr(Tables t, Soa s, int64_t r0, int64_t
     const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     int c = 0;
     for_candidates(
-        t, b, lane, [&](int S) { return hI + t.shmax[S]; }, [&](int J) { return hI + t.bhmax[J]; },
+        t, t.b_run[b], ib, lane, [&](int S) { return hI + t.shmax[S]; },
+        [&](int J) { return hI + t.bhmax[J]; },
         [&](int J) {
             int jc = t.b_cnt[J], j0 = t.b_beg[J];
             __syncthreads();

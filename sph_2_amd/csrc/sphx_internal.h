@@ -147,8 +147,11 @@ struct Tables {
     const int32_t *s_bbeg, *s_bend;             // supers: bucket ranges
     const int32_t *r_sbeg, *r_send;             // run -> super range
     const int32_t *r_pbeg, *r_pend;             // run -> particle range
+    const int32_t *r_lcut;                      // run -> coarsest Morton level a bucket may span
     double *bxl, *bxh, *byl, *byh, *bzl, *bzh, *bhmax, *bemax;  // bucket boxes
     double *sxl, *sxh, *syl, *syh, *szl, *szh, *shmax;          // super boxes
+    const uint64_t *keys;                       // sorted Morton keys of the particles
+    int32_t mbits;                              // Morton bits per axis in the keys
 };
 
 // Counters accumulated on device (int64 slots).
@@ -267,7 +270,7 @@ struct Engine {
 
     // tables
     std::vector<int32_t> hb_beg, hb_cnt, hb_run, hs_bbeg, hs_bend, hr_sbeg, hr_send, hr_pbeg,
-        hr_pend;
+        hr_pend, hr_lcut;
     int32_t *tab_i32 = nullptr;  // packed device copy of the int tables
     int64_t tab_i32_cap = 0;
     double *tab_d = nullptr;     // box arrays
@@ -292,6 +295,8 @@ struct Engine {
     Tables tview() const {
         Tables v = t;
         v.boff = rrun >= 0 ? rb0 : 0;
+        v.keys = keys;
+        v.mbits = morton_bits;
         return v;
     }
     unsigned nb_launch() const { return (unsigned)(rrun >= 0 ? rb1 - rb0 : t.nb); }
