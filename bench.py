@@ -404,7 +404,7 @@ def bench_serve(a) -> int:
         r.payload = build_payload(r)
     log(f"serve: {len(reqs)} requests built in {time.perf_counter() - t0:.1f}s")
     srv = Server(int(a.hbm_gb * 2 ** 30), max_runs=512, graph=bool(a.graph))
-    rep = srv.run(reqs, log=None)
+    rep = srv.run(reqs, log=lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True))
     rep["workload"] = (f"c5: {a.runs} runs, N log-uniform in [{a.nmin}, {a.nmax}], "
                        f"{a.run_steps} steps each, Poisson arrivals at {a.rate}/s, "
                        f"HBM budget {a.hbm_gb} GB, graph={bool(a.graph)}")

@@ -196,10 +196,11 @@ __global__ void k_bh_moments(BhTree T, int32_t l0, int32_t l1, const uint64_t *_
     T.link[i] = lk;
 }
 
-// Walk groups: every child (cnt <= 64) of a cell with more than 64
-// particles, the root when it has at most 64, and runs of 64 inside an
-// unsplittable leaf with more.  Each particle lands in exactly one group; the
-// list order (atomic append) does not affect any particle's result.
+// Walk groups: the children (cnt <= 64) of a cell with more than 64
+// particles, consecutive ones packed while their sum stays <= 64; the root
+// when it has at most 64; runs of 64 inside an unsplittable leaf with more.
+// Each particle lands in exactly one group; the list order (atomic append)
+// does not affect any particle's result.
 __global__ void k_bh_groups(BhTree T, int32_t ncell, int32_t *__restrict__ gbeg,
                             int32_t *__restrict__ gcnt, int32_t *__restrict__ counter) {
     int i = blockIdx.x * BHT + threadIdx.x;
@@ -212,13 +213,29 @@ __global__ void k_bh_groups(BhTree T, int32_t ncell, int32_t *__restrict__ gbeg,
     }
     if (c <= 64) return;
     if (T.child[i] >= 0) {
+        // consecutive small children are packed into one group of at most 64
+        // (their union stays inside this cell's cube)
+        int pb = -1, pc = 0;
         for (int q = T.child[i]; q < T.child[i] + T.nchild[i]; ++q) {
             int cq = T.cnt[q];
-            if (cq <= 64) {
-                int g = atomicAdd(counter, 1);
-                gbeg[g] = T.beg[q];
-                gcnt[g] = cq;
+            if (cq > 64 || pc + cq > 64) {
+                if (pc > 0) {
+                    int g = atomicAdd(counter, 1);
+                    gbeg[g] = pb;
+                    gcnt[g] = pc;
+                }
+                pb = -1;
+                pc = 0;
             }
+            if (cq <= 64) {
+                if (pc == 0) pb = T.beg[q];
+                pc += cq;
+            }
+        }
+        if (pc > 0) {
+            int g = atomicAdd(counter, 1);
+            gbeg[g] = pb;
+            gcnt[g] = pc;
         }
     } else {
         for (int o = 0; o < c; o += 64) {
