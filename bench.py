@@ -27,6 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
+# HBM bytes per k_grav launch at N = 1,048,576: 2 x FETCH_SIZE + WRITE_SIZE of the step's
+# dispatch in two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE reads half
+# of the bytes of that workload's 256 MiB calibration copy, hence the factor 2).
+GRAV_TRAFFIC_BYTES_1M = 11.982e9
 PEAK_HBM_GBS = 8000.0
 
 
@@ -135,7 +139,8 @@ def bench_evrard(a) -> int:
     roofline = {"bound": "valu", "kernel": "k_grav (direct-sum gravity, FP32)",
                 "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
-                "traffic": None,
+                "traffic": GRAV_TRAFFIC_BYTES_1M if n == 1 << 20 else None,
+                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
                 "algorithmic": "20 flop x N(N-1) pairs per launch",
                 "avg_ms": 1e3 * grav_s}
 
