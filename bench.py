@@ -75,6 +75,8 @@ def main(argv=None) -> int:
     ap.add_argument("--rate", type=float, default=4.0, help="serve: arrivals per second")
     ap.add_argument("--run-steps", type=int, default=200, help="serve: steps per request")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve: HBM budget")
+    ap.add_argument("--serve-runs", type=int, default=64,
+                    help="requests in the served-run latency sample of the default run (0: skip)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -202,6 +204,26 @@ def bench_evrard(a) -> int:
         print(f"[bench] cpu oracle {cpu['value']:.1f} particle-steps/s", file=sys.stderr,
               flush=True)
 
+    # ---------------- served-run sample: p50/p99 completion latency ----------
+    serving = None
+    if a.serve_runs > 0:
+        from sph_2_amd.serve import Server, build_payload, make_stream
+
+        reqs = make_stream(a.serve_runs, n_min=2048, n_max=65536, rate=8.0, steps=100,
+                           seed=a.seed)
+        for r in reqs:
+            r.payload = build_payload(r)
+        srv = Server(8 * 2 ** 30, max_runs=512, graph=True)
+        rep = srv.run(reqs)
+        srv.batch.close()
+        serving = {k: rep[k] for k in ("runs", "wall_s", "particle_steps_per_s", "p50_latency_s",
+                                       "p99_latency_s", "p50_queue_s", "p99_queue_s")}
+        serving["config"] = (f"{a.serve_runs} Evrard/cloud runs, N log-uniform in [2048, 65536], "
+                             f"100 steps each, Poisson arrivals at 8/s, continuous batching with "
+                             f"graph replay")
+        print(f"[bench] serving p50 {rep['p50_latency_s']:.2f}s p99 {rep['p99_latency_s']:.2f}s",
+              file=sys.stderr, flush=True)
+
     out = {
         "metric": "particle-steps/s at N=1M (mixed vs FP64 plain); p50/p99 latency batched runs",
         "value": value,
@@ -225,6 +247,7 @@ def bench_evrard(a) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_fp64_oracle": parity,
+        "serving": serving,
     }
     print(json.dumps(out))
     return 0

@@ -80,3 +80,31 @@ def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
         with open(_CACHE, "w") as f:
             json.dump(tab, f, indent=1)
     return best, {f"{k[0]},{k[1]},{k[2]}": v for k, v in res.items()}
+
+
+def main(argv=None) -> int:
+    """Sweep the gravity shapes for several run sizes on the device and write
+    the winners (per size class) to --out; copy that file next to this module
+    to make them the defaults."""
+    import argparse
+    import sys
+
+    ap = argparse.ArgumentParser(description=main.__doc__)
+    ap.add_argument("--sizes", type=int, nargs="+", default=[32768, 131072, 1048576])
+    ap.add_argument("--out", required=True)
+    a = ap.parse_args(argv)
+    tab = {}
+    for n in a.sizes:
+        best, res = tune_gravity(n, write=False)
+        tab[_size_class(n)] = list(best)
+        ranked = sorted(res.items(), key=lambda kv: kv[1])[:4]
+        print(f"[tune] n={n} best={best} " +
+              " ".join(f"{k}:{1e3 * v:.2f}ms" for k, v in ranked), file=sys.stderr, flush=True)
+    with open(a.out, "w") as f:
+        json.dump(tab, f, indent=1)
+    print(json.dumps(tab))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
