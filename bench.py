@@ -217,7 +217,7 @@ def bench_evrard(a) -> int:
         rep = srv.run(reqs)
         srv.batch.close()
         serving = {k: rep[k] for k in ("runs", "wall_s", "particle_steps_per_s", "p50_latency_s",
-                                       "p99_latency_s", "p50_queue_s", "p99_queue_s")}
+                                       "p99_latency_s", "p50_queue_s", "p99_queue_s", "host_s")}
         serving["config"] = (f"{a.serve_runs} Evrard/cloud runs, N log-uniform in [2048, 65536], "
                              f"100 steps each, Poisson arrivals at 8/s, continuous batching with "
                              f"graph replay")

@@ -220,11 +220,14 @@ class Server:
         busy = 0.0
         loops = 0
         psteps = 0
+        t_admit = t_step = t_retire = 0.0
         while pending or live:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 req = pending[0]
+                ta0 = time.perf_counter()
                 rid = self.batch.admit(req.payload, req.dt, cooling=req.kind == "cloud")
+                t_admit += time.perf_counter() - ta0
                 if rid is None:
                     break               # first come, first served: wait for room
                 pending.popleft()
@@ -235,6 +238,8 @@ class Server:
                 continue
             tb = time.perf_counter()
             self.batch.step(1)
+            torch.cuda.synchronize()
+            t_step += time.perf_counter() - tb
             loops += 1
             finished = []
             for rid, (req, ta, s) in list(live.items()):
@@ -245,7 +250,9 @@ class Server:
                     finished.append(rid)
             for rid in finished:
                 req, ta, s = live.pop(rid)
+                tr0 = time.perf_counter()
                 out = self.batch.retire(rid)      # synchronous copy-out
+                t_retire += time.perf_counter() - tr0
                 tc = time.perf_counter() - t0
                 done[req.rid] = {"arrival": req.arrival, "admitted": ta, "completed": tc,
                                  "n": req.n, "kind": req.kind}
@@ -265,7 +272,8 @@ class Server:
                "p99_latency_s": float(np.percentile(lat, 99)),
                "p50_queue_s": float(np.percentile(queue, 50)),
                "p99_queue_s": float(np.percentile(queue, 99)),
-               "capacity_particles": self.capacity}
+               "capacity_particles": self.capacity,
+               "host_s": {"admit": t_admit, "step": t_step, "retire": t_retire}}
         if keep_results:
             rep["results"] = results
             rep["done"] = done

@@ -151,6 +151,25 @@ def test_gravity(gpu, evr, mode):
     assert rel_rms(pot, pot_o) < tol
 
 
+@pytest.mark.parametrize("cfg", [(1, 256, 1), (1, 512, 3), (2, 256, 16), (2, 512, 8),
+                                 (2, 1024, 2), (4, 256, 4), (4, 512, 1)])
+def test_gravity_launch_shapes(gpu, evr, cfg):
+    """Every launch shape the tuner may pick (particles per lane, j-tile, j
+    split with partial sums) gives the same forces and potential."""
+    from sph_2_amd import _lib, libsx
+
+    acc_o, pot_o = oracle.gravity(evr["pos"], evr["m"], evr["h"], 1.0, potential=True)
+    ctx = libsx._context(0)
+    _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *cfg), "config")
+    try:
+        acc, pot = libsx.getAcc_g_smth(evr["pos"], evr["m"], 1.0, evr["h"], return_potential=True)
+    finally:
+        from sph_2_amd.autotune import _default
+        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(N)), "config")
+    assert rel_rms(acc, acc_o) < 1e-5
+    assert rel_rms(pot, pot_o) < 1e-5
+
+
 def test_gravity_rows_and_fixed_eps(gpu, evr):
     from sph_2_amd import libsx
 
