@@ -212,6 +212,14 @@ int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, const double *
                      const double *m, const double *u, const double *eps, double G,
                      sphx_diag *out, void *stream);
 
+/* do_smoothingX((pos, pos[row_beg:row_end])) (LIBSX:762-783; AUG:305-326):
+ * h = 0.5 x the kth smallest distance (0-based, self included; the reference
+ * uses kth = 64) from each row to all n particles, exact in FP64 (bit-equal
+ * to sorting the distances).  h_guess (device, n) seeds the search radius
+ * only.  Returns SPHX_CAPPED with NaN rows if a row's search did not resolve. */
+int sphx_knn_h(sphx_ctx *ctx, int64_t n, const double *pos, const double *h_guess, int32_t kth,
+               int64_t row_beg, int64_t row_end, double *h_out, void *stream);
+
 /* Global adaptive time step: dt_array (LIBV2:491-522) plus the driver's host
  * reduction (DRV2:517-561), done on the device without the O(N^2) v_sig loop
  * or the six host copies.  acc = total, acc_g = gravitational acceleration

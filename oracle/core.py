@@ -61,6 +61,7 @@ def lib():
         L.orc_cool_grid_apply.argtypes = [i64, dp, dp, C.c_double, i64, dp, i64, dp, dp,
                                           C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.orc_cool_grid_build.argtypes = [i64, dp, i64, dp, C.c_double, C.c_void_p, dp, dp, dp]
+        L.orc_knn_h.argtypes = [i64, dp, i64, lp, i64, dp]
         L.orc_global_dt.restype = C.c_double
         L.orc_global_dt.argtypes = [i64, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         _LIB = L
@@ -303,6 +304,18 @@ def pe(pos, m, eps, G=1.0):
     m, mp = _d(m)
     e, ep = _d(np.broadcast_to(eps, (n,)))
     return lib().orc_pe(n, pp, mp, ep, G)
+
+
+def knn_h(pos, rows=None, kth=64):
+    """do_smoothingX((pos, pos[rows])) (LIBSX:762-783): 0.5 x the kth smallest
+    distance (0-based, self included) to all particles."""
+    n = len(pos)
+    pos, pp = _d(pos)
+    r, rp, nr = _rows(rows, n)
+    out = np.zeros(nr)
+    out, op = _d(out)
+    lib().orc_knn_h(n, pp, kth, rp, nr, op)
+    return out
 
 
 def cool_grid_apply(rho, u, dt, u_nodes, rho_nodes, du, dt_ref_s, unit_u_cgs, unit_rho_cgs,

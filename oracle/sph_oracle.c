@@ -622,3 +622,52 @@ void orc_cool_grid_build(int64_t nu, const double *u_nodes, int64_t nrho, const 
             du[j * nrho + i] = (u_code - un) * cp->unit_u_cgs;
         }
 }
+
+/* kNN smoothing length of do_smoothingX((pos, subpos)) (LIBSX:762-783,
+ * AUG:305-326): h = 0.5 * (the kth smallest distance, 0-based, self included;
+ * the reference takes np.sort(dist)[64]) from each target row to all n
+ * particles, distances sqrt((dx^2 + dy^2) + dz^2) with dx = pos_j - pos_i.
+ * sqrt is monotonic, so this is 0.5 * sqrt(kth smallest r^2); found with a
+ * quickselect on r^2. */
+static double orc_kth_smallest(double *v, int64_t n, int64_t k) {
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        double piv = v[lo + (hi - lo) / 2];
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (v[i] < piv) ++i;
+            while (v[j] > piv) --j;
+            if (i <= j) {
+                double t = v[i];
+                v[i] = v[j];
+                v[j] = t;
+                ++i;
+                --j;
+            }
+        }
+        if (k <= j) hi = j;
+        else if (k >= i) lo = i;
+        else return v[k];
+    }
+    return v[k];
+}
+
+void orc_knn_h(int64_t n, const double *pos, int64_t kth, const int64_t *rows, int64_t nrows,
+               double *h_out) {
+#pragma omp parallel
+    {
+        double *r2 = (double *)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t t = 0; t < nrows; ++t) {
+            int64_t i = rows ? rows[t] : t;
+            for (int64_t j = 0; j < n; ++j) {
+                double dx = pos[3 * j] - pos[3 * i], dy = pos[3 * j + 1] - pos[3 * i + 1],
+                       dz = pos[3 * j + 2] - pos[3 * i + 2];
+                double a = dx * dx, b = dy * dy, c = dz * dz;
+                r2[j] = (a + b) + c;
+            }
+            h_out[t] = 0.5 * sqrt(orc_kth_smallest(r2, n, kth));
+        }
+        free(r2);
+    }
+}

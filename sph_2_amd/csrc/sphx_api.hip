@@ -705,6 +705,43 @@ extern "C" int sphx_smoothing(sphx_ctx *ctx, int mode, int64_t n, const double *
     return c.h_capped > 0 ? SPHX_CAPPED : SPHX_OK;
 }
 
+extern "C" int sphx_knn_h(sphx_ctx *ctx, int64_t n, const double *pos, const double *h_guess,
+                          int32_t kth, int64_t row_beg, int64_t row_end, double *h_out,
+                          void *stream) {
+    CTX_PROLOGUE(ctx);
+    SPHX_CHECK(check_rows(n, row_beg, row_end));
+    if (n == 0 || row_end == row_beg) return SPHX_OK;
+    if (!pos || !h_guess || !h_out) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
+    if (kth < 0 || kth >= n || kth > 4096) {
+        sphx_set_error("kth must be in [0, min(n, 4097))");
+        return SPHX_ERR_ARG;
+    }
+    Engine &E = ctx->E;
+    sphx_params p;
+    sphx_default_params(&p);
+    // count band [kth + 1, kth + 56] around a target of kth + 16: the search
+    // radius then holds at most 120 values, which the selection keeps in LDS
+    p.nngb = kth + 16;
+    p.ndown = kth + 1;
+    p.nup = kth + 56;
+    p.h_predictor = 0;
+    E.p = p;
+    SPHX_CHECK(E.reserve(std::max<int64_t>(n, 64)));
+    SPHX_CHECK(sphx_zero_counters(E.counters, st));
+    SPHX_CHECK(ctx_load(ctx, n, pos, nullptr, h_guess, nullptr, true, st));
+    E.run_params.assign(1, RunParams{0.0, 0, p.eps_from_h});
+    SPHX_CHECK(E.upload_run_params(st));
+    SPHX_CHECK(E.build_boxes(st, false));
+    SPHX_CHECK(E.smoothing(st, false));
+    SPHX_CHECK(sphx_zero_counters(E.counters, st));
+    SPHX_CHECK(sphx_knn_pass(E, kth, st));
+    SPHX_CHECK(scatter_out(E, E.s.h, row_beg, row_end, h_out, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    sphx_counters c{};
+    SPHX_CHECK(E.read_counters(&c));
+    return c.h_capped > 0 ? SPHX_CAPPED : SPHX_OK;
+}
+
 extern "C" int sphx_div_curlVel(sphx_ctx *ctx, int mode, int64_t n, const double *pos,
                                 const double *v, const double *rho, const double *m,
                                 const double *h, int64_t row_beg, int64_t row_end, double *divV,

@@ -555,6 +555,128 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
     }
 }
 
+// kNN smoothing length (do_smoothingX, LIBSX:762-783): h = 0.5 sqrt(v) with v
+// the kth smallest r^2 (0-based, self included) from particle i to all
+// particles of its run.  Each lane keeps a bracket: lo2 (exclusive, klo
+// values <= lo2), up2 (a value with >= kth + 1 values <= it, once known) and
+// the probe of the next walk.  A walk counts the values in (lo2, probe] and
+// keeps up to KNN_CAP of them in LDS; when the rank falls inside and they
+// fit, the value of that rank is selected exactly; otherwise the bracket
+// grows (too few) or is bisected in value (too many).  The first probe is the
+// radius of a count iteration whose band is [kth + 1, kth + 56], so one walk
+// usually suffices.
+constexpr int KNN_CAP = 128;
+
+struct KnnArgs {
+    int32_t kth;
+    int32_t max_walks;
+    int64_t *counters;  // lanes left unresolved -> CNT_H_CAPPED
+};
+
+__global__ __launch_bounds__(64) void k_knn(Tables t, Soa s, KnnArgs a) {
+    __shared__ double sx[64], sy[64], sz[64];
+    __shared__ double lst[KNN_CAP * 64];
+    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
+    int lane = threadIdx.x;
+    int cnt = t.b_cnt[b];
+    int i = t.b_beg[b] + lane;
+    bool ok = lane < cnt;
+    double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
+    const int r = t.b_run[b];
+    const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
+    double lo2 = -1.0, up2 = INFINITY;
+    double probe = ok ? sphx_sq(2.0 * s.h[i]) : 0.0;
+    int klo = 0;
+    bool done = !ok;
+    double res = 0.0;
+    for (int walk = 0; walk < a.max_walks && __any(!done); ++walk) {
+        int cin = 0;
+        for (int g0 = 0; g0 < cnt;) {
+            const int g1 = seg_end(cuts, g0, cnt);
+            const bool mine = lane >= g0 && lane < g1;
+            const bool act = mine && !done;
+            g0 = g1;
+            // a probe of 0 is legitimate (coincident particles): walk whenever
+            // a lane of the segment is active; reach 0 still finds the
+            // buckets whose box contains the particle
+            if (!__any(act)) continue;
+            double P = wave_max_d(act ? probe : 0.0);
+            double R = sqrt(P);
+            double gb[6], hdummy;
+            seg_box(ok && mine, xi, yi, zi, 0.0, gb, &hdummy);
+            for_candidates(
+                t, r, gb, lane, [&](int) { return R; }, [&](int) { return R; },
+                [&](int J) {
+                    int jc = t.b_cnt[J], j0 = t.b_beg[J];
+                    double xj = 0.0, yj = 0.0, zj = 0.0;
+                    bool keep = false;
+                    if (lane < jc) {
+                        xj = s.x[j0 + lane];
+                        yj = s.y[j0 + lane];
+                        zj = s.z[j0 + lane];
+                        keep = near_box(gb, xj, yj, zj, R);
+                    }
+                    Kept k = keep_mask(keep, lane);
+                    if (k.n == 0) return;
+                    __syncthreads();
+                    if (keep) {
+                        sx[k.slot] = xj;
+                        sy[k.slot] = yj;
+                        sz[k.slot] = zj;
+                    }
+                    __syncthreads();
+                    for (int q = 0; q < k.n; ++q) {
+                        double v = sphx_r2(sx[q] - xi, sy[q] - yi, sz[q] - zi);
+                        if (act && v > lo2 && v <= probe) {
+                            if (cin < KNN_CAP) lst[cin * 64 + lane] = v;
+                            ++cin;
+                        }
+                    }
+                });
+        }
+        if (!done) {
+            int have = klo + cin;
+            if (have <= a.kth) {
+                // too few values up to the probe: move the bracket up
+                lo2 = probe;
+                klo = have;
+                probe = up2 < INFINITY ? up2 : 4.0 * probe;
+            } else if (cin <= KNN_CAP) {
+                int rk = a.kth - klo;
+                for (int c = 0; c < cin; ++c) {
+                    double v = lst[c * 64 + lane];
+                    int less = 0, leq = 0;
+                    for (int d = 0; d < cin; ++d) {
+                        double w = lst[d * 64 + lane];
+                        less += w < v ? 1 : 0;
+                        leq += w <= v ? 1 : 0;
+                    }
+                    if (less <= rk && rk < leq) res = v;
+                }
+                done = true;
+            } else {
+                // too many to keep: bisect the bracket in the ordering of
+                // the IEEE bit patterns (monotonic for values >= 0), so ties
+                // and tiny values resolve within 64 halvings
+                up2 = probe;
+                unsigned long long lb = (unsigned long long)__double_as_longlong(lo2 < 0.0 ? 0.0 : lo2);
+                unsigned long long pb = (unsigned long long)__double_as_longlong(probe);
+                double mid = __longlong_as_double((long long)(lb + (pb - lb) / 2));
+                if (mid > lo2 && mid < probe) {
+                    probe = mid;
+                } else {
+                    res = probe;  // every value left in the bracket is the same
+                    done = true;
+                }
+            }
+        }
+    }
+    if (ok) {
+        if (!done) atomicAdd((unsigned long long *)&a.counters[CNT_H_CAPPED], 1ull);
+        s.h[i] = done ? 0.5 * sqrt(res) : NAN;
+    }
+}
+
 // Neighbour lists under r_ij^2 <= (h_i + h_j)^2 through the bucket culling.
 // counts == nullptr: fill mode, writing the neighbours' caller ids (s.id) at
 // offsets[id_i - r0] for targets with r0 <= id_i < r1.
@@ -604,6 +726,14 @@ __global__ __launch_bounds__(64) void k_nbr(Tables t, Soa s, int64_t r0, int64_t
 }
 
 }  // namespace
+
+int sphx_knn_pass(Engine &E, int kth, hipStream_t st) {
+    if (E.n == 0) return SPHX_OK;
+    KnnArgs a{kth, 200, E.counters};
+    k_knn<<<E.t.nb, 64, 0, st>>>(E.tview(), E.s, a);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
 
 int sphx_neighbor_pass(Engine &E, int64_t r0, int64_t r1, int32_t *counts, const int64_t *offsets,
                        int32_t *idx, hipStream_t st) {

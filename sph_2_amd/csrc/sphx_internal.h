@@ -364,6 +364,10 @@ struct Engine {
     int diagnostics(hipStream_t st, bool with_pe, sphx_diag *out);
 };
 
+// kNN smoothing lengths into E.s.h (sorted order), from the search radius 2 h
+// already in E.s.h (sphx_hydro.hip)
+int sphx_knn_pass(Engine &E, int kth, hipStream_t st);
+
 // Exact checkpoint of one resident run (sphx_io.hip)
 #define SPHX_CKPT_MAGIC "SPHXRUN1"
 #define SPHX_CKPT_VERSION 1

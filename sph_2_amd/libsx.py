@@ -359,6 +359,68 @@ def smoothing_h(pos, h, *, mode="mixed", device=0, **pkw):
     return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)
 
 
+def do_smoothingX(poz, *, kth=64, device=0):
+    """do_smoothingX((pos, subpos)) (LIBSX:762-783; AUG:305-326): h = 0.5 x
+    np.sort(distances to all of pos)[kth] for each row of subpos, exact.
+    subpos must be pos itself or a block of its rows (every call in the
+    reference passes (pos, pos))."""
+    pos_in, sub = poz
+    pos = _f64(pos_in)
+    sub = np.asarray(sub)
+    n = len(pos)
+    if sub is pos_in or (sub.shape == pos.shape and np.array_equal(sub, pos)):
+        r0, r1 = 0, n
+    else:
+        off = _row_offset(np.asarray(pos_in), sub)
+        if off is None:
+            raise ValueError("do_smoothingX: subpos must be pos or a contiguous block of its rows")
+        r0, r1 = off, off + len(sub)
+    t = _t()
+    lo, hi = pos.min(0), pos.max(0)
+    vol = float(np.prod(np.maximum(hi - lo, 1e-300)))
+    h0 = 0.5 * (vol * (kth + 1) / (n * 4.18879020478639)) ** (1.0 / 3.0)
+    dp = dev(pos)
+    dh = dev(np.full(n, h0))
+    out = t.empty(r1 - r0, dtype=t.float64, device="cuda")
+    rc = check(_lib.load().sphx_knn_h(_context(device), n, ptr(dp), ptr(dh), int(kth), r0, r1,
+                                      ptr(out), _lib.stream_handle()), "do_smoothingX")
+    if rc == _lib.CAPPED:
+        warnings.warn("kNN search did not resolve some rows (NaN)", RuntimeWarning)
+    return out.cpu().numpy()
+
+
+def _row_offset(pos, sub):
+    """Row offset of sub inside pos when sub is a contiguous block of pos's rows."""
+    if sub.ndim != 2 or sub.shape[1] != 3 or len(sub) > len(pos):
+        return None
+    if np.shares_memory(pos, sub) and sub.strides == pos.strides:
+        d = sub.__array_interface__["data"][0] - pos.__array_interface__["data"][0]
+        if d % pos.strides[0] == 0:
+            off = d // pos.strides[0]
+            if 0 <= off <= len(pos) - len(sub):
+                return int(off)
+    return None
+
+
+def do_smoothingX4(pos, h, *, mode="mixed", device=0, **pkw):
+    """do_smoothingX4(pos, h) (LIBSX:887-939): h in the 59..69 neighbour band,
+    started at 2h.  The reference steps the radius by +-0.01 x 2h; the band
+    and start are kept, the update is the canonical bracketed iteration."""
+    return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)[0]
+
+
+def h_smooth_fast(pos, h, *, mode="mixed", device=0, **pkw):
+    """h_smooth_fast(pos, h) (LIBSX:1131-1184): h with 59..69 neighbours within
+    2h.  The reference multiplies h by 1 -+ 0.003 per count; the band is kept,
+    the update is the canonical bracketed iteration."""
+    return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)[0]
+
+
+def getAcc_g_smthx(pos, mass, G, epsilon, **kw):
+    """getAcc_g_smthx (LIBSX:720-756): the same softened sum as getAcc_g_smth."""
+    return getAcc_g_smth(pos, mass, G, epsilon, **kw)
+
+
 def neighbors(pos, h, *, device=0, nbeg=None, nend=None):
     """Neighbour lists under r_ij <= h_i + h_j (CSR offsets, sorted indices per row)."""
     t = _t()

@@ -539,3 +539,29 @@ def test_cool_grid_reproduces_the_solve_at_its_nodes():
     want, _ = oracle.cool(rn[i] / units["unit_rho_cgs"], un[j] / units["unit_u_cgs"], dt, cp)
     want = np.maximum(want, un[0] / units["unit_u_cgs"])
     assert np.max(np.abs(got / want - 1)) < 1e-9
+
+
+# ---------------------------------------------------------------------------
+# kNN smoothing length (do_smoothingX, LIBSX:762-783)
+# ---------------------------------------------------------------------------
+def test_knn_h_equals_partition_of_distances():
+    rng = np.random.default_rng(12)
+    pos = rng.random((1500, 3)) ** 2           # uneven density
+    h = oracle.knn_h(pos)
+    d2 = ((pos[None, :, :] - pos[:, None, :]) ** 2).sum(-1)
+    ref = 0.5 * np.sqrt(np.partition(d2, 64, axis=1)[:, 64])
+    assert np.array_equal(h, ref)
+    rows = np.arange(100, 400)
+    assert np.array_equal(oracle.knn_h(pos, rows=rows), ref[rows])
+    assert np.array_equal(oracle.knn_h(pos, kth=10), 0.5 * np.sqrt(np.partition(d2, 10, axis=1)[:, 10]))
+
+
+def test_knn_h_cubic_lattice_shells():
+    """Interior lattice point: shells 0, 1, sqrt2, sqrt3, 2, sqrt5 hold 1+6+12+8+6+24 = 57
+    points, so rank 64 lies on the sqrt6 shell: h = sqrt(6) a / 2."""
+    a = 0.1
+    g = np.arange(12) * a
+    pos = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    centre = np.where(np.all(np.abs(pos - 0.55) < 0.06, axis=1))[0]
+    h = oracle.knn_h(pos, rows=centre)
+    assert np.allclose(h, 0.5 * np.sqrt(6.0) * a, rtol=1e-14)
