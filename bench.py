@@ -57,7 +57,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="evrard",
-                    choices=["evrard", "c2", "c3", "bh", "serve", "kernels"])
+                    choices=["evrard", "c2", "c3", "bh", "serve", "kernels", "aug"])
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None)
@@ -91,7 +91,7 @@ def main(argv=None) -> int:
         return 1
     if a.workload != "evrard":
         return {"c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve,
-                "kernels": bench_kernels}[a.workload](a)
+                "kernels": bench_kernels, "aug": bench_aug}[a.workload](a)
     if a.n is None:
         a.n = 1_048_576
     return bench_evrard(a)
@@ -437,6 +437,68 @@ def bench_serve(a) -> int:
                        f"{a.run_steps} steps each, Poisson arrivals at {a.rate}/s, "
                        f"HBM budget {a.hbm_gb} GB, graph={bool(a.graph)}")
     print(json.dumps(rep))
+    return 0
+
+
+def aug_run(n_steps, mode="mixed", log_every=0):
+    """The reference's own Python Evrard run that wrote Evrard_collapse/Energy.pkl
+    (augsphx6.5_Monagh_visc.py, the Monaghan-Gingold variant of augsphx6.5.py:406-567) on
+    this engine: its N = 1472 IC (float32 positions), v = 0, u = 0.05, m = 1/N, softening
+    0.05, kNN h each step (do_smoothingX), alpha = 1, beta = 2, eta = 0.1, dt = 1e-3.
+    Returns per step (U, KE at the half-kicked velocity, PE), as Energy.pkl records them."""
+    from sph_2_amd import ic as _ic, libsx
+    from sph_2_amd.plan import SphPlan
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    d = np.load(os.path.join(here, "tests", "golden", "evrard_1472_ic.npz"))
+    pos = np.stack([d["x"], d["y"], d["z"]], 1).astype(np.float64)
+    n = len(pos)
+    m = np.full(n, 1.0 / n)
+    u = np.full(n, 0.05)
+    eps = np.full(n, 0.05)
+    dt = 1e-3
+    h0 = libsx.do_smoothingX((pos, pos))
+    pl = SphPlan(n, mode=mode, h_mode=1, knn_kth=64, av_form=0, balsara=0, h_predictor=0,
+                 eps_from_h=0, alpha=1.0, beta=2.0, eta=0.1)
+    pl.load(pos, np.zeros((n, 3)), u, m, h0, eps)
+    pl.init()
+    out = []
+    for k in range(n_steps):
+        pl.step(1, dt)
+        st = pl.state(("pos", "vel", "u", "acc"))
+        vmid = st["vel"] - st["acc"] * (0.5 * dt)
+        ke = float(0.5 * (m * (vmid ** 2).sum(1)).sum())
+        pe = libsx.getPE(st["pos"], m, 1.0, eps)
+        out.append((float((m * st["u"]).sum()), ke, pe))
+        if log_every and (k + 1) % log_every == 0:
+            log(f"aug step {k + 1}")
+    pl.close()
+    return np.array(out)
+
+
+def bench_aug(a) -> int:
+    """Energy history of the reference's N = 1472 Python run against its recorded Energy.pkl."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    E = np.load(os.path.join(here, "tests", "golden", "evrard_1472_energy.npz"))
+    steps = min(a.steps if a.steps > 5 else 3000, len(E["U"]) - 1)
+    t0 = time.perf_counter()
+    hist = aug_run(steps, log_every=250)
+    wall = time.perf_counter() - t0
+    ref = np.stack([E["U"][1:steps + 1], E["KE"][1:steps + 1], E["PE"][1:steps + 1]], 1)
+    scale = np.abs(E["PE"][0])
+    res = {"workload": f"reference Python Evrard run (Energy.pkl): N=1472, {steps} steps at "
+                       f"dt=1e-3, kNN h, Monaghan-Gingold AV, eps=0.05, mixed precision",
+           "wall_s": wall, "steps": steps}
+    for tmax in (0.1, 0.5, 1.0, 2.0, 3.0):
+        k = min(steps, int(round(tmax / 1e-3)))
+        if k <= 0:
+            continue
+        dev = np.abs(hist[:k] - ref[:k])
+        res[f"t<={tmax}"] = {"U_rel": float((dev[:, 0] / np.abs(ref[:k, 0])).max()),
+                              "KE_over_|PE0|": float(dev[:, 1].max() / scale),
+                              "PE_rel": float((dev[:, 2] / np.abs(ref[:k, 2])).max())}
+    res["final"] = {"ours": hist[-1].tolist(), "reference": ref[-1].tolist()}
+    print(json.dumps(res))
     return 0
 
 

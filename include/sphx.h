@@ -63,6 +63,12 @@ typedef struct sphx_params {
     int32_t balsara;     /* 1: Balsara switch on Pi_ij (LIBV2:400-405) */
     int32_t eps_from_h;  /* 1: gravitational softening eps_i = h_i */
     int32_t h_predictor; /* 1: Gadget-2 eq. 31 start of the h iteration (LIBSX3:960) */
+    int32_t h_mode;      /* 0: neighbour-count band iteration; 1: kNN h = half the distance
+                            of rank knn_kth, as do_smoothingX (LIBSX:762-783) each step of
+                            augsphx6.5.py:510 (mixed precision only) */
+    int32_t knn_kth;     /* rank for h_mode 1 (64, self included at rank 0) */
+    double av_w_eps;     /* Gadget AV: w = v_ij.r_ij / (r + av_w_eps); 1e-5 in LIBSX:158-169
+                            and LIBV2, 1e-15 in augsphx6.5.py PI_ij */
 } sphx_params;
 
 /* Tabulated cooling: Lambda(T) on increasing T[ntab] (K) in erg cm^3 s^-1,

@@ -72,7 +72,7 @@ class _Params(C.Structure):
     _fields_ = [("gamma", C.c_double), ("alpha", C.c_double), ("beta", C.c_double),
                 ("eta", C.c_double), ("G", C.c_double), ("nngb", C.c_int32), ("ndown", C.c_int32),
                 ("nup", C.c_int32), ("max_iter", C.c_int32), ("av_form", C.c_int32),
-                ("balsara", C.c_int32)]
+                ("balsara", C.c_int32), ("av_w_eps", C.c_double)]
 
 
 class _CoolParams(C.Structure):
@@ -99,10 +99,13 @@ class Params:
     balsara: int = 0
     eps_from_h: bool = True   # softening eps_i = h_i
     h_predictor: bool = True  # Gadget-2 eq. 31 start for the h iteration
+    h_mode: int = 0           # 0 count-band iteration, 1 kNN (do_smoothingX)
+    knn_kth: int = 64
+    av_w_eps: float = 1e-5    # Gadget AV: w = v.r / (r + av_w_eps)
 
     def c(self) -> _Params:
         return _Params(self.gamma, self.alpha, self.beta, self.eta, self.G, self.nngb, self.ndown,
-                       self.nup, self.max_iter, self.av_form, self.balsara)
+                       self.nup, self.max_iter, self.av_form, self.balsara, self.av_w_eps)
 
 
 @dataclass
@@ -397,11 +400,16 @@ class State:
 
 
 def forces(s: State, p: Params, dt: float, use_pred: bool, cool_p: CoolParams | None = None):
-    if use_pred and p.h_predictor and s.k is not None and s.divV is not None:
-        h, k, it, st = smoothing(s.pos, s.h, s.k, s.divV, dt, p)
+    if p.h_mode == 1:
+        # kNN h each step, as augsphx6.5.py:510 (no iteration state)
+        s.h = knn_h(s.pos, kth=p.knn_kth)
+        s.k = np.full(len(s.h), p.knn_kth + 1, np.int32)
+        s.h_iters = np.zeros(len(s.h), np.int32)
+        s.h_status = np.zeros(len(s.h), np.int32)
+    elif use_pred and p.h_predictor and s.k is not None and s.divV is not None:
+        s.h, s.k, s.h_iters, s.h_status = smoothing(s.pos, s.h, s.k, s.divV, dt, p)
     else:
-        h, k, it, st = smoothing(s.pos, s.h, p=p)
-    s.h, s.k, s.h_iters, s.h_status = h, k, it, st
+        s.h, s.k, s.h_iters, s.h_status = smoothing(s.pos, s.h, p=p)
     s.rho, _ = density(s.pos, s.m, s.h)
     s.P, s.c = eos(s.rho, s.u, p.gamma)
     eps = s.h if p.eps_from_h else s.eps

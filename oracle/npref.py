@@ -54,7 +54,7 @@ def divcurl(pos, v, rho, m, h):
 
 
 def hydro(pos, v, rho, P, c, h, m, divV=None, curlV=None, gamma=5 / 3, alpha=1.0, beta=2.0,
-          eta=0.1, av_form=0, balsara=0):
+          eta=0.1, av_form=0, balsara=0, av_w_eps=1e-5):
     d, r2 = _pairs(pos)
     t = h[:, None] + h[None, :]
     nb = r2 <= t * t
@@ -68,7 +68,7 @@ def hydro(pos, v, rho, P, c, h, m, divV=None, curlV=None, gamma=5 / 3, alpha=1.0
         mu = hij * vr / (r2 + hij ** 2 * eta ** 2)
         pi = (-alpha * 0.5 * (c[:, None] + c[None, :]) * mu + beta * mu ** 2) / rhob
     else:
-        w = vr / (r + 1e-5)
+        w = vr / (r + av_w_eps)
         pi = -0.5 * alpha * (c[:, None] + c[None, :] - 3.0 * w) * w / rhob
     pi = np.where(vr <= 0, pi, 0.0)
     if balsara:

@@ -40,6 +40,7 @@ typedef struct {
     int32_t nngb, ndown, nup, max_iter;
     int32_t av_form; /* 0: Monaghan-Gingold 1983 (AUGX:103-139); 1: Gadget signal velocity (LIBSX:153-169) */
     int32_t balsara; /* 1: multiply Pi_ij by (f_i+f_j)/2 (LIBV2:400-405) */
+    double av_w_eps; /* Gadget form: w = v.r / (r + av_w_eps); 1e-5 LIBSX, 1e-15 augsphx6.5.py */
 } orc_params;
 
 typedef struct {
@@ -316,7 +317,7 @@ static inline double av_pi(const orc_params *p, double vr, double r2, double hij
         pi = (-p->alpha * cb * mu + p->beta * mu * mu) / rhob;
     } else {
         /* Gadget signal-velocity form (LIBSX:158-169) */
-        double w = vr / (sqrt(r2) + 1e-5);
+        double w = vr / (sqrt(r2) + p->av_w_eps);
         double vsig = ci + cj - 3.0 * w;
         pi = -0.5 * p->alpha * vsig * w / rhob;
     }

@@ -33,6 +33,9 @@ extern "C" void sphx_default_params(sphx_params *p) {
     p->balsara = 0;
     p->eps_from_h = 1;
     p->h_predictor = 1;
+    p->h_mode = 0;
+    p->knn_kth = 64;
+    p->av_w_eps = 1e-5;
 }
 
 namespace {
@@ -306,7 +309,20 @@ int Engine::tcollect() {
 int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
     SPHX_CHECK(tmark(st, PH_SMOOTH, 0));
     SPHX_CHECK(build_boxes(st, false));
-    SPHX_CHECK(smoothing(st, predictor));
+    if (p.h_mode == 1) {
+        // kNN h (do_smoothingX): a count iteration brackets the radius of rank
+        // knn_kth, then the selection pass makes it exact
+        sphx_params saved = p;
+        p.nngb = saved.knn_kth + 16;
+        p.ndown = saved.knn_kth + 1;
+        p.nup = saved.knn_kth + 56;
+        int rc = smoothing(st, false);
+        p = saved;
+        SPHX_CHECK(rc);
+        SPHX_CHECK(sphx_knn_pass(*this, p.knn_kth, st));
+    } else {
+        SPHX_CHECK(smoothing(st, predictor));
+    }
     SPHX_CHECK(tmark(st, PH_SMOOTH, 1));
     SPHX_CHECK(tmark(st, PH_DENS, 0));
     SPHX_CHECK(build_boxes(st, true));
@@ -331,6 +347,10 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
 }
 
 int Engine::plain_forces(hipStream_t st, bool predictor, bool with_pot) {
+    if (p.h_mode != 0) {
+        sphx_set_error("kNN smoothing lengths (h_mode 1) are a mixed-precision option");
+        return SPHX_ERR_ARG;
+    }
     bool pred = predictor && p.h_predictor;
     // the step's dt is read on the device (run 0's parameters), where the
     // adaptive time step leaves it
@@ -525,7 +545,9 @@ static int check_mode(int mode) {
 
 static int check_params(const sphx_params *p) {
     if (!p || p->nngb <= 0 || p->ndown > p->nngb || p->nup < p->nngb || p->max_iter <= 0 ||
-        !(p->gamma > 1.0) || (p->av_form != 0 && p->av_form != 1)) {
+        !(p->gamma > 1.0) || (p->av_form != 0 && p->av_form != 1) ||
+        (p->h_mode != 0 && p->h_mode != 1) || p->knn_kth < 0 || p->knn_kth > 4096 ||
+        !(p->av_w_eps >= 0.0)) {
         sphx_set_error("invalid sphx_params");
         return SPHX_ERR_ARG;
     }
@@ -1059,6 +1081,10 @@ extern "C" int sphx_plan_create(int device, int mode, const sphx_params *p,
     if (!out) { sphx_set_error("null output pointer"); return SPHX_ERR_ARG; }
     SPHX_CHECK(check_mode(mode));
     SPHX_CHECK(check_params(p));
+    if (mode == SPHX_FP64 && p->h_mode != 0) {
+        sphx_set_error("kNN smoothing lengths (h_mode 1) are a mixed-precision option");
+        return SPHX_ERR_ARG;
+    }
     if (capacity <= 0 || capacity > INT32_MAX) { sphx_set_error("bad capacity"); return SPHX_ERR_ARG; }
     int ndev = 0;
     SPHX_HIP(hipGetDeviceCount(&ndev));

@@ -445,7 +445,7 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
 }
 
 struct HydroArgs {
-    float alpha, beta, eta2;
+    float alpha, beta, eta2, w_eps;
     int32_t av_form, balsara;
 };
 
@@ -532,7 +532,7 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
                             float mu = hij * vr / (g.r2 + hij * hij * a.eta2);
                             pij = (-a.alpha * 0.5f * (ci + sc[q]) * mu + a.beta * mu * mu) / rhob;
                         } else {
-                            float w = vr / (__builtin_amdgcn_sqrtf(g.r2) + 1e-5f);
+                            float w = vr / (__builtin_amdgcn_sqrtf(g.r2) + a.w_eps);
                             pij = -0.5f * a.alpha * (ci + sc[q] - 3.0f * w) * w / rhob;
                         }
                         pij *= 0.5f * (fi + sf[q]);
@@ -729,8 +729,9 @@ __global__ __launch_bounds__(64) void k_nbr(Tables t, Soa s, int64_t r0, int64_t
 
 int sphx_knn_pass(Engine &E, int kth, hipStream_t st) {
     if (E.n == 0) return SPHX_OK;
+    if (E.nb_launch() == 0) return SPHX_OK;
     KnnArgs a{kth, 200, E.counters};
-    k_knn<<<E.t.nb, 64, 0, st>>>(E.tview(), E.s, a);
+    k_knn<<<E.nb_launch(), 64, 0, st>>>(E.tview(), E.s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
@@ -769,7 +770,8 @@ int Engine::divcurl(hipStream_t st) {
 
 int Engine::hydro(hipStream_t st) {
     if (n == 0 || nb_launch() == 0) return SPHX_OK;
-    HydroArgs a{(float)p.alpha, (float)p.beta, (float)(p.eta * p.eta), p.av_form, p.balsara};
+    HydroArgs a{(float)p.alpha, (float)p.beta, (float)(p.eta * p.eta), (float)p.av_w_eps,
+                p.av_form, p.balsara};
     k_hydro<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;

@@ -163,7 +163,7 @@ __global__ __launch_bounds__(PT) void kp_divcurl(
 }
 
 struct PlainAV {
-    double alpha, beta, eta;
+    double alpha, beta, eta, w_eps;
     int32_t av_form, balsara;
 };
 
@@ -182,7 +182,7 @@ __device__ __forceinline__ double pav(const PlainAV &p, double vr, double r2, do
         double mu = hij * vr / (r2 + hij * hij * p.eta * p.eta);
         pi = (-p.alpha * 0.5 * (ci + cj) * mu + p.beta * mu * mu) / rhob;
     } else {
-        double w = vr / (sqrt(r2) + 1e-5);
+        double w = vr / (sqrt(r2) + p.w_eps);
         pi = -0.5 * p.alpha * (ci + cj - 3.0 * w) * w / rhob;
     }
     if (p.balsara) pi *= 0.5 * (fi + fj);
@@ -363,7 +363,7 @@ int plain_divcurl(const double *x, const double *y, const double *z, const doubl
 int plain_acc_sph(const Soa &s, const sphx_params &p, int64_t n, int64_t r0, int64_t r1,
                   double *ax, double *ay, double *az, hipStream_t st) {
     if (r1 <= r0) return SPHX_OK;
-    PlainAV av{p.alpha, p.beta, p.eta, p.av_form, p.balsara};
+    PlainAV av{p.alpha, p.beta, p.eta, p.av_w_eps, p.av_form, p.balsara};
     kp_acc_sph<<<grid_rows(r0, r1), PT, 0, st>>>(s, av, n, r0, r1, ax, ay, az);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
@@ -372,7 +372,7 @@ int plain_acc_sph(const Soa &s, const sphx_params &p, int64_t n, int64_t r0, int
 int plain_du(const Soa &s, const sphx_params &p, int64_t n, int64_t r0, int64_t r1, double *du,
              hipStream_t st) {
     if (r1 <= r0) return SPHX_OK;
-    PlainAV av{p.alpha, p.beta, p.eta, p.av_form, p.balsara};
+    PlainAV av{p.alpha, p.beta, p.eta, p.av_w_eps, p.av_form, p.balsara};
     kp_du<<<grid_rows(r0, r1), PT, 0, st>>>(s, av, n, r0, r1, du);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
