@@ -57,7 +57,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="evrard",
-                    choices=["evrard", "c2", "c3", "bh", "serve", "kernels", "aug"])
+                    choices=["evrard", "c2", "c3", "bh", "serve", "kernels", "aug", "gadget"])
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None)
@@ -75,6 +75,8 @@ def main(argv=None) -> int:
     ap.add_argument("--rate", type=float, default=4.0, help="serve: arrivals per second")
     ap.add_argument("--run-steps", type=int, default=200, help="serve: steps per request")
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve: HBM budget")
+    ap.add_argument("--eps", type=float, default=None,
+                    help="gadget: fixed gravitational softening (default eps = h)")
     ap.add_argument("--serve-runs", type=int, default=64,
                     help="requests in the served-run latency sample of the default run (0: skip)")
     a = ap.parse_args(argv)
@@ -91,7 +93,7 @@ def main(argv=None) -> int:
         return 1
     if a.workload != "evrard":
         return {"c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve,
-                "kernels": bench_kernels, "aug": bench_aug}[a.workload](a)
+                "kernels": bench_kernels, "aug": bench_aug, "gadget": bench_gadget}[a.workload](a)
     if a.n is None:
         a.n = 1_048_576
     return bench_evrard(a)
@@ -499,6 +501,73 @@ def bench_aug(a) -> int:
                               "PE_rel": float((dev[:, 2] / np.abs(ref[:k, 2])).max())}
     res["final"] = {"ours": hist[-1].tolist(), "reference": ref[-1].tolist()}
     print(json.dumps(res))
+    return 0
+
+
+def bench_gadget(a) -> int:
+    """Evrard collapse to t = 0.8 with the canonical step and the device's adaptive dt, radial
+    profiles of rho, P and v_r against the Gadget-4 profiles the reference keeps
+    (Evrard_collapse/{rho,P,vr}Gadget.pkl, compared by its singlex.py:143-200)."""
+    import torch
+
+    from sph_2_amd import ic
+    from sph_2_amd.plan import SphPlan
+
+    here = os.path.dirname(os.path.abspath(__file__))
+    G4 = np.load(os.path.join(here, "tests", "golden", "gadget4_profiles.npz"))
+    n = a.n or 131072
+    e = ic.evrard(n, seed=a.seed)
+    fixed = a.eps is not None
+    pl = SphPlan(n, eps_from_h=0) if fixed else SphPlan(n)
+    pl.set_adaptive_dt(True, 0.25, 1e-5, 1e-3)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h, np.full(n, a.eps) if fixed else None)
+    pl.init()
+    t_end = 0.8
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        t, dt, _ = pl.time()
+        if t >= t_end - 1e-12:
+            break
+        pl.step(1, 1e-4)       # seeds the first step; later steps use the device's dt
+        steps += 1
+        if steps % 500 == 0:
+            log(f"gadget t={t:.3f} dt={dt:.2e} step {steps}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t, _, _ = pl.time()
+    st = pl.state(("pos", "vel", "u", "rho"))
+    pl.close()
+    r = np.linalg.norm(st["pos"], axis=1)
+    vr = (st["vel"] * st["pos"]).sum(1) / np.maximum(r, 1e-300)
+    P = (5.0 / 3.0 - 1.0) * st["rho"] * st["u"]
+    rg = G4["rho_r"]
+    edges = np.sqrt(rg[1:] * rg[:-1])
+    edges = np.concatenate([[rg[0] ** 2 / edges[0]], edges, [rg[-1] ** 2 / edges[-1]]])
+    k = np.digitize(r, edges) - 1
+    soft = f"eps={a.eps}" if fixed else "eps=h"
+    out = {"workload": f"Evrard N={n} to t={t:.4f} ({steps} adaptive steps), canonical step "
+                       f"(band h, Monaghan-Gingold AV, {soft}), vs Gadget-4 profiles",
+           "wall_s": wall, "t": t, "steps": steps}
+    for name, ours_v, ref in (("rho", st["rho"], G4["rho_rho"]), ("P", P, G4["P_P"]),
+                              ("vr", vr, G4["vr_vr"])):
+        rows = []
+        for b in range(len(rg)):
+            sel = k == b
+            if not np.isfinite(ref[b]) or sel.sum() < 8:
+                continue
+            rows.append((float(rg[b]), float(ours_v[sel].mean()), float(ref[b]), int(sel.sum())))
+        rows = np.array(rows)
+        if name == "vr":
+            dev = np.abs(rows[:, 1] - rows[:, 2])
+            out[name] = {"bins": len(rows), "mean_abs_diff": float(dev.mean()),
+                         "max_abs_diff": float(dev.max())}
+        else:
+            lr = np.abs(np.log(rows[:, 1] / rows[:, 2]))
+            out[name] = {"bins": len(rows), "median_abs_log_ratio": float(np.median(lr)),
+                         "max_abs_log_ratio": float(lr.max())}
+        out[name]["table_r_ours_gadget_count"] = rows.round(5).tolist()
+    print(json.dumps(out))
     return 0
 
 
