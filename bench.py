@@ -57,7 +57,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="evrard",
-                    choices=["evrard", "c2", "c3", "bh", "serve", "kernels", "aug", "gadget"])
+                    choices=["evrard", "c1", "c2", "c3", "bh", "serve", "kernels", "aug", "gadget"])
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None)
@@ -92,7 +92,7 @@ def main(argv=None) -> int:
         print(json.dumps({"error": "no HIP device"}))
         return 1
     if a.workload != "evrard":
-        return {"c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve,
+        return {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve,
                 "kernels": bench_kernels, "aug": bench_aug, "gadget": bench_gadget}[a.workload](a)
     if a.n is None:
         a.n = 1_048_576
@@ -282,6 +282,75 @@ def _plan_run(n_steps, e, mode, dt, cooling=None, record=(1,), graph=False):
     cnt = pl.counters()
     pl.close()
     return states, 1e3 * t / n_steps, diag, cnt
+
+
+def bench_c1(a) -> int:
+    """Evrard N=4096 seed 0: bootstrap + one KDK step timed in full on the FP64
+    CPU oracle (all host threads) and on the GPU (mixed and plain FP64), with the
+    GPU states compared against the oracle's after the bootstrap and the step.
+    With --n 131072 this is step 1 of the C2 configuration timed on the CPU."""
+    import torch
+
+    import oracle
+    from sph_2_amd import ic
+    from sph_2_amd.plan import SphPlan
+
+    n = a.n or 4096
+    e = ic.evrard(n, seed=a.seed)
+    p = oracle.Params()
+    cores = len(os.sched_getaffinity(0))
+    log(f"c1 Evrard N={n}: CPU oracle on {cores} threads")
+    t0 = time.perf_counter()
+    s0 = oracle.bootstrap(oracle.State(e.pos, e.vel, e.u, e.m, e.h), p)
+    t1 = time.perf_counter()
+    s1 = oracle.step(s0, a.dt, p)
+    t2 = time.perf_counter()
+    log(f"oracle bootstrap {t1 - t0:.3f} s, step {t2 - t1:.3f} s")
+
+    def rel_max(x, y):
+        return float(np.max(np.abs(x / y - 1)))
+
+    gpu = {}
+    for mode in ("mixed", "fp64"):
+        pl = SphPlan(n, mode=mode)
+        pl.load(e.pos, e.vel, e.u, e.m, e.h)
+        torch.cuda.synchronize()
+        g0 = time.perf_counter()
+        pl.init()
+        torch.cuda.synchronize()
+        g1 = time.perf_counter()
+        st0 = pl.state()
+        torch.cuda.synchronize()
+        g2 = time.perf_counter()
+        pl.step(1, a.dt)
+        torch.cuda.synchronize()
+        g3 = time.perf_counter()
+        st1 = pl.state()
+        pl.close()
+        gpu[mode] = {
+            "bootstrap_ms": 1e3 * (g1 - g0), "step_ms": 1e3 * (g3 - g2),
+            "particle_steps_per_s": n / (g3 - g2),
+            "bootstrap": {"h_identical": bool(np.array_equal(st0["h"], s0.h)),
+                          "density_rel_max": rel_max(st0["rho"], s0.rho),
+                          "acc_rel_rms": rel_rms(st0["acc"], s0.acc)},
+            "step1": {"k_mismatch_frac": float(np.mean(st1["k"] != s1.k)),
+                      "h_rel_max": rel_max(st1["h"], s1.h),
+                      "density_rel_max": rel_max(st1["rho"], s1.rho),
+                      "acc_rel_rms": rel_rms(st1["acc"], s1.acc),
+                      "acc_grav_rel_rms": rel_rms(st1["acc_g"], s1.acc_g),
+                      "dudt_rel_rms": rel_rms(st1["dudt"], s1.dudt),
+                      "u_rel_max": rel_max(st1["u"], s1.u),
+                      "pos_rel_rms": rel_rms(st1["pos"], s1.pos)},
+        }
+        log(f"{mode}: step {gpu[mode]['step_ms']:.2f} ms, "
+            f"acc rel rms {gpu[mode]['step1']['acc_rel_rms']:.2e}")
+    out = {"workload": f"c1: Evrard N={n} seed {a.seed}, bootstrap + one KDK step at dt={a.dt}",
+           "cpu_oracle": {"bootstrap_s": t1 - t0, "step_s": t2 - t1, "cores": cores,
+                          "cpu": cpu_model(), "particle_steps_per_s": n / (t2 - t1)},
+           "gpu": gpu,
+           "speedup_mixed_vs_cpu_step": (t2 - t1) * 1e3 / gpu["mixed"]["step_ms"]}
+    print(json.dumps(out))
+    return 0
 
 
 def bench_c2(a) -> int:

@@ -95,6 +95,25 @@ typedef struct sphx_cool_grid {
     double unit_u_cgs, unit_rho_cgs, unit_time_s;
 } sphx_cool_grid;
 
+/* 5-D CHIMES table (the hcoolingx method of GPU_sph_Type_cooling_II/
+ * hfvCLibs_v7.h:1077-1240, as its drivers hfvGpuSphTyp_v7.cu:35-79 and
+ * hfvGpuSph_mgpu_v5.cu load it from coolHeatGridZ.bin): DEVICE float32 arrays.
+ * Axes: kpc[n_kpc] (radius from the origin), nh[n_nh] (cm^-3), time_s[n_time]
+ * (s), all strictly increasing; u_evol and mu are [n_kpc][n_t][n_nh][n_z]
+ * [n_time], metalz [n_kpc][n_t][n_nh][n_z][n_m][n_time].  u_evol[.., 0] holds
+ * the node energies (erg/g).  z_index is the metallicity node used (the
+ * reference fixes 1), xh, m_h the hydrogen fraction and mass (0.70,
+ * 1.6726e-24), u_max_cgs the energy cap (2.0287e18, T = 1e10 K), kpc_cm the
+ * kpc in cm (3.086e21); code -> cgs factors for length, density, energy, time. */
+typedef struct sphx_chimes_table {
+    int32_t n_kpc, n_nh, n_z, n_t, n_m, n_time;
+    int32_t z_index, pad;
+    const float *kpc, *nh, *time_s;
+    const float *u_evol, *mu, *metalz;
+    double xh, m_h, u_max_cgs, kpc_cm;
+    double unit_length_cm, unit_rho_cgs, unit_u_cgs, unit_time_s;
+} sphx_chimes_table;
+
 typedef struct sphx_counters {
     int64_t n;              /* particles resident */
     int64_t steps;          /* steps taken since load */
@@ -212,6 +231,19 @@ int sphx_cool_grid_apply(sphx_ctx *ctx, int64_t n, const double *rho, const doub
 int sphx_cool_grid_build(sphx_ctx *ctx, const sphx_cooling *cool, const sphx_cool_grid *grid,
                          void *stream);
 
+/* hcoolingx (hfvCLibs_v7.h:1077-1240) on device arrays, in FP64 on the float32
+ * table: nearest kpc and n_H nodes (first node >= value, one down when the
+ * lower node is strictly closer, past the end the last), first T node whose
+ * energy (time index 0) is >= u and first time node >= dt (else the last),
+ * energy interpolated linearly in u between T nodes, capped at u_max_cgs.
+ * ion_out (n x n_m float32) and mu_out receive the species fractions and mu
+ * at the chosen nodes (both nullable; the reference kernel writes the
+ * fractions, its CPU twin hcooler in cooling_grids_use_EC2/checker1.py:53-146
+ * returns mu).  pos is (n, 3).  u_out may alias u_in. */
+int sphx_chimes_apply(sphx_ctx *ctx, int64_t n, const double *pos, const double *rho,
+                      const double *u_in, double dt, const sphx_chimes_table *table,
+                      double *u_out, float *ion_out, double *mu_out, void *stream);
+
 /* getKE (LIBSX:579-588), thermal energy sum m u, momentum and angular
  * momentum; pe only when pos/eps given (getPE, LIBSX:593-629).  Synchronous. */
 int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, const double *v,
@@ -257,7 +289,8 @@ int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, const double *
 int sphx_plan_init(sphx_plan *plan, void *stream);
 /* n_steps KDK steps at fixed dt.  use_cooling: 0 none, 1 the implicit solve
  * on the plan's Lambda(T) table, 2 the 2-D (u, rho) table of
- * sphx_plan_set_cooling_grid. */
+ * sphx_plan_set_cooling_grid, 3 the 5-D CHIMES table of
+ * sphx_plan_set_chimes_table. */
 int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32_t use_cooling,
                    void *stream);
 /* Copy state out in the original particle order; any pointer may be NULL.
@@ -306,6 +339,14 @@ int sphx_plan_set_gravity_bh(sphx_plan *plan, double theta);
  * 496-508, 600-609 apply it after the u update). */
 int sphx_plan_set_cooling_grid(sphx_plan *plan, const sphx_cool_grid *grid);
 
+/* Attach a 5-D CHIMES table (copied); runs stepped with use_cooling = 3 take
+ * their energy from the table after the u update, as hfvGpuSphTyp_v7.cu:
+ * 665-673 calls hcoolingx after u_updater.  The plan keeps each particle's
+ * species fractions of its last lookup; sphx_plan_chimes_ions copies them out
+ * (n x n_m float32, original particle order; zero before the first lookup). */
+int sphx_plan_set_chimes_table(sphx_plan *plan, const sphx_chimes_table *table);
+int sphx_plan_chimes_ions(sphx_plan *plan, float *out, void *stream);
+
 int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl, double dt_min,
                               double dt_max);
 int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts, void *stream);
@@ -325,7 +366,8 @@ int sphx_batch_create(int device, int mode, const sphx_params *p, const sphx_coo
                       int64_t capacity, int32_t max_runs, sphx_batch **out);
 /* Returns SPHX_REFUSED (2) without side effects when the particle capacity or
  * the run slots are exhausted.  Bootstraps the new run (forces at t = 0).
- * use_cooling as in sphx_plan_step (0 none, 1 solve, 2 2-D table). */
+ * use_cooling as in sphx_plan_step (0 none, 1 solve, 2 2-D table, 3 CHIMES
+ * table). */
 int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, const double *vel,
                      const double *u, const double *m, const double *h, const double *eps,
                      double dt, int32_t use_cooling, int64_t *run_id, void *stream);
@@ -344,6 +386,9 @@ int sphx_batch_runs(sphx_batch *b, int64_t *ids, int64_t *sizes, int64_t *steps,
  * counterpart in the reference, whose driver runs one simulation. */
 /* As sphx_plan_set_cooling_grid, for runs admitted with use_cooling = 2. */
 int sphx_batch_set_cooling_grid(sphx_batch *b, const sphx_cool_grid *grid);
+/* As sphx_plan_set_chimes_table, for runs admitted with use_cooling = 3 (the
+ * energy lookup only: batches keep no species fractions). */
+int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_table *table);
 int sphx_batch_set_adaptive_dt(sphx_batch *b, int64_t run_id, int32_t on, double c_cfl,
                                double dt_min, double dt_max, void *stream);
 int sphx_batch_time(sphx_batch *b, int64_t run_id, double *t, double *dt, double *dt_parts,
@@ -392,6 +437,18 @@ int sphx_write_snapshot_bin(const char *path, int64_t n, const int32_t *typ, con
 int sphx_read_snapshot_bin(const char *path, int64_t n_cap, int64_t *n_out, int32_t *typ,
                            double *pos, double *vel, double *rho, double *h, double *u,
                            double *m);
+
+/* coolHeatGridZ.bin, the CHIMES table file (written by GPU_sph_Type_cooling_II/
+ * cooling_grids_use_EC2/step_2x.py:101-140, read by hfvGpuSphTyp_v7.cu:35-79):
+ * int32 n_kpc, n_nh, n_z, n_t, n_m, n_time; float32 kpc, nh, z, temperatures,
+ * time_s; float32 u_evol, mu, metalz in the layouts of sphx_chimes_table.
+ * dims receives the six sizes in that order; with arrays null only dims is
+ * read.  HOST arrays sized from dims. */
+int sphx_read_chimes(const char *path, int32_t dims[6], float *kpc, float *nh, float *z,
+                     float *temp, float *time_s, float *u_evol, float *mu, float *metalz);
+int sphx_write_chimes(const char *path, const int32_t dims[6], const float *kpc,
+                      const float *nh, const float *z, const float *temp, const float *time_s,
+                      const float *u_evol, const float *mu, const float *metalz);
 
 /* Exact checkpoints (no counterpart in the reference, whose restart CSV makes
  * the driver recompute forces from rounded values): every per-particle array

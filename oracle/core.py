@@ -62,6 +62,9 @@ def lib():
                                           C.c_double, C.c_double, C.c_double, C.c_double, dp]
         L.orc_cool_grid_build.argtypes = [i64, dp, i64, dp, C.c_double, C.c_void_p, dp, dp, dp]
         L.orc_knn_h.argtypes = [i64, dp, i64, lp, i64, dp]
+        fp = C.POINTER(C.c_float)
+        L.orc_chimes_apply.argtypes = [i64, dp, dp, dp, C.c_double, C.c_void_p, fp, fp, fp, fp,
+                                       fp, fp, dp, fp, dp]
         L.orc_global_dt.restype = C.c_double
         L.orc_global_dt.argtypes = [i64, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         _LIB = L
@@ -351,6 +354,68 @@ def cool_grid_build(u_nodes, rho_nodes, dt_ref_s, cp: CoolParams):
     return du
 
 
+class _ChimesParams(C.Structure):
+    _fields_ = [("n_kpc", C.c_int32), ("n_nh", C.c_int32), ("n_z", C.c_int32),
+                ("n_t", C.c_int32), ("n_m", C.c_int32), ("n_time", C.c_int32),
+                ("z_index", C.c_int32), ("pad", C.c_int32), ("xh", C.c_double),
+                ("m_h", C.c_double), ("u_max_cgs", C.c_double), ("unit_length_cm", C.c_double),
+                ("kpc_cm", C.c_double), ("unit_rho_cgs", C.c_double), ("unit_u_cgs", C.c_double),
+                ("unit_time_s", C.c_double)]
+
+
+@dataclass
+class ChimesTable:
+    """The 5-D CHIMES table of the v7 GPU drivers (GPU_sph_Type_cooling_II/
+    hfvGpuSphTyp_v7.cu:35-79): float32 axes and arrays u_evol, mu of shape
+    (kpc, T, nH, Z, time) and metalz of shape (kpc, T, nH, Z, species, time),
+    with the lookup constants of hfvCLibs_v7.h:1077-1240 and the driver."""
+    kpc: np.ndarray
+    nh: np.ndarray
+    z: np.ndarray
+    temp: np.ndarray
+    time_s: np.ndarray
+    u_evol: np.ndarray
+    mu: np.ndarray
+    metalz: np.ndarray
+    unit_length_cm: float
+    unit_rho_cgs: float
+    unit_u_cgs: float
+    unit_time_s: float
+    z_index: int = 1
+    xh: float = 0.70
+    m_h: float = 1.6726e-24
+    u_max_cgs: float = 2.0287e18
+    kpc_cm: float = 3.086e21
+
+    def c(self) -> _ChimesParams:
+        nk, nt, nn, nz, nm, ntime = self.metalz.shape
+        return _ChimesParams(nk, nn, nz, nt, nm, ntime, self.z_index, 0, self.xh, self.m_h,
+                             self.u_max_cgs, self.unit_length_cm, self.kpc_cm, self.unit_rho_cgs,
+                             self.unit_u_cgs, self.unit_time_s)
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def chimes_apply(pos, rho, u, dt, tab: ChimesTable):
+    """hcoolingx (hfvCLibs_v7.h:1077-1240): returns (u_new, ion fractions
+    (n, n_species) float32, mu at the chosen node)."""
+    n = len(u)
+    pos, pp = _d(np.asarray(pos).reshape(n, 3))
+    rho, rp = _d(rho)
+    u, up = _d(u)
+    arrs = [_f(a) for a in (tab.kpc, tab.nh, tab.time_s, tab.u_evol, tab.mu, tab.metalz)]
+    nm = tab.metalz.shape[4]
+    out, op = _d(np.zeros(n))
+    ion, ip_ = _f(np.zeros((n, nm), np.float32))
+    mu, mp = _d(np.zeros(n))
+    c = tab.c()
+    lib().orc_chimes_apply(n, pp, rp, up, dt, C.byref(c), *[a[1] for a in arrs], op, ip_, mp)
+    return out, ion, mu
+
+
 def global_dt(h, c, acc, acc_g, divV, c_cfl=0.25, dt_min=1e-5, dt_max=1e-3):
     """Adaptive global dt of the GPU driver (myCppSPHLibs_v2.h:491-522 dt_array +
     hfv_sph_gpu_v2_evrard.cu:517-561).  Returns (dt, (dt_f, dt_kin, dt_cfl, dt_dens))."""
@@ -392,6 +457,7 @@ class State:
     ut_prev: np.ndarray | None = None
     h_iters: np.ndarray | None = None
     h_status: np.ndarray | None = None
+    ion: np.ndarray | None = None
     t: float = 0.0
 
     def copy(self):
@@ -428,10 +494,12 @@ def bootstrap(s: State, p: Params) -> State:
 
 
 def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None,
-         grid: dict | None = None) -> State:
+         grid: dict | None = None, chimes: "ChimesTable | None" = None) -> State:
     """One KDK step (hfv_sph_gpu_v2_evrard.cu:410-482).  grid (dict of
     cool_grid_apply's table arguments) cools by the 2-D table after the u
-    update, as hfv_sph_gpu_v3.cu:600-609 does."""
+    update, as hfv_sph_gpu_v3.cu:600-609 does; chimes looks the energy up in
+    the 5-D table after the u update, as hfvGpuSphTyp_v7.cu:665-673 does
+    (s.ion receives the species fractions)."""
     s = s.copy()
     s.vel = s.vel + s.acc * (0.5 * dt)
     s.pos = s.pos + s.vel * dt
@@ -443,6 +511,8 @@ def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None,
         s.u, _ = cool(s.rho, s.u, dt, cool_p)
     elif grid is not None:
         s.u = cool_grid_apply(s.rho, s.u, dt, **grid)
+    elif chimes is not None:
+        s.u, s.ion, _ = chimes_apply(s.pos, s.rho, s.u, dt, chimes)
     s.t = s.t + dt
     return s
 

@@ -672,3 +672,90 @@ void orc_knn_h(int64_t n, const double *pos, int64_t kth, const int64_t *rows, i
         free(r2);
     }
 }
+
+/* Lookup in the 5-D CHIMES table (hcoolingx, LIBV7:1077-1240; its CPU twin is
+ * hcooler in GPU_sph_Type_cooling_II/cooling_grids_use_EC2/checker1.py:53-146).
+ * The table holds, per (radius in kpc, T node, n_H node, metallicity node,
+ * elapsed time), the specific energy reached after that time from the node's
+ * initial energy, the mean molecular weight, and n_m species fractions;
+ * layouts [kpc][T][nH][Z][time] and [kpc][T][nH][Z][species][time], as the
+ * writer GPU_sph_Type_cooling_II/cooling_grids_use_EC2/step_2x.py lays them
+ * out.  Per particle, in FP64 on the float32 table values:
+ *   - radius r = |x| in kpc, n_H = rho_cgs X_H / m_H: the nearest node, found
+ *     as the reference does -- the first node >= the value, moved one down
+ *     when the node below is strictly closer; past the last node -> last;
+ *   - metallicity node fixed (the reference hard-codes index 1);
+ *   - U[t] = table energy at time index 0 of the column (the node energies);
+ *     j_u = first t with u_cgs <= U[t], else the last node;
+ *   - j_t = first time node >= dt in seconds, else the last;
+ *   - j_u > 0: linear interpolation in u between U[j_u-1] and U[j_u] of the
+ *     energies at j_t (extrapolating past the last node, as the reference);
+ *     j_u == 0: the node value;
+ *   - capped at u_max_cgs (T = 1e10 K in the reference), back to code units.
+ * ion_out (n x n_m) and mu_out receive the species fractions and mu at node
+ * j_u (nullable).  The reference stages U in one global buffer shared by all
+ * threads; here every particle reads its own column. */
+typedef struct {
+    int32_t n_kpc, n_nh, n_z, n_t, n_m, n_time;
+    int32_t z_index, pad;
+    double xh, m_h, u_max_cgs;
+    double unit_length_cm, kpc_cm, unit_rho_cgs, unit_u_cgs, unit_time_s;
+} orc_chimes_params;
+
+static int64_t orc_first_ge(const float *g, int64_t ng, int64_t stride, double x) {
+    for (int64_t k = 0; k < ng; ++k)
+        if (x <= (double)g[k * stride]) return k;
+    return -1;
+}
+
+static int64_t orc_nearest_node(const float *g, int64_t ng, double x) {
+    int64_t j = orc_first_ge(g, ng, 1, x);
+    if (j > 0) {
+        double d1 = fabs((double)g[j] - x), d2 = fabs((double)g[j - 1] - x);
+        if (d2 < d1) j -= 1;
+    }
+    return j < 0 ? ng - 1 : j;
+}
+
+void orc_chimes_apply(int64_t n, const double *pos, const double *rho, const double *u, double dt,
+                      const orc_chimes_params *cp, const float *kpc, const float *nh,
+                      const float *time_s, const float *u_evol, const float *mu,
+                      const float *metalz, double *u_out, float *ion_out, double *mu_out) {
+    const int64_t NT = cp->n_t, NH = cp->n_nh, NZ = cp->n_z, NM = cp->n_m, NTIME = cp->n_time;
+    const int64_t s_t = NH * NZ * NTIME, s_k = NT * s_t;  /* strides of [kpc][T][nH][Z][time] */
+    double dt_s = dt * cp->unit_time_s;
+    int64_t jt = orc_first_ge(time_s, NTIME, 1, dt_s);
+    if (jt < 0) jt = NTIME - 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+        double a = x * x, b = y * y, c = z * z;
+        double r = sqrt((a + b) + c);
+        double r_kpc = (r * cp->unit_length_cm) / cp->kpc_cm;
+        int64_t jk = orc_nearest_node(kpc, cp->n_kpc, r_kpc);
+        double nh_i = ((rho[i] * cp->unit_rho_cgs) * cp->xh) / cp->m_h;
+        int64_t jn = orc_nearest_node(nh, NH, nh_i);
+        int64_t jz = cp->z_index;
+        int64_t col = jk * s_k + jn * NZ * NTIME + jz * NTIME;  /* + T * s_t + time */
+        double u_i = u[i] * cp->unit_u_cgs;
+        int64_t ju = orc_first_ge(u_evol + col, NT, s_t, u_i);
+        if (ju < 0) ju = NT - 1;
+        double uev;
+        if (ju > 0) {
+            double U0 = u_evol[col + (ju - 1) * s_t], U1 = u_evol[col + ju * s_t];
+            double e0 = u_evol[col + (ju - 1) * s_t + jt], e1 = u_evol[col + ju * s_t + jt];
+            double fhi = (u_i - U0) / (U1 - U0);
+            double flo = 1.0 - fhi;
+            uev = flo * e0 + fhi * e1;
+        } else {
+            uev = u_evol[col + jt];
+        }
+        if (uev > cp->u_max_cgs) uev = cp->u_max_cgs;
+        u_out[i] = uev / cp->unit_u_cgs;
+        if (mu_out) mu_out[i] = mu[col + ju * s_t + jt];
+        if (ion_out) {
+            int64_t mcol = ((jk * NT + ju) * NH + jn) * NZ * NM * NTIME + jz * NM * NTIME + jt;
+            for (int64_t k = 0; k < NM; ++k) ion_out[i * NM + k] = metalz[mcol + k * NTIME];
+        }
+    }
+}

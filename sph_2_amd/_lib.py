@@ -73,6 +73,18 @@ class CoolGrid(C.Structure):
                 ("unit_time_s", C.c_double)]
 
 
+class ChimesTableC(C.Structure):
+    _fields_ = [("n_kpc", C.c_int32), ("n_nh", C.c_int32), ("n_z", C.c_int32),
+                ("n_t", C.c_int32), ("n_m", C.c_int32), ("n_time", C.c_int32),
+                ("z_index", C.c_int32), ("pad", C.c_int32),
+                ("kpc", C.c_void_p), ("nh", C.c_void_p), ("time_s", C.c_void_p),
+                ("u_evol", C.c_void_p), ("mu", C.c_void_p), ("metalz", C.c_void_p),
+                ("xh", C.c_double), ("m_h", C.c_double), ("u_max_cgs", C.c_double),
+                ("kpc_cm", C.c_double), ("unit_length_cm", C.c_double),
+                ("unit_rho_cgs", C.c_double), ("unit_u_cgs", C.c_double),
+                ("unit_time_s", C.c_double)]
+
+
 class Diag(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("ke", "ut", "pe", "px", "py", "pz", "lx", "ly", "lz",
                                           "mass")]
@@ -126,6 +138,12 @@ SIGNATURES = {
     "sphx_plan_set_cooling_grid": (C.c_int, [_V, _V]),
     "sphx_batch_set_cooling_grid": (C.c_int, [_V, _V]),
     "sphx_cool_grid_apply": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V]),
+    "sphx_chimes_apply": (C.c_int, [_V, _I64, _V, _V, _V, _D, _V, _V, _V, _V, _V]),
+    "sphx_plan_set_chimes_table": (C.c_int, [_V, _V]),
+    "sphx_plan_chimes_ions": (C.c_int, [_V, _V, _V]),
+    "sphx_batch_set_chimes_table": (C.c_int, [_V, _V]),
+    "sphx_read_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_write_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_cool_grid_build": (C.c_int, [_V, _V, _V, _V]),
     "sphx_plan_time": (C.c_int, [_V, _V, _V, _V, _V]),
     "sphx_plan_timing": (C.c_int, [_V, _V, _V, _I32]),
@@ -185,7 +203,10 @@ def check(rc: int, what: str = "") -> int:
 
 
 def cooling_mode(cooling) -> int:
-    """False/0: none; True/1/"solve": implicit solve; 2/"table": 2-D table."""
+    """False/0: none; True/1/"solve": implicit solve; 2/"table": 2-D table;
+    3/"chimes": 5-D CHIMES table."""
+    if cooling in ("chimes", 3):
+        return 3
     if cooling in ("table", 2):
         return 2
     if cooling in ("solve", True, 1):

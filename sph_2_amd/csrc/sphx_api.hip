@@ -123,6 +123,8 @@ Engine::~Engine() {
     if (cool_T) hipFree(cool_T);
     if (cool_L) hipFree(cool_L);
     if (grid_buf) hipFree(grid_buf);
+    if (chimes_buf) hipFree(chimes_buf);
+    if (chimes_ion) hipFree(chimes_ion);
     hipSetDevice(cur);
 }
 
@@ -951,6 +953,106 @@ int Engine::set_cool_grid(const sphx_cool_grid *g, hipStream_t st) {
     return SPHX_OK;
 }
 
+// Validate a CHIMES table (device arrays) on the host: sizes, constants and
+// strictly increasing axes (so that bisection finds the reference's node).
+int sphx_check_chimes(const sphx_chimes_table *t, hipStream_t st) {
+    if (!t || t->n_kpc < 1 || t->n_nh < 1 || t->n_z < 1 || t->n_t < 1 || t->n_m < 0 ||
+        t->n_time < 1 || !t->kpc || !t->nh || !t->time_s || !t->u_evol || !t->mu ||
+        (t->n_m > 0 && !t->metalz)) {
+        sphx_set_error("CHIMES table needs positive sizes and device arrays");
+        return SPHX_ERR_ARG;
+    }
+    if (t->z_index < 0 || t->z_index >= t->n_z) { sphx_set_error("CHIMES z_index outside the Z axis"); return SPHX_ERR_ARG; }
+    if (!(t->xh > 0 && t->m_h > 0 && t->kpc_cm > 0 && t->unit_length_cm > 0 &&
+          t->unit_rho_cgs > 0 && t->unit_u_cgs > 0 && t->unit_time_s > 0) ||
+        std::isnan(t->u_max_cgs)) {
+        sphx_set_error("CHIMES table needs positive constants and unit factors");
+        return SPHX_ERR_ARG;
+    }
+    const float *axes[3] = {t->kpc, t->nh, t->time_s};
+    int32_t lens[3] = {t->n_kpc, t->n_nh, t->n_time};
+    for (int a = 0; a < 3; ++a) {
+        std::vector<float> v(lens[a]);
+        SPHX_HIP(hipMemcpyAsync(v.data(), axes[a], sizeof(float) * lens[a], hipMemcpyDeviceToHost, st));
+        SPHX_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < lens[a]; ++k)
+            if (std::isnan(v[k]) || (k > 0 && !(v[k] > v[k - 1]))) {
+                sphx_set_error("CHIMES table axes (kpc, nH, time) must increase strictly");
+                return SPHX_ERR_ARG;
+            }
+    }
+    return SPHX_OK;
+}
+
+ChimesDev sphx_chimes_view(const sphx_chimes_table *t) {
+    ChimesDev c;
+    c.n_kpc = t->n_kpc;
+    c.n_nh = t->n_nh;
+    c.n_z = t->n_z;
+    c.n_t = t->n_t;
+    c.n_m = t->n_m;
+    c.n_time = t->n_time;
+    c.z_index = t->z_index;
+    c.kpc = t->kpc;
+    c.nh = t->nh;
+    c.time_s = t->time_s;
+    c.u_evol = t->u_evol;
+    c.mu = t->mu;
+    c.metalz = t->metalz;
+    c.xh = t->xh;
+    c.m_h = t->m_h;
+    c.u_max_cgs = t->u_max_cgs;
+    c.kpc_cm = t->kpc_cm;
+    c.unit_length_cm = t->unit_length_cm;
+    c.unit_rho_cgs = t->unit_rho_cgs;
+    c.unit_u_cgs = t->unit_u_cgs;
+    c.unit_time_s = t->unit_time_s;
+    return c;
+}
+
+// Copy a CHIMES table into engine-owned memory; keep_ions also sizes the
+// per-particle species fractions (cap x n_m, zeroed).
+int Engine::set_chimes(const sphx_chimes_table *t, bool keep_ions, hipStream_t st) {
+    SPHX_CHECK(sphx_check_chimes(t, st));
+    int64_t cells = (int64_t)t->n_kpc * t->n_t * t->n_nh * t->n_z * t->n_time;
+    int64_t need = t->n_kpc + t->n_nh + t->n_time + 2 * cells + cells * t->n_m;
+    if (need > chimes_cap) {
+        if (chimes_buf) hipFree(chimes_buf);
+        chimes_buf = nullptr;
+        chimes_cap = 0;
+        SPHX_HIP(hipMalloc(&chimes_buf, sizeof(float) * need));
+        chimes_cap = need;
+    }
+    float *q = chimes_buf;
+    ChimesDev c = sphx_chimes_view(t);
+    const float *src[6] = {t->kpc, t->nh, t->time_s, t->u_evol, t->mu, t->metalz};
+    int64_t len[6] = {t->n_kpc, t->n_nh, t->n_time, cells, cells, cells * t->n_m};
+    const float **dst[6] = {&c.kpc, &c.nh, &c.time_s, &c.u_evol, &c.mu, &c.metalz};
+    for (int a = 0; a < 6; ++a) {
+        if (len[a] > 0)
+            SPHX_HIP(hipMemcpyAsync(q, src[a], sizeof(float) * len[a], hipMemcpyDeviceToDevice, st));
+        *dst[a] = q;
+        q += len[a];
+    }
+    c.ion = nullptr;
+    if (keep_ions && t->n_m > 0) {
+        int64_t ion_need = cap * t->n_m;
+        if (ion_need > chimes_ion_cap) {
+            if (chimes_ion) hipFree(chimes_ion);
+            chimes_ion = nullptr;
+            chimes_ion_cap = 0;
+            SPHX_HIP(hipMalloc(&chimes_ion, sizeof(float) * ion_need));
+            chimes_ion_cap = ion_need;
+        }
+        SPHX_HIP(hipMemsetAsync(chimes_ion, 0, sizeof(float) * ion_need, st));
+        c.ion = chimes_ion;
+    }
+    SPHX_HIP(hipStreamSynchronize(st));
+    chimes = c;
+    drop_graph();
+    return SPHX_OK;
+}
+
 int sphx_copy_cool(Engine &E, const sphx_cooling *cool) {
     if (!cool) {
         E.has_cool = false;
@@ -1007,6 +1109,20 @@ extern "C" int sphx_cool_grid_apply(sphx_ctx *ctx, int64_t n, const double *rho,
     std::vector<double> un, rn;
     SPHX_CHECK(check_grid(grid, &un, &rn, st));
     return sphx_grid_apply_launch(grid_view(grid), rho, u_in, dt, n, u_out, st);
+}
+
+extern "C" int sphx_chimes_apply(sphx_ctx *ctx, int64_t n, const double *pos, const double *rho,
+                                 const double *u_in, double dt, const sphx_chimes_table *table,
+                                 double *u_out, float *ion_out, double *mu_out, void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n < 0 || (n > 0 && (!pos || !rho || !u_in || !u_out))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    if (std::isnan(dt)) { sphx_set_error("dt is NaN"); return SPHX_ERR_ARG; }
+    SPHX_CHECK(sphx_check_chimes(table, st));
+    if (ion_out && table->n_m == 0) ion_out = nullptr;
+    SPHX_CHECK(sphx_chimes_apply_launch(sphx_chimes_view(table), pos, rho, u_in, dt, n, u_out,
+                                        ion_out, mu_out, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    return SPHX_OK;
 }
 
 extern "C" int sphx_cool_grid_build(sphx_ctx *ctx, const sphx_cooling *cool,
@@ -1164,9 +1280,10 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
     if (!plan->inited) { sphx_set_error("call sphx_plan_init first"); return SPHX_ERR_STATE; }
     if (n_steps < 0 || !(dt > 0.0)) { sphx_set_error("bad step count or dt"); return SPHX_ERR_ARG; }
     Engine &E = plan->E;
-    if (use_cooling < 0 || use_cooling > 2) { sphx_set_error("use_cooling is 0, 1 (solve) or 2 (table)"); return SPHX_ERR_ARG; }
+    if (use_cooling < 0 || use_cooling > 3) { sphx_set_error("use_cooling is 0, 1 (solve), 2 (2-D table) or 3 (CHIMES table)"); return SPHX_ERR_ARG; }
     if (use_cooling == 1 && !E.has_cool) { sphx_set_error("plan has no cooling table"); return SPHX_ERR_ARG; }
     if (use_cooling == 2 && !E.grid.du) { sphx_set_error("plan has no 2-D cooling table"); return SPHX_ERR_ARG; }
+    if (use_cooling == 3 && !E.chimes.u_evol) { sphx_set_error("plan has no CHIMES table"); return SPHX_ERR_ARG; }
     // The host copy of the run parameters is current except for t and, with
     // the adaptive step, dt: pull the device copy before changing anything.
     RunParams &hp = E.run_params[0];
@@ -1259,7 +1376,8 @@ extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream
     SPHX_CHECK(sphx_ckpt_read_header(path, &h));
     if (h.mode != E.mode) { sphx_set_error("checkpoint precision mode differs from the plan's"); return SPHX_ERR_ARG; }
     if (h.n <= 0 || h.n > E.cap) { sphx_set_error("checkpoint particle count exceeds the plan capacity"); return SPHX_ERR_ARG; }
-    if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du)) {
+    if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du) ||
+        (h.rp.use_cool == 3 && !E.chimes.u_evol)) {
         sphx_set_error("checkpoint run cools with a table the plan does not have");
         return SPHX_ERR_ARG;
     }
@@ -1291,6 +1409,24 @@ extern "C" int sphx_plan_set_cooling_grid(sphx_plan *plan, const sphx_cool_grid 
     void *stream = nullptr;
     PLAN_PROLOGUE(plan);
     return plan->E.set_cool_grid(grid, st);
+}
+
+extern "C" int sphx_plan_set_chimes_table(sphx_plan *plan, const sphx_chimes_table *table) {
+    void *stream = nullptr;
+    PLAN_PROLOGUE(plan);
+    return plan->E.set_chimes(table, true, st);
+}
+
+extern "C" int sphx_plan_chimes_ions(sphx_plan *plan, float *out, void *stream) {
+    PLAN_PROLOGUE(plan);
+    Engine &E = plan->E;
+    if (!E.chimes.ion) { sphx_set_error("plan has no CHIMES table with species"); return SPHX_ERR_STATE; }
+    if (!out && E.n > 0) { sphx_set_error("null output"); return SPHX_ERR_ARG; }
+    if (E.n > 0)
+        SPHX_HIP(hipMemcpyAsync(out, E.chimes.ion, sizeof(float) * E.n * E.chimes.n_m,
+                                hipMemcpyDeviceToDevice, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    return SPHX_OK;
 }
 
 extern "C" int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts,

@@ -200,9 +200,10 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
         return SPHX_ERR_ARG;
     }
     if (!E.p.eps_from_h && !eps) { sphx_set_error("eps required when eps_from_h == 0"); return SPHX_ERR_ARG; }
-    if (use_cooling < 0 || use_cooling > 2) { sphx_set_error("use_cooling is 0, 1 (solve) or 2 (table)"); return SPHX_ERR_ARG; }
+    if (use_cooling < 0 || use_cooling > 3) { sphx_set_error("use_cooling is 0, 1 (solve), 2 (2-D table) or 3 (CHIMES table)"); return SPHX_ERR_ARG; }
     if (use_cooling == 1 && !E.has_cool) { sphx_set_error("batch has no cooling table"); return SPHX_ERR_ARG; }
     if (use_cooling == 2 && !E.grid.du) { sphx_set_error("batch has no 2-D cooling table"); return SPHX_ERR_ARG; }
+    if (use_cooling == 3 && !E.chimes.u_evol) { sphx_set_error("batch has no CHIMES table"); return SPHX_ERR_ARG; }
     if (E.n + n > E.cap || (int)b->runs.size() >= E.max_runs) {
         sphx_set_error("admission refused: particle capacity or run slots exhausted");
         return SPHX_REFUSED;
@@ -321,6 +322,12 @@ extern "C" int sphx_batch_step(sphx_batch *b, int32_t n_steps, void *stream) {
     return SPHX_OK;
 }
 
+extern "C" int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_table *table) {
+    void *stream = nullptr;
+    BATCH_PROLOGUE(b);
+    return b->E.set_chimes(table, false, st);
+}
+
 extern "C" int sphx_batch_set_cooling_grid(sphx_batch *b, const sphx_cool_grid *grid) {
     void *stream = nullptr;
     BATCH_PROLOGUE(b);
@@ -404,7 +411,8 @@ extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *
         sphx_set_error("checkpoint from an engine with another key layout");
         return SPHX_ERR_ARG;
     }
-    if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du)) {
+    if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du) ||
+        (h.rp.use_cool == 3 && !E.chimes.u_evol)) {
         sphx_set_error("checkpoint run cools with a table the batch does not have");
         return SPHX_ERR_ARG;
     }

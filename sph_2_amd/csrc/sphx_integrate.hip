@@ -199,8 +199,86 @@ __global__ void k_grid_build(GridDev g, CoolDev cd, double *__restrict__ du) {
     du[t] = (u_code - un) * cd.unit_u_cgs;
 }
 
+// hcoolingx of the 5-D CHIMES table (hfvCLibs_v7.h:1077-1240), FP64 on the
+// float32 table.  Sorted axes are searched by bisection for the first node >=
+// the value (the reference's linear scans find the same node); the T column
+// is scanned in order, since its node energies need not increase.
+__device__ __forceinline__ int chimes_first_ge(const float *__restrict__ g, int n, double x) {
+    int lo = 0, hi = n;  // first k with x <= g[k], else n (also for NaN)
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (x <= (double)g[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo < n ? lo : -1;
+}
+
+__device__ __forceinline__ int chimes_nearest(const float *__restrict__ g, int n, double x) {
+    int j = chimes_first_ge(g, n, x);
+    if (j > 0) {
+        double d1 = fabs((double)g[j] - x), d2 = fabs((double)g[j - 1] - x);
+        if (d2 < d1) j -= 1;
+    }
+    return j < 0 ? n - 1 : j;
+}
+
+// New energy (code units) of one particle; *cell receives the flat index of
+// the chosen (kpc, T, nH, Z, time) node in u_evol / mu and *mcell that of
+// species 0 in metalz (species k at + k * n_time).
+__device__ double chimes_one(double x, double y, double z, double rho, double u_code, double dt,
+                             const ChimesDev &c, int64_t *cell, int64_t *mcell) {
+#pragma clang fp contract(off)
+    const int64_t NT = c.n_t, NH = c.n_nh, NZ = c.n_z, NM = c.n_m, NTIME = c.n_time;
+    const int64_t s_t = NH * NZ * NTIME, s_k = NT * s_t;
+    double dt_s = dt * c.unit_time_s;
+    int jt = chimes_first_ge(c.time_s, c.n_time, dt_s);
+    if (jt < 0) jt = c.n_time - 1;
+    double a = x * x, b = y * y, cz = z * z;
+    double r = sqrt((a + b) + cz);
+    double r_kpc = (r * c.unit_length_cm) / c.kpc_cm;
+    int64_t jk = chimes_nearest(c.kpc, c.n_kpc, r_kpc);
+    double nh_i = ((rho * c.unit_rho_cgs) * c.xh) / c.m_h;
+    int64_t jn = chimes_nearest(c.nh, c.n_nh, nh_i);
+    int64_t jz = c.z_index;
+    int64_t col = jk * s_k + jn * NZ * NTIME + jz * NTIME;
+    double u_i = u_code * c.unit_u_cgs;
+    int64_t ju = NT - 1;
+    for (int64_t t = 0; t < NT; ++t)
+        if (u_i <= (double)c.u_evol[col + t * s_t]) { ju = t; break; }
+    double uev;
+    if (ju > 0) {
+        double U0 = c.u_evol[col + (ju - 1) * s_t], U1 = c.u_evol[col + ju * s_t];
+        double e0 = c.u_evol[col + (ju - 1) * s_t + jt], e1 = c.u_evol[col + ju * s_t + jt];
+        double fhi = (u_i - U0) / (U1 - U0);
+        double flo = 1.0 - fhi;
+        uev = flo * e0 + fhi * e1;
+    } else {
+        uev = c.u_evol[col + jt];
+    }
+    if (uev > c.u_max_cgs) uev = c.u_max_cgs;
+    *cell = col + ju * s_t + jt;
+    *mcell = ((jk * NT + ju) * NH + jn) * NZ * NM * NTIME + jz * NM * NTIME + jt;
+    return uev / c.unit_u_cgs;
+}
+
+__global__ void k_chimes_apply(ChimesDev c, const double *__restrict__ pos,
+                               const double *__restrict__ rho, const double *u_in, double dt,
+                               int64_t n, double *u_out, float *__restrict__ ion_out,
+                               double *__restrict__ mu_out) {
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    int64_t cell, mcell;
+    double un = chimes_one(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], rho[i], u_in[i], dt, c,
+                           &cell, &mcell);
+    u_out[i] = un;
+    if (mu_out) mu_out[i] = c.mu[cell];
+    if (ion_out)
+        for (int k = 0; k < c.n_m; ++k)
+            ion_out[i * c.n_m + k] = c.metalz[mcell + (int64_t)k * c.n_time];
+}
+
 __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, CoolDev cd,
-                         int cool_on, GridDev gd, int64_t *counters) {
+                         int cool_on, GridDev gd, ChimesDev ch, int64_t *counters) {
 #pragma clang fp contract(off)
     int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
     if (i >= n) return;
@@ -219,6 +297,13 @@ __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, Coo
         if (it >= cd.max_iter) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
     } else if (r.use_cool == 2 && gd.du) {
         u = grid_apply(s.rho[i], u, dt, gd);
+    } else if (r.use_cool == 3 && ch.u_evol) {
+        int64_t cell, mcell;
+        u = chimes_one(s.x[i], s.y[i], s.z[i], s.rho[i], u, dt, ch, &cell, &mcell);
+        if (ch.ion) {
+            float *o = ch.ion + (int64_t)s.id[i] * ch.n_m;
+            for (int k = 0; k < ch.n_m; ++k) o[k] = ch.metalz[mcell + (int64_t)k * ch.n_time];
+        }
     }
     s.u[i] = u;
 }
@@ -287,6 +372,15 @@ int sphx_grid_apply_launch(const GridDev &g, const double *rho, const double *u_
     return SPHX_OK;
 }
 
+int sphx_chimes_apply_launch(const ChimesDev &c, const double *pos, const double *rho,
+                             const double *u_in, double dt, int64_t n, double *u_out,
+                             float *ion_out, double *mu_out, hipStream_t st) {
+    if (n <= 0) return SPHX_OK;
+    k_chimes_apply<<<g1(n), IT, 0, st>>>(c, pos, rho, u_in, dt, n, u_out, ion_out, mu_out);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
 int sphx_grid_build_launch(const GridDev &g, const CoolDev &cd, double *du, hipStream_t st) {
     int64_t nn = g.nu * g.nrho;
     if (nn <= 0) return SPHX_OK;
@@ -345,7 +439,7 @@ CoolDev sphx_engine_cool(const Engine &E) { return make_cool(E); }
 int Engine::finish_step(hipStream_t st) {
     if (n == 0) return SPHX_OK;
     k_finish<<<g1(n), IT, 0, st>>>(s, d_run, n, make_cool(*this), has_cool ? 1 : 0, grid,
-                                   counters);
+                                   chimes, counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -213,6 +213,18 @@ struct GridDev {
     double dt_ref_s = 0, unit_u_cgs = 0, unit_rho_cgs = 0, unit_time_s = 0;
 };
 
+// Device view of a 5-D CHIMES table (sphx_chimes_table).  ion: per-particle
+// species fractions indexed by the particle's caller id (plans only; null in
+// batches and stateless calls use their own output).
+struct ChimesDev {
+    int32_t n_kpc = 0, n_nh = 0, n_z = 0, n_t = 0, n_m = 0, n_time = 0, z_index = 0;
+    const float *kpc = nullptr, *nh = nullptr, *time_s = nullptr;
+    const float *u_evol = nullptr, *mu = nullptr, *metalz = nullptr;
+    double xh = 0, m_h = 0, u_max_cgs = 0, kpc_cm = 0;
+    double unit_length_cm = 0, unit_rho_cgs = 0, unit_u_cgs = 0, unit_time_s = 0;
+    float *ion = nullptr;
+};
+
 // ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
@@ -246,6 +258,12 @@ struct Engine {
     double *grid_buf = nullptr;
     int64_t grid_cap = 0;
     int set_cool_grid(const sphx_cool_grid *g, hipStream_t st);
+    ChimesDev chimes{};        // optional 5-D CHIMES table (runs with use_cool == 3)
+    float *chimes_buf = nullptr;
+    int64_t chimes_cap = 0;
+    float *chimes_ion = nullptr;  // cap x n_m species fractions (plans)
+    int64_t chimes_ion_cap = 0;
+    int set_chimes(const sphx_chimes_table *t, bool keep_ions, hipStream_t st);
 
     int64_t cap = 0, n = 0;
     int R = 0;
@@ -451,6 +469,11 @@ struct CoolDev {
 int sphx_grid_apply_launch(const GridDev &g, const double *rho, const double *u_in, double dt,
                            int64_t n, double *u_out, hipStream_t st);
 int sphx_grid_build_launch(const GridDev &g, const CoolDev &cd, double *du, hipStream_t st);
+int sphx_chimes_apply_launch(const ChimesDev &c, const double *pos, const double *rho,
+                             const double *u_in, double dt, int64_t n, double *u_out,
+                             float *ion_out, double *mu_out, hipStream_t st);
+int sphx_check_chimes(const sphx_chimes_table *t, hipStream_t st);
+ChimesDev sphx_chimes_view(const sphx_chimes_table *t);
 int sphx_cool_launch(const CoolDev &cd, const double *rho, const double *u_in, double dt,
                      const double *dt_p, const int32_t *run, const RunParams *rp, int64_t n,
                      double *u_out, int32_t *iters, int64_t *counters, hipStream_t st);

@@ -334,6 +334,70 @@ extern "C" int sphx_read_snapshot_bin(const char *path, int64_t n_cap, int64_t *
 }
 
 // ---------------------------------------------------------------------------
+// coolHeatGridZ.bin, the CHIMES table file (written by GPU_sph_Type_cooling_II/
+// cooling_grids_use_EC2/step_2x.py:101-140, read by hfvGpuSphTyp_v7.cu:35-79):
+// int32 n_kpc, n_nh, n_z, n_t, n_m, n_time; float32 axes kpc, nh, z, T, time;
+// float32 u_evol and mu [kpc][T][nH][Z][time]; float32 metalz
+// [kpc][T][nH][Z][species][time].  The file size must match the sizes.
+// ---------------------------------------------------------------------------
+static bool chimes_sizes(const int32_t d[6], int64_t *axes, int64_t *cells, int64_t *mcells) {
+    for (int k = 0; k < 6; ++k)
+        if (d[k] < (k == 4 ? 0 : 1) || d[k] > (1 << 20)) return false;
+    *axes = (int64_t)d[0] + d[1] + d[2] + d[3] + d[5];
+    *cells = (int64_t)d[0] * d[3] * d[1] * d[2] * d[5];
+    *mcells = *cells * d[4];
+    return *cells < ((int64_t)1 << 40) && *mcells < ((int64_t)1 << 40);
+}
+
+extern "C" int sphx_read_chimes(const char *path, int32_t dims[6], float *kpc, float *nh, float *z,
+                                float *temp, float *time_s, float *u_evol, float *mu,
+                                float *metalz) {
+    if (!dims) { sphx_set_error("null argument"); return SPHX_ERR_ARG; }
+    File F(path, "rb");
+    if (!F.f) return io_error("cannot open CHIMES table", path, true);
+    int32_t d[6];
+    if (!read_all(F.f, d, 6)) return io_error("CHIMES table shorter than its header", path);
+    int64_t axes, cells, mcells;
+    if (!chimes_sizes(d, &axes, &cells, &mcells)) return io_error("CHIMES table header sizes out of range", path);
+    std::fseek(F.f, 0, SEEK_END);
+    long long size = std::ftell(F.f);
+    if (size != (long long)(24 + 4 * (axes + 2 * cells + mcells)))
+        return io_error("CHIMES table size does not match its header", path);
+    std::memcpy(dims, d, sizeof(d));
+    if (!kpc && !nh && !z && !temp && !time_s && !u_evol && !mu && !metalz) return SPHX_OK;
+    if (!kpc || !nh || !z || !temp || !time_s || !u_evol || !mu || (mcells > 0 && !metalz)) {
+        sphx_set_error("CHIMES arrays must all be given");
+        return SPHX_ERR_ARG;
+    }
+    std::fseek(F.f, 24, SEEK_SET);
+    bool ok = read_all(F.f, kpc, d[0]) && read_all(F.f, nh, d[1]) && read_all(F.f, z, d[2]) &&
+              read_all(F.f, temp, d[3]) && read_all(F.f, time_s, d[5]) &&
+              read_all(F.f, u_evol, cells) && read_all(F.f, mu, cells) &&
+              (mcells == 0 || read_all(F.f, metalz, mcells));
+    return ok ? SPHX_OK : io_error("read failed", path);
+}
+
+extern "C" int sphx_write_chimes(const char *path, const int32_t dims[6], const float *kpc,
+                                 const float *nh, const float *z, const float *temp,
+                                 const float *time_s, const float *u_evol, const float *mu,
+                                 const float *metalz) {
+    int64_t axes, cells, mcells;
+    if (!dims || !chimes_sizes(dims, &axes, &cells, &mcells)) { sphx_set_error("bad CHIMES sizes"); return SPHX_ERR_ARG; }
+    if (!kpc || !nh || !z || !temp || !time_s || !u_evol || !mu || (mcells > 0 && !metalz)) {
+        sphx_set_error("null argument");
+        return SPHX_ERR_ARG;
+    }
+    File F(path, "wb");
+    if (!F.f) return io_error("cannot create CHIMES table", path, true);
+    bool ok = write_all(F.f, dims, 6) && write_all(F.f, kpc, dims[0]) &&
+              write_all(F.f, nh, dims[1]) && write_all(F.f, z, dims[2]) &&
+              write_all(F.f, temp, dims[3]) && write_all(F.f, time_s, dims[5]) &&
+              write_all(F.f, u_evol, cells) && write_all(F.f, mu, cells) &&
+              (mcells == 0 || write_all(F.f, metalz, mcells));
+    return ok ? SPHX_OK : io_error("write failed", path, true);
+}
+
+// ---------------------------------------------------------------------------
 // Exact checkpoint of a resident run (no counterpart in the reference, whose
 // restart CSV holds 6-digit floats and makes the driver recompute the forces).
 // Layout (little-endian): CkptHeader, then SPHX_CKPT_ND arrays of n float64

@@ -14,7 +14,10 @@ Names follow the reference's own readers and writers (paths relative to
 * ``read_ic_binary`` / ``write_ic_binary``: int32 Typ then float32 arrays
   (GPU_sph_Type/Evrard_collapse/IC_evrard_binary.py:60-82, hfvCppLibs.h:8-63);
 * ``read_snapshot_binary`` / ``write_snapshot_binary``: int32 N, Typ, float32
-  x,y,z,vx,vy,vz,rho,h,u,mass (hfvCppLibs.h:68-96).
+  x,y,z,vx,vy,vz,rho,h,u,mass (hfvCppLibs.h:68-96);
+* ``read_chimes_table`` / ``write_chimes_table``: coolHeatGridZ.bin, the 5-D
+  CHIMES table (GPU_sph_Type_cooling_II/cooling_grids_use_EC2/step_2x.py:101-140
+  writes it, hfvGpuSphTyp_v7.cu:35-79 reads it).
 
 All arrays are float64 numpy arrays in and out (binary files hold float32, as
 the reference's do).  Errors raise ``SphxError`` with the file and the reason.
@@ -142,3 +145,41 @@ def read_snapshot_binary(path) -> dict:
                                        _p(o["vel"]), _p(o["rho"]), _p(o["h"]), _p(o["u"]),
                                        _p(o["m"])), "read_snapshot_binary")
     return o
+
+
+_CHIMES_KEYS = ("kpc", "nh", "z", "temp", "time_s", "u_evol", "mu", "metalz")
+
+
+def read_chimes_table(path) -> dict:
+    """coolHeatGridZ.bin: float32 axes kpc, nh (densities), z (metallicities),
+    temp (temperatures), time_s and arrays u_evol, mu of shape
+    (kpc, T, nH, Z, time), metalz of shape (kpc, T, nH, Z, species, time)."""
+    L = _lib.load()
+    dims = (C.c_int32 * 6)()
+    check(L.sphx_read_chimes(_path(path), dims, None, None, None, None, None, None, None, None),
+          "read_chimes_table")
+    nk, nn, nz, nt, nm, ntime = (int(v) for v in dims)
+    out = {"kpc": np.empty(nk, np.float32), "nh": np.empty(nn, np.float32),
+           "z": np.empty(nz, np.float32), "temp": np.empty(nt, np.float32),
+           "time_s": np.empty(ntime, np.float32),
+           "u_evol": np.empty((nk, nt, nn, nz, ntime), np.float32),
+           "mu": np.empty((nk, nt, nn, nz, ntime), np.float32),
+           "metalz": np.empty((nk, nt, nn, nz, nm, ntime), np.float32)}
+    check(L.sphx_read_chimes(_path(path), dims, *[_p(out[k]) for k in _CHIMES_KEYS]),
+          "read_chimes_table")
+    return out
+
+
+def write_chimes_table(path, kpc, nh, z, temp, time_s, u_evol, mu, metalz):
+    """Write coolHeatGridZ.bin from arrays shaped as read_chimes_table returns them."""
+    a = {k: np.ascontiguousarray(v, dtype=np.float32)
+         for k, v in zip(_CHIMES_KEYS, (kpc, nh, z, temp, time_s, u_evol, mu, metalz))}
+    nk, nn, nz, nt, ntime = (len(a[k]) for k in ("kpc", "nh", "z", "temp", "time_s"))
+    if a["u_evol"].shape != (nk, nt, nn, nz, ntime) or a["mu"].shape != a["u_evol"].shape:
+        raise ValueError("u_evol and mu must have shape (kpc, T, nH, Z, time)")
+    if a["metalz"].ndim != 6 or a["metalz"].shape[:4] != (nk, nt, nn, nz) or \
+            a["metalz"].shape[5] != ntime:
+        raise ValueError("metalz must have shape (kpc, T, nH, Z, species, time)")
+    dims = (C.c_int32 * 6)(nk, nn, nz, nt, a["metalz"].shape[4], ntime)
+    check(_lib.load().sphx_write_chimes(_path(path), dims, *[_p(a[k]) for k in _CHIMES_KEYS]),
+          "write_chimes_table")

@@ -561,6 +561,122 @@ def applyCooling(rho, u, dt, grid: CoolingGrid, *, device=0):
     return out.cpu().numpy()
 
 
+class ChimesTable:
+    """5-D CHIMES table of the reference's v7 drivers (GPU_sph_Type_cooling_II/
+    hfvGpuSphTyp_v7.cu:35-79, looked up by hcoolingx in hfvCLibs_v7.h:1077-1240):
+    float32 axes kpc, nh, z, temp, time_s and arrays u_evol, mu of shape
+    (kpc, T, nH, Z, time), metalz of shape (kpc, T, nH, Z, species, time), all
+    held on the device; code -> cgs unit factors and the lookup constants
+    (hydrogen fraction xh, hydrogen mass m_h, energy cap u_max_cgs, kpc in cm,
+    metallicity node z_index) with the reference's values as defaults."""
+
+    def __init__(self, kpc, nh, z, temp, time_s, u_evol, mu, metalz, *, unit_length_cm,
+                 unit_rho_cgs, unit_u_cgs, unit_time_s, z_index=1, xh=0.70, m_h=1.6726e-24,
+                 u_max_cgs=2.0287e18, kpc_cm=3.086e21):
+        f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)  # noqa: E731
+        self.host = {"kpc": f32(kpc), "nh": f32(nh), "z": f32(z), "temp": f32(temp),
+                     "time_s": f32(time_s), "u_evol": f32(u_evol), "mu": f32(mu),
+                     "metalz": f32(metalz)}
+        h = self.host
+        shape = (len(h["kpc"]), len(h["temp"]), len(h["nh"]), len(h["z"]), len(h["time_s"]))
+        if h["u_evol"].shape != shape or h["mu"].shape != shape:
+            raise ValueError(f"u_evol and mu must have shape (kpc, T, nH, Z, time) = {shape}")
+        if h["metalz"].ndim != 6 or h["metalz"].shape[:4] != shape[:4] or \
+                h["metalz"].shape[5] != shape[4]:
+            raise ValueError("metalz must have shape (kpc, T, nH, Z, species, time)")
+        self.n_species = int(h["metalz"].shape[4])
+        self.shape = shape
+        self.d = {k: dev(v.ravel()) for k, v in h.items() if k not in ("z", "temp")}
+        self.kw = dict(z_index=int(z_index), xh=float(xh), m_h=float(m_h),
+                       u_max_cgs=float(u_max_cgs), kpc_cm=float(kpc_cm),
+                       unit_length_cm=float(unit_length_cm), unit_rho_cgs=float(unit_rho_cgs),
+                       unit_u_cgs=float(unit_u_cgs), unit_time_s=float(unit_time_s))
+
+    def c(self) -> _lib.ChimesTableC:
+        nk, nt, nn, nz, ntime = self.shape
+        d = self.d
+        return _lib.ChimesTableC(nk, nn, nz, nt, self.n_species, ntime, self.kw["z_index"], 0,
+                                 ptr(d["kpc"]), ptr(d["nh"]), ptr(d["time_s"]), ptr(d["u_evol"]),
+                                 ptr(d["mu"]), ptr(d["metalz"]) if self.n_species else None,
+                                 self.kw["xh"], self.kw["m_h"], self.kw["u_max_cgs"],
+                                 self.kw["kpc_cm"], self.kw["unit_length_cm"],
+                                 self.kw["unit_rho_cgs"], self.kw["unit_u_cgs"],
+                                 self.kw["unit_time_s"])
+
+    @classmethod
+    def read(cls, path, **kw):
+        """Load coolHeatGridZ.bin; kw: the unit factors and lookup constants."""
+        from .formats import read_chimes_table
+        return cls(**read_chimes_table(path), **kw)
+
+    def write(self, path):
+        from .formats import write_chimes_table
+        write_chimes_table(path, **self.host)
+
+    @classmethod
+    def from_solve(cls, table: CoolingTable, temp, nh, time_s, kpc=(0.0,), z=(-2.0, -1.0, 0.0),
+                   z_index=1, *, unit_length_cm, device=0):
+        """A table filled from the library's own implicit solve over Lambda(T)
+        (the role the CHIMES runs of cooling_grids_use_EC2/step_1x.py play for
+        the reference): node energies u = k_B T / ((gamma-1) mu m_H) at
+        time_s[0] = 0, the solve's energy after each later time, repeated over
+        the kpc and Z axes (the solve depends on neither); mu constant, no
+        species.  xh and m_h follow the solve, so that n_H agrees."""
+        kw = table.kw
+        temp = np.asarray(temp, np.float64)
+        nh = np.asarray(nh, np.float64)
+        ts = np.asarray(time_s, np.float64)
+        nk, nz = len(kpc), len(z)
+        tfac = (kw["gamma"] - 1.0) * kw["mu"] * MH_CGS / KB_CGS
+        u0 = np.float32(temp / tfac).astype(np.float64)      # the stored node energies
+        nh32 = np.float32(nh).astype(np.float64)
+        U, NH = np.meshgrid(u0, nh32, indexing="ij")          # (T, nH)
+        rho_code = NH * MH_CGS / kw["XH"] / kw["unit_rho_cgs"]
+        evol = np.empty(U.shape + (len(ts),))
+        for j, tj in enumerate(ts):
+            if tj == 0.0:
+                evol[..., j] = U
+            else:
+                evol[..., j] = DoCooling(rho_code.ravel(), (U / kw["unit_u_cgs"]).ravel(),
+                                         tj / kw["unit_time_s"], table,
+                                         device=device).reshape(U.shape) * kw["unit_u_cgs"]
+        u_evol = np.broadcast_to(evol[None, :, :, None, :], (nk, len(temp), len(nh), nz, len(ts)))
+        mu = np.full(u_evol.shape, kw["mu"], np.float32)
+        metalz = np.zeros(u_evol.shape[:4] + (0, len(ts)), np.float32)
+        return cls(kpc, nh, z, temp, ts, u_evol, mu, metalz, unit_length_cm=unit_length_cm,
+                   unit_rho_cgs=kw["unit_rho_cgs"], unit_u_cgs=kw["unit_u_cgs"],
+                   unit_time_s=kw["unit_time_s"], z_index=z_index, xh=kw["XH"], m_h=MH_CGS)
+
+
+def hcoolingx(pos, rho, u, dt, table: ChimesTable, *, device=0):
+    """hcoolingx (hfvCLibs_v7.h:1077-1240) on numpy arrays in code units:
+    returns (u after the lookup, species fractions (n, n_species) float32, mu at
+    the chosen node)."""
+    t = _t()
+    dp, dr, du = dev(_f64(pos).reshape(-1, 3)), dev(_f64(rho)), dev(_f64(u))
+    n = len(du)
+    out = t.empty(n, dtype=t.float64, device="cuda")
+    mu = t.empty(n, dtype=t.float64, device="cuda")
+    ion = t.empty((n, max(table.n_species, 1)), dtype=t.float32, device="cuda")
+    c = table.c()
+    check(_lib.load().sphx_chimes_apply(_context(device), n, ptr(dp), ptr(dr), ptr(du), float(dt),
+                                        C.byref(c), ptr(out), ptr(ion) if table.n_species else None,
+                                        ptr(mu), _lib.stream_handle()), "hcoolingx")
+    return (out.cpu().numpy(), ion[:, :table.n_species].cpu().numpy(), mu.cpu().numpy())
+
+
+def hcooler(r_p, u_p, nH_p, dt_sec, table: ChimesTable, *, device=0):
+    """Scalar cgs form of the reference's CPU twin hcooler (GPU_sph_Type_cooling_II/
+    cooling_grids_use_EC2/checker1.py:53-146): radius in kpc, u in erg/g, n_H in
+    cm^-3, dt in s; returns (u after the lookup in erg/g, mu)."""
+    kw = table.kw
+    pos = np.array([[r_p * kw["kpc_cm"] / kw["unit_length_cm"], 0.0, 0.0]])
+    rho = np.array([nH_p * kw["m_h"] / kw["xh"] / kw["unit_rho_cgs"]])
+    u = np.array([u_p / kw["unit_u_cgs"]])
+    un, _, mu = hcoolingx(pos, rho, u, dt_sec / kw["unit_time_s"], table, device=device)
+    return float(un[0] * kw["unit_u_cgs"]), float(mu[0])
+
+
 def DoCooling_h(rho, u_old, dt, XH, table: CoolingTable | None = None):
     """Scalar cgs form of DoCooling_h(rho, u_old, dt, XH) (PHOTOPY:489): returns
     u in cgs after an implicit step of dt seconds over the tabulated Lambda(T)."""

@@ -94,8 +94,25 @@ class SphPlan:
         g = grid.c()
         check(_lib.load().sphx_plan_set_cooling_grid(self._h, C.byref(g)), "set_cooling_grid")
 
+    def set_chimes_table(self, table):
+        """Attach a libsx.ChimesTable; step(..., cooling="chimes") uses it."""
+        c = table.c()
+        check(_lib.load().sphx_plan_set_chimes_table(self._h, C.byref(c)), "set_chimes_table")
+        self._n_species = table.n_species
+
+    def chimes_ion_fractions(self):
+        """Species fractions of every particle at its last CHIMES lookup,
+        (n, n_species) float32 in the original particle order (the ionFrac
+        array of hfvGpuSphTyp_v7.cu)."""
+        t = _lib.torch()
+        out = t.empty((self.n, getattr(self, "_n_species", 0)), dtype=t.float32, device="cuda")
+        check(_lib.load().sphx_plan_chimes_ions(self._h, ptr(out), _lib.stream_handle()),
+              "chimes_ions")
+        return out.cpu().numpy()
+
     def step(self, n_steps: int, dt: float, cooling=False) -> int:
-        """cooling: False, True / "solve" (implicit Lambda(T) solve) or "table"."""
+        """cooling: False, True / "solve" (implicit Lambda(T) solve), "table"
+        (2-D table) or "chimes" (5-D CHIMES table)."""
         return check(_lib.load().sphx_plan_step(self._h, int(n_steps), float(dt),
                                                 _lib.cooling_mode(cooling), _lib.stream_handle()),
                      "sphx_plan_step")

@@ -69,6 +69,11 @@ class Batch:
         g = grid.c()
         check(_lib.load().sphx_batch_set_cooling_grid(self._h, C.byref(g)), "set_cooling_grid")
 
+    def set_chimes_table(self, table):
+        """Attach a libsx.ChimesTable for runs admitted with cooling="chimes"."""
+        c = table.c()
+        check(_lib.load().sphx_batch_set_chimes_table(self._h, C.byref(c)), "set_chimes_table")
+
     def admit(self, ic_: ic.IC, dt: float, cooling=False) -> int | None:
         """Admit one run; None when the budget or the run slots are full."""
         t = _lib.torch()

@@ -131,3 +131,47 @@ def test_errors_name_the_file(tmp_path):
     short.write_text("5\n1.0\n")
     with pytest.raises(_lib.SphxError, match="fewer than 10"):
         formats.read_params_gpu(short)
+
+
+def _chimes_arrays(rng, nk=3, nt=7, nn=5, nz=3, nm=2, ntime=4):
+    return dict(kpc=np.sort(rng.uniform(0, 5, nk)).astype(np.float32),
+                nh=np.sort(rng.uniform(0, 1e3, nn)).astype(np.float32),
+                z=np.array([-2, -1, 0], np.float32)[:nz],
+                temp=np.sort(rng.uniform(1, 1e8, nt)).astype(np.float32),
+                time_s=np.sort(rng.uniform(0, 1e10, ntime)).astype(np.float32),
+                u_evol=rng.random((nk, nt, nn, nz, ntime), np.float32),
+                mu=rng.random((nk, nt, nn, nz, ntime), np.float32),
+                metalz=rng.random((nk, nt, nn, nz, nm, ntime), np.float32))
+
+
+def test_chimes_table_bytes_match_the_reference_recipe(tmp_path):
+    """coolHeatGridZ.bin as step_2x.py:101-140 writes it: struct.pack('i') of
+    N_kpc, N_nH, N_Z, N_T, N_M, N_time, then 'f' values of the axes and of the
+    three arrays in nested-loop (C) order."""
+    import struct
+
+    a = _chimes_arrays(np.random.default_rng(5))
+    nk, nt, nn, nz, nm, ntime = a["metalz"].shape
+    want = b"".join(struct.pack("i", v) for v in (nk, nn, nz, nt, nm, ntime))
+    for k in ("kpc", "nh", "z", "temp", "time_s", "u_evol", "mu", "metalz"):
+        want += b"".join(struct.pack("f", v) for v in a[k].ravel())
+    f = tmp_path / "coolHeatGridZ.bin"
+    formats.write_chimes_table(f, **a)
+    assert f.read_bytes() == want
+    back = formats.read_chimes_table(f)
+    for k, v in a.items():
+        assert back[k].shape == v.shape and np.array_equal(back[k], v), k
+
+
+def test_chimes_table_errors(tmp_path):
+    a = _chimes_arrays(np.random.default_rng(6))
+    f = tmp_path / "t.bin"
+    formats.write_chimes_table(f, **a)
+    data = f.read_bytes()
+    (tmp_path / "short.bin").write_bytes(data[:-4])
+    with pytest.raises(_lib.SphxError, match="size does not match"):
+        formats.read_chimes_table(tmp_path / "short.bin")
+    with pytest.raises(_lib.SphxError, match="cannot open"):
+        formats.read_chimes_table(tmp_path / "missing.bin")
+    with pytest.raises(ValueError):
+        formats.write_chimes_table(f, **{**a, "mu": a["mu"][:, :-1]})
