@@ -27,10 +27,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
-# HBM bytes per k_grav launch at N = 1,048,576: 2 x FETCH_SIZE + WRITE_SIZE of the step's
-# dispatch in two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE reads half
-# of the bytes of that workload's 256 MiB calibration copy, hence the factor 2).
-GRAV_TRAFFIC_BYTES_1M = 11.982e9
+# HBM bytes per gravity-kernel launch at N = 1,048,576: 2 x FETCH_SIZE + WRITE_SIZE of the
+# step's dispatch in two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE reads
+# half of the bytes of that workload's 256 MiB calibration copy, hence the factor 2).
+GRAV_TRAFFIC_BYTES_1M = {"k_grav": 11.982e9, "k_grav_sym": 50.694e9}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -112,12 +112,11 @@ def bench_evrard(a) -> int:
     # ---------------- mixed precision, resident -----------------------------
     pl = SphPlan(n, mode="mixed")
     if a.grav:
-        ipl, tj, js = (int(x) for x in a.grav.split(","))
-        pl.set_gravity_config(ipl, tj, js)
+        cfg_used = tuple(int(x) for x in a.grav.split(","))
     else:
         from sph_2_amd.autotune import choose_gravity_config
-        cfg = choose_gravity_config(n)
-        pl.set_gravity_config(*cfg)
+        cfg_used = tuple(choose_gravity_config(n))
+    pl.set_gravity_config(*cfg_used)
     pl.load(e.pos, e.vel, e.u, e.m, e.h)
     pl.init()
     pl.step(a.warmup, a.dt)
@@ -136,17 +135,24 @@ def bench_evrard(a) -> int:
     counters = pl.counters()
     print(f"[bench] mixed {ms_per_step:.1f} ms/step", file=sys.stderr, flush=True)
 
+    # the gravity kernel alone (HIP events on the plan's stream around its launch)
+    k_ms, k_cnt = phases["gravity_kernel"]
     g_ms, g_cnt = phases["gravity"]
-    grav_s = g_ms / max(g_cnt, 1) / 1e3
+    kern_s = k_ms / max(k_cnt, 1) / 1e3
     flops = 20.0 * n * (n - 1)
-    achieved = flops / grav_s / 1e12 if grav_s > 0 else None
-    roofline = {"bound": "valu", "kernel": "k_grav (direct-sum gravity, FP32)",
+    achieved = flops / kern_s / 1e12 if kern_s > 0 else None
+    kname = "k_grav_sym" if cfg_used[0] == 8 else "k_grav"
+    what = ("symmetric direct sum, each pair once for both particles" if kname == "k_grav_sym"
+            else "tiled direct sum")
+    roofline = {"bound": "valu", "kernel": f"{kname} ({what}, FP32)",
                 "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
-                "traffic": GRAV_TRAFFIC_BYTES_1M if n == 1 << 20 else None,
+                "traffic": GRAV_TRAFFIC_BYTES_1M[kname] if n == 1 << 20 else None,
                 "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE)",
-                "algorithmic": "20 flop x N(N-1) pairs per launch",
-                "avg_ms": 1e3 * grav_s}
+                "algorithmic": "20 flop x N(N-1) pair interactions per launch",
+                "avg_ms": 1e3 * kern_s,
+                "gravity_phase_ms": g_ms / max(g_cnt, 1),
+                "launch_config": list(cfg_used)}
 
     state = pl.state(("pos", "vel", "u", "h", "rho", "acc_g"))
     pl.close()
@@ -662,6 +668,8 @@ def bench_kernels(a) -> int:
     del x, y
     e = ic.evrard(n, seed=a.seed)
     pl = SphPlan(n)
+    if a.grav:
+        pl.set_gravity_config(*(int(x) for x in a.grav.split(",")))
     pl.load(e.pos, e.vel, e.u, e.m, e.h)
     pl.init()
     pl.step(a.steps, a.dt)
@@ -669,7 +677,7 @@ def bench_kernels(a) -> int:
     pl.close()
     print(json.dumps({"workload": f"counter driver: {COPY_BYTES}-byte device copy, then Evrard "
                                   f"N={n} bootstrap + {a.steps} mixed steps", "n": n,
-                      "steps": a.steps, "copy_bytes": COPY_BYTES}))
+                      "steps": a.steps, "copy_bytes": COPY_BYTES, "grav": a.grav}))
     return 0
 
 

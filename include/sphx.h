@@ -386,6 +386,17 @@ int sphx_batch_runs(sphx_batch *b, int64_t *ids, int64_t *sizes, int64_t *steps,
  * counterpart in the reference, whose driver runs one simulation. */
 /* As sphx_plan_set_cooling_grid, for runs admitted with use_cooling = 2. */
 int sphx_batch_set_cooling_grid(sphx_batch *b, const sphx_cool_grid *grid);
+/* Gravity of the batch's runs from the symmetric direct sum (each pair
+ * evaluated once for both particles; block_pairs = 512-particle block pairs
+ * per wavefront) instead of the fixed tiled shape.  Its partial sums take
+ * sphx_gravity_symmetric_bytes(n, block_pairs) per run of n particles
+ * (about 12 n^2 / 1024 bytes); budget_bytes of device memory are reserved
+ * here and admissions that would exceed them are refused (SPHX_REFUSED).
+ * Only before the first admission.  A run's results do not depend on the
+ * other runs either way.  No counterpart in the reference (one simulation per
+ * process, acc_g of LIBV2:119-168 over all pairs). */
+int sphx_batch_set_gravity_symmetric(sphx_batch *b, int32_t block_pairs, int64_t budget_bytes);
+int64_t sphx_gravity_symmetric_bytes(int64_t n, int32_t block_pairs);
 /* As sphx_plan_set_chimes_table, for runs admitted with use_cooling = 3 (the
  * energy lookup only: batches keep no species fractions). */
 int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_table *table);

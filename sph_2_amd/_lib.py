@@ -142,6 +142,8 @@ SIGNATURES = {
     "sphx_plan_set_chimes_table": (C.c_int, [_V, _V]),
     "sphx_plan_chimes_ions": (C.c_int, [_V, _V, _V]),
     "sphx_batch_set_chimes_table": (C.c_int, [_V, _V]),
+    "sphx_batch_set_gravity_symmetric": (C.c_int, [_V, _I32, _I64]),
+    "sphx_gravity_symmetric_bytes": (C.c_int64, [_I64, _I32]),
     "sphx_read_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_write_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_cool_grid_build": (C.c_int, [_V, _V, _V, _V]),
@@ -175,7 +177,8 @@ SIGNATURES = {
     "sphx_batch_destroy": (C.c_int, [_V]),
 }
 
-PHASES = ("sort", "smoothing", "density", "gravity", "divcurl", "hydro", "finish")
+PHASES = ("sort", "smoothing", "density", "gravity", "divcurl", "hydro", "finish",
+          "gravity_kernel")
 
 
 def load(path: str = LIB_PATH):

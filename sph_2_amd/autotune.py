@@ -1,7 +1,8 @@
 """Launch-shape selection for the gravity kernel.
 
 choose_gravity_config(n) returns (ipl, tj, js): particles per lane, j-tile
-size and j split.  The defaults below keep >= 4 workgroups per CU on the
+size and j split of the tiled sum, or (8, 512, block pairs per wave) for the
+symmetric sum.  The defaults below keep >= 4 workgroups per CU on the
 256-CU MI355X; ``tune_gravity`` measures the candidates on the device and
 caches the winner per size class in a JSON file.
 """
@@ -13,6 +14,10 @@ import time
 
 CUS = 256
 CANDIDATES = [(1, 256), (1, 512), (2, 256), (2, 512), (2, 1024), (4, 256), (4, 512)]
+# Symmetric sum (ipl 8, blocks of 512): js = block pairs per wave.  Its
+# partial sums take 12 B x n^2 / 1024, so it is offered up to 2M particles.
+SYM_SEGMENTS = (1, 2, 4, 8, 16, 32, 64)
+SYM_MAX_N = 1 << 21
 _CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gravity_tuning.json")
 
 
@@ -68,6 +73,16 @@ def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
                 libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
             torch.cuda.synchronize()
             res[(ipl, tj, js)] = (time.perf_counter() - t0) / reps
+    if n <= SYM_MAX_N:
+        for S in SYM_SEGMENTS:
+            check(load().sphx_ctx_set_gravity_config(ctx, 8, 512, S), "config")
+            libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+            torch.cuda.synchronize()
+            res[(8, 512, S)] = (time.perf_counter() - t0) / reps
     best = min(res, key=res.get)
     check(load().sphx_ctx_set_gravity_config(ctx, *_default(n)), "config")
     if write:

@@ -113,6 +113,7 @@ Engine::~Engine() {
     hipGetDevice(&cur);
     hipSetDevice(device);
     if (bh) sphx_bh_host_free(bh);
+    if (grav.sym) sphx_sym_host_free(grav.sym);
     for (void *q : owned) hipFree(q);
     for (hipEvent_t e : tev)
         if (e) hipEventDestroy(e);
@@ -290,6 +291,7 @@ int Engine::tmark(hipStream_t st, int phase, int end) {
     hipEvent_t &e = tev[2 * phase + end];
     if (!e) SPHX_HIP(hipEventCreate(&e));
     SPHX_HIP(hipEventRecord(e, st));
+    if (end) tmarked[phase] = true;
     return SPHX_OK;
 }
 
@@ -297,7 +299,8 @@ int Engine::tcollect() {
     if (!timing) return SPHX_OK;
     for (int ph = 0; ph < PH_COUNT; ++ph) {
         hipEvent_t a = tev[2 * ph], b = tev[2 * ph + 1];
-        if (!a || !b) continue;
+        if (!a || !b || !tmarked[ph]) continue;
+        tmarked[ph] = false;
         SPHX_HIP(hipEventSynchronize(b));
         float ms = 0.f;
         if (hipEventElapsedTime(&ms, a, b) == hipSuccess) {
@@ -1503,10 +1506,18 @@ extern "C" int sphx_plan_destroy(sphx_plan *plan) {
 }
 
 // Gravity launch configuration of a plan (used by the autotuner).
+// Tiled sum: ipl 1/2/4 particles per lane, j tiles of 256/512/1024, js j
+// splits (1..64).  Symmetric sum: ipl 8, tj 512, js = block pairs per wave
+// (1..4096).
+static bool gravity_config_ok(int32_t ipl, int32_t tj, int32_t js) {
+    if (ipl == 8) return tj == 512 && js >= 1 && js <= 4096;
+    return (ipl == 1 || ipl == 2 || ipl == 4) && (tj == 256 || tj == 512 || tj == 1024) &&
+           js >= 1 && js <= 64 && !(ipl == 1 && tj == 1024) && !(ipl == 4 && tj == 1024);
+}
+
 extern "C" int sphx_plan_set_gravity_config(sphx_plan *plan, int32_t ipl, int32_t tj, int32_t js) {
     if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
-    if (!(ipl == 1 || ipl == 2 || ipl == 4) || !(tj == 256 || tj == 512 || tj == 1024) || js < 1 ||
-        js > 64 || (ipl == 1 && tj == 1024) || (ipl == 4 && tj == 1024)) {
+    if (!gravity_config_ok(ipl, tj, js)) {
         sphx_set_error("unsupported gravity configuration");
         return SPHX_ERR_ARG;
     }
@@ -1516,8 +1527,7 @@ extern "C" int sphx_plan_set_gravity_config(sphx_plan *plan, int32_t ipl, int32_
 
 extern "C" int sphx_ctx_set_gravity_config(sphx_ctx *ctx, int32_t ipl, int32_t tj, int32_t js) {
     if (!ctx) { sphx_set_error("null context"); return SPHX_ERR_ARG; }
-    if (!(ipl == 1 || ipl == 2 || ipl == 4) || !(tj == 256 || tj == 512 || tj == 1024) || js < 1 ||
-        js > 64 || (ipl == 1 && tj == 1024) || (ipl == 4 && tj == 1024)) {
+    if (!gravity_config_ok(ipl, tj, js)) {
         sphx_set_error("unsupported gravity configuration");
         return SPHX_ERR_ARG;
     }

@@ -208,6 +208,14 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
         sphx_set_error("admission refused: particle capacity or run slots exhausted");
         return SPHX_REFUSED;
     }
+    if (E.gcfg.ipl == 8) {
+        std::vector<int64_t> with_new = E.run_n;
+        with_new.push_back(n);
+        if (!sphx_sym_fits(E, with_new, E.gcfg.js)) {
+            sphx_set_error("admission refused: gravity partial-sum memory exhausted");
+            return SPHX_REFUSED;
+        }
+    }
     int64_t p0 = E.n;
     int r = (int)b->runs.size();
     kb_aos_to_soa<<<gb(n), BT, 0, st>>>(pos, n, E.s.x + p0, E.s.y + p0, E.s.z + p0);
@@ -322,6 +330,27 @@ extern "C" int sphx_batch_step(sphx_batch *b, int32_t n_steps, void *stream) {
     return SPHX_OK;
 }
 
+extern "C" int sphx_batch_set_gravity_symmetric(sphx_batch *b, int32_t block_pairs,
+                                                int64_t budget_bytes) {
+    void *stream = nullptr;
+    BATCH_PROLOGUE(b);
+    (void)st;
+    if (!b->runs.empty()) { sphx_set_error("set the gravity sum before admitting runs"); return SPHX_ERR_STATE; }
+    if (block_pairs < 1 || block_pairs > 4096 || budget_bytes < 0) {
+        sphx_set_error("block_pairs must be 1..4096 and the budget non-negative");
+        return SPHX_ERR_ARG;
+    }
+    SPHX_CHECK(sphx_sym_reserve(b->E, block_pairs, budget_bytes));
+    b->E.gcfg = GravConfig{8, 512, block_pairs};
+    b->E.drop_graph();
+    return SPHX_OK;
+}
+
+extern "C" int64_t sphx_gravity_symmetric_bytes(int64_t n, int32_t block_pairs) {
+    if (n <= 0 || block_pairs < 1) return 0;
+    return sphx_sym_run_bytes(n, block_pairs);
+}
+
 extern "C" int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_table *table) {
     void *stream = nullptr;
     BATCH_PROLOGUE(b);
@@ -420,6 +449,14 @@ extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *
     if (E.n + h.n > E.cap || (int)b->runs.size() >= E.max_runs) {
         sphx_set_error("admission refused: particle capacity or run slots exhausted");
         return SPHX_REFUSED;
+    }
+    if (E.gcfg.ipl == 8) {
+        std::vector<int64_t> with_new = E.run_n;
+        with_new.push_back(h.n);
+        if (!sphx_sym_fits(E, with_new, E.gcfg.js)) {
+            sphx_set_error("admission refused: gravity partial-sum memory exhausted");
+            return SPHX_REFUSED;
+        }
     }
     int64_t p0 = E.n, n = h.n;
     int r = (int)b->runs.size();

@@ -14,6 +14,9 @@
 // A j-tile whose AABB is farther than eps_max(I) + eps_max(J) from the
 // i-block's AABB has q > 2 for every pair and takes the branch-free 1/r^3
 // path (rsq + 12 VALU per pair); the few near tiles take the spline path.
+// A far tile whose particles all have the same mass (the usual SPH set-up)
+// sums r^-3 d without the mass and scales the tile's partial once (rsq + 11
+// VALU per pair).
 #include "sphx_internal.h"
 
 namespace {
@@ -35,7 +38,7 @@ struct GravIO {
 };
 
 __global__ void k_tile_boxes(Tables t, const int32_t *__restrict__ r_tbeg, int ntiles_total,
-                             int tj, double *__restrict__ tbox) {
+                             int tj, const double *__restrict__ m, double *__restrict__ tbox) {
     int tile = blockIdx.x;
     int lane = threadIdx.x;
     // find the run of this tile (runs are few; linear scan by lane 0 is enough)
@@ -65,9 +68,18 @@ __global__ void k_tile_boxes(Tables t, const int32_t *__restrict__ r_tbeg, int n
         v[a + 1] = wave_max_d(v[a + 1]);
     }
     v[6] = wave_max_d(v[6]);
+    // the tile's common mass, or -1 when its masses differ
+    int p0 = t.r_pbeg[r] + k * tj, p1 = min(t.r_pend[r], p0 + tj);
+    double mlo = INFINITY, mhi = -INFINITY;
+    for (int q = p0 + lane; q < p1; q += 64) {
+        mlo = fmin(mlo, m[q]);
+        mhi = fmax(mhi, m[q]);
+    }
+    mlo = wave_min_d(mlo);
+    mhi = wave_max_d(mhi);
     if (lane == 0) {
         for (int a = 0; a < 7; ++a) tbox[8 * tile + a] = v[a];
-        tbox[8 * tile + 7] = 0.0;
+        tbox[8 * tile + 7] = (mlo == mhi && mlo >= 0.0) ? mlo : -1.0;
     }
 }
 
@@ -79,7 +91,9 @@ struct IState {
     KahanF kx[IPL], ky[IPL], kz[IPL], kp[IPL];
 };
 
-template <int IPL, bool POT>
+// UNIF: every j of the tile has the same mass; the mass is left out here and
+// applied to the tile's partial sums in flush().
+template <int IPL, bool POT, bool UNIF>
 __device__ __forceinline__ void far_pair(IState<IPL> &S, float4 j) {
 #pragma unroll
     for (int p = 0; p < IPL; ++p) {
@@ -88,7 +102,7 @@ __device__ __forceinline__ void far_pair(IState<IPL> &S, float4 j) {
         float dz = j.z - S.z[p];
         float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         float ri = __builtin_amdgcn_rsqf(r2);
-        float mri = j.w * ri;
+        float mri = UNIF ? ri : j.w * ri;
         float s = mri * (ri * ri);
         S.ax[p] = __builtin_fmaf(s, dx, S.ax[p]);
         S.ay[p] = __builtin_fmaf(s, dy, S.ay[p]);
@@ -142,14 +156,16 @@ __device__ __forceinline__ void near_pair(IState<IPL> &S, float4 j, float ej, in
     }
 }
 
+// Fold the tile's FP32 partials, scaled by the tile's common mass when the
+// tile was summed without it (scale 1 otherwise), into the Kahan sums.
 template <int IPL>
-__device__ __forceinline__ void flush(IState<IPL> &S) {
+__device__ __forceinline__ void flush(IState<IPL> &S, float scale) {
 #pragma unroll
     for (int p = 0; p < IPL; ++p) {
-        S.kx[p].add(S.ax[p]); S.ax[p] = 0.f;
-        S.ky[p].add(S.ay[p]); S.ay[p] = 0.f;
-        S.kz[p].add(S.az[p]); S.az[p] = 0.f;
-        S.kp[p].add(S.ph[p]); S.ph[p] = 0.f;
+        S.kx[p].add(scale * S.ax[p]); S.ax[p] = 0.f;
+        S.ky[p].add(scale * S.ay[p]); S.ay[p] = 0.f;
+        S.kz[p].add(scale * S.az[p]); S.az[p] = 0.f;
+        S.kp[p].add(scale * S.ph[p]); S.ph[p] = 0.f;
     }
 }
 
@@ -249,13 +265,18 @@ __global__ __launch_bounds__(256) void k_grav(GravTabs gt, GravIO io) {
         double d2 = sphx_box_d2(lx, hx, ly, hy, lz, hz, tb[0], tb[1], tb[2], tb[3], tb[4], tb[5]);
         double reach = em + tb[6];
         bool far = d2 > reach * reach * (1.0 + 1e-6);
-        if (far) {
+        float scale = 1.0f;
+        if (far && tb[7] >= 0.0) {
+            scale = (float)tb[7];
 #pragma unroll 8
-            for (int q = 0; q < TJ; ++q) far_pair<IPL, POT>(S, jt[q]);
+            for (int q = 0; q < TJ; ++q) far_pair<IPL, POT, true>(S, jt[q]);
+        } else if (far) {
+#pragma unroll 8
+            for (int q = 0; q < TJ; ++q) far_pair<IPL, POT, false>(S, jt[q]);
         } else {
             for (int q = 0; q < TJ; ++q) near_pair<IPL, POT>(S, jt[q], je[q], j0 + q);
         }
-        flush<IPL>(S);
+        flush<IPL>(S, scale);
     }
 
 #pragma unroll
@@ -298,9 +319,457 @@ int launch_grav(const GravTabs &gt, const GravIO &io, int nblocks, bool pot, hip
     return SPHX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric direct sum (configuration ipl = 8).  Each pair is evaluated once
+// and applied to both particles (Newton's third law).  The run's particles
+// form blocks of SYM_B; a wave holds one block as its i-set (8 per lane) and
+// walks a segment of the upper triangle of block pairs (row I, blocks
+// J0..J1-1 >= I).  Per 128-particle chunk of block J, two sets of 64 j
+// particles sit one per lane and rotate one lane per step (ds_bpermute, on the
+// LDS pipe) together with their own force accumulators, so after 64 steps
+// every (i, j) pair of the chunk has been met once and each j is back in its
+// lane with its sum over the 512 i.  The j-side sums of row I go to their own
+// slots of a partial buffer and the i-side sums of each segment to theirs;
+// k_grav_sym_reduce adds them in a fixed order, so results do not depend on
+// scheduling.  The diagonal block pair (J = I) is summed one-sided.  Far block
+// pairs whose blocks each have one common mass use no mass in the pair and
+// scale the chunk sums (rsq + 14 VALU per pair for two interactions).
+// ---------------------------------------------------------------------------
+constexpr int SYM_B = 512;
+
+struct SymTabs {
+    const int32_t *b_beg, *b_cnt;  // blocks (SYM_B particles; a run's last block partial)
+    const int32_t *run_b0;         // first block of each run (+ total)
+    const int32_t *seg_i, *seg_j0, *seg_j1;  // segments of the block-pair triangle
+    const int64_t *row_off;        // per block: first j-side partial of its row
+    const int32_t *row_j0;         // per block: first particle after the block in its run
+    const int32_t *row_nseg;       // per block: segments of its row
+    const int32_t *row_seg0;       // per block: first (global) segment of its row
+    const double *bbox;            // per block: box(6), max eps, common mass or -1
+    float *px, *py, *pz;           // j-side partials
+    double *qx, *qy, *qz;          // i-side partials, SYM_B slots per segment
+    int32_t seg0;                  // global index of the launch's first segment
+    const int32_t *run;            // run of each particle
+    const int32_t *r_pbeg;         // first particle of each run
+    int64_t n;
+};
+
+__device__ __forceinline__ float sym_rot(float v, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
+// The spline-softened force factor of near_pair, without mass or potential.
+__device__ __forceinline__ float sym_spline_f(float r2, float e) {
+    if (r2 >= 4.0f * e * e) {
+        float ri = __builtin_amdgcn_rsqf(r2);
+        return ri * ri * ri;
+    }
+    float r = r2 > 0.0f ? __builtin_amdgcn_sqrtf(r2) : 0.0f;
+    float ie = __builtin_amdgcn_rcpf(e);
+    float q = r * ie;
+    if (q <= 1.0f) {
+        float q2 = q * q;
+        return ie * ie * ie * (4.0f / 3.0f - 1.2f * q2 + 0.5f * q2 * q);
+    }
+    float w = 2.0f - q;
+    float w2 = w * w;
+    float ri = __builtin_amdgcn_rcpf(r);
+    return ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
+}
+
+struct SymI {
+    float x[8], y[8], z[8], m[8], e[8];
+    float ax[8], ay[8], az[8];
+};
+
+struct SymJ {
+    float x, y, z, m, e, fx, fy, fz;
+};
+
+// j particle q of a block (lane-relative), relative to the i-block centre;
+// positions past the block's end hold massless particles far from any i.
+__device__ __forceinline__ SymJ sym_load(const double *x, const double *y, const double *z,
+                                         const double *m, const double *eps, int jb, int jc,
+                                         int q, double ox, double oy, double oz) {
+    SymJ J;
+    if (q < jc) {
+        int j = jb + q;
+        J.x = (float)(x[j] - ox);
+        J.y = (float)(y[j] - oy);
+        J.z = (float)(z[j] - oz);
+        J.m = (float)m[j];
+        J.e = (float)eps[j];
+    } else {
+        J.x = J.y = J.z = -1.0e18f;
+        J.m = 0.0f;
+        J.e = 1.0f;
+    }
+    J.fx = J.fy = J.fz = 0.0f;
+    return J;
+}
+
+// MODE 0: far, common masses (applied later); 1: far; 2: near; 3: diagonal
+// block (one-sided, spline).
+template <int MODE>
+__device__ __forceinline__ void sym_pairs(SymI &S, SymJ &J) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        float dx = J.x - S.x[p];
+        float dy = J.y - S.y[p];
+        float dz = J.z - S.z[p];
+        float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        float f;
+        if (MODE <= 1) {
+            float ri = __builtin_amdgcn_rsqf(r2);
+            f = (ri * ri) * ri;
+        } else {
+            f = sym_spline_f(r2, 0.5f * (S.e[p] + J.e));
+        }
+        float si = MODE == 0 ? f : J.m * f;
+        S.ax[p] = __builtin_fmaf(si, dx, S.ax[p]);
+        S.ay[p] = __builtin_fmaf(si, dy, S.ay[p]);
+        S.az[p] = __builtin_fmaf(si, dz, S.az[p]);
+        if (MODE != 3) {
+            float sj = MODE == 0 ? f : S.m[p] * f;
+            J.fx = __builtin_fmaf(-sj, dx, J.fx);
+            J.fy = __builtin_fmaf(-sj, dy, J.fy);
+            J.fz = __builtin_fmaf(-sj, dz, J.fz);
+        }
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void sym_rotate(SymJ &J, int addr) {
+    J.x = sym_rot(J.x, addr);
+    J.y = sym_rot(J.y, addr);
+    J.z = sym_rot(J.z, addr);
+    if (MODE != 0) J.m = sym_rot(J.m, addr);
+    if (MODE >= 2) J.e = sym_rot(J.e, addr);
+    if (MODE != 3) {
+        J.fx = sym_rot(J.fx, addr);
+        J.fy = sym_rot(J.fy, addr);
+        J.fz = sym_rot(J.fz, addr);
+    }
+}
+
+template <int MODE>
+__device__ __forceinline__ void sym_chunk(SymI &S, SymJ &A, SymJ &B, int addr) {
+    for (int s = 0; s < 64; ++s) {
+        sym_pairs<MODE>(S, A);
+        sym_rotate<MODE>(A, addr);
+        sym_pairs<MODE>(S, B);
+        sym_rotate<MODE>(B, addr);
+    }
+}
+
+// Three waves per SIMD: measured 4.5 % faster than the two the registers allow
+// by default (the spills it forces sit outside the pair loops) and than four.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grav_sym(
+    SymTabs T, const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ z,
+                                                 const double *__restrict__ m,
+                                                 const double *__restrict__ eps) {
+    const int lane = threadIdx.x;
+    const int sg = blockIdx.x;
+    const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
+    const int beg = T.b_beg[I], cnt = T.b_cnt[I];
+    const double *bI = T.bbox + 8 * (int64_t)I;
+    const double ox = 0.5 * (bI[0] + bI[1]), oy = 0.5 * (bI[2] + bI[3]), oz = 0.5 * (bI[4] + bI[5]);
+    const double mI = bI[7];
+    const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
+
+    SymI S;
+    KahanF kx[8], ky[8], kz[8];
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        int li = lane + 64 * p;
+        bool ok = li < cnt;
+        int i = beg + (ok ? li : 0);
+        S.x[p] = ok ? (float)(x[i] - ox) : 1.0e18f;
+        S.y[p] = ok ? (float)(y[i] - oy) : 1.0e18f;
+        S.z[p] = ok ? (float)(z[i] - oz) : 1.0e18f;
+        S.m[p] = ok ? (float)m[i] : 0.0f;
+        S.e[p] = ok ? (float)eps[i] : 1.0f;
+        S.ax[p] = S.ay[p] = S.az[p] = 0.0f;
+        kx[p].init(); ky[p].init(); kz[p].init();
+    }
+    const int64_t roff = T.row_off[I];
+    const int rj0 = T.row_j0[I];
+
+    for (int J = J0; J < J1; ++J) {
+        const int jb = T.b_beg[J], jc = T.b_cnt[J];
+        const double *bJ = T.bbox + 8 * (int64_t)J;
+        int mode = 3;
+        if (J != I) {
+            double d2 = sphx_box_d2(bI[0], bI[1], bI[2], bI[3], bI[4], bI[5], bJ[0], bJ[1], bJ[2],
+                                    bJ[3], bJ[4], bJ[5]);
+            double reach = bI[6] + bJ[6];
+            bool far = d2 > reach * reach * (1.0 + 1e-6);
+            mode = far ? ((mI >= 0.0 && bJ[7] >= 0.0) ? 0 : 1) : 2;
+        }
+        const float si = mode == 0 ? (float)bJ[7] : 1.0f;  // i-side scale: the j block's mass
+        const float sj = mode == 0 ? (float)mI : 1.0f;     // j-side scale: the i block's mass
+        for (int c = 0; c < jc; c += 128) {
+            SymJ A = sym_load(x, y, z, m, eps, jb, jc, c + lane, ox, oy, oz);
+            SymJ B = sym_load(x, y, z, m, eps, jb, jc, c + 64 + lane, ox, oy, oz);
+            if (mode == 0) sym_chunk<0>(S, A, B, addr);
+            else if (mode == 1) sym_chunk<1>(S, A, B, addr);
+            else if (mode == 2) sym_chunk<2>(S, A, B, addr);
+            else sym_chunk<3>(S, A, B, addr);
+            if (mode != 3) {
+                int qa = c + lane, qb = c + 64 + lane;
+                if (qa < jc) {
+                    int64_t o = roff + (jb + qa - rj0);
+                    T.px[o] = sj * A.fx; T.py[o] = sj * A.fy; T.pz[o] = sj * A.fz;
+                }
+                if (qb < jc) {
+                    int64_t o = roff + (jb + qb - rj0);
+                    T.px[o] = sj * B.fx; T.py[o] = sj * B.fy; T.pz[o] = sj * B.fz;
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                kx[p].add(si * S.ax[p]); S.ax[p] = 0.0f;
+                ky[p].add(si * S.ay[p]); S.ay[p] = 0.0f;
+                kz[p].add(si * S.az[p]); S.az[p] = 0.0f;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < 8; ++p) {
+        int li = lane + 64 * p;
+        if (li < cnt) {
+            int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
+            T.qx[o] = kx[p].value();
+            T.qy[o] = ky[p].value();
+            T.qz[o] = kz[p].value();
+        }
+    }
+}
+
+// a_i = G (sum over the segments of i's row of their i-side sums, in order,
+// + sum over the earlier rows of the run of their j-side sums for i, in order).
+__global__ void k_grav_sym_reduce(SymTabs T, double G, int64_t p0, int64_t p1, double *gx,
+                                  double *gy, double *gz) {
+    int64_t i = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p1) return;
+    int r = T.run[i];
+    int b0 = T.run_b0[r];
+    int I = b0 + (int)((i - T.r_pbeg[r]) / SYM_B);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    int ns = T.row_nseg[I], sq = T.row_seg0[I];
+    int64_t li = i - T.b_beg[I];
+    for (int k = 0; k < ns; ++k) {
+        int64_t o = (int64_t)(sq + k) * SYM_B + li;
+        sx += T.qx[o]; sy += T.qy[o]; sz += T.qz[o];
+    }
+    for (int I2 = b0; I2 < I; ++I2) {
+        int64_t o = T.row_off[I2] + (i - T.row_j0[I2]);
+        sx += (double)T.px[o]; sy += (double)T.py[o]; sz += (double)T.pz[o];
+    }
+    gx[i] = G * sx;
+    gy[i] = G * sy;
+    gz[i] = G * sz;
+}
+
 }  // namespace
 
+// Host tables of the symmetric sum; rebuilt when the run sizes or the segment
+// length change.
+struct SymHost {
+    int seg_len = 0;
+    std::vector<int64_t> sizes;
+    int nblocks = 0, nseg = 0;
+    std::vector<int32_t> run_seg0;  // first segment of each run (+ total)
+    int32_t *dev_i = nullptr;       // int32 tables
+    int64_t dev_i_cap = 0;
+    int64_t *row_off = nullptr;
+    int64_t row_off_cap = 0;
+    double *bbox = nullptr;
+    int64_t bbox_cap = 0;
+    float *part = nullptr;          // 3 x total row length
+    int64_t part_cap = 0, part_len = 0;
+    double *q = nullptr;            // 3 x SYM_B per segment
+    int64_t q_cap = 0;
+    int64_t budget = -1;            // bytes the partial sums may take (-1: no limit)
+    double *pool = nullptr;         // reserved memory of a budget (i-side, then j-side sums)
+    int o_bbeg = 0, o_bcnt = 0, o_runb0 = 0, o_si = 0, o_sj0 = 0, o_sj1 = 0, o_rj0 = 0, o_rns = 0,
+        o_rs0 = 0;
+};
+
+void sphx_sym_host_free(SymHost *h) { delete h; }
+
+static int64_t sym_blocks(int64_t n) { return (n + SYM_B - 1) / SYM_B; }
+
+// Bytes of partial sums one run of n particles needs at segment length S:
+// 12 per j-side slot (row a of nb blocks holds the particles after block a)
+// and 24 x SYM_B per segment.
+int64_t sphx_sym_run_bytes(int64_t n, int S) {
+    int64_t nb = sym_blocks(n), slots = 0, segs = 0;
+    for (int64_t a = 0; a < nb; ++a) {
+        slots += n - std::min(n, (a + 1) * SYM_B);
+        segs += (nb - a + S - 1) / S;
+    }
+    return 12 * slots + 24 * SYM_B * segs;
+}
+
+// Fix the partial-sum memory of an engine (served batches): allocated once;
+// runs whose partials would not fit are refused at admission.
+int sphx_sym_reserve(Engine &E, int S, int64_t budget_bytes) {
+    if (!E.grav.sym) E.grav.sym = new SymHost();
+    SymHost &H = *E.grav.sym;
+    if (H.budget >= 0) { sphx_set_error("symmetric gravity memory already reserved"); return SPHX_ERR_STATE; }
+    SPHX_CHECK(E.alloc((void **)&H.pool, (size_t)std::max<int64_t>(budget_bytes, 64)));
+    H.budget = budget_bytes;
+    H.seg_len = 0;  // rebuild the tables at the next launch
+    (void)S;
+    return SPHX_OK;
+}
+
+// Whether the runs `sizes` fit the reserved partial-sum memory at segment
+// length S (always true without a reservation).
+bool sphx_sym_fits(const Engine &E, const std::vector<int64_t> &sizes, int S) {
+    if (!E.grav.sym || E.grav.sym->budget < 0) return true;
+    int64_t bytes = 0;
+    for (int64_t n : sizes) bytes += sphx_sym_run_bytes(n, S);
+    return bytes <= E.grav.sym->budget;
+}
+
+static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
+    if (!E.grav.sym) E.grav.sym = new SymHost();
+    SymHost &H = *E.grav.sym;
+    const int S = cfg.js;
+    if (H.seg_len == S && H.sizes == E.run_n) return SPHX_OK;
+    std::vector<int32_t> bbeg, bcnt, runb0, si, sj0, sj1, rj0, rns, rs0;
+    std::vector<int64_t> roff;
+    std::vector<int32_t> run_seg0;
+    int64_t off = 0, plen = 0;
+    for (int r = 0; r < E.R; ++r) {
+        int64_t nr = E.run_n[r];
+        int b0 = (int)bbeg.size();
+        runb0.push_back(b0);
+        run_seg0.push_back((int32_t)si.size());
+        for (int64_t a = 0; a < nr; a += SYM_B) {
+            bbeg.push_back((int32_t)(off + a));
+            bcnt.push_back((int32_t)std::min<int64_t>(SYM_B, nr - a));
+        }
+        int b1 = (int)bbeg.size();
+        for (int I = b0; I < b1; ++I) {
+            int64_t j0 = (int64_t)bbeg[I] + bcnt[I];  // first particle after block I
+            rj0.push_back((int32_t)j0);
+            roff.push_back(plen);
+            plen += (off + nr) - j0;
+            rs0.push_back((int32_t)si.size());
+            int k = 0;
+            for (int J = I; J < b1; J += S, ++k) {
+                si.push_back(I);
+                sj0.push_back(J);
+                sj1.push_back(std::min(b1, J + S));
+            }
+            rns.push_back(k);
+        }
+        off += nr;
+    }
+    runb0.push_back((int32_t)bbeg.size());
+    run_seg0.push_back((int32_t)si.size());
+    int nb = (int)bbeg.size(), ns = (int)si.size();
+    int64_t need_p = 3 * plen, need_q = 3 * (int64_t)SYM_B * ns;
+    if (H.budget >= 0) {
+        if (8 * need_q + 4 * need_p > H.budget) {
+            sphx_set_error("runs exceed the reserved memory of the symmetric gravity sum");
+            return SPHX_ERR_STATE;
+        }
+    } else {
+        double need_gb = (4.0 * need_p + 8.0 * need_q) / 1e9;
+        if (need_gb > 96.0) {
+            sphx_set_error("symmetric gravity would need " + std::to_string((int)need_gb) +
+                           " GB of partial sums for these runs; use a tiled configuration");
+            return SPHX_ERR_ARG;
+        }
+    }
+    std::vector<int32_t> all;
+    auto put = [&](const std::vector<int32_t> &v) {
+        int o = (int)all.size();
+        all.insert(all.end(), v.begin(), v.end());
+        return o;
+    };
+    int o_bbeg = put(bbeg), o_bcnt = put(bcnt), o_runb0 = put(runb0), o_si = put(si),
+        o_sj0 = put(sj0), o_sj1 = put(sj1), o_rj0 = put(rj0), o_rns = put(rns), o_rs0 = put(rs0);
+    if ((int64_t)all.size() > H.dev_i_cap) {
+        SPHX_CHECK(E.alloc((void **)&H.dev_i, sizeof(int32_t) * (all.size() + 1024)));
+        H.dev_i_cap = (int64_t)all.size() + 1024;
+    }
+    if (nb > H.row_off_cap) {
+        SPHX_CHECK(E.alloc((void **)&H.row_off, sizeof(int64_t) * (nb + 64)));
+        H.row_off_cap = nb + 64;
+    }
+    if (8 * (int64_t)nb > H.bbox_cap) {
+        SPHX_CHECK(E.alloc((void **)&H.bbox, sizeof(double) * (8 * (int64_t)nb + 64)));
+        H.bbox_cap = 8 * (int64_t)nb + 64;
+    }
+    if (H.budget >= 0) {
+        H.q = H.pool;                                // carved from the reservation
+        H.part = reinterpret_cast<float *>(H.pool + need_q);
+    } else {
+        if (need_p > H.part_cap) {
+            SPHX_CHECK(E.alloc((void **)&H.part, sizeof(float) * (need_p + 64)));
+            H.part_cap = need_p + 64;
+        }
+        if (need_q > H.q_cap) {
+            SPHX_CHECK(E.alloc((void **)&H.q, sizeof(double) * (need_q + 64)));
+            H.q_cap = need_q + 64;
+        }
+    }
+    if (!all.empty())
+        SPHX_HIP(hipMemcpyAsync(H.dev_i, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice, st));
+    if (nb > 0)
+        SPHX_HIP(hipMemcpyAsync(H.row_off, roff.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    H.o_bbeg = o_bbeg; H.o_bcnt = o_bcnt; H.o_runb0 = o_runb0; H.o_si = o_si; H.o_sj0 = o_sj0;
+    H.o_sj1 = o_sj1; H.o_rj0 = o_rj0; H.o_rns = o_rns; H.o_rs0 = o_rs0;
+    H.run_seg0 = run_seg0;
+    H.seg_len = S;
+    H.sizes = E.run_n;
+    H.nblocks = nb;
+    H.nseg = ns;
+    H.part_len = plen;
+    return SPHX_OK;
+}
+
+static int sym_run(Engine &E, hipStream_t st) {
+    SymHost &H = *E.grav.sym;
+    int32_t *d = H.dev_i;
+    int s0 = 0, s1 = H.nseg;
+    int64_t p0 = 0, p1 = E.n;
+    if (E.rrun >= 0) {
+        s0 = H.run_seg0[E.rrun];
+        s1 = H.run_seg0[E.rrun + 1];
+        p0 = E.rp0;
+        p1 = E.rp1;
+    }
+    if (s1 <= s0) return SPHX_OK;
+    k_tile_boxes<<<H.nblocks, 64, 0, st>>>(E.t, d + H.o_runb0, H.nblocks, SYM_B, E.s.m, H.bbox);
+    SPHX_HIP(hipGetLastError());
+    int64_t pl = H.part_len, ql = (int64_t)SYM_B * H.nseg;
+    SymTabs T{d + H.o_bbeg, d + H.o_bcnt, d + H.o_runb0, d + H.o_si + s0, d + H.o_sj0 + s0,
+              d + H.o_sj1 + s0, H.row_off, d + H.o_rj0, d + H.o_rns, d + H.o_rs0, H.bbox,
+              H.part, H.part + pl, H.part + 2 * pl, H.q, H.q + ql, H.q + 2 * ql, s0,
+              E.s.run, E.t.r_pbeg, E.n};
+    const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
+    SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
+    k_grav_sym<<<(unsigned)(s1 - s0), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
+    SPHX_HIP(hipGetLastError());
+    SPHX_CHECK(E.tmark(st, PH_GRAVK, 1));
+    k_grav_sym_reduce<<<(unsigned)((p1 - p0 + 255) / 256), 256, 0, st>>>(T, E.p.G, p0, p1, E.s.gx,
+                                                                         E.s.gy, E.s.gz);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
 int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
+    if (E.n == 0) return SPHX_OK;
+    if (cfg.ipl == 8) return sym_prepare(E, cfg, st);
     if (E.n == 0) return SPHX_OK;
     GravHost &H = E.grav;
     int ib = 256 * cfg.ipl;
@@ -358,8 +827,14 @@ int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     return SPHX_OK;
 }
 
-int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_t st) {
+int sphx_gravity_run(Engine &E, const GravConfig &cfg_in, bool with_pot, hipStream_t st) {
     if (E.n == 0) return SPHX_OK;
+    if (cfg_in.ipl == 8 && !with_pot) {
+        SPHX_CHECK(sym_prepare(E, cfg_in, st));
+        return sym_run(E, st);
+    }
+    // the symmetric sum carries no potential: that call uses the tiled sum
+    const GravConfig cfg = cfg_in.ipl == 8 ? GravConfig{2, 512, 1} : cfg_in;
     SPHX_CHECK(sphx_gravity_prepare(E, cfg, st));
     GravHost &H = E.grav;
     int32_t *d = H.dev;
@@ -374,13 +849,15 @@ int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_
     GravTabs gt{d + ib0, d + H.nblocks + ib0, d + 2 * H.nblocks + ib0, E.t.r_pbeg, E.t.r_pend,
                 d + 3 * H.nblocks, H.tbox, cfg.js};
     if (H.ntiles > 0) {
-        k_tile_boxes<<<H.ntiles, 64, 0, st>>>(E.t, d + 3 * H.nblocks, H.ntiles, cfg.tj, H.tbox);
+        k_tile_boxes<<<H.ntiles, 64, 0, st>>>(E.t, d + 3 * H.nblocks, H.ntiles, cfg.tj, E.s.m,
+                                              H.tbox);
         SPHX_HIP(hipGetLastError());
     }
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
     GravIO io{E.s.x, E.s.y, E.s.z, E.s.m, eps, E.p.G, E.s.gx, E.s.gy, E.s.gz, E.s.pot, H.part, E.n};
     int nblk = ib1 - ib0;
     int rc;
+    SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
     switch (cfg.ipl * 10000 + cfg.tj) {
         case 10256: rc = launch_grav<1, 256>(gt, io, nblk, with_pot, st); break;
         case 10512: rc = launch_grav<1, 512>(gt, io, nblk, with_pot, st); break;
@@ -394,6 +871,7 @@ int sphx_gravity_run(Engine &E, const GravConfig &cfg, bool with_pot, hipStream_
             return SPHX_ERR_ARG;
     }
     SPHX_CHECK(rc);
+    SPHX_CHECK(E.tmark(st, PH_GRAVK, 1));
     if (cfg.js > 1) {
         int64_t p0 = 0, p1 = E.n;
         if (E.rrun >= 0) {

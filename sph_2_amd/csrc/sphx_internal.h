@@ -174,6 +174,7 @@ enum {
     PH_DIVCURL,
     PH_HYDRO,
     PH_FINISH,     // combine + half kick + u update + cooling
+    PH_GRAVK,      // the direct-sum gravity kernel alone (inside PH_GRAV)
     PH_COUNT
 };
 
@@ -245,7 +246,12 @@ struct GravHost {
     double *part = nullptr;  // j-split partial sums
     int64_t part_cap = 0;
     std::vector<int64_t> sizes;
+    struct SymHost *sym = nullptr;  // tables of the symmetric sum (configuration ipl = 8)
 };
+void sphx_sym_host_free(struct SymHost *h);
+int64_t sphx_sym_run_bytes(int64_t n, int S);
+int sphx_sym_reserve(struct Engine &E, int S, int64_t budget_bytes);
+bool sphx_sym_fits(const struct Engine &E, const std::vector<int64_t> &sizes, int S);
 
 struct Engine {
     int device = 0;
@@ -342,6 +348,7 @@ struct Engine {
     // optional per-phase HIP-event timing of the step
     bool timing = false;
     hipEvent_t tev[2 * 8] = {};
+    bool tmarked[8] = {};      // phase recorded since the last collection
     double phase_ms[8] = {};
     int64_t phase_n[8] = {};
     int tmark(hipStream_t st, int phase, int end);

@@ -69,6 +69,18 @@ class Batch:
         g = grid.c()
         check(_lib.load().sphx_batch_set_cooling_grid(self._h, C.byref(g)), "set_cooling_grid")
 
+    def set_gravity_symmetric(self, block_pairs: int, budget_bytes: int):
+        """Symmetric direct-sum gravity for every run (before the first admission),
+        with budget_bytes reserved for its partial sums."""
+        check(_lib.load().sphx_batch_set_gravity_symmetric(self._h, int(block_pairs),
+                                                           int(budget_bytes)),
+              "set_gravity_symmetric")
+        self.sym = (int(block_pairs), int(budget_bytes))
+
+    @staticmethod
+    def symmetric_bytes(n: int, block_pairs: int) -> int:
+        return int(_lib.load().sphx_gravity_symmetric_bytes(int(n), int(block_pairs)))
+
     def set_chimes_table(self, table):
         """Attach a libsx.ChimesTable for runs admitted with cooling="chimes"."""
         c = table.c()
@@ -204,17 +216,25 @@ class Server:
     """Admit by HBM budget, step every live run together, retire when done."""
 
     def __init__(self, hbm_budget_bytes: int, max_runs: int = 512, graph: bool = True,
-                 cooling: CoolingTable | None = None):
-        self.capacity = int(hbm_budget_bytes // Batch.bytes_per_particle())
+                 cooling: CoolingTable | None = None, symmetric: tuple | None = (4, 0.5)):
+        """symmetric: (block pairs per wave, share of the budget for the
+        symmetric gravity sum's partials), or None for the tiled sum."""
+        share = symmetric[1] if symmetric else 0.0
+        self.capacity = int(hbm_budget_bytes * (1.0 - share) // Batch.bytes_per_particle())
         self.batch = Batch(self.capacity, max_runs, cooling=cooling or default_cooling(),
                            graph=graph)
+        self.symmetric = symmetric
+        if symmetric:
+            self.sym_budget = int(hbm_budget_bytes * share)
+            self.batch.set_gravity_symmetric(symmetric[0], self.sym_budget)
 
     def run(self, requests: list[Request], keep_results: bool = False, log=None) -> dict:
         torch = _lib.torch()
         for r in requests:
             if r.payload is None:
                 r.payload = build_payload(r)
-            if r.n > self.capacity:
+            if r.n > self.capacity or (self.symmetric and Batch.symmetric_bytes(
+                    r.n, self.symmetric[0]) > self.sym_budget):
                 raise ValueError(f"request {r.rid} (N={r.n}) exceeds the whole budget")
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         live: dict[int, tuple[Request, float, int]] = {}   # run id -> (req, admit time, steps)

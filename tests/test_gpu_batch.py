@@ -126,8 +126,40 @@ def test_serving_loop_small(gpu):
 
     for r in reqs[:3]:
         solo = Batch(20000, 8, cooling=default_cooling())
+        solo.set_gravity_symmetric(srv.symmetric[0], srv.sym_budget)
         rid = solo.admit(build_payload(r), r.dt, cooling=r.kind == "cloud")
         solo.step(r.steps)
         alone = solo.retire(rid)
         solo.close()
         _same(alone, rep["results"][r.rid])
+
+
+def test_symmetric_gravity_alone_equals_in_the_mix(gpu, runs):
+    """With the symmetric gravity sum a run's result still does not depend on
+    the other runs; admissions beyond the reserved partial-sum memory are
+    refused."""
+    from sph_2_amd.serve import Batch
+
+    a, b, c, cool = runs
+    need = Batch.symmetric_bytes(a.n, 2)
+    outs = []
+    for mixed in (False, True):
+        bt = Batch(60000, 8, cooling=cool)
+        bt.set_gravity_symmetric(2, 64 * 2 ** 20)
+        if mixed:
+            rb = bt.admit(b, 1e-4)
+            bt.step(2)
+        ra = bt.admit(a, 2e-4)
+        bt.step(2)
+        if mixed:
+            bt.retire(rb)
+            rc = bt.admit(c, 1e-4, cooling=True)
+        bt.step(2)
+        outs.append(bt.retire(ra))
+        bt.close()
+    _same(outs[0], outs[1])
+    small = Batch(60000, 8)
+    small.set_gravity_symmetric(2, int(need * 1.5))
+    assert small.admit(a, 1e-4) is not None
+    assert small.admit(a, 1e-4) is None            # the second copy does not fit
+    small.close()
