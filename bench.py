@@ -77,6 +77,9 @@ def main(argv=None) -> int:
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve: HBM budget")
     ap.add_argument("--eps", type=float, default=None,
                     help="gadget: fixed gravitational softening (default eps = h)")
+    ap.add_argument("--serve-gravity", default="4,0.5",
+                    help="serve: 'S,share' for the symmetric gravity sum (S block pairs per "
+                         "wave, share of the budget for its partials) or 'tiled'")
     ap.add_argument("--serve-runs", type=int, default=64,
                     help="requests in the served-run latency sample of the default run (0: skip)")
     a = ap.parse_args(argv)
@@ -508,11 +511,14 @@ def bench_serve(a) -> int:
     for r in reqs:
         r.payload = build_payload(r)
     log(f"serve: {len(reqs)} requests built in {time.perf_counter() - t0:.1f}s")
-    srv = Server(int(a.hbm_gb * 2 ** 30), max_runs=512, graph=bool(a.graph))
+    sym = None if a.serve_gravity == "tiled" else (int(a.serve_gravity.split(",")[0]),
+                                                    float(a.serve_gravity.split(",")[1]))
+    srv = Server(int(a.hbm_gb * 2 ** 30), max_runs=512, graph=bool(a.graph), symmetric=sym)
     rep = srv.run(reqs, log=lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True))
     rep["workload"] = (f"c5: {a.runs} runs, N log-uniform in [{a.nmin}, {a.nmax}], "
                        f"{a.run_steps} steps each, Poisson arrivals at {a.rate}/s, "
-                       f"HBM budget {a.hbm_gb} GB, graph={bool(a.graph)}")
+                       f"HBM budget {a.hbm_gb} GB, graph={bool(a.graph)}, gravity "
+                       f"{'tiled' if sym is None else f'symmetric ({sym[0]} block pairs per wave, {sym[1]:.0%} of the budget for its partial sums)'}")
     print(json.dumps(rep))
     return 0
 
