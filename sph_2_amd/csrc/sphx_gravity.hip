@@ -462,19 +462,42 @@ __device__ __forceinline__ void sym_chunk(SymI &S, SymJ &A, SymJ &B, int addr) {
     }
 }
 
-// Three waves per SIMD: measured 4.5 % faster than the two the registers allow
-// by default (the spills it forces sit outside the pair loops) and than four.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grav_sym(
-    SymTabs T, const double *__restrict__ x,
-                                                 const double *__restrict__ y,
-                                                 const double *__restrict__ z,
-                                                 const double *__restrict__ m,
-                                                 const double *__restrict__ eps) {
+// Block-pair mode of tile J for row I (wave-uniform): 3 diagonal, 2 near
+// (spline path), 0 far with common masses on both sides, 1 far.
+__device__ __forceinline__ int sym_mode(const double *bI, const double *bJ, bool diag) {
+    if (diag) return 3;
+    double d2 = sphx_box_d2(bI[0], bI[1], bI[2], bI[3], bI[4], bI[5], bJ[0], bJ[1], bJ[2], bJ[3],
+                            bJ[4], bJ[5]);
+    double reach = bI[6] + bJ[6];
+    bool far = d2 > reach * reach * (1.0 + 1e-6);
+    return far ? ((bI[7] >= 0.0 && bJ[7] >= 0.0) ? 0 : 1) : 2;
+}
+
+// One segment of row I, restricted to the far block pairs (FAR) or to the
+// near and diagonal ones.  The two passes run as separate kernels so that
+// the far pass, nearly all of the work, keeps the register budget of its own
+// code (three waves per SIMD without spills in the pair loops).  The far pass
+// writes the segment's i-side sums; the near pass, launched after it on the
+// same stream, adds its own to them.  Every j-side slot belongs to exactly
+// one block pair and so to exactly one pass.
+template <bool FAR>
+__device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__restrict__ x,
+                                            const double *__restrict__ y,
+                                            const double *__restrict__ z,
+                                            const double *__restrict__ m,
+                                            const double *__restrict__ eps) {
     const int lane = threadIdx.x;
     const int sg = blockIdx.x;
     const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
     const int beg = T.b_beg[I], cnt = T.b_cnt[I];
     const double *bI = T.bbox + 8 * (int64_t)I;
+    if (!FAR) {
+        // most segments hold no near pair: leave before loading the i block
+        bool any = false;
+        for (int J = J0; J < J1 && !any; ++J)
+            any = sym_mode(bI, T.bbox + 8 * (int64_t)J, J == I) >= 2;
+        if (!any) return;
+    }
     const double ox = 0.5 * (bI[0] + bI[1]), oy = 0.5 * (bI[2] + bI[3]), oz = 0.5 * (bI[4] + bI[5]);
     const double mI = bI[7];
     const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
@@ -500,23 +523,43 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     for (int J = J0; J < J1; ++J) {
         const int jb = T.b_beg[J], jc = T.b_cnt[J];
         const double *bJ = T.bbox + 8 * (int64_t)J;
-        int mode = 3;
-        if (J != I) {
-            double d2 = sphx_box_d2(bI[0], bI[1], bI[2], bI[3], bI[4], bI[5], bJ[0], bJ[1], bJ[2],
-                                    bJ[3], bJ[4], bJ[5]);
-            double reach = bI[6] + bJ[6];
-            bool far = d2 > reach * reach * (1.0 + 1e-6);
-            mode = far ? ((mI >= 0.0 && bJ[7] >= 0.0) ? 0 : 1) : 2;
-        }
+        const int mode = sym_mode(bI, bJ, J == I);
+        if (FAR != (mode <= 1)) continue;
         const float si = mode == 0 ? (float)bJ[7] : 1.0f;  // i-side scale: the j block's mass
         const float sj = mode == 0 ? (float)mI : 1.0f;     // j-side scale: the i block's mass
         for (int c = 0; c < jc; c += 128) {
             SymJ A = sym_load(x, y, z, m, eps, jb, jc, c + lane, ox, oy, oz);
             SymJ B = sym_load(x, y, z, m, eps, jb, jc, c + 64 + lane, ox, oy, oz);
-            if (mode == 0) sym_chunk<0>(S, A, B, addr);
-            else if (mode == 1) sym_chunk<1>(S, A, B, addr);
-            else if (mode == 2) sym_chunk<2>(S, A, B, addr);
-            else sym_chunk<3>(S, A, B, addr);
+            int cm = mode;
+            if (!FAR && mode == 2) {
+                // A near block pair can still hold far chunks: test this chunk's own box
+                // (from the FP32 relative coordinates, with a margin for their rounding).
+                bool va = c + lane < jc, vb = c + 64 + lane < jc;
+                double cb[6] = {
+                    wave_min_d(fmin(va ? (double)A.x : INFINITY, vb ? (double)B.x : INFINITY)),
+                    wave_max_d(fmax(va ? (double)A.x : -INFINITY, vb ? (double)B.x : -INFINITY)),
+                    wave_min_d(fmin(va ? (double)A.y : INFINITY, vb ? (double)B.y : INFINITY)),
+                    wave_max_d(fmax(va ? (double)A.y : -INFINITY, vb ? (double)B.y : -INFINITY)),
+                    wave_min_d(fmin(va ? (double)A.z : INFINITY, vb ? (double)B.z : INFINITY)),
+                    wave_max_d(fmax(va ? (double)A.z : -INFINITY, vb ? (double)B.z : -INFINITY))};
+                double ce = wave_max_d(fmax(va ? (double)A.e : 0.0, vb ? (double)B.e : 0.0));
+                double ib[6] = {bI[0] - ox, bI[1] - ox, bI[2] - oy, bI[3] - oy, bI[4] - oz,
+                                bI[5] - oz};
+                double big = 1.0;
+                for (int k = 0; k < 6; ++k) big = fmax(big, fmax(fabs(cb[k]), fabs(ib[k])));
+                double d2 = sphx_box_d2(ib[0], ib[1], ib[2], ib[3], ib[4], ib[5], cb[0], cb[1],
+                                        cb[2], cb[3], cb[4], cb[5]);
+                double reach = (bI[6] + ce) * (1.0 + 1e-5) + 4e-6 * big;
+                if (d2 > reach * reach) cm = 1;
+            }
+            if (FAR) {
+                if (mode == 0) sym_chunk<0>(S, A, B, addr);
+                else sym_chunk<1>(S, A, B, addr);
+            } else {
+                if (cm == 1) sym_chunk<1>(S, A, B, addr);
+                else if (cm == 2) sym_chunk<2>(S, A, B, addr);
+                else sym_chunk<3>(S, A, B, addr);
+            }
             if (mode != 3) {
                 int qa = c + lane, qb = c + 64 + lane;
                 if (qa < jc) {
@@ -541,11 +584,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         int li = lane + 64 * p;
         if (li < cnt) {
             int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
-            T.qx[o] = kx[p].value();
-            T.qy[o] = ky[p].value();
-            T.qz[o] = kz[p].value();
+            if (FAR) {
+                T.qx[o] = kx[p].value();
+                T.qy[o] = ky[p].value();
+                T.qz[o] = kz[p].value();
+            } else {
+                T.qx[o] += kx[p].value();
+                T.qy[o] += ky[p].value();
+                T.qz[o] += kz[p].value();
+            }
         }
     }
+}
+
+// The far pass at three waves per SIMD (its registers fit without spills).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grav_sym(
+    SymTabs T, const double *__restrict__ x, const double *__restrict__ y,
+    const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ eps) {
+    sym_segment<true>(T, x, y, z, m, eps);
+}
+
+__global__ __launch_bounds__(64) void k_grav_sym_near(SymTabs T, const double *__restrict__ x,
+                                                      const double *__restrict__ y,
+                                                      const double *__restrict__ z,
+                                                      const double *__restrict__ m,
+                                                      const double *__restrict__ eps) {
+    sym_segment<false>(T, x, y, z, m, eps);
 }
 
 // a_i = G (sum over the segments of i's row of their i-side sums, in order,
@@ -759,6 +823,8 @@ static int sym_run(Engine &E, hipStream_t st) {
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
     SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
     k_grav_sym<<<(unsigned)(s1 - s0), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
+    SPHX_HIP(hipGetLastError());
+    k_grav_sym_near<<<(unsigned)(s1 - s0), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
     SPHX_HIP(hipGetLastError());
     SPHX_CHECK(E.tmark(st, PH_GRAVK, 1));
     k_grav_sym_reduce<<<(unsigned)((p1 - p0 + 255) / 256), 256, 0, st>>>(T, E.p.G, p0, p1, E.s.gx,
