@@ -77,6 +77,9 @@ def main(argv=None) -> int:
     ap.add_argument("--hbm-gb", type=float, default=64.0, help="serve: HBM budget")
     ap.add_argument("--eps", type=float, default=None,
                     help="gadget: fixed gravitational softening (default eps = h)")
+    ap.add_argument("--bh-theta", type=float, default=0.5,
+                    help="also time the step with Barnes-Hut gravity at this opening angle "
+                         "(0: skip); reported beside the direct-sum headline")
     ap.add_argument("--serve-gravity", default="4,0.5",
                     help="serve: 'S,share' for the symmetric gravity sum (S block pairs per "
                          "wave, share of the budget for its partials) or 'tiled'")
@@ -181,6 +184,35 @@ def bench_evrard(a) -> int:
         torch.cuda.empty_cache()
         print(f"[bench] fp64 {fp64['ms_per_step']:.1f} ms/step", file=sys.stderr, flush=True)
 
+    # ---------------- the same step with Barnes-Hut gravity ------------------
+    bh_step = None
+    if a.bh_theta > 0:
+        from sph_2_amd import libsx
+
+        pb = SphPlan(n, mode="mixed")
+        pb.set_gravity_bh(a.bh_theta)
+        pb.load(e.pos, e.vel, e.u, e.m, e.h)
+        pb.init()
+        pb.step(a.warmup, a.dt)
+        torch.cuda.synchronize()
+        pb.set_timing(True)
+        t0 = time.perf_counter()
+        pb.step(a.steps, a.dt)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ph = pb.timing()
+        sb = pb.state(("pos", "h", "acc_g"))
+        pb.close()
+        del pb
+        torch.cuda.empty_cache()
+        g_direct = libsx.getAcc_g_smth(sb["pos"], e.m, 1.0, sb["h"])
+        bh_step = {"theta": a.bh_theta, "ms_per_step": 1e3 * (t1 - t0) / a.steps,
+                   "particle_steps_per_s": n * a.steps / (t1 - t0),
+                   "gravity_ms_per_step": ph["gravity"][0] / max(ph["gravity"][1], 1),
+                   "gravity_rel_rms_vs_direct_sum": rel_rms(sb["acc_g"], g_direct)}
+        log(f"bh step {bh_step['ms_per_step']:.1f} ms/step, force error "
+            f"{bh_step['gravity_rel_rms_vs_direct_sum']:.2e}")
+
     # ---------------- FP64 CPU oracle on a sample of rows --------------------
     cpu = None
     parity = {}
@@ -259,6 +291,7 @@ def bench_evrard(a) -> int:
         "cpu_baseline": cpu,
         "parity_vs_fp64_oracle": parity,
         "serving": serving,
+        "barnes_hut_step": bh_step,
     }
     print(json.dumps(out))
     return 0
