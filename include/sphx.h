@@ -124,6 +124,7 @@ typedef struct sphx_counters {
     int64_t cand_buckets;   /* candidate bucket pairs in the last neighbour build */
     int64_t buckets;        /* number of 64-particle buckets */
     int64_t bh_overflow;    /* Barnes-Hut walk groups whose cell stack overflowed */
+    int64_t nonfinite;      /* particles whose u, v or x was not finite after the last step */
 } sphx_counters;
 
 typedef struct sphx_diag {

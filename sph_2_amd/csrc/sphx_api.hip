@@ -511,6 +511,7 @@ int Engine::read_counters(sphx_counters *out) {
     out->cand_buckets = c[CNT_CAND];
     out->buckets = t.nb;
     out->bh_overflow = c[CNT_BH_OVERFLOW];
+    out->nonfinite = c[CNT_NONFINITE];
     return SPHX_OK;
 }
 
@@ -1522,6 +1523,7 @@ extern "C" int sphx_plan_set_gravity_config(sphx_plan *plan, int32_t ipl, int32_
         return SPHX_ERR_ARG;
     }
     plan->E.gcfg = GravConfig{ipl, tj, js};
+    plan->E.drop_graph();  // a captured step holds the previous launch shape
     return SPHX_OK;
 }
 

@@ -306,6 +306,11 @@ __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, Coo
         }
     }
     s.u[i] = u;
+    // a particle whose state stopped being finite is counted (the step goes on;
+    // the caller reads the count), as the reference has no such check at all
+    if (!(isfinite(u) && isfinite(s.vx[i]) && isfinite(s.vy[i]) && isfinite(s.vz[i]) &&
+          isfinite(s.x[i]) && isfinite(s.y[i]) && isfinite(s.z[i])))
+        atomicAdd((unsigned long long *)&counters[CNT_NONFINITE], 1ull);
 }
 
 __global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t p0,

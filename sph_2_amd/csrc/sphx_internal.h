@@ -162,6 +162,7 @@ enum {
     CNT_COOL_CAPPED,
     CNT_CAND,
     CNT_BH_OVERFLOW,
+    CNT_NONFINITE,
     CNT_SLOTS = 8
 };
 

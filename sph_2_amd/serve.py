@@ -69,6 +69,13 @@ class Batch:
         g = grid.c()
         check(_lib.load().sphx_batch_set_cooling_grid(self._h, C.byref(g)), "set_cooling_grid")
 
+    def counters(self) -> dict:
+        """Counters of the last step over all live runs (h-iteration caps,
+        cooling caps, non-finite particles, ...)."""
+        c = _lib.Counters()
+        check(_lib.load().sphx_batch_query(self._h, C.byref(c)), "sphx_batch_query")
+        return c.as_dict()
+
     def set_gravity_symmetric(self, block_pairs: int, budget_bytes: int):
         """Symmetric direct-sum gravity for every run (before the first admission),
         with budget_bytes reserved for its partial sums."""
