@@ -348,6 +348,14 @@ int sphx_plan_set_cooling_grid(sphx_plan *plan, const sphx_cool_grid *grid);
 int sphx_plan_set_chimes_table(sphx_plan *plan, const sphx_chimes_table *table);
 int sphx_plan_chimes_ions(sphx_plan *plan, float *out, void *stream);
 
+/* Shift (-3..3) of the Morton level at which 64-particle buckets are cut into
+ * segments for the neighbour passes (default 0: level floor(log8(n/64)) - 1).
+ * A tuning knob only: every sum is met in the same order at any level, so
+ * results do not change.  No counterpart in the reference (brute-force pair
+ * loops, LIBV2:27-38). */
+int sphx_plan_set_cell_level(sphx_plan *plan, int32_t bias);
+int sphx_batch_set_cell_level(sphx_batch *b, int32_t bias);
+
 int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl, double dt_min,
                               double dt_max);
 int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts, void *stream);

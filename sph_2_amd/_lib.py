@@ -145,6 +145,8 @@ SIGNATURES = {
     "sphx_batch_set_chimes_table": (C.c_int, [_V, _V]),
     "sphx_batch_set_gravity_symmetric": (C.c_int, [_V, _I32, _I64]),
     "sphx_gravity_symmetric_bytes": (C.c_int64, [_I64, _I32]),
+    "sphx_plan_set_cell_level": (C.c_int, [_V, _I32]),
+    "sphx_batch_set_cell_level": (C.c_int, [_V, _I32]),
     "sphx_read_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_write_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_cool_grid_build": (C.c_int, [_V, _V, _V, _V]),

@@ -49,6 +49,47 @@ def choose_gravity_config(n: int):
     return _default(n)
 
 
+_CELL_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cell_tuning.json")
+CELL_BIASES = (-2, -1, 0, 1, 2)
+
+
+def choose_cell_level(n: int) -> int:
+    """Bias of the bucket cut level for a run of n particles (0 when untuned)."""
+    try:
+        with open(_CELL_CACHE) as f:
+            return int(json.load(f).get(_size_class(n), 0))
+    except (OSError, ValueError):
+        return 0
+
+
+def tune_cells(n: int, seed: int = 0, steps: int = 3):
+    """Time the four neighbour passes of an Evrard step of n particles at every
+    cut-level bias (results are the same at every bias; only the candidate
+    walks change).  Returns (best bias, {bias: ms per step})."""
+    import torch
+
+    from . import ic
+    from .plan import SphPlan
+
+    e = ic.evrard(n, seed=seed)
+    pl = SphPlan(n)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h)
+    pl.init()
+    res = {}
+    for b in CELL_BIASES:
+        pl.set_cell_level(b)
+        pl.step(1, 1e-4)
+        torch.cuda.synchronize()
+        pl.set_timing(True)
+        pl.step(steps, 1e-4)
+        ph = pl.timing()
+        pl.set_timing(False)
+        res[b] = sum(ph[k][0] / max(ph[k][1], 1)
+                     for k in ("smoothing", "density", "divcurl", "hydro"))
+    pl.close()
+    return min(res, key=res.get), res
+
+
 def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
     """Time every candidate shape on an Evrard sphere of n particles."""
     import numpy as np
@@ -107,7 +148,18 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=main.__doc__)
     ap.add_argument("--sizes", type=int, nargs="+", default=[32768, 131072, 1048576])
     ap.add_argument("--out", required=True)
+    ap.add_argument("--cells-out", default=None, help="also tune the bucket cut level")
     a = ap.parse_args(argv)
+    if a.cells_out:
+        ctab = {}
+        for n in a.sizes:
+            best, res = tune_cells(n)
+            ctab[_size_class(n)] = best
+            print(f"[tune] cells n={n} best={best} " +
+                  " ".join(f"{k}:{v:.3f}ms" for k, v in sorted(res.items())),
+                  file=sys.stderr, flush=True)
+        with open(a.cells_out, "w") as f:
+            json.dump(ctab, f, indent=1)
     tab = {}
     for n in a.sizes:
         best, res = tune_gravity(n, write=False)

@@ -199,7 +199,7 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
         {
             int l = 0;
             while (((int64_t)64 << (3 * (l + 1))) <= nr) ++l;
-            hr_lcut.push_back(std::max(1, l - 1));
+            hr_lcut.push_back(std::max(1, l - 1 + lcut_bias));
         }
         int b0 = (int)hb_beg.size();
         for (int64_t a = 0; a < nr; a += SPHX_BUCKET) {
@@ -1535,6 +1535,20 @@ extern "C" int sphx_ctx_set_gravity_config(sphx_ctx *ctx, int32_t ipl, int32_t t
     }
     ctx->E.gcfg = GravConfig{ipl, tj, js};
     return SPHX_OK;
+}
+
+int sphx_set_cell_level(Engine &E, int32_t bias) {
+    if (bias < -3 || bias > 3) { sphx_set_error("cell level bias must be in -3..3"); return SPHX_ERR_ARG; }
+    E.lcut_bias = bias;
+    if (E.R > 0) SPHX_CHECK(E.set_runs(E.run_n));  // rebuild the run tables with the new cut levels
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_set_cell_level(sphx_plan *plan, int32_t bias) {
+    void *stream = nullptr;
+    PLAN_PROLOGUE(plan);
+    (void)st;
+    return sphx_set_cell_level(plan->E, bias);
 }
 
 extern "C" int sphx_plan_set_timing(sphx_plan *plan, int32_t on) {

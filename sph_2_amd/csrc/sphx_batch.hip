@@ -346,6 +346,13 @@ extern "C" int sphx_batch_set_gravity_symmetric(sphx_batch *b, int32_t block_pai
     return SPHX_OK;
 }
 
+extern "C" int sphx_batch_set_cell_level(sphx_batch *b, int32_t bias) {
+    void *stream = nullptr;
+    BATCH_PROLOGUE(b);
+    (void)st;
+    return sphx_set_cell_level(b->E, bias);
+}
+
 extern "C" int64_t sphx_gravity_symmetric_bytes(int64_t n, int32_t block_pairs) {
     if (n <= 0 || block_pairs < 1) return 0;
     return sphx_sym_run_bytes(n, block_pairs);

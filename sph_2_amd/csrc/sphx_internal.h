@@ -251,6 +251,7 @@ struct GravHost {
 };
 void sphx_sym_host_free(struct SymHost *h);
 int64_t sphx_sym_run_bytes(int64_t n, int S);
+int sphx_set_cell_level(struct Engine &E, int32_t bias);
 int sphx_sym_reserve(struct Engine &E, int S, int64_t budget_bytes);
 bool sphx_sym_fits(const struct Engine &E, const std::vector<int64_t> &sizes, int S);
 
@@ -304,6 +305,7 @@ struct Engine {
     int32_t *cand_off = nullptr, *cand_cnt = nullptr, *cand = nullptr;
     int64_t cand_cap = 0;
     int64_t cand_total = 0;
+    int lcut_bias = 0;  // shift of the per-run bucket cut level (tuning only: sums do not change)
 
     std::vector<void *> owned;  // every hipMalloc'd pointer
 

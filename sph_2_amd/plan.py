@@ -61,11 +61,21 @@ class SphPlan:
         check(_lib.load().sphx_plan_set_gravity_config(self._h, ipl, tj, js), "gravity config")
         self._gcfg_set = True
 
+    def set_cell_level(self, bias: int):
+        """Shift of the bucket cut level of the neighbour passes (-3..3); results
+        are the same at any level."""
+        check(_lib.load().sphx_plan_set_cell_level(self._h, int(bias)), "set_cell_level")
+        self._cell_set = True
+
     def load(self, pos, vel, u, m, h, eps=None):
         t = _lib.torch()
         if self.mode == _lib.MIXED and not getattr(self, "_gcfg_set", False):
             from .autotune import choose_gravity_config
             self.set_gravity_config(*choose_gravity_config(len(m)))
+        if self.mode == _lib.MIXED and not getattr(self, "_cell_set", False):
+            from .autotune import choose_cell_level
+            check(_lib.load().sphx_plan_set_cell_level(self._h, choose_cell_level(len(m))),
+                  "set_cell_level")
         self._bufs = [dev(np.asarray(a, np.float64) if not isinstance(a, t.Tensor) else a,
                           t.float64) for a in (pos, vel, u, m, h)]
         de = dev(np.asarray(eps, np.float64), t.float64) if eps is not None else None

@@ -322,3 +322,38 @@ def test_diagnostics(gpu, evr):
     assert abs(d["ut"] - float((evr["m"] * evr["u"]).sum())) < 1e-13
     assert abs(libsx.getKE(evr["v"], evr["m"]) - ke) < 1e-13 * ke
     assert abs(libsx.getPE(evr["pos"], evr["m"], 1.0, evr["h"]) - pe) < 1e-12 * abs(pe)
+
+
+def test_cell_level_does_not_change_results(gpu):
+    """The bucket cut level only changes which candidate buckets each segment
+    walks; every kept pair is met in the same order, so a plan gives the same
+    bits at any level."""
+    from sph_2_amd.plan import SphPlan
+
+    n = 32768
+    e = ic.evrard(n, seed=4)
+    outs = []
+    for b in (-1, 0, 2):                      # cut levels 1, 2 and 4 at this size
+        pl = SphPlan(n)
+        pl.set_cell_level(b)
+        pl.load(e.pos, e.vel, e.u, e.m, e.h)
+        pl.init()
+        pl.step(2, 1e-4)
+        outs.append(pl.state())
+        if b == 0:
+            pl.set_cell_level(2)              # also when changed between steps
+            pl.step(1, 1e-4)
+            later = pl.state()
+        pl.close()
+    for o in outs[1:]:
+        for f in o:
+            assert np.array_equal(outs[0][f], o[f]), f
+    pl = SphPlan(n)
+    pl.set_cell_level(0)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h)
+    pl.init()
+    pl.step(3, 1e-4)
+    ref = pl.state()
+    pl.close()
+    for f in ref:
+        assert np.array_equal(ref[f], later[f]), f
