@@ -30,7 +30,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
 # HBM bytes per gravity-kernel launch at N = 1,048,576: 2 x FETCH_SIZE + WRITE_SIZE of the
 # step's dispatch in two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE reads
 # half of the bytes of that workload's 256 MiB calibration copy, hence the factor 2).
-GRAV_TRAFFIC_BYTES_1M = {"k_grav": 11.982e9, "k_grav_sym": 50.694e9}
+GRAV_TRAFFIC_BYTES_1M = {"k_grav": 11.982e9, "k_grav_sym": 45.587e9}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -148,8 +148,8 @@ def bench_evrard(a) -> int:
     flops = 20.0 * n * (n - 1)
     achieved = flops / kern_s / 1e12 if kern_s > 0 else None
     kname = "k_grav_sym" if cfg_used[0] == 8 else "k_grav"
-    what = ("symmetric direct sum, each pair once for both particles" if kname == "k_grav_sym"
-            else "tiled direct sum")
+    what = ("+ k_grav_sym_near: symmetric direct sum, each pair once for both particles"
+            if kname == "k_grav_sym" else "tiled direct sum")
     roofline = {"bound": "valu", "kernel": f"{kname} ({what}, FP32)",
                 "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": (achieved / PEAK_FP32_TFLOPS) if achieved else None,
