@@ -357,3 +357,27 @@ def test_cell_level_does_not_change_results(gpu):
     pl.close()
     for f in ref:
         assert np.array_equal(ref[f], later[f]), f
+
+
+def test_symmetric_gravity_conserves_momentum_at_scale(gpu):
+    """Size-independent check at N = 262,144 (beyond the oracle's reach): the
+    symmetric sum applies each pair's force to both particles, so the total
+    gravitational force sum_i m_i a_i vanishes to round-off, and the result
+    agrees with the tiled sum on the same input."""
+    from sph_2_amd import _lib, libsx
+    from sph_2_amd.autotune import _default
+
+    n = 262144
+    e = ic.evrard(n, seed=9)
+    ctx = libsx._context(0)
+    try:
+        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 8, 512, 2), "config")
+        a_sym = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 2, 512, 4), "config")
+        a_til = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+    finally:
+        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(N)), "config")
+    scale = float((e.m[:, None] * np.abs(a_sym)).sum())
+    net = np.abs((e.m[:, None] * a_sym).sum(0)).max()
+    assert net / scale < 1e-6
+    assert rel_rms(a_sym, a_til) < 1e-6
