@@ -125,6 +125,8 @@ typedef struct sphx_counters {
     int64_t buckets;        /* number of 64-particle buckets */
     int64_t bh_overflow;    /* Barnes-Hut walk groups whose cell stack overflowed */
     int64_t nonfinite;      /* particles whose u, v or x was not finite after the last step */
+    int64_t bin_floor;      /* block time steps: particles whose criterion asked for a step
+                               shorter than the shortest level in the last substep */
 } sphx_counters;
 
 typedef struct sphx_diag {
@@ -328,6 +330,24 @@ int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int32_t nphase
  * sphx_plan_time returns (host scalars) the simulated time -- the sum of the
  * dt each step used --, the next step's dt and (nullable, 4 doubles) the
  * last dt_f, dt_kin, dt_cfl, dt_dens.  Synchronous. */
+/* Block time steps (levels > 0; 0 switches them off): particle i steps with
+ * dt / 2^b_i, b_i in 0..levels, where dt is sphx_plan_step's argument (the
+ * longest step, fixed until the next sphx_plan_init) and each sphx_plan_step
+ * step is one substep dt / 2^levels.  b_i is the longest level whose step is
+ * at most the per-particle form of the global criterion (0.25 min(sqrt(h/|a_g|),
+ * sqrt(h/|a|), c_cfl h/c, c_cfl h/|h div v / 3|)); a particle moves to a
+ * shorter level at once, to a longer one a level at a time where the substep
+ * count allows.  Kick-drift-kick per particle: every particle drifts every
+ * substep, the SPH passes run for all, gravity only for the particle blocks
+ * holding a particle whose step ends.  The reference's prototype of individual
+ * steps (GPU_sph_Type/Evrard_collapse/individual_dt_not_completed/AWS/
+ * main.cu:472-520 with hfvCppLibs_indiv_dt.h:819-980) is unfinished; this is
+ * this library's own scheme.  Mixed precision; not with the adaptive global
+ * dt, graphs or checkpoints.  Call before the first step after init. */
+int sphx_plan_set_block_steps(sphx_plan *plan, int32_t levels, double c_cfl);
+/* Level b_i of every particle (int32, n, original order; device array). */
+int sphx_plan_block_bins(sphx_plan *plan, int32_t *out, void *stream);
+
 /* Gravity of the step from the Barnes-Hut tree of sphx_bh_gravity instead of
  * the direct sum (theta > 0; 0 restores the direct sum).  Mixed precision,
  * single-run plans; replaces the BH gravity call of the reference's tree

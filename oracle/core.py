@@ -517,6 +517,84 @@ def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None,
     return s
 
 
+def particle_dt(h, c, acc, acc_g, divV, c_cfl=0.25):
+    """Per-particle form of the global criterion (hfv_sph_gpu_v2_evrard.cu:
+    555-561 limits, each from the particle's own h, c, |a|, |a_g|, div v):
+    0.25 min(sqrt(h/|a_g|), sqrt(h/|a|), c_cfl h/c, c_cfl h/|h div v/3|)."""
+    g, a = acc_g, acc
+    ag = np.sqrt((g[:, 0] * g[:, 0] + g[:, 1] * g[:, 1]) + g[:, 2] * g[:, 2])
+    at = np.sqrt((a[:, 0] * a[:, 0] + a[:, 1] * a[:, 1]) + a[:, 2] * a[:, 2])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dd = h / np.abs((1.0 / 3.0) * h * divV)
+        dt_f = np.sqrt(h / ag)
+        dt_kin = np.sqrt(h / at)
+        dt_cfl = c_cfl * h / c
+    dt_dens = c_cfl * dd
+    m = dt_f
+    m = np.where(dt_kin < m, dt_kin, m)
+    m = np.where(dt_cfl < m, dt_cfl, m)
+    m = np.where(dt_dens < m, dt_dens, m)
+    return 0.25 * m
+
+
+def block_level(dt_i, dt_max, levels):
+    """The smallest b in 0..levels with dt_max / 2^b <= dt_i (the longest step
+    not above dt_i); `levels` and a floor flag when there is none."""
+    b = np.full(len(dt_i), levels, np.int64)
+    floor = np.ones(len(dt_i), bool)
+    for k in range(levels, -1, -1):
+        ok = (dt_max * 2.0 ** -k) <= dt_i
+        b[ok] = k
+        floor[ok] = False
+    return b, floor
+
+
+def block_run(s: State, dt_max: float, levels: int, n_sub: int, p: Params, c_cfl=0.25,
+              cool_p: CoolParams | None = None):
+    """Block time steps (the plan's sphx_plan_set_block_steps): s is a
+    bootstrapped state; n_sub substeps of dt_max / 2^levels.  Particle i steps
+    with dt_max / 2^b_i; every particle drifts every substep, forces come for
+    all (only the active particles' are used), the active particles take the
+    closing half kick, the u trapezoid and cooling over their step, a new level
+    (shorter at once, longer one level at a time where the substep count is a
+    multiple of the longer step), then the opening half kick of the next step.
+    Returns (state, levels, floor counts per substep)."""
+    s = s.copy()
+    d = dt_max * 2.0 ** -levels
+    bins, fl = block_level(particle_dt(s.h, s.c, s.acc, s.acc_g, s.divV, c_cfl), dt_max, levels)
+    floors = [int(fl.sum())]
+    half = 0.5 * (dt_max * 2.0 ** -bins)
+    s.vel = s.vel + s.acc * half[:, None]
+    tick = 0
+    for _ in range(n_sub):
+        s.pos = s.pos + s.vel * d
+        forces(s, p, d, use_pred=True)
+        t1 = tick + 1
+        act = (t1 & ((1 << (levels - bins)) - 1)) == 0
+        dto = dt_max * 2.0 ** -bins
+        half = 0.5 * dto
+        v = s.vel + s.acc * half[:, None]
+        u = s.u + half * (s.dudt + s.ut_prev)
+        if cool_p is not None:
+            for b in np.unique(bins[act]):
+                sel = act & (bins == b)
+                u[sel], _ = cool(s.rho[sel], u[sel], dt_max * 2.0 ** -int(b), cool_p)
+        bt, fl = block_level(particle_dt(s.h, s.c, s.acc, s.acc_g, s.divV, c_cfl), dt_max,
+                             levels)
+        up = (bins > 0) & ((t1 & ((1 << np.maximum(levels - (bins - 1), 0)) - 1)) == 0)
+        bn = np.where(bt >= bins, bt, np.where(up, bins - 1, bins))
+        h2 = 0.5 * (dt_max * 2.0 ** -bn)
+        v2 = v + s.acc * h2[:, None]
+        s.vel = np.where(act[:, None], v2, s.vel)
+        s.u = np.where(act, u, s.u)
+        s.ut_prev = np.where(act, s.dudt, s.ut_prev)
+        bins = np.where(act, bn, bins)
+        floors.append(int((fl & act).sum()))
+        s.t = s.t + d
+        tick = t1 & ((1 << levels) - 1)
+    return s, bins, floors
+
+
 def next_dt(s: State, c_cfl=0.25, dt_min=1e-5, dt_max=1e-3) -> float:
     """dt of the next step from this step's forces (hfv_sph_gpu_v2_evrard.cu:517-561)."""
     return global_dt(s.h, s.c, s.acc, s.acc_g, s.divV, c_cfl, dt_min, dt_max)[0]

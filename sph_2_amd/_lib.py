@@ -50,7 +50,7 @@ class Counters(C.Structure):
     _fields_ = [("n", C.c_int64), ("steps", C.c_int64), ("h_iters_max", C.c_int64),
                 ("h_iters_sum", C.c_int64), ("h_capped", C.c_int64), ("cool_capped", C.c_int64),
                 ("cand_buckets", C.c_int64), ("buckets", C.c_int64), ("bh_overflow", C.c_int64),
-                ("nonfinite", C.c_int64)]
+                ("nonfinite", C.c_int64), ("bin_floor", C.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -146,6 +146,8 @@ SIGNATURES = {
     "sphx_batch_set_gravity_symmetric": (C.c_int, [_V, _I32, _I64]),
     "sphx_gravity_symmetric_bytes": (C.c_int64, [_I64, _I32]),
     "sphx_plan_set_cell_level": (C.c_int, [_V, _I32]),
+    "sphx_plan_set_block_steps": (C.c_int, [_V, _I32, _D]),
+    "sphx_plan_block_bins": (C.c_int, [_V, _V, _V]),
     "sphx_batch_set_cell_level": (C.c_int, [_V, _I32]),
     "sphx_read_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_write_chimes": (C.c_int, [_S, _V, _V, _V, _V, _V, _V, _V, _V, _V]),

@@ -1,6 +1,7 @@
 // C ABI of libsphx: engine lifecycle, run tables, step orchestration, the
 // stateless drop-in calls and the resident plan.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -162,7 +163,7 @@ int Engine::reserve(int64_t capacity) {
     for (double **q : dptr) SPHX_CHECK(alloc((void **)q, D));
     // tmpd also holds the 256 x 11 diagnostic block partials
     SPHX_CHECK(alloc((void **)&tmpd, sizeof(double) * (size_t)std::max<int64_t>(capacity, 4096)));
-    int32_t **iptr[] = {&s.k, &s.id, &s.run, &perm, &perm2, &tmpi, &h_iters};
+    int32_t **iptr[] = {&s.k, &s.id, &s.run, &s.bin, &bin_spare, &perm, &perm2, &tmpi, &h_iters};
     for (int32_t **q : iptr) SPHX_CHECK(alloc((void **)q, I));
     SPHX_CHECK(alloc((void **)&keys, sizeof(uint64_t) * capacity));
     SPHX_CHECK(alloc((void **)&keys2, sizeof(uint64_t) * capacity));
@@ -401,9 +402,37 @@ int Engine::step_body(hipStream_t st) {
     return SPHX_OK;
 }
 
+// One substep of the block time steps (dt_max / 2^B): opening kicks at the
+// first substep, drift of every particle, sort, forces (gravity only for the
+// blocks holding a particle whose step ends here; the SPH passes for all, as
+// the neighbours of active particles need their current h, rho, P, c),
+// closing and opening kicks of the active particles, t += substep.
+int Engine::block_substep(hipStream_t st) {
+    SPHX_CHECK(sphx_zero_counters(counters, st));
+    SPHX_CHECK(tmark(st, PH_SORT, 0));
+    if (!blk_started) {
+        SPHX_CHECK(block_open(st));
+        blk_started = true;
+    }
+    SPHX_CHECK(block_drift(std::ldexp(blk_dtmax, -blk_B), st));
+    SPHX_CHECK(sort_and_permute(st, true));
+    SPHX_CHECK(tmark(st, PH_SORT, 1));
+    int64_t t1 = blk_tick + 1;
+    blk_t1 = t1;
+    int rc = forces(st, true, false);
+    blk_t1 = 0;
+    SPHX_CHECK(rc);
+    SPHX_CHECK(tmark(st, PH_FINISH, 0));
+    SPHX_CHECK(block_close(t1, st));
+    SPHX_CHECK(global_dt(st));  // t += the substep (run dt)
+    SPHX_CHECK(tmark(st, PH_FINISH, 1));
+    blk_tick = t1 & ((1ll << blk_B) - 1);
+    return SPHX_OK;
+}
+
 int Engine::step(hipStream_t st, int nsteps) {
     for (int k = 0; k < nsteps; ++k) {
-        SPHX_CHECK(step_body(st));
+        SPHX_CHECK(blk_B > 0 ? block_substep(st) : step_body(st));
         SPHX_CHECK(tcollect());
         ++steps;
     }
@@ -512,6 +541,7 @@ int Engine::read_counters(sphx_counters *out) {
     out->buckets = t.nb;
     out->bh_overflow = c[CNT_BH_OVERFLOW];
     out->nonfinite = c[CNT_NONFINITE];
+    out->bin_floor = c[CNT_BIN_FLOOR];
     return SPHX_OK;
 }
 
@@ -1274,6 +1304,8 @@ extern "C" int sphx_plan_init(sphx_plan *plan, void *stream) {
         SPHX_CHECK(E.plain_forces(st, false, false));
     }
     SPHX_CHECK(E.bootstrap_finish(st));
+    E.blk_started = false;
+    E.blk_tick = 0;
     plan->inited = true;
     return SPHX_OK;
 }
@@ -1292,10 +1324,20 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
     // the adaptive step, dt: pull the device copy before changing anything.
     RunParams &hp = E.run_params[0];
     int32_t uc = use_cooling;
-    bool set_dt = plan->adaptive ? plan->seed_pending : hp.dt != dt;
+    // block time steps: dt is the longest step, the run advances by substeps
+    double run_dt = dt;
+    if (E.blk_B > 0) {
+        if (E.blk_started && dt != E.blk_dtmax) {
+            sphx_set_error("block time steps keep the longest step fixed; call sphx_plan_init to change it");
+            return SPHX_ERR_ARG;
+        }
+        E.blk_dtmax = dt;
+        run_dt = std::ldexp(dt, -E.blk_B);
+    }
+    bool set_dt = plan->adaptive ? plan->seed_pending : hp.dt != run_dt;
     if (set_dt || hp.use_cool != uc) {
         SPHX_CHECK(E.pull_run_params(st));
-        if (set_dt) E.run_params[0].dt = dt;
+        if (set_dt) E.run_params[0].dt = run_dt;
         E.run_params[0].use_cool = uc;
         plan->seed_pending = false;
         SPHX_CHECK(E.upload_run_params(st));
@@ -1303,8 +1345,9 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
     // cooling is gated by the run's use_cool flag (device memory), so a
     // captured step graph stays valid when the flag changes
     // the Barnes-Hut build sizes its levels on the host, so those steps run eagerly
-    return (E.use_graph && !E.timing && E.bh_theta == 0.0) ? E.step_graphs(st, n_steps)
-                                                            : E.step(st, n_steps);
+    return (E.use_graph && !E.timing && E.bh_theta == 0.0 && E.blk_B == 0)
+               ? E.step_graphs(st, n_steps)
+               : E.step(st, n_steps);
 }
 
 extern "C" int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl,
@@ -1313,6 +1356,10 @@ extern "C" int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_c
     PLAN_PROLOGUE(plan);
     if (on && !(c_cfl > 0.0 && dt_min > 0.0 && dt_max >= dt_min)) {
         sphx_set_error("adaptive dt needs c_cfl > 0 and 0 < dt_min <= dt_max");
+        return SPHX_ERR_ARG;
+    }
+    if (on && plan->E.blk_B > 0) {
+        sphx_set_error("the adaptive global dt and block time steps exclude each other");
         return SPHX_ERR_ARG;
     }
     plan->adaptive = on ? 1 : 0;
@@ -1333,6 +1380,34 @@ extern "C" int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_c
         SPHX_CHECK(E.upload_run_params(st));
         SPHX_HIP(hipStreamSynchronize(st));
     }
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_set_block_steps(sphx_plan *plan, int32_t levels, double c_cfl) {
+    if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
+    Engine &E = plan->E;
+    if (levels < 0 || levels > 20) { sphx_set_error("block levels must be 0..20"); return SPHX_ERR_ARG; }
+    if (levels > 0 && E.mode != SPHX_MIXED) { sphx_set_error("block time steps are a mixed-precision option"); return SPHX_ERR_ARG; }
+    if (levels > 0 && plan->adaptive) { sphx_set_error("block time steps replace the adaptive global dt; switch that off first"); return SPHX_ERR_ARG; }
+    if (levels > 0 && !(c_cfl > 0.0)) { sphx_set_error("c_cfl must be positive"); return SPHX_ERR_ARG; }
+    if (E.blk_started) { sphx_set_error("the run is inside block steps; call sphx_plan_init first"); return SPHX_ERR_STATE; }
+    E.blk_B = levels;
+    E.blk_c_cfl = c_cfl;
+    E.blk_tick = 0;
+    E.drop_graph();
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_block_bins(sphx_plan *plan, int32_t *out, void *stream) {
+    PLAN_PROLOGUE(plan);
+    Engine &E = plan->E;
+    if (!plan->inited || E.blk_B == 0 || !E.blk_started) {
+        sphx_set_error("no block steps taken since sphx_plan_init");
+        return SPHX_ERR_STATE;
+    }
+    if (!out) { sphx_set_error("null output"); return SPHX_ERR_ARG; }
+    SPHX_CHECK(scatter_out_i(E, E.s.bin, 0, E.n, out, st));
+    SPHX_HIP(hipStreamSynchronize(st));
     return SPHX_OK;
 }
 
@@ -1362,6 +1437,7 @@ static void ckpt_header(CkptHeader &h, const Engine &E) {
 extern "C" int sphx_plan_save(sphx_plan *plan, const char *path, void *stream) {
     PLAN_PROLOGUE(plan);
     if (!plan->inited) { sphx_set_error("nothing to save: call sphx_plan_init first"); return SPHX_ERR_STATE; }
+    if (plan->E.blk_B > 0) { sphx_set_error("checkpoints of block-step runs are not supported"); return SPHX_ERR_STATE; }
     Engine &E = plan->E;
     SPHX_CHECK(E.pull_run_params(st));
     CkptHeader h;

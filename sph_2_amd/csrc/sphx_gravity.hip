@@ -35,6 +35,11 @@ struct GravIO {
     double *ax, *ay, *az, *pot;  // js == 1: final outputs
     double *part;                // js > 1: [js][n][4] partial sums
     int64_t n;
+    // block time steps: only i-blocks holding a particle whose step ends at
+    // substep t1 are computed (blk_B == 0: all)
+    const int32_t *bin;
+    int32_t blk_B;
+    int64_t blk_t1;
 };
 
 __global__ void k_tile_boxes(Tables t, const int32_t *__restrict__ r_tbeg, int ntiles_total,
@@ -214,6 +219,15 @@ __global__ __launch_bounds__(256) void k_grav(GravTabs gt, GravIO io) {
             lz = fmin(lz, zd[p]); hz = fmax(hz, zd[p]);
             em = fmax(em, ed[p]);
         }
+    }
+    if (io.blk_B > 0) {
+        bool any = false;
+#pragma unroll
+        for (int p = 0; p < IPL; ++p) {
+            int b = io.bin[beg + (ok[p] ? tid + p * 256 : 0)];
+            any = any || (ok[p] && (io.blk_t1 & ((1ll << (io.blk_B - b)) - 1)) == 0);
+        }
+        if (!__syncthreads_or(any)) return;  // the same answer in every lane of the block
     }
     lx = block_min(lx, red); hx = block_max(hx, red);
     ly = block_min(ly, red); hy = block_max(hy, red);
@@ -895,11 +909,13 @@ int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
 
 int sphx_gravity_run(Engine &E, const GravConfig &cfg_in, bool with_pot, hipStream_t st) {
     if (E.n == 0) return SPHX_OK;
-    if (cfg_in.ipl == 8 && !with_pot) {
+    if (cfg_in.ipl == 8 && !with_pot && E.blk_t1 == 0) {
         SPHX_CHECK(sym_prepare(E, cfg_in, st));
         return sym_run(E, st);
     }
-    // the symmetric sum carries no potential: that call uses the tiled sum
+    // the symmetric sum carries no potential and computes every pair: those
+    // calls and block substeps (gravity only for active i-blocks) use the
+    // tiled sum
     const GravConfig cfg = cfg_in.ipl == 8 ? GravConfig{2, 512, 1} : cfg_in;
     SPHX_CHECK(sphx_gravity_prepare(E, cfg, st));
     GravHost &H = E.grav;
@@ -920,7 +936,8 @@ int sphx_gravity_run(Engine &E, const GravConfig &cfg_in, bool with_pot, hipStre
         SPHX_HIP(hipGetLastError());
     }
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
-    GravIO io{E.s.x, E.s.y, E.s.z, E.s.m, eps, E.p.G, E.s.gx, E.s.gy, E.s.gz, E.s.pot, H.part, E.n};
+    GravIO io{E.s.x, E.s.y, E.s.z, E.s.m, eps, E.p.G, E.s.gx, E.s.gy, E.s.gz, E.s.pot, H.part, E.n,
+              E.s.bin, E.blk_t1 > 0 ? E.blk_B : 0, E.blk_t1};
     int nblk = ib1 - ib0;
     int rc;
     SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));

@@ -277,6 +277,38 @@ __global__ void k_chimes_apply(ChimesDev c, const double *__restrict__ pos,
             ion_out[i * c.n_m + k] = c.metalz[mcell + (int64_t)k * c.n_time];
 }
 
+// Cooling of particle i's updated energy u over dt by the run's method (0
+// none, 1 implicit solve, 2 2-D table, 3 CHIMES table).
+__device__ __forceinline__ double cool_particle(const Soa &s, int64_t i, double u, double dt,
+                                                int use_cool, int cool_on, const CoolDev &cd,
+                                                const GridDev &gd, const ChimesDev &ch,
+                                                int64_t *counters) {
+    if (cool_on && use_cool == 1) {
+        int it = 0;
+        u = cool_one(s.rho[i], u, dt, cd, &it);
+        if (it >= cd.max_iter) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
+    } else if (use_cool == 2 && gd.du) {
+        u = grid_apply(s.rho[i], u, dt, gd);
+    } else if (use_cool == 3 && ch.u_evol) {
+        int64_t cell, mcell;
+        u = chimes_one(s.x[i], s.y[i], s.z[i], s.rho[i], u, dt, ch, &cell, &mcell);
+        if (ch.ion) {
+            float *o = ch.ion + (int64_t)s.id[i] * ch.n_m;
+            for (int k = 0; k < ch.n_m; ++k) o[k] = ch.metalz[mcell + (int64_t)k * ch.n_time];
+        }
+    }
+    return u;
+}
+
+// A particle whose state stopped being finite is counted (the step goes on;
+// the caller reads the count), as the reference has no such check at all.
+__device__ __forceinline__ void count_nonfinite(const Soa &s, int64_t i, double u,
+                                                int64_t *counters) {
+    if (!(isfinite(u) && isfinite(s.vx[i]) && isfinite(s.vy[i]) && isfinite(s.vz[i]) &&
+          isfinite(s.x[i]) && isfinite(s.y[i]) && isfinite(s.z[i])))
+        atomicAdd((unsigned long long *)&counters[CNT_NONFINITE], 1ull);
+}
+
 __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, CoolDev cd,
                          int cool_on, GridDev gd, ChimesDev ch, int64_t *counters) {
 #pragma clang fp contract(off)
@@ -291,26 +323,108 @@ __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, Coo
     double du = s.du[i];
     double u = s.u[i] + half * (du + s.ut_prev[i]);
     s.ut_prev[i] = du;
-    if (cool_on && r.use_cool == 1) {
-        int it = 0;
-        u = cool_one(s.rho[i], u, dt, cd, &it);
-        if (it >= cd.max_iter) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
-    } else if (r.use_cool == 2 && gd.du) {
-        u = grid_apply(s.rho[i], u, dt, gd);
-    } else if (r.use_cool == 3 && ch.u_evol) {
-        int64_t cell, mcell;
-        u = chimes_one(s.x[i], s.y[i], s.z[i], s.rho[i], u, dt, ch, &cell, &mcell);
-        if (ch.ion) {
-            float *o = ch.ion + (int64_t)s.id[i] * ch.n_m;
-            for (int k = 0; k < ch.n_m; ++k) o[k] = ch.metalz[mcell + (int64_t)k * ch.n_time];
-        }
-    }
+    u = cool_particle(s, i, u, dt, r.use_cool, cool_on, cd, gd, ch, counters);
     s.u[i] = u;
-    // a particle whose state stopped being finite is counted (the step goes on;
-    // the caller reads the count), as the reference has no such check at all
-    if (!(isfinite(u) && isfinite(s.vx[i]) && isfinite(s.vy[i]) && isfinite(s.vz[i]) &&
-          isfinite(s.x[i]) && isfinite(s.y[i]) && isfinite(s.z[i])))
-        atomicAdd((unsigned long long *)&counters[CNT_NONFINITE], 1ull);
+    count_nonfinite(s, i, u, counters);
+}
+
+// ---------------------------------------------------------------------------
+// Block time steps: particle i steps with dt_max / 2^bin[i]; the substep is
+// dt_max / 2^B.  A particle is active at the end of a substep (tick t1, in
+// substeps) when t1 is a multiple of its step.  Kick-drift-kick per particle:
+// the opening half kick when its step starts, every particle drifts every
+// substep, forces at the end of its step, the closing half kick, a new bin.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool blk_active(int b, int B, int64_t t1) {
+    return (t1 & ((1ll << (B - b)) - 1)) == 0;
+}
+
+// The per-particle form of the step criterion of the global dt
+// (myCppSPHLibs_v2.h:491-522 with hfv_sph_gpu_v2_evrard.cu:555-561: the same
+// four limits, each from this particle's own h, c, |a|, |a_g|, div v).
+__device__ __forceinline__ double blk_dt_crit(const Soa &s, int64_t i, double c_cfl) {
+#pragma clang fp contract(off)
+    double h = s.h[i];
+    double ag = sqrt((s.gx[i] * s.gx[i] + s.gy[i] * s.gy[i]) + s.gz[i] * s.gz[i]);
+    double at = sqrt((s.ax[i] * s.ax[i] + s.ay[i] * s.ay[i]) + s.az[i] * s.az[i]);
+    double dd = h / fabs((1.0 / 3.0) * h * s.divv[i]);
+    double dt_f = sqrt(h / ag), dt_kin = sqrt(h / at);
+    double dt_cfl = c_cfl * h / s.c[i], dt_dens = c_cfl * dd;
+    double m = dt_f;
+    m = dt_kin < m ? dt_kin : m;
+    m = dt_cfl < m ? dt_cfl : m;
+    m = dt_dens < m ? dt_dens : m;
+    return 0.25 * m;
+}
+
+// The longest step dt_max / 2^b (b in 0..B) not above dt_i; B (and a floor
+// count) when even the shortest is too long or dt_i is not a number.
+__device__ __forceinline__ int blk_target(double dt_i, double dt_max, int B, bool *floor) {
+    for (int b = 0; b <= B; ++b)
+        if (ldexp(dt_max, -b) <= dt_i) return b;
+    *floor = true;
+    return B;
+}
+
+// Tick 0: every particle takes its first bin and its opening half kick.
+__global__ void k_blk_open(Soa s, int64_t n, double dt_max, int B, double c_cfl,
+                           int64_t *counters) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    bool fl = false;
+    int b = blk_target(blk_dt_crit(s, i, c_cfl), dt_max, B, &fl);
+    s.bin[i] = b;
+    double half = 0.5 * ldexp(dt_max, -b);
+    s.vx[i] = s.vx[i] + s.ax[i] * half;
+    s.vy[i] = s.vy[i] + s.ay[i] * half;
+    s.vz[i] = s.vz[i] + s.az[i] * half;
+    if (fl) atomicAdd((unsigned long long *)&counters[CNT_BIN_FLOOR], 1ull);
+}
+
+__global__ void k_blk_drift(Soa s, int64_t n, double d) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    s.x[i] = s.x[i] + s.vx[i] * d;
+    s.y[i] = s.y[i] + s.vy[i] * d;
+    s.z[i] = s.z[i] + s.vz[i] * d;
+}
+
+// End of substep t1: the active particles take the closing half kick, the u
+// trapezoid and cooling over their step, a new bin (a shorter step at once, a
+// longer one only one level at a time and only where t1 is a multiple of the
+// longer step), then the opening half kick of their next step.
+__global__ void k_blk_close(Soa s, const RunParams *__restrict__ rp, int64_t n, double dt_max,
+                            int B, int64_t t1, double c_cfl, CoolDev cd, int cool_on, GridDev gd,
+                            ChimesDev ch, int64_t *counters) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    int b = s.bin[i];
+    if (!blk_active(b, B, t1)) return;
+    double dto = ldexp(dt_max, -b);
+    double half = 0.5 * dto;
+    s.vx[i] = s.vx[i] + s.ax[i] * half;
+    s.vy[i] = s.vy[i] + s.ay[i] * half;
+    s.vz[i] = s.vz[i] + s.az[i] * half;
+    double du = s.du[i];
+    double u = s.u[i] + half * (du + s.ut_prev[i]);
+    s.ut_prev[i] = du;
+    u = cool_particle(s, i, u, dto, rp[s.run[i]].use_cool, cool_on, cd, gd, ch, counters);
+    s.u[i] = u;
+    bool fl = false;
+    int bt = blk_target(blk_dt_crit(s, i, c_cfl), dt_max, B, &fl);
+    int bn = b;
+    if (bt >= b) bn = bt;
+    else if (blk_active(b - 1, B, t1)) bn = b - 1;
+    s.bin[i] = bn;
+    double h2 = 0.5 * ldexp(dt_max, -bn);
+    s.vx[i] = s.vx[i] + s.ax[i] * h2;
+    s.vy[i] = s.vy[i] + s.ay[i] * h2;
+    s.vz[i] = s.vz[i] + s.az[i] * h2;
+    if (fl) atomicAdd((unsigned long long *)&counters[CNT_BIN_FLOOR], 1ull);
+    count_nonfinite(s, i, u, counters);
 }
 
 __global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t p0,
@@ -445,6 +559,28 @@ int Engine::finish_step(hipStream_t st) {
     if (n == 0) return SPHX_OK;
     k_finish<<<g1(n), IT, 0, st>>>(s, d_run, n, make_cool(*this), has_cool ? 1 : 0, grid,
                                    chimes, counters);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::block_open(hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    k_blk_open<<<g1(n), IT, 0, st>>>(s, n, blk_dtmax, blk_B, blk_c_cfl, counters);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::block_drift(double d, hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    k_blk_drift<<<g1(n), IT, 0, st>>>(s, n, d);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::block_close(int64_t t1, hipStream_t st) {
+    if (n == 0) return SPHX_OK;
+    k_blk_close<<<g1(n), IT, 0, st>>>(s, d_run, n, blk_dtmax, blk_B, t1, blk_c_cfl,
+                                      make_cool(*this), has_cool ? 1 : 0, grid, chimes, counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -138,6 +138,7 @@ struct Soa {
     double *x, *y, *z, *vx, *vy, *vz, *u, *m, *h, *eps, *ut_prev;
     double *ax, *ay, *az, *gx, *gy, *gz, *rho, *P, *c, *divv, *curlv, *du, *pot;
     int32_t *k, *id, *run;
+    int32_t *bin;  // block time-step level of each particle (block mode)
 };
 
 struct Tables {
@@ -163,6 +164,7 @@ enum {
     CNT_CAND,
     CNT_BH_OVERFLOW,
     CNT_NONFINITE,
+    CNT_BIN_FLOOR,
     CNT_SLOTS = 8
 };
 
@@ -306,6 +308,18 @@ struct Engine {
     int64_t cand_cap = 0;
     int64_t cand_total = 0;
     int lcut_bias = 0;  // shift of the per-run bucket cut level (tuning only: sums do not change)
+
+    // block time steps (plans): levels B (0: off), longest step, substep tick
+    int blk_B = 0;
+    double blk_dtmax = 0.0, blk_c_cfl = 0.25;
+    int64_t blk_tick = 0;   // substeps since the last tick that every level shares
+    int64_t blk_t1 = 0;     // > 0 while the forces of substep end t1 are computed
+    bool blk_started = false;
+    int32_t *bin_spare = nullptr;
+    int block_open(hipStream_t st);
+    int block_drift(double d, hipStream_t st);
+    int block_close(int64_t t1, hipStream_t st);
+    int block_substep(hipStream_t st);
 
     std::vector<void *> owned;  // every hipMalloc'd pointer
 

@@ -340,25 +340,25 @@ int sphx_launch_morton(Engine &E, hipStream_t st) {
 // Persistent arrays gathered by the sort, and the derived arrays that serve
 // as their destinations (recomputed after every sort).  After the gather the
 // roles swap; applying the swap twice restores the original assignment.
-static void sort_roles(Engine &E, double **dst[12], double **spare[12], int32_t **idst[3],
-                       int32_t **ispare[3]) {
+static void sort_roles(Engine &E, double **dst[12], double **spare[12], int32_t **idst[4],
+                       int32_t **ispare[4]) {
     Soa &s = E.s;
     double **d[12] = {&s.x, &s.y, &s.z, &s.vx, &s.vy, &s.vz, &s.u, &s.m,
                       &s.h, &s.eps, &s.ut_prev, &s.divv};
     double **sp[12] = {&s.gx, &s.gy, &s.gz, &s.rho, &s.P, &s.c, &s.curlv, &s.du,
                        &s.pot, &s.ax, &s.ay, &s.az};
-    int32_t **id[3] = {&s.k, &s.id, &s.run};
-    int32_t **isp[3] = {&E.tmpi, &E.perm2, &E.h_iters};
+    int32_t **id[4] = {&s.k, &s.id, &s.run, &s.bin};
+    int32_t **isp[4] = {&E.tmpi, &E.perm2, &E.h_iters, &E.bin_spare};
     for (int a = 0; a < 12; ++a) { dst[a] = d[a]; spare[a] = sp[a]; }
-    for (int a = 0; a < 3; ++a) { idst[a] = id[a]; ispare[a] = isp[a]; }
+    for (int a = 0; a < 4; ++a) { idst[a] = id[a]; ispare[a] = isp[a]; }
 }
 
 void Engine::swap_sorted() {
     double **d[12], **sp[12];
-    int32_t **id[3], **isp[3];
+    int32_t **id[4], **isp[4];
     sort_roles(*this, d, sp, id, isp);
     for (int a = 0; a < 12; ++a) std::swap(*d[a], *sp[a]);
-    for (int a = 0; a < 3; ++a) std::swap(*id[a], *isp[a]);
+    for (int a = 0; a < 4; ++a) std::swap(*id[a], *isp[a]);
     gpar ^= 1;
 }
 
@@ -368,16 +368,16 @@ int Engine::sort_and_permute(hipStream_t st, bool gather_state) {
     SPHX_CHECK(sphx_radix_sort(keys, perm, keys2, perm2, hist, n, run_key_bits(max_runs), st));
     if (!gather_state) return SPHX_OK;
     double **d[12], **sp[12];
-    int32_t **id[3], **isp[3];
+    int32_t **id[4], **isp[4];
     sort_roles(*this, d, sp, id, isp);
     GatherList g{};
     g.nd = 12;
-    g.ni = 3;
+    g.ni = 4;
     for (int a = 0; a < 12; ++a) {
         g.src[a] = *d[a];
         g.dst[a] = *sp[a];
     }
-    for (int a = 0; a < 3; ++a) {
+    for (int a = 0; a < 4; ++a) {
         g.isrc[a] = *id[a];
         g.idst[a] = *isp[a];
     }

@@ -156,6 +156,22 @@ class SphPlan:
         """Barnes-Hut gravity in the step (theta > 0); 0 restores the direct sum."""
         check(_lib.load().sphx_plan_set_gravity_bh(self._h, float(theta)), "set_gravity_bh")
 
+    def set_block_steps(self, levels: int, c_cfl: float = 0.25):
+        """Block time steps with `levels` halvings of the longest step (0: off).
+        Then step(n, dt) takes n substeps of dt / 2**levels; particle i steps
+        with dt / 2**b_i chosen from its own criterion (before the first step
+        after init)."""
+        check(_lib.load().sphx_plan_set_block_steps(self._h, int(levels), float(c_cfl)),
+              "set_block_steps")
+
+    def block_bins(self):
+        """Level b_i of every particle (original order)."""
+        t = _lib.torch()
+        out = t.empty(self.n, dtype=t.int32, device="cuda")
+        check(_lib.load().sphx_plan_block_bins(self._h, ptr(out), _lib.stream_handle()),
+              "block_bins")
+        return out.cpu().numpy()
+
     def set_adaptive_dt(self, on: bool = True, c_cfl: float = 0.25, dt_min: float = 1e-5,
                         dt_max: float = 1e-3):
         """Global adaptive dt (hfv_sph_gpu_v2_evrard.cu:517-561) evaluated on the

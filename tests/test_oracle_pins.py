@@ -599,3 +599,56 @@ def test_reference_recorded_energy_history():
             assert abs(u / en["U"][k + 1] - 1) < 1e-7, k
             assert abs(ke / en["KE"][k + 1] - 1) < 1e-6, k
             assert abs(pe / en["PE"][k + 1] - 1) < 1e-7, k
+
+
+# ---------------------------------------------------------------- block time steps
+def _boot(n=512, seed=5):
+    e = ic.evrard(n, seed=seed)
+    p = oracle.Params()
+    return oracle.bootstrap(oracle.State(e.pos, e.vel, e.u, e.m, e.h), p), p
+
+
+def test_block_levels_closed_form():
+    """The level is the smallest b with dt_max / 2^b <= dt_i: b = ceil(log2(dt_max /
+    dt_i)) clamped to [0, levels], with a floor flag below the shortest step."""
+    rng = np.random.default_rng(3)
+    dt_i = 10.0 ** rng.uniform(-4, 0.5, 5000)
+    b, fl = oracle.block_level(dt_i, 1.0, 6)
+    want = np.clip(np.ceil(np.log2(1.0 / dt_i)), 0, 6).astype(int)
+    assert np.array_equal(b, want)
+    assert np.array_equal(fl, dt_i < 2.0 ** -6)
+    b, fl = oracle.block_level(np.array([0.5, 0.25, np.nan, np.inf]), 1.0, 3)
+    assert list(b) == [1, 2, 3, 0] and list(fl) == [False, False, True, False]
+
+
+def test_block_steps_with_one_level_are_the_global_step():
+    """levels = 0: every particle steps with dt_max, which is the fixed-dt KDK
+    step, bit for bit."""
+    s0, p = _boot()
+    want = s0
+    for _ in range(3):
+        want = oracle.step(want, 2e-3, p)
+    got, bins, _ = oracle.block_run(s0, 2e-3, 0, 3, p)
+    assert np.all(bins == 0)
+    for f in ("pos", "u", "h", "rho"):
+        assert np.array_equal(getattr(got, f), getattr(want, f)), f
+    # between substeps a block-stepped particle carries the opening half kick of
+    # its next step (its drift velocity)
+    assert np.array_equal(got.vel, want.vel + want.acc * (0.5 * 2e-3))
+
+
+def test_block_steps_all_on_the_shortest_level_are_the_global_step():
+    """With dt_max far above every particle's criterion all particles sit on
+    the shortest level: again the fixed-dt step with dt_max / 2^levels."""
+    s0, p = _boot()
+    dti = oracle.particle_dt(s0.h, s0.c, s0.acc, s0.acc_g, s0.divV)
+    big = 8.0 * 1.25 * dti.max()          # even the shortest step, big / 8, is too long
+    got, bins, floors = oracle.block_run(s0, big, 3, 4, p)
+    assert floors[0] == len(bins) and np.all(bins == 3)
+    want = s0
+    for _ in range(4):
+        want = oracle.step(want, big / 8, p)
+    assert np.all(np.isfinite(want.pos))
+    for f in ("pos", "u"):
+        assert np.array_equal(getattr(got, f), getattr(want, f)), f
+    assert np.array_equal(got.vel, want.vel + want.acc * (0.5 * (big / 8)))
