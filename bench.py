@@ -57,7 +57,8 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="evrard",
-                    choices=["evrard", "c1", "c2", "c3", "bh", "serve", "kernels", "aug", "gadget"])
+                    choices=["evrard", "c1", "c2", "c3", "bh", "serve", "kernels", "aug", "gadget",
+                             "block"])
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=None)
@@ -99,7 +100,8 @@ def main(argv=None) -> int:
         return 1
     if a.workload != "evrard":
         return {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "bh": bench_bh, "serve": bench_serve,
-                "kernels": bench_kernels, "aug": bench_aug, "gadget": bench_gadget}[a.workload](a)
+                "kernels": bench_kernels, "aug": bench_aug, "gadget": bench_gadget,
+                "block": bench_block}[a.workload](a)
     if a.n is None:
         a.n = 1_048_576
     return bench_evrard(a)
@@ -686,6 +688,67 @@ def bench_gadget(a) -> int:
 
 
 COPY_BYTES = 256 * 2 ** 20
+
+
+def bench_block(a) -> int:
+    """Block time steps at Evrard N (default 1,048,576): a probe substep finds
+    the spread of the per-particle step criterion, which sets the longest step
+    (the coarsest occupied level becomes level 0) and the number of levels;
+    then one full cycle of the longest step with block steps is timed against
+    the fixed-dt step at the shortest level over the same simulated time (what
+    a single global step must use so that every particle meets its criterion)."""
+    import torch
+
+    from sph_2_amd import ic
+    from sph_2_amd.plan import SphPlan
+
+    n = a.n or 1_048_576
+    e = ic.evrard(n, seed=a.seed)
+    probe_dt, probe_levels = 0.4, 10
+    pr = SphPlan(n)
+    pr.set_block_steps(probe_levels)
+    pr.load(e.pos, e.vel, e.u, e.m, e.h)
+    pr.init()
+    pr.step(1, probe_dt)
+    b = pr.block_bins()
+    pr.close()
+    lo, hi = int(b.min()), int(b.max())
+    levels = max(hi - lo, 1)
+    dt_max = probe_dt * 2.0 ** -lo
+    dt_min = dt_max * 2.0 ** -levels
+    log(f"block: levels {lo}..{hi} of the probe -> dt_max {dt_max:.3e}, {levels} levels")
+    res = {}
+    for mode in ("block", "fixed"):
+        pl = SphPlan(n)
+        if mode == "block":
+            pl.set_block_steps(levels)
+        pl.load(e.pos, e.vel, e.u, e.m, e.h)
+        pl.init()
+        torch.cuda.synchronize()
+        pl.set_timing(True)
+        t0 = time.perf_counter()
+        pl.step(2 ** levels, dt_max if mode == "block" else dt_min)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ph = pl.timing()
+        entry = {"wall_s": wall, "substeps": 2 ** levels, "simulated_time": dt_max,
+                 "gravity_ms_per_substep": ph["gravity"][0] / max(ph["gravity"][1], 1),
+                 "ms_per_substep": 1e3 * wall / 2 ** levels}
+        if mode == "block":
+            bins = pl.block_bins()
+            entry["levels_population"] = {int(k): int(v) for k, v in
+                                          zip(*np.unique(bins, return_counts=True))}
+            entry["counters"] = pl.counters()
+        res[mode] = entry
+        pl.close()
+        log(f"{mode}: {wall:.2f} s for {dt_max:.3e} of simulated time")
+    out = {"workload": f"block time steps: Evrard N={n} seed {a.seed}, {levels} levels below "
+                       f"dt_max={dt_max:.4e}, one cycle ({2 ** levels} substeps) vs the fixed "
+                       f"step dt_max/2^{levels}", "n": n, "levels": levels, "dt_max": dt_max,
+           "block": res["block"], "fixed": res["fixed"],
+           "speedup_simulated_time_per_wall": res["fixed"]["wall_s"] / res["block"]["wall_s"]}
+    print(json.dumps(out))
+    return 0
 
 
 def bench_kernels(a) -> int:

@@ -322,14 +322,6 @@ int sphx_ctx_set_gravity_config(sphx_ctx *ctx, int32_t ipl, int32_t tj, int32_t 
 int sphx_plan_set_timing(sphx_plan *plan, int32_t on);
 int sphx_plan_timing(sphx_plan *plan, double *ms, int64_t *count, int32_t nphase);
 
-/* Adaptive global time step inside the step loop (DRV2:517-561): after each
- * step's forces the device evaluates the criterion of sphx_global_dt and
- * uses it for the next step, with no host round trip.  The dt argument of
- * the first sphx_plan_step after enabling seeds the first step (the driver
- * starts from dt = 2e-4, DRV2:303); later dt arguments are ignored.
- * sphx_plan_time returns (host scalars) the simulated time -- the sum of the
- * dt each step used --, the next step's dt and (nullable, 4 doubles) the
- * last dt_f, dt_kin, dt_cfl, dt_dens.  Synchronous. */
 /* Block time steps (levels > 0; 0 switches them off): particle i steps with
  * dt / 2^b_i, b_i in 0..levels, where dt is sphx_plan_step's argument (the
  * longest step, fixed until the next sphx_plan_init) and each sphx_plan_step
@@ -376,6 +368,14 @@ int sphx_plan_chimes_ions(sphx_plan *plan, float *out, void *stream);
 int sphx_plan_set_cell_level(sphx_plan *plan, int32_t bias);
 int sphx_batch_set_cell_level(sphx_batch *b, int32_t bias);
 
+/* Adaptive global time step inside the step loop (DRV2:517-561): after each
+ * step's forces the device evaluates the criterion of sphx_global_dt and
+ * uses it for the next step, with no host round trip.  The dt argument of
+ * the first sphx_plan_step after enabling seeds the first step (the driver
+ * starts from dt = 2e-4, DRV2:303); later dt arguments are ignored.
+ * sphx_plan_time returns (host scalars) the simulated time -- the sum of the
+ * dt each step used --, the next step's dt and (nullable, 4 doubles) the
+ * last dt_f, dt_kin, dt_cfl, dt_dens.  Synchronous. */
 int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl, double dt_min,
                               double dt_max);
 int sphx_plan_time(sphx_plan *plan, double *t, double *dt, double *dt_parts, void *stream);

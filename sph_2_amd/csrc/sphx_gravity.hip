@@ -42,8 +42,12 @@ struct GravIO {
     int64_t blk_t1;
 };
 
+// Per tile of tj particles: box, max eps, common mass; with bin given (block
+// time steps) also whether the tile holds a particle whose step ends at t1.
 __global__ void k_tile_boxes(Tables t, const int32_t *__restrict__ r_tbeg, int ntiles_total,
-                             int tj, const double *__restrict__ m, double *__restrict__ tbox) {
+                             int tj, const double *__restrict__ m, double *__restrict__ tbox,
+                             const int32_t *__restrict__ bin, int blk_B, int64_t blk_t1,
+                             int32_t *__restrict__ act) {
     int tile = blockIdx.x;
     int lane = threadIdx.x;
     // find the run of this tile (runs are few; linear scan by lane 0 is enough)
@@ -82,9 +86,15 @@ __global__ void k_tile_boxes(Tables t, const int32_t *__restrict__ r_tbeg, int n
     }
     mlo = wave_min_d(mlo);
     mhi = wave_max_d(mhi);
+    bool any = false;
+    if (bin)
+        for (int q = p0 + lane; q < p1; q += 64)
+            any = any || (blk_t1 & ((1ll << (blk_B - bin[q])) - 1)) == 0;
+    any = __any(any);
     if (lane == 0) {
         for (int a = 0; a < 7; ++a) tbox[8 * tile + a] = v[a];
         tbox[8 * tile + 7] = (mlo == mhi && mlo >= 0.0) ? mlo : -1.0;
+        if (bin) act[tile] = any ? 1 : 0;
     }
 }
 
@@ -363,6 +373,8 @@ struct SymTabs {
     float *px, *py, *pz;           // j-side partials
     double *qx, *qy, *qz;          // i-side partials, SYM_B slots per segment
     int32_t seg0;                  // global index of the launch's first segment
+    const int32_t *act;            // block time steps: blocks holding an active particle
+                                   // (nullptr: every block pair is computed)
     const int32_t *run;            // run of each particle
     const int32_t *r_pbeg;         // first particle of each run
     int64_t n;
@@ -505,11 +517,14 @@ __device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__re
     const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
     const int beg = T.b_beg[I], cnt = T.b_cnt[I];
     const double *bI = T.bbox + 8 * (int64_t)I;
+    // block time steps: a block pair with no active particle on either side is
+    // not needed (its sums feed only inactive particles)
+    const bool actI = !T.act || T.act[I];
     if (!FAR) {
         // most segments hold no near pair: leave before loading the i block
         bool any = false;
         for (int J = J0; J < J1 && !any; ++J)
-            any = sym_mode(bI, T.bbox + 8 * (int64_t)J, J == I) >= 2;
+            any = (actI || T.act[J]) && sym_mode(bI, T.bbox + 8 * (int64_t)J, J == I) >= 2;
         if (!any) return;
     }
     const double ox = 0.5 * (bI[0] + bI[1]), oy = 0.5 * (bI[2] + bI[3]), oz = 0.5 * (bI[4] + bI[5]);
@@ -537,6 +552,7 @@ __device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__re
     for (int J = J0; J < J1; ++J) {
         const int jb = T.b_beg[J], jc = T.b_cnt[J];
         const double *bJ = T.bbox + 8 * (int64_t)J;
+        if (!actI && !T.act[J]) continue;
         const int mode = sym_mode(bI, bJ, J == I);
         if (FAR != (mode <= 1)) continue;
         const float si = mode == 0 ? (float)bJ[7] : 1.0f;  // i-side scale: the j block's mass
@@ -670,6 +686,8 @@ struct SymHost {
     int64_t part_cap = 0, part_len = 0;
     double *q = nullptr;            // 3 x SYM_B per segment
     int64_t q_cap = 0;
+    int32_t *act = nullptr;         // per block: holds an active particle (block time steps)
+    int64_t act_cap = 0;
     int64_t budget = -1;            // bytes the partial sums may take (-1: no limit)
     double *pool = nullptr;         // reserved memory of a budget (i-side, then j-side sums)
     int o_bbeg = 0, o_bcnt = 0, o_runb0 = 0, o_si = 0, o_sj0 = 0, o_sj1 = 0, o_rj0 = 0, o_rns = 0,
@@ -827,13 +845,19 @@ static int sym_run(Engine &E, hipStream_t st) {
         p1 = E.rp1;
     }
     if (s1 <= s0) return SPHX_OK;
-    k_tile_boxes<<<H.nblocks, 64, 0, st>>>(E.t, d + H.o_runb0, H.nblocks, SYM_B, E.s.m, H.bbox);
+    const bool blk = E.blk_t1 > 0;
+    if (blk && H.act_cap < H.nblocks) {
+        SPHX_CHECK(E.alloc((void **)&H.act, sizeof(int32_t) * (H.nblocks + 64)));
+        H.act_cap = H.nblocks + 64;
+    }
+    k_tile_boxes<<<H.nblocks, 64, 0, st>>>(E.t, d + H.o_runb0, H.nblocks, SYM_B, E.s.m, H.bbox,
+                                           blk ? E.s.bin : nullptr, E.blk_B, E.blk_t1, H.act);
     SPHX_HIP(hipGetLastError());
     int64_t pl = H.part_len, ql = (int64_t)SYM_B * H.nseg;
     SymTabs T{d + H.o_bbeg, d + H.o_bcnt, d + H.o_runb0, d + H.o_si + s0, d + H.o_sj0 + s0,
               d + H.o_sj1 + s0, H.row_off, d + H.o_rj0, d + H.o_rns, d + H.o_rs0, H.bbox,
               H.part, H.part + pl, H.part + 2 * pl, H.q, H.q + ql, H.q + 2 * ql, s0,
-              E.s.run, E.t.r_pbeg, E.n};
+              blk ? H.act : nullptr, E.s.run, E.t.r_pbeg, E.n};
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
     SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
     k_grav_sym<<<(unsigned)(s1 - s0), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
@@ -909,13 +933,11 @@ int sphx_gravity_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
 
 int sphx_gravity_run(Engine &E, const GravConfig &cfg_in, bool with_pot, hipStream_t st) {
     if (E.n == 0) return SPHX_OK;
-    if (cfg_in.ipl == 8 && !with_pot && E.blk_t1 == 0) {
+    if (cfg_in.ipl == 8 && !with_pot) {
         SPHX_CHECK(sym_prepare(E, cfg_in, st));
         return sym_run(E, st);
     }
-    // the symmetric sum carries no potential and computes every pair: those
-    // calls and block substeps (gravity only for active i-blocks) use the
-    // tiled sum
+    // the symmetric sum carries no potential: that call uses the tiled sum
     const GravConfig cfg = cfg_in.ipl == 8 ? GravConfig{2, 512, 1} : cfg_in;
     SPHX_CHECK(sphx_gravity_prepare(E, cfg, st));
     GravHost &H = E.grav;
@@ -932,7 +954,7 @@ int sphx_gravity_run(Engine &E, const GravConfig &cfg_in, bool with_pot, hipStre
                 d + 3 * H.nblocks, H.tbox, cfg.js};
     if (H.ntiles > 0) {
         k_tile_boxes<<<H.ntiles, 64, 0, st>>>(E.t, d + 3 * H.nblocks, H.ntiles, cfg.tj, E.s.m,
-                                              H.tbox);
+                                              H.tbox, nullptr, 0, 0, nullptr);
         SPHX_HIP(hipGetLastError());
     }
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
