@@ -161,6 +161,17 @@ def bench_evrard(a) -> int:
                 "avg_ms": 1e3 * kern_s,
                 "gravity_phase_ms": g_ms / max(g_cnt, 1),
                 "launch_config": list(cfg_used)}
+    # the far-pair instruction mix on registers alone: the rate the symmetric
+    # sum could reach if its memory, LDS rotation and near pairs cost nothing
+    issue = None
+    if kname == "k_grav_sym" and kern_s > 0:
+        from sph_2_amd import libsx
+        ceil = libsx.pair_issue_rate()
+        rate = 0.5 * n * (n - 1) / kern_s
+        issue = {"pairs_per_s": rate, "ceiling_pairs_per_s": ceil, "frac": rate / ceil,
+                 "ceiling": "k_pair_probe: the far-pair instruction mix (8 i x 2 j per lane), "
+                            "registers only, 8 waves per SIMD",
+                 "pairs": "N(N-1)/2 evaluations per launch, each for both particles"}
 
     state = pl.state(("pos", "vel", "u", "h", "rho", "acc_g"))
     pl.close()
@@ -290,6 +301,7 @@ def bench_evrard(a) -> int:
         "phases_ms_per_step": {k: (v[0] / max(v[1], 1)) for k, v in phases.items()},
         "counters": counters,
         "roofline": roofline,
+        "gravity_issue_ceiling": issue,
         "cpu_baseline": cpu,
         "parity_vs_fp64_oracle": parity,
         "serving": serving,

@@ -196,6 +196,16 @@ int sphx_getAcc_g_smth(sphx_ctx *ctx, int mode, int64_t n, const double *pos, co
                        const double *eps, double G, int64_t row_beg, int64_t row_end,
                        double *acc, double *pot, void *stream);
 
+/* Issue-rate ceiling of the direct sum's far pairs on this device: a kernel
+ * with the far-pair instruction mix of the symmetric sum (3 sub, 1 mul + 2 fma
+ * for r^2, rsq, 2 mul, 3 + 3 fma to both particles' sums), 8 i and 2 j per
+ * lane in registers, no memory traffic, eight waves per SIMD.  Writes the pair
+ * evaluations per second it sustains.  A measurement aid with no counterpart
+ * in the reference: it is what the direct sum's pair rate is compared with.
+ * iters in 1..65536 (16 pairs per lane per iteration); the fastest of three
+ * timed launches after a warm-up.  Synchronous. */
+int sphx_pair_issue_rate(sphx_ctx *ctx, int32_t iters, double *pairs_per_s, void *stream);
+
 /* Barnes-Hut gravity with the spline softening of getAcc_g_smth: Morton-built
  * octree (cells = key ranges), monopole moments, acceptance when a cell's
  * distance from the 64-particle Morton group's box exceeds s/theta + |com -

@@ -116,6 +116,7 @@ SIGNATURES = {
                                   _I64, _I64, _V, _V, _V]),
     "sphx_getAcc_g_smth": (C.c_int, [_V, C.c_int, _I64, _V, _V, _V, _D, _I64, _I64, _V, _V,
                                      _V]),
+    "sphx_pair_issue_rate": (C.c_int, [_V, _I32, _V, _V]),
     "sphx_bh_gravity": (C.c_int, [_V, _I64, _V, _V, _V, _D, _D, _I64, _I64, _V, _V, _V, _V]),
     "sphx_neighbors": (C.c_int, [_V, _I64, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V]),
     "sphx_cool": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V, _V]),
@@ -194,6 +195,10 @@ def load(path: str = LIB_PATH):
             return _lib
         if not os.path.exists(path):
             raise SphxError(f"{path} is missing: build it with `python -m sph_2_amd.build`")
+        # torch first: its HIP runtime is then the one libsphx binds to (by
+        # soname).  Loaded the other way round, the process would hold two HIP
+        # runtimes, and the library's would see no device.
+        torch()
         L = C.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

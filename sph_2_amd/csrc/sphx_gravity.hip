@@ -667,6 +667,59 @@ __global__ void k_grav_sym_reduce(SymTabs T, double G, int64_t p0, int64_t p1, d
     gz[i] = G * sz;
 }
 
+// Issue-rate reference for the far block pairs of k_grav_sym: the same
+// instruction mix per pair (3 sub, r^2 as 1 mul + 2 fma, rsq, 2 mul, 3 fma to
+// the i-side sums, 3 fma to the j-side sums) and the same shape (8 i and 2
+// sets of j per lane), on register-resident particles with no memory or LDS
+// traffic.  Each step moves every j by s so that nothing is loop-invariant.
+// Each lane evaluates 16 pairs per step; one store per lane at the end.
+__global__ __launch_bounds__(256) void k_pair_probe(float *out, int iters, float s) {
+    float xi[8], yi[8], zi[8], ax[8], ay[8], az[8];
+    const float t = threadIdx.x * 1e-3f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        xi[k] = t + k;
+        yi[k] = t - k;
+        zi[k] = 0.5f * k;
+        ax[k] = ay[k] = az[k] = 0.f;
+    }
+    float xj[2], yj[2], zj[2], bx[2], by[2], bz[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        xj[q] = 10.f + q;
+        yj[q] = 11.f;
+        zj[q] = 12.f - q;
+        bx[q] = by[q] = bz[q] = 0.f;
+    }
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float dx = xj[q] - xi[k], dy = yj[q] - yi[k], dz = zj[q] - zi[k];
+                const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float inv3 = inv * inv * inv;
+                ax[k] = __builtin_fmaf(inv3, dx, ax[k]);
+                ay[k] = __builtin_fmaf(inv3, dy, ay[k]);
+                az[k] = __builtin_fmaf(inv3, dz, az[k]);
+                bx[q] = __builtin_fmaf(-inv3, dx, bx[q]);
+                by[q] = __builtin_fmaf(-inv3, dy, by[q]);
+                bz[q] = __builtin_fmaf(-inv3, dz, bz[q]);
+            }
+            xj[q] += s;
+            yj[q] += s;
+            zj[q] += s;
+        }
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += ax[k] + ay[k] + az[k];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) acc += bx[q] + by[q] + bz[q];
+    out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+
 }  // namespace
 
 // Host tables of the symmetric sum; rebuilt when the run sizes or the segment
@@ -987,5 +1040,13 @@ int sphx_gravity_run(Engine &E, const GravConfig &cfg_in, bool with_pot, hipStre
             H.part, cfg.js, E.n, p0, p1, E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr);
         SPHX_HIP(hipGetLastError());
     }
+    return SPHX_OK;
+}
+
+// Launch the pair-rate reference: blocks x 256 lanes, 16 pairs per lane per
+// step; out holds blocks * 256 floats.
+int sphx_pair_probe_launch(int64_t blocks, int iters, float *out, hipStream_t st) {
+    k_pair_probe<<<(unsigned)blocks, 256, 0, st>>>(out, iters, 1e-3f);
+    SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

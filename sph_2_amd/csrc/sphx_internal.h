@@ -254,6 +254,7 @@ struct GravHost {
 void sphx_sym_host_free(struct SymHost *h);
 int64_t sphx_sym_run_bytes(int64_t n, int S);
 int sphx_set_cell_level(struct Engine &E, int32_t bias);
+int sphx_pair_probe_launch(int64_t blocks, int iters, float *out, hipStream_t st);
 int sphx_sym_reserve(struct Engine &E, int S, int64_t budget_bytes);
 bool sphx_sym_fits(const struct Engine &E, const std::vector<int64_t> &sizes, int S);
 

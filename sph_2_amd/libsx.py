@@ -310,6 +310,19 @@ def global_dt(h, csnd, acc_tot, acc_g, divV, C_CFL=0.25, dt_min=1e-5, dt_max=1e-
     return float(o[0]), tuple(float(v) for v in o[1:])
 
 
+def pair_issue_rate(iters=2048, *, device=0):
+    """Pair evaluations per second that this device sustains on the far-pair
+    instruction mix of the symmetric direct sum with registers only (no
+    memory traffic): the ceiling the direct sum's pair rate is compared with.
+    No counterpart in the reference."""
+    import ctypes as C
+
+    out = C.c_double(0.0)
+    check(_lib.load().sphx_pair_issue_rate(_context(device), int(iters), C.byref(out),
+                                           _lib.stream_handle()), "pair_issue_rate")
+    return out.value
+
+
 def diagnostics(pos, v, m, u=None, epsilon=None, G=1.0, *, device=0):
     """Kinetic, thermal, potential energy, momentum and angular momentum."""
     return _diag(pos, v, m, u=u, eps=epsilon, G=G, device=device)

@@ -381,3 +381,39 @@ def test_symmetric_gravity_conserves_momentum_at_scale(gpu):
     net = np.abs((e.m[:, None] * a_sym).sum(0)).max()
     assert net / scale < 1e-6
     assert rel_rms(a_sym, a_til) < 1e-6
+
+
+def test_pair_issue_rate(gpu):
+    """The far-pair issue-rate reference runs, returns a rate in a physical range
+    for one MI355X (10^11..10^14 pair evaluations per second) and rejects
+    iteration counts outside 1..65536."""
+    from sph_2_amd import libsx
+    from sph_2_amd._lib import SphxError
+
+    r = libsx.pair_issue_rate(64)
+    assert 1e11 < r < 1e14, r
+    for bad in (0, 70000):
+        with pytest.raises(SphxError, match="iters"):
+            libsx.pair_issue_rate(bad)
+
+
+def test_library_first_in_a_fresh_process(gpu):
+    """A caller whose first step is loading the library (before anything has
+    imported torch) still reaches the device: the process holds one HIP
+    runtime.  Runs in a child process so that nothing is imported beforehand."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import ctypes\n"
+            "from sph_2_amd import _lib\n"
+            "L = _lib.load()\n"
+            "c = ctypes.c_void_p()\n"
+            "rc = L.sphx_ctx_create(0, ctypes.byref(c))\n"
+            "assert rc == 0, L.sphx_last_error()\n"
+            "assert L.sphx_ctx_destroy(c) == 0\n"
+            "print('ok')\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True,
+                       timeout=150)
+    assert r.returncode == 0 and "ok" in r.stdout, r.stdout + r.stderr
