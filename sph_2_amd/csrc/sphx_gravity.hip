@@ -403,10 +403,14 @@ __device__ __forceinline__ float sym_spline_f(float r2, float e) {
     return ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
 }
 
-struct SymI {
-    float x[8], y[8], z[8], m[8], e[8];
-    float ax[8], ay[8], az[8];
+// The i particles a lane holds: P of them (8 in the far pass, 4 in the near
+// one), with their FP32 force sums of the current j chunk.
+template <int P>
+struct SymIP {
+    float x[P], y[P], z[P], m[P], e[P];
+    float ax[P], ay[P], az[P];
 };
+using SymI = SymIP<8>;
 
 struct SymJ {
     float x, y, z, m, e, fx, fy, fz;
@@ -436,10 +440,10 @@ __device__ __forceinline__ SymJ sym_load(const double *x, const double *y, const
 
 // MODE 0: far, common masses (applied later); 1: far; 2: near; 3: diagonal
 // block (one-sided, spline).
-template <int MODE>
-__device__ __forceinline__ void sym_pairs(SymI &S, SymJ &J) {
+template <int MODE, int P>
+__device__ __forceinline__ void sym_pairs(SymIP<P> &S, SymJ &J) {
 #pragma unroll
-    for (int p = 0; p < 8; ++p) {
+    for (int p = 0; p < P; ++p) {
         float dx = J.x - S.x[p];
         float dy = J.y - S.y[p];
         float dz = J.z - S.z[p];
@@ -478,8 +482,8 @@ __device__ __forceinline__ void sym_rotate(SymJ &J, int addr) {
     }
 }
 
-template <int MODE>
-__device__ __forceinline__ void sym_chunk(SymI &S, SymJ &A, SymJ &B, int addr) {
+template <int MODE, int P = 8>
+__device__ __forceinline__ void sym_chunk(SymIP<P> &S, SymJ &A, SymJ &B, int addr) {
     for (int s = 0; s < 64; ++s) {
         sym_pairs<MODE>(S, A);
         sym_rotate<MODE>(A, addr);
@@ -499,19 +503,18 @@ __device__ __forceinline__ int sym_mode(const double *bI, const double *bJ, bool
     return far ? ((bI[7] >= 0.0 && bJ[7] >= 0.0) ? 0 : 1) : 2;
 }
 
-// One segment of row I, restricted to the far block pairs (FAR) or to the
-// near and diagonal ones.  The two passes run as separate kernels so that
-// the far pass, nearly all of the work, keeps the register budget of its own
-// code (three waves per SIMD without spills in the pair loops).  The far pass
-// writes the segment's i-side sums; the near pass, launched after it on the
-// same stream, adds its own to them.  Every j-side slot belongs to exactly
-// one block pair and so to exactly one pass.
-template <bool FAR>
-__device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__restrict__ x,
-                                            const double *__restrict__ y,
-                                            const double *__restrict__ z,
-                                            const double *__restrict__ m,
-                                            const double *__restrict__ eps) {
+// One segment of row I, restricted to its far block pairs.  Far and near
+// block pairs run as separate kernels so that the far pass, nearly all of the
+// work, keeps the register budget of its own code (three waves per SIMD
+// without spills in the pair loops).  The far pass writes the segment's
+// i-side sums; the near pass, launched after it on the same stream, adds its
+// own to them.  Every j-side slot belongs to exactly one block pair and so to
+// exactly one pass.
+__device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                const double *__restrict__ z,
+                                                const double *__restrict__ m,
+                                                const double *__restrict__ eps) {
     const int lane = threadIdx.x;
     const int sg = blockIdx.x;
     const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
@@ -520,13 +523,6 @@ __device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__re
     // block time steps: a block pair with no active particle on either side is
     // not needed (its sums feed only inactive particles)
     const bool actI = !T.act || T.act[I];
-    if (!FAR) {
-        // most segments hold no near pair: leave before loading the i block
-        bool any = false;
-        for (int J = J0; J < J1 && !any; ++J)
-            any = (actI || T.act[J]) && sym_mode(bI, T.bbox + 8 * (int64_t)J, J == I) >= 2;
-        if (!any) return;
-    }
     const double ox = 0.5 * (bI[0] + bI[1]), oy = 0.5 * (bI[2] + bI[3]), oz = 0.5 * (bI[4] + bI[5]);
     const double mI = bI[7];
     const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
@@ -554,52 +550,22 @@ __device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__re
         const double *bJ = T.bbox + 8 * (int64_t)J;
         if (!actI && !T.act[J]) continue;
         const int mode = sym_mode(bI, bJ, J == I);
-        if (FAR != (mode <= 1)) continue;
+        if (mode > 1) continue;
         const float si = mode == 0 ? (float)bJ[7] : 1.0f;  // i-side scale: the j block's mass
         const float sj = mode == 0 ? (float)mI : 1.0f;     // j-side scale: the i block's mass
         for (int c = 0; c < jc; c += 128) {
             SymJ A = sym_load(x, y, z, m, eps, jb, jc, c + lane, ox, oy, oz);
             SymJ B = sym_load(x, y, z, m, eps, jb, jc, c + 64 + lane, ox, oy, oz);
-            int cm = mode;
-            if (!FAR && mode == 2) {
-                // A near block pair can still hold far chunks: test this chunk's own box
-                // (from the FP32 relative coordinates, with a margin for their rounding).
-                bool va = c + lane < jc, vb = c + 64 + lane < jc;
-                double cb[6] = {
-                    wave_min_d(fmin(va ? (double)A.x : INFINITY, vb ? (double)B.x : INFINITY)),
-                    wave_max_d(fmax(va ? (double)A.x : -INFINITY, vb ? (double)B.x : -INFINITY)),
-                    wave_min_d(fmin(va ? (double)A.y : INFINITY, vb ? (double)B.y : INFINITY)),
-                    wave_max_d(fmax(va ? (double)A.y : -INFINITY, vb ? (double)B.y : -INFINITY)),
-                    wave_min_d(fmin(va ? (double)A.z : INFINITY, vb ? (double)B.z : INFINITY)),
-                    wave_max_d(fmax(va ? (double)A.z : -INFINITY, vb ? (double)B.z : -INFINITY))};
-                double ce = wave_max_d(fmax(va ? (double)A.e : 0.0, vb ? (double)B.e : 0.0));
-                double ib[6] = {bI[0] - ox, bI[1] - ox, bI[2] - oy, bI[3] - oy, bI[4] - oz,
-                                bI[5] - oz};
-                double big = 1.0;
-                for (int k = 0; k < 6; ++k) big = fmax(big, fmax(fabs(cb[k]), fabs(ib[k])));
-                double d2 = sphx_box_d2(ib[0], ib[1], ib[2], ib[3], ib[4], ib[5], cb[0], cb[1],
-                                        cb[2], cb[3], cb[4], cb[5]);
-                double reach = (bI[6] + ce) * (1.0 + 1e-5) + 4e-6 * big;
-                if (d2 > reach * reach) cm = 1;
+            if (mode == 0) sym_chunk<0>(S, A, B, addr);
+            else sym_chunk<1>(S, A, B, addr);
+            int qa = c + lane, qb = c + 64 + lane;
+            if (qa < jc) {
+                int64_t o = roff + (jb + qa - rj0);
+                T.px[o] = sj * A.fx; T.py[o] = sj * A.fy; T.pz[o] = sj * A.fz;
             }
-            if (FAR) {
-                if (mode == 0) sym_chunk<0>(S, A, B, addr);
-                else sym_chunk<1>(S, A, B, addr);
-            } else {
-                if (cm == 1) sym_chunk<1>(S, A, B, addr);
-                else if (cm == 2) sym_chunk<2>(S, A, B, addr);
-                else sym_chunk<3>(S, A, B, addr);
-            }
-            if (mode != 3) {
-                int qa = c + lane, qb = c + 64 + lane;
-                if (qa < jc) {
-                    int64_t o = roff + (jb + qa - rj0);
-                    T.px[o] = sj * A.fx; T.py[o] = sj * A.fy; T.pz[o] = sj * A.fz;
-                }
-                if (qb < jc) {
-                    int64_t o = roff + (jb + qb - rj0);
-                    T.px[o] = sj * B.fx; T.py[o] = sj * B.fy; T.pz[o] = sj * B.fz;
-                }
+            if (qb < jc) {
+                int64_t o = roff + (jb + qb - rj0);
+                T.px[o] = sj * B.fx; T.py[o] = sj * B.fy; T.pz[o] = sj * B.fz;
             }
 #pragma unroll
             for (int p = 0; p < 8; ++p) {
@@ -614,11 +580,135 @@ __device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__re
         int li = lane + 64 * p;
         if (li < cnt) {
             int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
-            if (FAR) {
-                T.qx[o] = kx[p].value();
-                T.qy[o] = ky[p].value();
-                T.qz[o] = kz[p].value();
-            } else {
+            T.qx[o] = kx[p].value();
+            T.qy[o] = ky[p].value();
+            T.qz[o] = kz[p].value();
+        }
+    }
+}
+
+// The same segment restricted to its near and diagonal block pairs.  The
+// block's 512 i are taken as two halves of 256 (4 per lane), one after the
+// other: half the i-side registers of the far pass, so that more waves stay
+// resident on this latency-bound path, and each half tests the 128-particle j
+// chunks against its own box, so that more of them take the r^-3 path.  The j
+// sets rotate once per half; the first half stores their j-side sums and the
+// second adds to them (lane L owns the same j slots in both).
+__device__ __forceinline__ void sym_segment_near(const SymTabs &T, const double *__restrict__ x,
+                                                 const double *__restrict__ y,
+                                                 const double *__restrict__ z,
+                                                 const double *__restrict__ m,
+                                                 const double *__restrict__ eps) {
+    const int lane = threadIdx.x;
+    const int sg = blockIdx.x;
+    const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
+    const int beg = T.b_beg[I], cnt = T.b_cnt[I];
+    const double *bI = T.bbox + 8 * (int64_t)I;
+    const bool actI = !T.act || T.act[I];
+    // most segments hold no near pair: leave before loading the i block
+    bool any = false;
+    for (int J = J0; J < J1 && !any; ++J)
+        any = (actI || T.act[J]) && sym_mode(bI, T.bbox + 8 * (int64_t)J, J == I) >= 2;
+    if (!any) return;
+    const double ox = 0.5 * (bI[0] + bI[1]), oy = 0.5 * (bI[2] + bI[3]), oz = 0.5 * (bI[4] + bI[5]);
+    const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
+    const int64_t roff = T.row_off[I];
+    const int rj0 = T.row_j0[I];
+
+    for (int h = 0; 256 * h < cnt; ++h) {
+        SymIP<4> S;
+        KahanF kx[4], ky[4], kz[4];
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        double emax = 0.0;
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            int li = lane + 64 * (4 * h + p);
+            bool ok = li < cnt;
+            int i = beg + (ok ? li : 0);
+            S.x[p] = ok ? (float)(x[i] - ox) : 1.0e18f;
+            S.y[p] = ok ? (float)(y[i] - oy) : 1.0e18f;
+            S.z[p] = ok ? (float)(z[i] - oz) : 1.0e18f;
+            S.m[p] = ok ? (float)m[i] : 0.0f;
+            S.e[p] = ok ? (float)eps[i] : 1.0f;
+            S.ax[p] = S.ay[p] = S.az[p] = 0.0f;
+            kx[p].init(); ky[p].init(); kz[p].init();
+            if (ok) {
+                lo[0] = fmin(lo[0], (double)S.x[p]); hi[0] = fmax(hi[0], (double)S.x[p]);
+                lo[1] = fmin(lo[1], (double)S.y[p]); hi[1] = fmax(hi[1], (double)S.y[p]);
+                lo[2] = fmin(lo[2], (double)S.z[p]); hi[2] = fmax(hi[2], (double)S.z[p]);
+                emax = fmax(emax, (double)S.e[p]);
+            }
+        }
+        // the half's box (block-relative FP32 coordinates) and largest softening
+        const double hb[6] = {wave_min_d(lo[0]), wave_max_d(hi[0]), wave_min_d(lo[1]),
+                              wave_max_d(hi[1]), wave_min_d(lo[2]), wave_max_d(hi[2])};
+        const double he = wave_max_d(emax);
+
+        for (int J = J0; J < J1; ++J) {
+            const int jb = T.b_beg[J], jc = T.b_cnt[J];
+            const double *bJ = T.bbox + 8 * (int64_t)J;
+            if (!actI && !T.act[J]) continue;
+            const int mode = sym_mode(bI, bJ, J == I);
+            if (mode <= 1) continue;
+            for (int c = 0; c < jc; c += 128) {
+                SymJ A = sym_load(x, y, z, m, eps, jb, jc, c + lane, ox, oy, oz);
+                SymJ B = sym_load(x, y, z, m, eps, jb, jc, c + 64 + lane, ox, oy, oz);
+                int cm = mode;
+                if (mode == 2) {
+                    // A near block pair can still hold far (half, chunk) pairs: test the
+                    // chunk's box against the half's (both from the FP32 relative
+                    // coordinates the pairs use, with a margin for their rounding).
+                    bool va = c + lane < jc, vb = c + 64 + lane < jc;
+                    double cb[6] = {
+                        wave_min_d(fmin(va ? (double)A.x : INFINITY, vb ? (double)B.x : INFINITY)),
+                        wave_max_d(fmax(va ? (double)A.x : -INFINITY, vb ? (double)B.x : -INFINITY)),
+                        wave_min_d(fmin(va ? (double)A.y : INFINITY, vb ? (double)B.y : INFINITY)),
+                        wave_max_d(fmax(va ? (double)A.y : -INFINITY, vb ? (double)B.y : -INFINITY)),
+                        wave_min_d(fmin(va ? (double)A.z : INFINITY, vb ? (double)B.z : INFINITY)),
+                        wave_max_d(fmax(va ? (double)A.z : -INFINITY, vb ? (double)B.z : -INFINITY))};
+                    double ce = wave_max_d(fmax(va ? (double)A.e : 0.0, vb ? (double)B.e : 0.0));
+                    double big = 1.0;
+                    for (int k = 0; k < 6; ++k) big = fmax(big, fmax(fabs(cb[k]), fabs(hb[k])));
+                    double d2 = sphx_box_d2(hb[0], hb[1], hb[2], hb[3], hb[4], hb[5], cb[0], cb[1],
+                                            cb[2], cb[3], cb[4], cb[5]);
+                    double reach = (he + ce) * (1.0 + 1e-5) + 4e-6 * big;
+                    if (d2 > reach * reach) cm = 1;
+                }
+                if (cm == 1) sym_chunk<1, 4>(S, A, B, addr);
+                else if (cm == 2) sym_chunk<2, 4>(S, A, B, addr);
+                else sym_chunk<3, 4>(S, A, B, addr);
+                if (mode != 3) {
+                    int qa = c + lane, qb = c + 64 + lane;
+                    if (qa < jc) {
+                        int64_t o = roff + (jb + qa - rj0);
+                        if (h == 0) {
+                            T.px[o] = A.fx; T.py[o] = A.fy; T.pz[o] = A.fz;
+                        } else {
+                            T.px[o] += A.fx; T.py[o] += A.fy; T.pz[o] += A.fz;
+                        }
+                    }
+                    if (qb < jc) {
+                        int64_t o = roff + (jb + qb - rj0);
+                        if (h == 0) {
+                            T.px[o] = B.fx; T.py[o] = B.fy; T.pz[o] = B.fz;
+                        } else {
+                            T.px[o] += B.fx; T.py[o] += B.fy; T.pz[o] += B.fz;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    kx[p].add(S.ax[p]); S.ax[p] = 0.0f;
+                    ky[p].add(S.ay[p]); S.ay[p] = 0.0f;
+                    kz[p].add(S.az[p]); S.az[p] = 0.0f;
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            int li = lane + 64 * (4 * h + p);
+            if (li < cnt) {
+                int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
                 T.qx[o] += kx[p].value();
                 T.qy[o] += ky[p].value();
                 T.qz[o] += kz[p].value();
@@ -631,7 +721,7 @@ __device__ __forceinline__ void sym_segment(const SymTabs &T, const double *__re
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grav_sym(
     SymTabs T, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ eps) {
-    sym_segment<true>(T, x, y, z, m, eps);
+    sym_segment_far(T, x, y, z, m, eps);
 }
 
 __global__ __launch_bounds__(64) void k_grav_sym_near(SymTabs T, const double *__restrict__ x,
@@ -639,7 +729,7 @@ __global__ __launch_bounds__(64) void k_grav_sym_near(SymTabs T, const double *_
                                                       const double *__restrict__ z,
                                                       const double *__restrict__ m,
                                                       const double *__restrict__ eps) {
-    sym_segment<false>(T, x, y, z, m, eps);
+    sym_segment_near(T, x, y, z, m, eps);
 }
 
 // a_i = G (sum over the segments of i's row of their i-side sums, in order,
