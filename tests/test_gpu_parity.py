@@ -170,13 +170,15 @@ def test_gravity_launch_shapes(gpu, evr, cfg):
     assert rel_rms(pot, pot_o) < 1e-5
 
 
-@pytest.mark.parametrize("n", [4096, 3001, 300])
+@pytest.mark.parametrize("n", [4096, 3001, 2200, 300])
 @pytest.mark.parametrize("masses", ["equal", "random"])
 @pytest.mark.parametrize("eps", ["h", 0.002])
 def test_gravity_symmetric(gpu, evr, n, masses, eps):
     """The symmetric sum (each pair once, applied to both particles) against
     the oracle: far block pairs with one common mass, with mixed masses, near
-    and diagonal block pairs, partial blocks, segment lengths 1, 3 and 64."""
+    and diagonal block pairs, partial blocks (n = 3001: a last block of 441,
+    whose second half of 256 is partial; 2200: a last block of 152, with no
+    second half; 300: one partial block), segment lengths 1, 3 and 64."""
     from sph_2_amd import _lib, libsx
     from sph_2_amd.autotune import _default
 
