@@ -254,6 +254,18 @@ def require_gpu():
     return t
 
 
+def check_run_arrays(pos, vel, u, m, h, eps=None) -> int:
+    """Shapes of one run's particle arrays (numpy or torch): pos and vel
+    (n, 3), u, m, h and eps (n,).  The library reads n rows of each, so a
+    shorter array would be read past its end; returns n."""
+    n = len(m)
+    for name, a, shape in (("pos", pos, (n, 3)), ("vel", vel, (n, 3)), ("u", u, (n,)),
+                           ("m", m, (n,)), ("h", h, (n,)), ("eps", eps, (n,))):
+        if a is not None and tuple(np.shape(a) if not hasattr(a, "shape") else a.shape) != shape:
+            raise ValueError(f"{name} must have shape {shape}, got {tuple(a.shape)}")
+    return n
+
+
 def dev(a, dtype=None):
     """numpy / torch -> contiguous device tensor."""
     t = require_gpu()

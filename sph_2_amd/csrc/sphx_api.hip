@@ -1220,7 +1220,11 @@ extern "C" int sphx_diagnostics(sphx_ctx *ctx, int64_t n, const double *pos, con
                                 const double *m, const double *u, const double *eps, double G,
                                 sphx_diag *out, void *stream) {
     CTX_PROLOGUE(ctx);
-    if (n < 0 || !v || !m || !out) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    if (n < 0 || !out || (n > 0 && (!v || !m))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    if (n == 0) {  // no particles: every sum is zero
+        *out = sphx_diag{};
+        return SPHX_OK;
+    }
     Engine &E = ctx->E;
     SPHX_CHECK(ctx_load(ctx, n, pos ? pos : v, m, nullptr, v, false, st));
     const double *pot = nullptr;
@@ -1302,7 +1306,7 @@ extern "C" int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, con
                               void *stream) {
     PLAN_PROLOGUE(plan);
     Engine &E = plan->E;
-    if (n <= 0 || n > E.cap) { sphx_set_error("particle count exceeds the plan capacity"); return SPHX_ERR_ARG; }
+    if (n <= 0 || n > E.cap) { sphx_set_error("particle count must be 1 .. the plan capacity"); return SPHX_ERR_ARG; }
     if (!pos || !vel || !u || !m || !h) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
     if (!E.p.eps_from_h && !eps) { sphx_set_error("eps required when eps_from_h == 0"); return SPHX_ERR_ARG; }
     SPHX_CHECK(E.set_runs({n}));

@@ -47,6 +47,35 @@ def _f64(a, shape_tail=()):
     return a
 
 
+def _xyz(a, n=None, name="pos"):
+    """Contiguous float64 (n, 3) array; n is taken from it when None.  The
+    kernels read n rows of every array they are given, so a shorter one
+    would be read past its end: refused here instead."""
+    a = _f64(a)
+    if a.ndim != 2 or a.shape[1] != 3 or (n is not None and a.shape[0] != n):
+        want = "(N, 3)" if n is None else f"({n}, 3)"
+        raise ValueError(f"{name} must have shape {want}, got {a.shape}")
+    return a
+
+
+def _vec(a, n, name):
+    """Contiguous float64 (n,) array (see _xyz)."""
+    a = _f64(a)
+    if a.shape != (n,):
+        raise ValueError(f"{name} must have shape ({n},), got {a.shape}")
+    return a
+
+
+def _rho_u(rho, u):
+    """rho and u as device arrays of one length n (1-D float64)."""
+    rho = _f64(rho)
+    if rho.ndim != 1:
+        raise ValueError(f"rho must be 1-D, got shape {rho.shape}")
+    n = len(rho)
+    u = _vec(u, n, "u")
+    return dev(rho), dev(u)
+
+
 def _rows(n, nbeg, nend):
     if nbeg is None:
         return 0, n
@@ -70,10 +99,11 @@ def _t():
 def getDensity(pos, m, h, *, mode="mixed", device=0, return_count=False, nbeg=None, nend=None):
     """rho_i = sum_j m_j W(r_ij, h_ij), self included (LIBSX:17-51)."""
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     r0, r1 = _rows(n, nbeg, nend)
-    dp, dm, dh = dev(pos), dev(_f64(m)), dev(_f64(h))
+    m, h = _vec(m, n, "m"), _vec(h, n, "h")
+    dp, dm, dh = dev(pos), dev(m), dev(h)
     rho = t.empty(r1 - r0, dtype=t.float64, device="cuda")
     cnt = t.empty(r1 - r0, dtype=t.int32, device="cuda") if return_count else None
     check(_lib.load().sphx_getDensity(_context(device), _mode(mode), n, ptr(dp), ptr(dm), ptr(dh),
@@ -96,7 +126,7 @@ def getPressure(rho, u, gama):
 def getPressure_csound(rho, u, gama, *, device=0):
     """P and the adiabatic sound speed c = sqrt(gama (gama - 1) u) (LIBV2:766-789)."""
     t = _t()
-    dr, du = dev(_f64(rho)), dev(_f64(u))
+    dr, du = _rho_u(rho, u)
     n = len(dr)
     P = t.empty(n, dtype=t.float64, device="cuda")
     c = t.empty(n, dtype=t.float64, device="cuda")
@@ -110,12 +140,16 @@ def getPressure_csound(rho, u, gama, *, device=0):
 # ---------------------------------------------------------------------------
 def _hydro(pos, v, rho, P, c, h, m, divV, curlV, p, mode, device, nbeg, nend, want_acc, want_du):
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     r0, r1 = _rows(n, nbeg, nend)
-    arrs = [dev(_f64(a)) for a in (pos, v, rho, P, c, h, m)]
-    dv = dev(_f64(divV)) if divV is not None else None
-    cv = dev(_f64(curlV)) if curlV is not None else None
+    host = [pos, _xyz(v, n, "v")] + [
+        _vec(a, n, name) for a, name in ((rho, "rho"), (P, "P"), (c, "c"), (h, "h"), (m, "m"))]
+    divV = _vec(divV, n, "divV") if divV is not None else None
+    curlV = _vec(curlV, n, "curlV") if curlV is not None else None
+    arrs = [dev(a) for a in host]
+    dv = dev(divV) if divV is not None else None
+    cv = dev(curlV) if curlV is not None else None
     acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda") if want_acc else None
     du = t.empty(r1 - r0, dtype=t.float64, device="cuda") if want_du else None
     check(_lib.load().sphx_getAcc_sph(_context(device), _mode(mode), n, *[ptr(a) for a in arrs],
@@ -188,10 +222,12 @@ def get_dU_shear_mpi(nbeg, nend, pos, v, rho, P, c, h, m, divV, curlV, alpha, **
 def div_curlVel(pos, v, rho, m, h, *, mode="mixed", device=0, nbeg=None, nend=None):
     """Signed div v and |curl v| (SHEAR:8-69)."""
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     r0, r1 = _rows(n, nbeg, nend)
-    arrs = [dev(_f64(a)) for a in (pos, v, rho, m, h)]
+    host = [pos, _xyz(v, n, "v")] + [
+        _vec(a, n, name) for a, name in ((rho, "rho"), (m, "m"), (h, "h"))]
+    arrs = [dev(a) for a in host]
     dv = t.empty(r1 - r0, dtype=t.float64, device="cuda")
     cv = t.empty(r1 - r0, dtype=t.float64, device="cuda")
     check(_lib.load().sphx_div_curlVel(_context(device), _mode(mode), n, *[ptr(a) for a in arrs],
@@ -211,11 +247,12 @@ def getAcc_g_smth(pos, mass, G, epsilon, *, mode="mixed", device=0, nbeg=None, n
                   return_potential=False):
     """Spline-softened gravity, eps_ij = (eps_i + eps_j)/2 (LIBSX:633-671)."""
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     r0, r1 = _rows(n, nbeg, nend)
     eps = np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (n,))
-    dp, dm, de = dev(pos), dev(_f64(mass)), dev(np.ascontiguousarray(eps))
+    mass = _vec(mass, n, "mass")
+    dp, dm, de = dev(pos), dev(mass), dev(np.ascontiguousarray(eps))
     acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda")
     pot = t.empty(r1 - r0, dtype=t.float64, device="cuda") if return_potential else None
     check(_lib.load().sphx_getAcc_g_smth(_context(device), _mode(mode), n, ptr(dp), ptr(dm),
@@ -249,11 +286,12 @@ def getAcc_g_bh(pos, mass, G, epsilon, theta=0.5, *, device=0, nbeg=None, nend=N
     """Barnes-Hut spline-softened gravity (Morton-built octree, monopoles,
     opening angle theta); the direct-sum counterpart is getAcc_g_smth."""
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     r0, r1 = _rows(n, nbeg, nend)
     eps = np.ascontiguousarray(np.broadcast_to(np.asarray(epsilon, dtype=np.float64), (n,)))
-    dp, dm, de = dev(pos), dev(_f64(mass)), dev(eps)
+    mass = _vec(mass, n, "mass")
+    dp, dm, de = dev(pos), dev(mass), dev(eps)
     acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda")
     pot = t.empty(r1 - r0, dtype=t.float64, device="cuda") if return_potential else None
     st = (C.c_int64 * 8)()
@@ -271,9 +309,14 @@ def getAcc_g_bh(pos, mass, G, epsilon, theta=0.5, *, device=0, nbeg=None, nend=N
 
 def _diag(pos, v, m, u=None, eps=None, G=1.0, device=0):
     d = _lib.Diag()
-    dv, dm = dev(_f64(v)), dev(_f64(m))
-    dp = dev(_f64(pos)) if pos is not None else None
-    du = dev(_f64(u)) if u is not None else None
+    m = _f64(m)
+    n = len(m)
+    v = _xyz(v, n, "v")
+    pos = _xyz(pos, n, "pos") if pos is not None else None
+    u = _vec(u, n, "u") if u is not None else None
+    dv, dm = dev(v), dev(m)
+    dp = dev(pos) if pos is not None else None
+    du = dev(u) if u is not None else None
     de = dev(np.ascontiguousarray(np.broadcast_to(np.asarray(eps, np.float64), (len(dm),)))) \
         if eps is not None else None
     check(_lib.load().sphx_diagnostics(_context(device), len(dm), ptr(dp), ptr(dv), ptr(dm),
@@ -299,9 +342,12 @@ def global_dt(h, csnd, acc_tot, acc_g, divV, C_CFL=0.25, dt_min=1e-5, dt_max=1e-
     and the host reduction after it (hfv_sph_gpu_v2_evrard.cu:517-561), on the
     device.  Returns (dt, (dt_f, dt_kin, dt_cfl, dt_dens))."""
     t = _t()
-    dh = dev(_f64(h))
-    n = len(dh)
-    bufs = [dev(_f64(a)) for a in (csnd, acc_tot, acc_g, divV)]
+    h = _f64(h)
+    n = len(h)
+    host = [_vec(csnd, n, "csnd"), _xyz(acc_tot, n, "acc_tot"), _xyz(acc_g, n, "acc_g"),
+            _vec(divV, n, "divV")]
+    dh = dev(h)
+    bufs = [dev(a) for a in host]
     out = t.empty(5, dtype=t.float64, device="cuda")
     check(_lib.load().sphx_global_dt(_context(device), n, ptr(dh), *[ptr(b) for b in bufs],
                                      float(C_CFL), float(dt_min), float(dt_max), ptr(out),
@@ -333,13 +379,19 @@ def diagnostics(pos, v, m, u=None, epsilon=None, G=1.0, *, device=0):
 # ---------------------------------------------------------------------------
 def _smoothing(pos, h, k_prev, divV, dt, nbeg, nend, mode, device, **pkw):
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     r0, r1 = _rows(n, nbeg, nend)
     p = _lib.default_params(**pkw)
-    dp, dh = dev(pos), dev(_f64(h))
-    dk = dev(np.ascontiguousarray(k_prev, dtype=np.int32)) if k_prev is not None else None
-    ddv = dev(_f64(divV)) if divV is not None else None
+    h = _vec(h, n, "h")
+    if k_prev is not None:
+        k_prev = np.ascontiguousarray(k_prev, dtype=np.int32)
+        if k_prev.shape != (n,):
+            raise ValueError(f"Nngb_previous must have shape ({n},), got {k_prev.shape}")
+    divV = _vec(divV, n, "divV") if divV is not None else None
+    dp, dh = dev(pos), dev(h)
+    dk = dev(k_prev) if k_prev is not None else None
+    ddv = dev(divV) if divV is not None else None
     hout = t.empty(r1 - r0, dtype=t.float64, device="cuda")
     kout = t.empty(r1 - r0, dtype=t.int32, device="cuda")
     it = t.empty(r1 - r0, dtype=t.int32, device="cuda")
@@ -378,7 +430,7 @@ def do_smoothingX(poz, *, kth=64, device=0):
     subpos must be pos itself or a block of its rows (every call in the
     reference passes (pos, pos))."""
     pos_in, sub = poz
-    pos = _f64(pos_in)
+    pos = _xyz(pos_in)
     sub = np.asarray(sub)
     n = len(pos)
     if sub is pos_in or (sub.shape == pos.shape and np.array_equal(sub, pos)):
@@ -389,6 +441,8 @@ def do_smoothingX(poz, *, kth=64, device=0):
             raise ValueError("do_smoothingX: subpos must be pos or a contiguous block of its rows")
         r0, r1 = off, off + len(sub)
     t = _t()
+    if r1 == r0:
+        return np.zeros(0)
     lo, hi = pos.min(0), pos.max(0)
     vol = float(np.prod(np.maximum(hi - lo, 1e-300)))
     h0 = 0.5 * (vol * (kth + 1) / (n * 4.18879020478639)) ** (1.0 / 3.0)
@@ -437,10 +491,11 @@ def getAcc_g_smthx(pos, mass, G, epsilon, **kw):
 def neighbors(pos, h, *, device=0, nbeg=None, nend=None):
     """Neighbour lists under r_ij <= h_i + h_j (CSR offsets, sorted indices per row)."""
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     r0, r1 = _rows(n, nbeg, nend)
-    dp, dh = dev(pos), dev(_f64(h))
+    h = _vec(h, n, "h")
+    dp, dh = dev(pos), dev(h)
     off = t.zeros(r1 - r0 + 1, dtype=t.int64, device="cuda")
     total = C.c_int64()
     L = _lib.load()
@@ -460,7 +515,7 @@ def morton_sort(pos, *, device=0):
     """63-bit Morton keys of pos over its bounding box and the stable sort
     permutation (keys_sorted, perm)."""
     t = _t()
-    pos = _f64(pos)
+    pos = _xyz(pos)
     n = len(pos)
     dp = dev(pos)
     keys = t.empty(n, dtype=t.int64, device="cuda")
@@ -495,7 +550,7 @@ class CoolingTable:
 def DoCooling(rho, u, dt, table: CoolingTable, *, device=0, return_iters=False):
     """Implicit per-particle cooling update in code units (PHOTOPY:489-543 structure)."""
     t = _t()
-    dr, du = dev(_f64(rho)), dev(_f64(u))
+    dr, du = _rho_u(rho, u)
     n = len(dr)
     out = t.empty(n, dtype=t.float64, device="cuda")
     it = t.empty(n, dtype=t.int32, device="cuda")
@@ -565,7 +620,7 @@ class CoolingGrid:
 def applyCooling(rho, u, dt, grid: CoolingGrid, *, device=0):
     """applyCooling (myCppSPHLibs_v3.h:794-859) in code units: returns the cooled u."""
     t = _t()
-    dr, du = dev(_f64(rho)), dev(_f64(u))
+    dr, du = _rho_u(rho, u)
     out = t.empty(len(dr), dtype=t.float64, device="cuda")
     g = grid.c()
     check(_lib.load().sphx_cool_grid_apply(_context(device), len(dr), ptr(dr), ptr(du), float(dt),
@@ -666,8 +721,10 @@ def hcoolingx(pos, rho, u, dt, table: ChimesTable, *, device=0):
     returns (u after the lookup, species fractions (n, n_species) float32, mu at
     the chosen node)."""
     t = _t()
-    dp, dr, du = dev(_f64(pos).reshape(-1, 3)), dev(_f64(rho)), dev(_f64(u))
+    pos = _xyz(pos, len(np.asarray(rho)), "pos")
+    dr, du = _rho_u(rho, u)
     n = len(du)
+    dp = dev(pos)
     out = t.empty(n, dtype=t.float64, device="cuda")
     mu = t.empty(n, dtype=t.float64, device="cuda")
     ion = t.empty((n, max(table.n_species, 1)), dtype=t.float32, device="cuda")

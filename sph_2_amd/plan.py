@@ -69,6 +69,7 @@ class SphPlan:
 
     def load(self, pos, vel, u, m, h, eps=None):
         t = _lib.torch()
+        _lib.check_run_arrays(pos, vel, u, m, h, eps)
         if self.mode == _lib.MIXED and not getattr(self, "_gcfg_set", False):
             from .autotune import choose_gravity_config
             self.set_gravity_config(*choose_gravity_config(len(m)))

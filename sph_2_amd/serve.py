@@ -96,6 +96,9 @@ class Batch:
     def admit(self, ic_: ic.IC, dt: float, cooling=False) -> int | None:
         """Admit one run; None when the budget or the run slots are full."""
         t = _lib.torch()
+        _lib.check_run_arrays(ic_.pos, ic_.vel, ic_.u, ic_.m, ic_.h, ic_.eps)
+        if ic_.n != len(ic_.m):
+            raise ValueError(f"the run declares n = {ic_.n} but holds {len(ic_.m)} particles")
         bufs = [dev(np.asarray(a, np.float64), t.float64) for a in
                 (ic_.pos, ic_.vel, ic_.u, ic_.m, ic_.h)]
         de = dev(np.asarray(ic_.eps, np.float64), t.float64) if ic_.eps is not None else None

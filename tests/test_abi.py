@@ -43,6 +43,59 @@ def test_default_params_and_errors_without_gpu():
     assert "output" in lib.sphx_last_error().decode() or "mode" in lib.sphx_last_error().decode()
 
 
+def test_null_context_and_bad_rows_are_rejected_without_gpu():
+    """Every stateless entry point checks its context first, and the row
+    range before any device work: both come back as SPHX_ERR_ARG with a
+    message, on a machine with no GPU at all."""
+    lib = _lib.load()
+    assert lib.sphx_getDensity(None, 0, 10, None, None, None, 0, 10, None, None, None) == -1
+    assert "context" in lib.sphx_last_error().decode()
+    assert lib.sphx_getAcc_g_smth(None, 0, 10, None, None, None, 1.0, 0, 10, None, None,
+                                  None) == -1
+    assert lib.sphx_pair_issue_rate(None, 16, None, None) == -1
+    assert "context" in lib.sphx_last_error().decode()
+
+
+def test_wrappers_refuse_arrays_of_the_wrong_shape():
+    """The kernels read n rows of every array; the Python layer refuses an
+    array of another length before any of it reaches the device (where it
+    would be read past its end), as numpy indexing would have in the
+    reference's functions."""
+    import pytest
+
+    from sph_2_amd import libsx
+
+    e = ic.evrard(64, seed=0)
+    g = 5.0 / 3.0
+    bad = [
+        lambda: libsx.getDensity(e.pos, e.m[:-1], e.h),
+        lambda: libsx.getDensity(e.pos[:, :2], e.m, e.h),
+        lambda: libsx.getAcc_sph(e.pos, e.vel[:-1], e.h, e.h, e.h, e.h, e.m, g, 0.1, 1.0, 2.0),
+        lambda: libsx.get_dU(e.pos, e.vel, e.h, e.h, e.h, e.h, e.m[:9], g, 0.1, 1.0, 2.0),
+        lambda: libsx.getAcc_sph_shear(e.pos, e.vel, e.h, e.h, e.h, e.h, e.m, e.h[:3], e.h, 1.0),
+        lambda: libsx.div_curlVel(e.pos, e.vel, e.h[:10], e.m, e.h),
+        lambda: libsx.getAcc_g_smth(e.pos, e.m[:3], 1.0, e.h),
+        lambda: libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h[:3]),
+        lambda: libsx.getAcc_g_bh(e.pos, e.m[:3], 1.0, e.h),
+        lambda: libsx.getKE(e.vel, e.m[:5]),
+        lambda: libsx.getPE(e.pos[:7], e.m, 1.0, e.h),
+        lambda: libsx.global_dt(e.h, e.h, e.vel, e.vel[:4], e.h),
+        lambda: libsx.smoothing_h(e.pos, e.h[:5]),
+        lambda: libsx.smoothing_fast_new_mpi3(0, 64, e.pos, e.h, np.zeros(3), e.h, 1e-3),
+        lambda: libsx.neighbors(e.pos, e.h[:5]),
+        lambda: libsx.getPressure_csound(e.h, e.h[:3], g),
+        lambda: libsx.getDensity_mpi(10, 70, e.pos, e.m, e.h),
+    ]
+    for k, f in enumerate(bad):
+        with pytest.raises(ValueError):
+            f()
+    with pytest.raises(ValueError, match="vel"):
+        _lib.check_run_arrays(e.pos, e.vel[:-1], e.u, e.m, e.h)
+    with pytest.raises(ValueError, match="eps"):
+        _lib.check_run_arrays(e.pos, e.vel, e.u, e.m, e.h, np.ones(3))
+    assert _lib.check_run_arrays(e.pos, e.vel, e.u, e.m, e.h, np.ones(64)) == 64
+
+
 def test_product_never_imports_the_oracle():
     pkg = os.path.join(ROOT, "sph_2_amd")
     for dirpath, _, files in os.walk(pkg):
