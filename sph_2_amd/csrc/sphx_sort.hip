@@ -120,26 +120,55 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint64_t *__restri
     hist[(int64_t)tid * nblocks + blockIdx.x] = h[tid];
 }
 
-// Exclusive scan of `total` uint32 entries with one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t *__restrict__ hist, int64_t total) {
-    __shared__ uint32_t part[1024];
-    int tid = threadIdx.x;
-    int64_t per = (total + 1023) / 1024;
-    int64_t b = tid * per, e = min(total, b + per);
+// The scatter offsets are the exclusive scan of the block histograms in
+// digit-major order (entry d * nblocks + b counts digit d in block b).  Two
+// kernels with one workgroup per digit: the first sums each digit's row, the
+// second scans its row on top of the totals of the digits before it.
+__global__ __launch_bounds__(256) void k_rs_digit_total(const uint32_t *__restrict__ hist,
+                                                        int nblocks, uint32_t *__restrict__ tot) {
+    __shared__ uint32_t red[256];
+    const int d = blockIdx.x, tid = threadIdx.x;
     uint32_t s = 0;
-    for (int64_t i = b; i < e; ++i) s += hist[i];
+    for (int b = tid; b < nblocks; b += 256) s += hist[(int64_t)d * nblocks + b];
+    red[tid] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) tot[d] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_rs_scan_rows(uint32_t *__restrict__ hist, int nblocks,
+                                                      const uint32_t *__restrict__ tot) {
+    __shared__ uint32_t part[256];
+    const int d = blockIdx.x, tid = threadIdx.x;
+    part[tid] = tid < d ? tot[tid] : 0u;  // the digits before this one
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) part[tid] += part[tid + o];
+        __syncthreads();
+    }
+    const uint32_t base = part[0];
+    __syncthreads();
+    // each thread a consecutive piece of the row
+    const int per = (nblocks + 255) / 256;
+    const int b0 = min(nblocks, tid * per), b1 = min(nblocks, b0 + per);
+    uint32_t *row = hist + (int64_t)d * nblocks;
+    uint32_t s = 0;
+    for (int b = b0; b < b1; ++b) s += row[b];
     part[tid] = s;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
+    for (int off = 1; off < 256; off <<= 1) {
         uint32_t v = tid >= off ? part[tid - off] : 0u;
         __syncthreads();
         part[tid] += v;
         __syncthreads();
     }
-    uint32_t run = tid > 0 ? part[tid - 1] : 0u;
-    for (int64_t i = b; i < e; ++i) {
-        uint32_t v = hist[i];
-        hist[i] = run;
+    uint32_t run = base + (tid > 0 ? part[tid - 1] : 0u);
+    for (int b = b0; b < b1; ++b) {
+        uint32_t v = row[b];
+        row[b] = run;
         run += v;
     }
 }
@@ -285,17 +314,21 @@ int sphx_soa_to_aos(const double *x, const double *y, const double *z, int64_t n
     return SPHX_OK;
 }
 
+// hist holds 256 x (nblocks + 1) entries: the block histograms, then the
+// digit totals.
 int sphx_radix_sort(uint64_t *keys, int32_t *vals, uint64_t *keys_tmp, int32_t *vals_tmp,
                     uint32_t *hist, int64_t n, int key_bits, hipStream_t st) {
     if (n <= 1) return SPHX_OK;
     int nblocks = (int)((n + RS_TILE - 1) / RS_TILE);
+    uint32_t *tot = hist + (int64_t)256 * nblocks;
     int passes = (key_bits + 7) / 8;
     uint64_t *ka = keys, *kb = keys_tmp;
     int32_t *va = vals, *vb = vals_tmp;
     for (int p = 0; p < passes; ++p) {
         int shift = 8 * p;
         k_rs_hist<<<nblocks, RS_THREADS, 0, st>>>(ka, n, shift, hist, nblocks);
-        k_rs_scan<<<1, 1024, 0, st>>>(hist, (int64_t)256 * nblocks);
+        k_rs_digit_total<<<256, 256, 0, st>>>(hist, nblocks, tot);
+        k_rs_scan_rows<<<256, 256, 0, st>>>(hist, nblocks, tot);
         k_rs_scatter<<<nblocks, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, hist, nblocks);
         SPHX_HIP(hipGetLastError());
         uint64_t *tk = ka; ka = kb; kb = tk;
