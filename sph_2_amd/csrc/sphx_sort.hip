@@ -238,41 +238,50 @@ __global__ void k_gather(GatherList g, const int32_t *__restrict__ perm, int64_t
     for (int a = 0; a < g.ni; ++a) g.idst[a][i] = g.isrc[a][p];
 }
 
-// Bucket boxes (+ max h, max eps) and super boxes (+ max h); one wavefront
-// per super-bucket walking its buckets.
-__global__ void k_boxes(Tables t, const double *__restrict__ x, const double *__restrict__ y,
-                        const double *__restrict__ z, const double *__restrict__ h,
-                        const double *__restrict__ eps, int with_h) {
-    int sidx = blockIdx.x;
-    int lane = threadIdx.x;
-    double S[7] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
-    for (int b = t.s_bbeg[sidx]; b < t.s_bend[sidx]; ++b) {
-        int cnt = t.b_cnt[b];
-        int i = t.b_beg[b] + lane;
-        bool ok = lane < cnt;
-        double a = ok ? x[i] : INFINITY, bb = ok ? y[i] : INFINITY, c = ok ? z[i] : INFINITY;
-        double xl = wave_min_d(a), yl = wave_min_d(bb), zl = wave_min_d(c);
-        a = ok ? x[i] : -INFINITY; bb = ok ? y[i] : -INFINITY; c = ok ? z[i] : -INFINITY;
-        double xh = wave_max_d(a), yh = wave_max_d(bb), zh = wave_max_d(c);
-        double hm = 0.0, em = 0.0;
-        if (with_h) {
-            hm = wave_max_d(ok ? h[i] : 0.0);
-            em = wave_max_d(ok ? (eps ? eps[i] : h[i]) : 0.0);
-        }
-        if (lane == 0) {
-            t.bxl[b] = xl; t.bxh[b] = xh; t.byl[b] = yl; t.byh[b] = yh;
-            t.bzl[b] = zl; t.bzh[b] = zh;
-            if (with_h) { t.bhmax[b] = hm; t.bemax[b] = em; }
-        }
-        S[0] = fmin(S[0], xl); S[1] = fmax(S[1], xh);
-        S[2] = fmin(S[2], yl); S[3] = fmax(S[3], yh);
-        S[4] = fmin(S[4], zl); S[5] = fmax(S[5], zh);
-        S[6] = fmax(S[6], hm);
+// Box and largest h and softening of each 64-particle bucket: one wave each.
+__global__ __launch_bounds__(64) void k_bucket_boxes(Tables t, const double *__restrict__ x,
+                                                     const double *__restrict__ y,
+                                                     const double *__restrict__ z,
+                                                     const double *__restrict__ h,
+                                                     const double *__restrict__ eps, int with_h) {
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int cnt = t.b_cnt[b];
+    const int i = t.b_beg[b] + lane;
+    const bool ok = lane < cnt;
+    double a = ok ? x[i] : INFINITY, bb = ok ? y[i] : INFINITY, c = ok ? z[i] : INFINITY;
+    const double xl = wave_min_d(a), yl = wave_min_d(bb), zl = wave_min_d(c);
+    a = ok ? x[i] : -INFINITY; bb = ok ? y[i] : -INFINITY; c = ok ? z[i] : -INFINITY;
+    const double xh = wave_max_d(a), yh = wave_max_d(bb), zh = wave_max_d(c);
+    double hm = 0.0, em = 0.0;
+    if (with_h) {
+        hm = wave_max_d(ok ? h[i] : 0.0);
+        em = wave_max_d(ok ? (eps ? eps[i] : h[i]) : 0.0);
     }
     if (lane == 0) {
-        t.sxl[sidx] = S[0]; t.sxh[sidx] = S[1]; t.syl[sidx] = S[2]; t.syh[sidx] = S[3];
-        t.szl[sidx] = S[4]; t.szh[sidx] = S[5];
-        if (with_h) t.shmax[sidx] = S[6];
+        t.bxl[b] = xl; t.bxh[b] = xh; t.byl[b] = yl; t.byh[b] = yh;
+        t.bzl[b] = zl; t.bzh[b] = zh;
+        if (with_h) { t.bhmax[b] = hm; t.bemax[b] = em; }
+    }
+}
+
+// Box and largest h of each super (up to SPHX_SUPER consecutive buckets) from
+// its buckets' boxes: lane k takes buckets k, k + 64, ... of the super.
+__global__ __launch_bounds__(64) void k_super_boxes(Tables t, int with_h) {
+    const int sidx = blockIdx.x, lane = threadIdx.x;
+    double S[7] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY, 0.0};
+    for (int b = t.s_bbeg[sidx] + lane; b < t.s_bend[sidx]; b += 64) {
+        S[0] = fmin(S[0], t.bxl[b]); S[1] = fmax(S[1], t.bxh[b]);
+        S[2] = fmin(S[2], t.byl[b]); S[3] = fmax(S[3], t.byh[b]);
+        S[4] = fmin(S[4], t.bzl[b]); S[5] = fmax(S[5], t.bzh[b]);
+        if (with_h) S[6] = fmax(S[6], t.bhmax[b]);
+    }
+    const double xl = wave_min_d(S[0]), xh = wave_max_d(S[1]), yl = wave_min_d(S[2]),
+                 yh = wave_max_d(S[3]), zl = wave_min_d(S[4]), zh = wave_max_d(S[5]);
+    const double hm = with_h ? wave_max_d(S[6]) : 0.0;
+    if (lane == 0) {
+        t.sxl[sidx] = xl; t.sxh[sidx] = xh; t.syl[sidx] = yl; t.syh[sidx] = yh;
+        t.szl[sidx] = zl; t.szh[sidx] = zh;
+        if (with_h) t.shmax[sidx] = hm;
     }
 }
 
@@ -423,7 +432,8 @@ int Engine::sort_and_permute(hipStream_t st, bool gather_state) {
 int Engine::build_boxes(hipStream_t st, bool with_h) {
     if (n == 0) return SPHX_OK;
     const double *e = p.eps_from_h ? nullptr : s.eps;
-    k_boxes<<<t.ns, 64, 0, st>>>(t, s.x, s.y, s.z, s.h, e, with_h ? 1 : 0);
+    k_bucket_boxes<<<t.nb, 64, 0, st>>>(t, s.x, s.y, s.z, s.h, e, with_h ? 1 : 0);
+    k_super_boxes<<<t.ns, 64, 0, st>>>(t, with_h ? 1 : 0);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
