@@ -28,29 +28,34 @@ __device__ __forceinline__ uint64_t spread3(uint64_t v) {
 
 // Bounding box of each super-bucket straight from the particles (before the
 // sort, in the current order; supers never straddle runs).
-__global__ void k_super_box_pos(Tables t, const double *__restrict__ x,
-                                const double *__restrict__ y, const double *__restrict__ z,
-                                double *__restrict__ out /* 6 per super */) {
-    int sidx = blockIdx.x;
-    int lane = threadIdx.x;
-    int b0 = t.s_bbeg[sidx], b1 = t.s_bend[sidx];
-    int p0 = t.b_beg[b0];
-    int p1 = t.b_beg[b1 - 1] + t.b_cnt[b1 - 1];
-    double xl = INFINITY, xh = -INFINITY, yl = INFINITY, yh = -INFINITY, zl = INFINITY,
-           zh = -INFINITY;
-    for (int i = p0 + lane; i < p1; i += 64) {
-        double a = x[i], b = y[i], c = z[i];
-        xl = fmin(xl, a); xh = fmax(xh, a);
-        yl = fmin(yl, b); yh = fmax(yh, b);
-        zl = fmin(zl, c); zh = fmax(zh, c);
+__global__ __launch_bounds__(256) void k_super_box_pos(Tables t, const double *__restrict__ x,
+                                                       const double *__restrict__ y,
+                                                       const double *__restrict__ z,
+                                                       double *__restrict__ out /* 6 per super */) {
+    __shared__ double part[4][6];
+    const int sidx = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int b0 = t.s_bbeg[sidx], b1 = t.s_bend[sidx];
+    const int p0 = t.b_beg[b0];
+    const int p1 = t.b_beg[b1 - 1] + t.b_cnt[b1 - 1];
+    double v[6] = {INFINITY, -INFINITY, INFINITY, -INFINITY, INFINITY, -INFINITY};
+    for (int i = p0 + tid; i < p1; i += 256) {
+        const double a = x[i], b = y[i], c = z[i];
+        v[0] = fmin(v[0], a); v[1] = fmax(v[1], a);
+        v[2] = fmin(v[2], b); v[3] = fmax(v[3], b);
+        v[4] = fmin(v[4], c); v[5] = fmax(v[5], c);
     }
-    xl = wave_min_d(xl); xh = wave_max_d(xh);
-    yl = wave_min_d(yl); yh = wave_max_d(yh);
-    zl = wave_min_d(zl); zh = wave_max_d(zh);
-    if (lane == 0) {
-        out[6 * sidx + 0] = xl; out[6 * sidx + 1] = xh;
-        out[6 * sidx + 2] = yl; out[6 * sidx + 3] = yh;
-        out[6 * sidx + 4] = zl; out[6 * sidx + 5] = zh;
+    for (int k = 0; k < 6; k += 2) {
+        v[k] = wave_min_d(v[k]);
+        v[k + 1] = wave_max_d(v[k + 1]);
+    }
+    if (lane == 0)
+        for (int k = 0; k < 6; ++k) part[w][k] = v[k];
+    __syncthreads();
+    if (tid < 6) {
+        double r = part[0][tid];
+        for (int q = 1; q < 4; ++q) r = (tid & 1) ? fmax(r, part[q][tid]) : fmin(r, part[q][tid]);
+        out[6 * sidx + tid] = r;
     }
 }
 
@@ -370,7 +375,7 @@ int sphx_launch_morton(Engine &E, hipStream_t st) {
     Tables &t = E.t;
     double *sbox = E.bbox;                 // 6 per super
     double *rbox = E.bbox + 6 * (int64_t)t.ns;  // 6 per run
-    k_super_box_pos<<<t.ns, 64, 0, st>>>(t, E.s.x, E.s.y, E.s.z, sbox);
+    k_super_box_pos<<<t.ns, 256, 0, st>>>(t, E.s.x, E.s.y, E.s.z, sbox);
     k_run_box<<<t.R, 64, 0, st>>>(t, sbox, rbox);
     k_morton<<<grid_for(E.n, 256), 256, 0, st>>>(E.s.x, E.s.y, E.s.z,
                                                 E.max_runs > 1 ? E.s.run : nullptr, rbox, E.n,
