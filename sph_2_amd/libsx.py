@@ -14,6 +14,7 @@ printed (iteration caps), these functions warn.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 import warnings
 
 import numpy as np
@@ -23,15 +24,20 @@ from ._lib import check, dev, ptr
 
 _MODES = {"mixed": _lib.MIXED, "fp64": _lib.FP64}
 _ctx = {}
+_ctx_lock = threading.Lock()
 
 
 def _context(device: int = 0):
-    c = _ctx.get(device)
-    if c is None:
-        _lib.require_gpu()
-        h = C.c_void_p()
-        check(_lib.load().sphx_ctx_create(device, C.byref(h)), "sphx_ctx_create")
-        c = _ctx[device] = h
+    """The per-device context of the stateless calls (created once; calls
+    that share it must not overlap, as the library's contexts are not
+    re-entrant)."""
+    with _ctx_lock:
+        c = _ctx.get(device)
+        if c is None:
+            _lib.require_gpu()
+            h = C.c_void_p()
+            check(_lib.load().sphx_ctx_create(device, C.byref(h)), "sphx_ctx_create")
+            c = _ctx[device] = h
     return c
 
 
