@@ -150,7 +150,7 @@ def bench_evrard(a) -> int:
     flops = 20.0 * n * (n - 1)
     achieved = flops / kern_s / 1e12 if kern_s > 0 else None
     kname = "k_grav_sym" if cfg_used[0] == 8 else "k_grav"
-    what = ("+ k_grav_sym_near: symmetric direct sum, each pair once for both particles"
+    what = ("symmetric direct sum, near and far block pairs, each pair once for both particles"
             if kname == "k_grav_sym" else "tiled direct sum")
     roofline = {"bound": "valu", "kernel": f"{kname} ({what}, FP32)",
                 "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

@@ -403,7 +403,7 @@ __device__ __forceinline__ float sym_spline_f(float r2, float e) {
     return ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
 }
 
-// The i particles a lane holds: P of them (8 in the far pass, 4 in the near
+// The i particles a lane holds: P of them (8 on the far path, 4 on the near
 // one), with their FP32 force sums of the current j chunk.
 template <int P>
 struct SymIP {
@@ -503,18 +503,15 @@ __device__ __forceinline__ int sym_mode(const double *bI, const double *bJ, bool
     return far ? ((bI[7] >= 0.0 && bJ[7] >= 0.0) ? 0 : 1) : 2;
 }
 
-// One segment of row I, restricted to its far block pairs.  Far and near
-// block pairs run as separate kernels so that the far pass, nearly all of the
-// work, keeps the register budget of its own code (three waves per SIMD
-// without spills in the pair loops).  The far pass writes the segment's
-// i-side sums; the near pass, launched after it on the same stream, adds its
-// own to them.  Every j-side slot belongs to exactly one block pair and so to
-// exactly one pass.
+// One segment of row I, restricted to its far block pairs: the i block held
+// as 8 per lane.  Its i-side sums are stored in the segment's slots, or added
+// to what the segment's near pairs stored there before (add).  Every j-side
+// slot belongs to exactly one block pair, near or far.
 __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *__restrict__ x,
                                                 const double *__restrict__ y,
                                                 const double *__restrict__ z,
                                                 const double *__restrict__ m,
-                                                const double *__restrict__ eps) {
+                                                const double *__restrict__ eps, bool add) {
     const int lane = threadIdx.x;
     const int sg = blockIdx.x;
     const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
@@ -580,21 +577,28 @@ __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *
         int li = lane + 64 * p;
         if (li < cnt) {
             int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
-            T.qx[o] = kx[p].value();
-            T.qy[o] = ky[p].value();
-            T.qz[o] = kz[p].value();
+            if (add) {
+                T.qx[o] += kx[p].value();
+                T.qy[o] += ky[p].value();
+                T.qz[o] += kz[p].value();
+            } else {
+                T.qx[o] = kx[p].value();
+                T.qy[o] = ky[p].value();
+                T.qz[o] = kz[p].value();
+            }
         }
     }
 }
 
 // The same segment restricted to its near and diagonal block pairs.  The
 // block's 512 i are taken as two halves of 256 (4 per lane), one after the
-// other: half the i-side registers of the far pass, so that more waves stay
-// resident on this latency-bound path, and each half tests the 128-particle j
-// chunks against its own box, so that more of them take the r^-3 path.  The j
-// sets rotate once per half; the first half stores their j-side sums and the
-// second adds to them (lane L owns the same j slots in both).
-__device__ __forceinline__ void sym_segment_near(const SymTabs &T, const double *__restrict__ x,
+// other: the registers of this path then stay within the far path's budget,
+// and each half tests the 128-particle j chunks against its own box, so that
+// more of them take the r^-3 path.  The j sets rotate once per half; the
+// first half stores their j-side sums and the second adds to them (lane L
+// owns the same j slots in both).  Stores the segment's i-side sums and
+// returns true, or returns false at once when the segment has no near pair.
+__device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double *__restrict__ x,
                                                  const double *__restrict__ y,
                                                  const double *__restrict__ z,
                                                  const double *__restrict__ m,
@@ -609,7 +613,7 @@ __device__ __forceinline__ void sym_segment_near(const SymTabs &T, const double 
     bool any = false;
     for (int J = J0; J < J1 && !any; ++J)
         any = (actI || T.act[J]) && sym_mode(bI, T.bbox + 8 * (int64_t)J, J == I) >= 2;
-    if (!any) return;
+    if (!any) return false;
     const double ox = 0.5 * (bI[0] + bI[1]), oy = 0.5 * (bI[2] + bI[3]), oz = 0.5 * (bI[4] + bI[5]);
     const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
     const int64_t roff = T.row_off[I];
@@ -709,28 +713,26 @@ __device__ __forceinline__ void sym_segment_near(const SymTabs &T, const double 
             int li = lane + 64 * (4 * h + p);
             if (li < cnt) {
                 int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
-                T.qx[o] += kx[p].value();
-                T.qy[o] += ky[p].value();
-                T.qz[o] += kz[p].value();
+                T.qx[o] = kx[p].value();
+                T.qy[o] = ky[p].value();
+                T.qz[o] = kz[p].value();
             }
         }
     }
+    return true;
 }
 
-// The far pass at three waves per SIMD (its registers fit without spills).
+// One wave per segment: its near block pairs, then its far ones (three waves
+// per SIMD: both paths fit 168 VGPRs without spills).  Near work, a few
+// percent of the pairs at 1M particles but a third at 65K, is spread over the
+// same launch as the far work instead of trailing it.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void k_grav_sym(
     SymTabs T, const double *__restrict__ x, const double *__restrict__ y,
     const double *__restrict__ z, const double *__restrict__ m, const double *__restrict__ eps) {
-    sym_segment_far(T, x, y, z, m, eps);
+    const bool near = sym_segment_near(T, x, y, z, m, eps);
+    sym_segment_far(T, x, y, z, m, eps, near);
 }
 
-__global__ __launch_bounds__(64) void k_grav_sym_near(SymTabs T, const double *__restrict__ x,
-                                                      const double *__restrict__ y,
-                                                      const double *__restrict__ z,
-                                                      const double *__restrict__ m,
-                                                      const double *__restrict__ eps) {
-    sym_segment_near(T, x, y, z, m, eps);
-}
 
 // a_i = G (sum over the segments of i's row of their i-side sums, in order,
 // + sum over the earlier rows of the run of their j-side sums for i, in order).
@@ -1004,8 +1006,6 @@ static int sym_run(Engine &E, hipStream_t st) {
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
     SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
     k_grav_sym<<<(unsigned)(s1 - s0), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
-    SPHX_HIP(hipGetLastError());
-    k_grav_sym_near<<<(unsigned)(s1 - s0), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
     SPHX_HIP(hipGetLastError());
     SPHX_CHECK(E.tmark(st, PH_GRAVK, 1));
     k_grav_sym_reduce<<<(unsigned)((p1 - p0 + 255) / 256), 256, 0, st>>>(T, E.p.G, p0, p1, E.s.gx,
