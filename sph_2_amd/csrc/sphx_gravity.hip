@@ -746,10 +746,14 @@ __global__ void k_grav_sym_reduce(SymTabs T, double G, int64_t p0, int64_t p1, d
     double sx = 0.0, sy = 0.0, sz = 0.0;
     int ns = T.row_nseg[I], sq = T.row_seg0[I];
     int64_t li = i - T.b_beg[I];
+    // unrolled so that the loads of several slots are in flight at once (the
+    // additions keep their order)
+#pragma unroll 8
     for (int k = 0; k < ns; ++k) {
         int64_t o = (int64_t)(sq + k) * SYM_B + li;
         sx += T.qx[o]; sy += T.qy[o]; sz += T.qz[o];
     }
+#pragma unroll 8
     for (int I2 = b0; I2 < I; ++I2) {
         int64_t o = T.row_off[I2] + (i - T.row_j0[I2]);
         sx += (double)T.px[o]; sy += (double)T.py[o]; sz += (double)T.pz[o];
