@@ -23,9 +23,11 @@
 namespace {
 
 constexpr int BH_LEAF = 8;        // particles per leaf at most
-constexpr int BH_SCAP = 3072;     // cell stack capacity per group
+constexpr int BH_SCAP = 1024;     // cell stack capacity per group
 constexpr int BH_SLACK = 160;     // > 7 x 21 levels of pure depth-first growth
-constexpr int BH_ICAP = 768;      // interaction list capacity per group
+constexpr int BH_ICAP = 512;      // interaction list capacity per group (>= 64 x BH_LEAF)
+static_assert(BH_ICAP >= 64 * BH_LEAF, "one round of leaf particles must fit the list");
+static_assert(BH_SCAP > BH_SLACK + 7 * 64, "a full batch must fit above the slack");
 constexpr int BHT = 256;
 
 // What the walk reads of a cell, packed so that one cell costs two 32-byte
@@ -332,7 +334,18 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
 
     auto flush = [&]() {
         __syncthreads();
-        for (int q = 0; q < icnt; ++q) bh_pair<POT>(xi, yi, zi, ei, il[q], ie[q], ax, ay, az, ph);
+        // entries four at a time, their LDS loads issued before the pairs are
+        // evaluated (in list order, as one at a time would)
+        int q = 0;
+        for (; q + 4 <= icnt; q += 4) {
+            const float4 j0 = il[q], j1 = il[q + 1], j2 = il[q + 2], j3 = il[q + 3];
+            const float e0 = ie[q], e1 = ie[q + 1], e2 = ie[q + 2], e3 = ie[q + 3];
+            bh_pair<POT>(xi, yi, zi, ei, j0, e0, ax, ay, az, ph);
+            bh_pair<POT>(xi, yi, zi, ei, j1, e1, ax, ay, az, ph);
+            bh_pair<POT>(xi, yi, zi, ei, j2, e2, ax, ay, az, ph);
+            bh_pair<POT>(xi, yi, zi, ei, j3, e3, ax, ay, az, ph);
+        }
+        for (; q < icnt; ++q) bh_pair<POT>(xi, yi, zi, ei, il[q], ie[q], ax, ay, az, ph);
         kx.add(ax); ky.add(ay); kz.add(az); kp.add(ph);
         ax = ay = az = ph = 0.f;
         ninter += icnt;
