@@ -334,16 +334,16 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
 
     auto flush = [&]() {
         __syncthreads();
-        // entries four at a time, their LDS loads issued before the pairs are
+        // entries eight at a time, their LDS loads issued before the pairs are
         // evaluated (in list order, as one at a time would)
         int q = 0;
-        for (; q + 4 <= icnt; q += 4) {
-            const float4 j0 = il[q], j1 = il[q + 1], j2 = il[q + 2], j3 = il[q + 3];
-            const float e0 = ie[q], e1 = ie[q + 1], e2 = ie[q + 2], e3 = ie[q + 3];
-            bh_pair<POT>(xi, yi, zi, ei, j0, e0, ax, ay, az, ph);
-            bh_pair<POT>(xi, yi, zi, ei, j1, e1, ax, ay, az, ph);
-            bh_pair<POT>(xi, yi, zi, ei, j2, e2, ax, ay, az, ph);
-            bh_pair<POT>(xi, yi, zi, ei, j3, e3, ax, ay, az, ph);
+        for (; q + 8 <= icnt; q += 8) {
+            float4 j[8];
+            float e[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) { j[k] = il[q + k]; e[k] = ie[q + k]; }
+#pragma unroll
+            for (int k = 0; k < 8; ++k) bh_pair<POT>(xi, yi, zi, ei, j[k], e[k], ax, ay, az, ph);
         }
         for (; q < icnt; ++q) bh_pair<POT>(xi, yi, zi, ei, il[q], ie[q], ax, ay, az, ph);
         kx.add(ax); ky.add(ay); kz.add(az); kp.add(ph);
