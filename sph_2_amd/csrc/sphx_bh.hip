@@ -82,15 +82,36 @@ __global__ void k_bh_split(BhTree T, int32_t f0, int32_t fc, const uint64_t *__r
 }
 
 // Exclusive scan of fc counts (one workgroup), total written to *total.
-__global__ __launch_bounds__(1024) void k_bh_scan(const int32_t *__restrict__ in,
-                                                  int32_t *__restrict__ out, int32_t fc,
-                                                  int32_t *__restrict__ total) {
-    __shared__ int32_t part[1024];
-    int tid = threadIdx.x;
-    int per = (fc + 1023) / 1024;
-    int b = tid * per, e = min(fc, b + per);
+// Exclusive scan of the fc child counts of a level, in three launches of
+// 1024-entry tiles: the tile sums, a scan of the tile sums (writing the
+// total), then each tile's scan on top of its offset.
+constexpr int BH_SCAN_TILE = 1024;
+
+__global__ __launch_bounds__(256) void k_bh_scan_tiles(const int32_t *__restrict__ in, int32_t fc,
+                                                       int32_t *__restrict__ tsum) {
+    __shared__ int32_t red[256];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int64_t b = (int64_t)t * BH_SCAN_TILE;
     int s = 0;
-    for (int i = b; i < e; ++i) s += in[i];
+    for (int k = tid; k < BH_SCAN_TILE; k += 256)
+        if (b + k < fc) s += in[b + k];
+    red[tid] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) tsum[t] = red[0];
+}
+
+__global__ __launch_bounds__(1024) void k_bh_scan_sums(int32_t *__restrict__ tsum, int ntiles,
+                                                       int32_t *__restrict__ total) {
+    __shared__ int32_t part[1024];
+    const int tid = threadIdx.x;
+    const int per = (ntiles + 1023) / 1024;
+    const int b = min(ntiles, tid * per), e = min(ntiles, b + per);
+    int s = 0;
+    for (int i = b; i < e; ++i) s += tsum[i];
     part[tid] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -101,10 +122,37 @@ __global__ __launch_bounds__(1024) void k_bh_scan(const int32_t *__restrict__ in
     }
     int run = tid > 0 ? part[tid - 1] : 0;
     for (int i = b; i < e; ++i) {
-        out[i] = run;
-        run += in[i];
+        int v = tsum[i];
+        tsum[i] = run;
+        run += v;
     }
     if (tid == 1023) *total = part[1023];
+}
+
+__global__ __launch_bounds__(256) void k_bh_scan_apply(const int32_t *__restrict__ in, int32_t fc,
+                                                       const int32_t *__restrict__ tsum,
+                                                       int32_t *__restrict__ out) {
+    __shared__ int32_t part[256];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const int64_t b = (int64_t)t * BH_SCAN_TILE + 4 * tid;  // four consecutive entries per thread
+    int v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = b + k < fc ? in[b + k] : 0;
+        s += v[k];
+    }
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int u = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += u;
+        __syncthreads();
+    }
+    int run = tsum[t] + (tid > 0 ? part[tid - 1] : 0);
+    for (int k = 0; k < 4; ++k) {
+        if (b + k < fc) out[b + k] = run;
+        run += v[k];
+    }
 }
 
 __global__ void k_bh_emit(BhTree T, int32_t f0, int32_t fc, const int32_t *__restrict__ off,
@@ -454,7 +502,7 @@ struct BhHost {
     int64_t cap = 0;
     int32_t *gbeg = nullptr, *gcnt = nullptr;
     int64_t gcap = 0;
-    int32_t *nk = nullptr, *off = nullptr, *bounds = nullptr, *total = nullptr;
+    int32_t *nk = nullptr, *off = nullptr, *bounds = nullptr, *total = nullptr, *tsum = nullptr;
     int64_t scratch_cap = 0;
     int64_t *stats = nullptr;
     std::vector<void *> owned;
@@ -534,11 +582,15 @@ int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t s
             SPHX_CHECK(H.alloc((void **)&H.nk, sizeof(int32_t) * c));
             SPHX_CHECK(H.alloc((void **)&H.off, sizeof(int32_t) * c));
             SPHX_CHECK(H.alloc((void **)&H.bounds, sizeof(int32_t) * 9 * c));
+            SPHX_CHECK(H.alloc((void **)&H.tsum, sizeof(int32_t) * ((c + BH_SCAN_TILE - 1) / BH_SCAN_TILE)));
             H.scratch_cap = c;
         }
         unsigned g = (unsigned)((fc + BHT - 1) / BHT);
         k_bh_split<<<g, BHT, 0, st>>>(H.T, f0, fc, E.keys, level, H.nk, H.bounds);
-        k_bh_scan<<<1, 1024, 0, st>>>(H.nk, H.off, fc, H.total);
+        const int ntiles = (int)((fc + BH_SCAN_TILE - 1) / BH_SCAN_TILE);
+        k_bh_scan_tiles<<<ntiles, 256, 0, st>>>(H.nk, fc, H.tsum);
+        k_bh_scan_sums<<<1, 1024, 0, st>>>(H.tsum, ntiles, H.total);
+        k_bh_scan_apply<<<ntiles, 256, 0, st>>>(H.nk, fc, H.tsum, H.off);
         int32_t tot = 0;
         SPHX_HIP(hipMemcpyAsync(&tot, H.total, 4, hipMemcpyDeviceToHost, st));
         SPHX_HIP(hipStreamSynchronize(st));
