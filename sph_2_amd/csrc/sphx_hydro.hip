@@ -196,7 +196,21 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
                         sz[k.slot] = zj;
                     }
                     __syncthreads();
-                    for (int q = 0; q < k.n; ++q) {
+                    // candidates four at a time, their LDS loads issued first
+                    int q = 0;
+                    for (; q + 4 <= k.n; q += 4) {
+                        double cx[4], cy[4], cz[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            cx[u] = sx[q + u]; cy[u] = sy[q + u]; cz[u] = sz[q + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            double r2 = sphx_r2(xi - cx[u], yi - cy[u], zi - cz[u]);
+                            kc += (mine && r2 <= s2) ? 1 : 0;
+                        }
+                    }
+                    for (; q < k.n; ++q) {
                         double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
                         kc += (mine && r2 <= s2) ? 1 : 0;
                     }
@@ -343,16 +357,34 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                 sm[k.slot] = (float)s.m[j0 + lane];
             }
             __syncthreads();
-            for (int q = 0; q < k.n; ++q) {
-                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
-                double r2 = sphx_r2(dx, dy, dz);
-                double tt = hi + sh[q];
-                if (mine && r2 <= sphx_sq(tt)) {
+            // candidates four at a time, their LDS loads issued before the tests
+            // (each pair is met in the same order as one at a time)
+            auto pair = [&](int q, double dx, double dy, double dz, double r2, double tt) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h3 = g.hinv * g.hinv * g.hinv;
                     acc.add(sm[q] * (SPHX_SIGMA_F * h3 * w_shape(g.q)));
                     ++nn;
+                };
+            int q0 = 0;
+            for (; q0 + 4 <= k.n; q0 += 4) {
+                double cx[4], cy[4], cz[4], ch[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cx[u] = sx[q0 + u]; cy[u] = sy[q0 + u]; cz[u] = sz[q0 + u]; ch[u] = sh[q0 + u];
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    double dx = xi - cx[u], dy = yi - cy[u], dz = zi - cz[u];
+                    double r2 = sphx_r2(dx, dy, dz);
+                    double tt = hi + ch[u];
+                    if (mine && r2 <= sphx_sq(tt)) pair(q0 + u, dx, dy, dz, r2, tt);
+                }
+            }
+            for (int q = q0; q < k.n; ++q) {
+                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
+                double r2 = sphx_r2(dx, dy, dz);
+                double tt = hi + sh[q];
+                if (mine && r2 <= sphx_sq(tt)) pair(q, dx, dy, dz, r2, tt);
             }
         });
     }
@@ -417,11 +449,9 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
                 sm[q] = (float)s.m[j];
             }
             __syncthreads();
-            for (int q = 0; q < k.n; ++q) {
-                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
-                double r2 = sphx_r2(dx, dy, dz);
-                double tt = hi + sh[q];
-                if (mine && r2 <= sphx_sq(tt)) {
+            // candidates four at a time, their LDS loads issued before the tests
+            // (each pair is met in the same order as one at a time)
+            auto pair = [&](int q, double dx, double dy, double dz, double r2, double tt) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h5 = g.hinv * g.hinv;
                     h5 = h5 * h5 * g.hinv;
@@ -432,7 +462,27 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
                     kx.add(F * (vy * g.dz - vz * g.dy));
                     ky.add(F * (vz * g.dx - vx * g.dz));
                     kz.add(F * (vx * g.dy - vy * g.dx));
+                };
+            int q0 = 0;
+            for (; q0 + 4 <= k.n; q0 += 4) {
+                double cx[4], cy[4], cz[4], ch[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cx[u] = sx[q0 + u]; cy[u] = sy[q0 + u]; cz[u] = sz[q0 + u]; ch[u] = sh[q0 + u];
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    double dx = xi - cx[u], dy = yi - cy[u], dz = zi - cz[u];
+                    double r2 = sphx_r2(dx, dy, dz);
+                    double tt = hi + ch[u];
+                    if (mine && r2 <= sphx_sq(tt)) pair(q0 + u, dx, dy, dz, r2, tt);
+                }
+            }
+            for (int q = q0; q < k.n; ++q) {
+                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
+                double r2 = sphx_r2(dx, dy, dz);
+                double tt = hi + sh[q];
+                if (mine && r2 <= sphx_sq(tt)) pair(q, dx, dy, dz, r2, tt);
             }
         });
     }
@@ -512,11 +562,9 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
                 sf[q] = a.balsara ? (float)balsara_f(s.divv[j], s.curlv[j], s.c[j], s.h[j]) : 1.0f;
             }
             __syncthreads();
-            for (int q = 0; q < k.n; ++q) {
-                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
-                double r2 = sphx_r2(dx, dy, dz);
-                double tt = hi + sh[q];
-                if (mine && r2 <= sphx_sq(tt)) {
+            // candidates four at a time, their LDS loads issued before the tests
+            // (each pair is met in the same order as one at a time)
+            auto pair = [&](int q, double dx, double dy, double dz, double r2, double tt) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h5 = g.hinv * g.hinv;
                     h5 = h5 * h5 * g.hinv;
@@ -543,7 +591,27 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
                     ky.add(-sc_ * g.dy);
                     kz.add(-sc_ * g.dz);
                     kdu.add(mF * (pri + 0.5f * pij) * vr);
+                };
+            int q0 = 0;
+            for (; q0 + 4 <= k.n; q0 += 4) {
+                double cx[4], cy[4], cz[4], ch[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    cx[u] = sx[q0 + u]; cy[u] = sy[q0 + u]; cz[u] = sz[q0 + u]; ch[u] = sh[q0 + u];
                 }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    double dx = xi - cx[u], dy = yi - cy[u], dz = zi - cz[u];
+                    double r2 = sphx_r2(dx, dy, dz);
+                    double tt = hi + ch[u];
+                    if (mine && r2 <= sphx_sq(tt)) pair(q0 + u, dx, dy, dz, r2, tt);
+                }
+            }
+            for (int q = q0; q < k.n; ++q) {
+                double dx = xi - sx[q], dy = yi - sy[q], dz = zi - sz[q];
+                double r2 = sphx_r2(dx, dy, dz);
+                double tt = hi + sh[q];
+                if (mine && r2 <= sphx_sq(tt)) pair(q, dx, dy, dz, r2, tt);
             }
         });
     }
