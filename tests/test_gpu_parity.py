@@ -241,11 +241,14 @@ def test_gravity_rows_and_fixed_eps(gpu, evr):
     assert rel_rms(acc64, acc_o) < 1e-12
 
 
-def test_morton_keys_and_sort(gpu):
+@pytest.mark.parametrize("n", [50_000, 1_500_001])
+def test_morton_keys_and_sort(gpu, n):
+    """Keys and the stable sort permutation against a numpy restatement; at
+    1,500,001 particles the radix sort's 367 tile histograms give its
+    per-digit scan two entries per thread and a partial last tile."""
     from sph_2_amd import libsx
 
     rng = np.random.default_rng(3)
-    n = 50_000
     pos = rng.random((n, 3))
     pos[::7] = pos[3]              # ties: equal keys must keep their order
     keys, perm = libsx.morton_sort(pos)
