@@ -693,7 +693,25 @@ __global__ __launch_bounds__(64) void k_knn(Tables t, Soa s, KnnArgs a) {
                         sz[k.slot] = zj;
                     }
                     __syncthreads();
-                    for (int q = 0; q < k.n; ++q) {
+                    // candidates four at a time, their LDS loads issued first (the
+                    // values are appended in the same order)
+                    int q = 0;
+                    for (; q + 4 <= k.n; q += 4) {
+                        double cx[4], cy[4], cz[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            cx[u] = sx[q + u]; cy[u] = sy[q + u]; cz[u] = sz[q + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            double v = sphx_r2(cx[u] - xi, cy[u] - yi, cz[u] - zi);
+                            if (act && v > lo2 && v <= probe) {
+                                if (cin < KNN_CAP) lst[cin * 64 + lane] = v;
+                                ++cin;
+                            }
+                        }
+                    }
+                    for (; q < k.n; ++q) {
                         double v = sphx_r2(sx[q] - xi, sy[q] - yi, sz[q] - zi);
                         if (act && v > lo2 && v <= probe) {
                             if (cin < KNN_CAP) lst[cin * 64 + lane] = v;
