@@ -913,7 +913,10 @@ extern "C" int sphx_pair_issue_rate(sphx_ctx *ctx, int32_t iters, double *pairs_
     // the fastest of three timed launches
     for (int rep = 0; rep < 3 && rc == SPHX_OK; ++rep) {
         float t = 0.f;
-        if (hipEventRecord(e0, st) != hipSuccess) rc = SPHX_ERR_HIP;
+        if (hipEventRecord(e0, st) != hipSuccess) {
+            sphx_set_error("hipEventRecord failed");
+            rc = SPHX_ERR_HIP;
+        }
         if (rc == SPHX_OK) rc = sphx_pair_probe_launch(blocks, iters, out, st);
         if (rc == SPHX_OK && (hipEventRecord(e1, st) != hipSuccess ||
                               hipEventSynchronize(e1) != hipSuccess ||
@@ -928,6 +931,10 @@ extern "C" int sphx_pair_issue_rate(sphx_ctx *ctx, int32_t iters, double *pairs_
     hipStreamSynchronize(st);
     hipFree(out);
     if (rc != SPHX_OK) return rc;
+    if (!(ms > 0.f)) {
+        sphx_set_error("the pair-rate reference reported no elapsed time");
+        return SPHX_ERR_HIP;
+    }
     *pairs_per_s = (double)blocks * 256.0 * iters * 16.0 / (ms * 1e-3);
     return SPHX_OK;
 }
