@@ -9,8 +9,11 @@ hydro acceleration + du/dt, half kick, u update.
 
 Prints one JSON line: mixed-precision particle-steps/s (value), the plain
 FP64 GPU step's throughput on the same inputs and the speedup, the gravity
-kernel's roofline (HIP events around its launches, 20 flop per pair), and
-the FP64 CPU oracle timed on a sample of rows of the same step.
+kernel's roofline (HIP events around its launches, 20 flop per pair), the
+FP64 CPU oracle timed on a sample of rows of the same step with the step's
+parity against it on those rows, and the served stream (512 runs, N
+log-uniform in [2048, 262144], 200 steps each, Poisson arrivals) with its
+p50/p99 completion latency and aggregate particle-steps/s.
 """
 from __future__ import annotations
 
@@ -69,7 +72,9 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-rows", type=int, default=32768,
                     help="rows of the step timed on the FP64 CPU oracle (0: skip)")
     ap.add_argument("--grav", default=None, help="gravity config ipl,tj,js (default: autotuned)")
-    ap.add_argument("--graph", type=int, default=0, help="replay steps from hipGraphs")
+    ap.add_argument("--graph", type=int, default=0, help="c2/c3: replay steps from hipGraphs")
+    ap.add_argument("--serve-graph", type=int, default=1,
+                    help="served stream: replay steps from hipGraphs")
     ap.add_argument("--runs", type=int, default=512, help="serve: requests in the stream")
     ap.add_argument("--nmin", type=int, default=2048)
     ap.add_argument("--nmax", type=int, default=262144)
@@ -84,8 +89,8 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-gravity", default="4,0.5",
                     help="serve: 'S,share' for the symmetric gravity sum (S block pairs per "
                          "wave, share of the budget for its partials) or 'tiled'")
-    ap.add_argument("--serve-runs", type=int, default=64,
-                    help="requests in the served-run latency sample of the default run (0: skip)")
+    ap.add_argument("--serve-runs", type=int, default=512,
+                    help="requests in the served stream of the default run (0: skip)")
     a = ap.parse_args(argv)
 
     if a.gpus != 1:
@@ -231,19 +236,25 @@ def bench_evrard(a) -> int:
     parity = {}
     if a.cpu_rows > 0:
         import oracle
+        from sph_2_amd import libsx
 
         rows = np.arange(min(a.cpu_rows, n))
+        r1 = len(rows)
         p = oracle.Params()
-        P, c = oracle.eos(state["rho"], state["u"], p.gamma)
+        pos, vel, hh, rho = state["pos"], state["vel"], state["h"], state["rho"]
+        P, c = oracle.eos(rho, state["u"], p.gamma)
         t0 = time.perf_counter()
-        oracle.smoothing(state["pos"], state["h"], rows=rows, p=p)
-        rho_o, _ = oracle.density(state["pos"], e.m, state["h"], rows=rows)
-        g_o = oracle.gravity(state["pos"], e.m, state["h"], 1.0, rows=rows)
-        dv, cv = oracle.divcurl(state["pos"], state["vel"], state["rho"], e.m, state["h"],
-                                rows=rows)
-        oracle.hydro(state["pos"], state["vel"], state["rho"], P, c, state["h"], e.m, p=p,
-                     rows=rows)
+        h_o, k_o, _, _ = oracle.smoothing(pos, hh, rows=rows, p=p)
+        rho_o, _ = oracle.density(pos, e.m, hh, rows=rows)
+        g_o = oracle.gravity(pos, e.m, hh, 1.0, rows=rows)
+        dv_o, cv_o = oracle.divcurl(pos, vel, rho, e.m, hh, rows=rows)
+        acc_o, du_o = oracle.hydro(pos, vel, rho, P, c, hh, e.m, p=p, rows=rows)
         t1 = time.perf_counter()
+        # the same rows of the same final state through the GPU drop-ins: the h
+        # iteration (from 2h), div/curl and the fused hydro pass (Monaghan AV)
+        h_g, k_g, _ = libsx.smoothing_h(pos, hh, nbeg=0, nend=r1)
+        dv_g, cv_g = libsx.div_curlVel_mpi(0, r1, pos, vel, rho, e.m, hh)
+        acc_g, du_g = libsx.getAcc_sph_dU(pos, vel, rho, P, c, hh, e.m, nbeg=0, nend=r1)
         cores = len(os.sched_getaffinity(0))
         threads = int(os.environ.get("OMP_NUM_THREADS", cores))
         cpu = {"value": len(rows) / (t1 - t0), "unit": "particle-steps/s",
@@ -253,32 +264,28 @@ def bench_evrard(a) -> int:
                          f"oracle, {(t1 - t0):.1f} s on {cpu_model()}"}
         parity = {
             "rows": int(len(rows)),
-            "density_rel_max": float(np.max(np.abs(state["rho"][rows] / rho_o - 1.0))),
-            "gravity_rel_rms": float(np.sqrt(((state["acc_g"][rows] - g_o) ** 2).sum() /
-                                             (g_o ** 2).sum())),
+            "what": "the step's final state (rho, acc_g from the step itself; h/k, div/curl "
+                    "and hydro from the drop-ins on that state) against the FP64 oracle on "
+                    "the same inputs",
+            "h_k_bit_exact": bool(np.array_equal(h_g, h_o) and np.array_equal(k_g, k_o)),
+            "k_mismatch": int(np.sum(k_g != k_o)),
+            "h_rel_max": float(np.max(np.abs(h_g / h_o - 1.0))),
+            "density_rel_max": float(np.max(np.abs(rho[rows] / rho_o - 1.0))),
+            "gravity_rel_rms": rel_rms(state["acc_g"][rows], g_o),
+            "divV_rel_rms": rel_rms(dv_g, dv_o),
+            "curlV_rel_rms": rel_rms(cv_g, cv_o),
+            "hydro_acc_rel_rms": rel_rms(acc_g, acc_o),
+            "dudt_rel_rms": rel_rms(du_g, du_o),
         }
         print(f"[bench] cpu oracle {cpu['value']:.1f} particle-steps/s", file=sys.stderr,
               flush=True)
 
-    # ---------------- served-run sample: p50/p99 completion latency ----------
+    # ---------------- the served stream: p50/p99 completion latency ---------
     serving = None
     if a.serve_runs > 0:
-        from sph_2_amd.serve import Server, build_payload, make_stream
-
-        reqs = make_stream(a.serve_runs, n_min=2048, n_max=65536, rate=8.0, steps=100,
-                           seed=a.seed)
-        for r in reqs:
-            r.payload = build_payload(r)
-        srv = Server(8 * 2 ** 30, max_runs=512, graph=True)
-        rep = srv.run(reqs)
-        srv.batch.close()
-        serving = {k: rep[k] for k in ("runs", "wall_s", "particle_steps_per_s", "p50_latency_s",
-                                       "p99_latency_s", "p50_queue_s", "p99_queue_s", "host_s")}
-        serving["config"] = (f"{a.serve_runs} Evrard/cloud runs, N log-uniform in [2048, 65536], "
-                             f"100 steps each, Poisson arrivals at 8/s, continuous batching with "
-                             f"graph replay")
-        print(f"[bench] serving p50 {rep['p50_latency_s']:.2f}s p99 {rep['p99_latency_s']:.2f}s",
-              file=sys.stderr, flush=True)
+        serving = serve_stream(a, a.serve_runs)
+        log(f"serving p50 {serving['p50_latency_s']:.2f}s p99 {serving['p99_latency_s']:.2f}s, "
+            f"{serving['particle_steps_per_s']:.3e} particle-steps/s")
 
     out = {
         "metric": "particle-steps/s at N=1M (mixed vs FP64 plain); p50/p99 latency batched runs",
@@ -497,7 +504,7 @@ def bench_c3(a) -> int:
     units = e.units
     tfac = (5.0 / 3.0 - 1.0) * cool.kw["mu"] * ic.MH / ic.KB * units["unit_u_cgs"]
     out = {"workload": f"c3: turbulent cooling cloud N={n}, rms Mach 2, T0=1e4 K, {steps} steps "
-                       f"at dt={a.dt} (code units: 1e4 Msun, 1 pc), 200-node Lambda(T) table",
+                       f"at dt={a.dt} (code units: 1e4 Msun, 20 pc), 200-node Lambda(T) table",
            "ms_per_step": 1e3 * wall / steps, "particle_steps_per_s": n * steps / wall,
            "T_median_K_start": float(np.median(st0["u"]) * tfac),
            "T_median_K_end": float(np.median(st1["u"]) * tfac),
@@ -546,26 +553,51 @@ def bench_bh(a) -> int:
     return 0
 
 
-def bench_serve(a) -> int:
-    """Served stream: Poisson arrivals, N log-uniform, 200 steps each, admitted
-    by HBM budget, every live run advanced by one launch per pass."""
-    from sph_2_amd.serve import Server, make_stream
+def serve_stream(a, runs: int, verbose: bool = False) -> dict:
+    """The served stream: `runs` Evrard/cloud requests, N log-uniform in
+    [nmin, nmax], run_steps steps each, seeded Poisson arrivals at `rate`/s,
+    admitted by the HBM budget, every live run advanced by one launch per
+    pass (graph replay), retired as soon as it finishes."""
+    from sph_2_amd.serve import Server, build_payload, make_stream
 
-    reqs = make_stream(a.runs, n_min=a.nmin, n_max=a.nmax, rate=a.rate, steps=a.run_steps,
+    reqs = make_stream(runs, n_min=a.nmin, n_max=a.nmax, rate=a.rate, steps=a.run_steps,
                        seed=a.seed)
     t0 = time.perf_counter()
-    from sph_2_amd.serve import build_payload
     for r in reqs:
         r.payload = build_payload(r)
     log(f"serve: {len(reqs)} requests built in {time.perf_counter() - t0:.1f}s")
     sym = None if a.serve_gravity == "tiled" else (int(a.serve_gravity.split(",")[0]),
                                                     float(a.serve_gravity.split(",")[1]))
-    srv = Server(int(a.hbm_gb * 2 ** 30), max_runs=512, graph=bool(a.graph), symmetric=sym)
-    rep = srv.run(reqs, log=lambda m: print(f"[serve] {m}", file=sys.stderr, flush=True))
-    rep["workload"] = (f"c5: {a.runs} runs, N log-uniform in [{a.nmin}, {a.nmax}], "
-                       f"{a.run_steps} steps each, Poisson arrivals at {a.rate}/s, "
-                       f"HBM budget {a.hbm_gb} GB, graph={bool(a.graph)}, gravity "
-                       f"{'tiled' if sym is None else f'symmetric ({sym[0]} block pairs per wave, {sym[1]:.0%} of the budget for its partial sums)'}")
+    srv = Server(int(a.hbm_gb * 2 ** 30), max_runs=512, graph=bool(a.serve_graph), symmetric=sym)
+    done = [0]
+
+    def progress(m):
+        done[0] += 1
+        if verbose or done[0] % 32 == 0:
+            log(f"serve: {done[0]} runs done; {m}")
+
+    rep = srv.run(reqs, log=progress)
+    srv.batch.close()
+    keep = ("runs", "wall_s", "particle_steps", "particle_steps_per_s", "p50_latency_s",
+            "p99_latency_s", "p50_queue_s", "p99_queue_s", "capacity_particles", "host_s",
+            "loop_steps", "max_steps_in_flight")
+    out = {k: rep[k] for k in keep if k in rep}
+    out["config"] = {
+        "workload": f"{runs} Evrard/cooling-cloud runs (half each), N = floor({a.nmin} * "
+                    f"({a.nmax}/{a.nmin})^U) log-uniform, {a.run_steps} KDK steps each at "
+                    f"dt=1e-4, seeded Poisson arrivals (seed {a.seed})",
+        "runs": runs, "n_min": a.nmin, "n_max": a.nmax, "steps": a.run_steps,
+        "arrival_rate_per_s": a.rate, "hbm_budget_gb": a.hbm_gb,
+        "graph_replay": bool(a.serve_graph),
+        "gravity": ("tiled" if sym is None else
+                    f"symmetric ({sym[0]} block pairs per wave, {sym[1]:.0%} of the budget for "
+                    f"its partial sums)")}
+    return out
+
+
+def bench_serve(a) -> int:
+    """Served stream alone (the same as the default run's serving block)."""
+    rep = serve_stream(a, a.runs, verbose=True)
     print(json.dumps(rep))
     return 0
 

@@ -127,6 +127,10 @@ typedef struct sphx_counters {
     int64_t nonfinite;      /* particles whose u, v or x was not finite after the last step */
     int64_t bin_floor;      /* block time steps: particles whose criterion asked for a step
                                shorter than the shortest level in the last substep */
+    int64_t cool_iters;     /* bisection iterations of the implicit cooling solve, summed
+                               over the particles of the last step */
+    int64_t cool_bisected;  /* particles whose cooling solve bracketed and bisected a root in
+                               the last step (the others left at the table floor or unchanged) */
 } sphx_counters;
 
 typedef struct sphx_diag {

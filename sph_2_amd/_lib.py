@@ -50,7 +50,8 @@ class Counters(C.Structure):
     _fields_ = [("n", C.c_int64), ("steps", C.c_int64), ("h_iters_max", C.c_int64),
                 ("h_iters_sum", C.c_int64), ("h_capped", C.c_int64), ("cool_capped", C.c_int64),
                 ("cand_buckets", C.c_int64), ("buckets", C.c_int64), ("bh_overflow", C.c_int64),
-                ("nonfinite", C.c_int64), ("bin_floor", C.c_int64)]
+                ("nonfinite", C.c_int64), ("bin_floor", C.c_int64),
+                ("cool_iters", C.c_int64), ("cool_bisected", C.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -542,6 +542,8 @@ int Engine::read_counters(sphx_counters *out) {
     out->bh_overflow = c[CNT_BH_OVERFLOW];
     out->nonfinite = c[CNT_NONFINITE];
     out->bin_floor = c[CNT_BIN_FLOOR];
+    out->cool_iters = c[CNT_COOL_ITERS];
+    out->cool_bisected = c[CNT_COOL_BISECT];
     return SPHX_OK;
 }
 

@@ -279,13 +279,16 @@ __global__ void k_chimes_apply(ChimesDev c, const double *__restrict__ pos,
 
 // Cooling of particle i's updated energy u over dt by the run's method (0
 // none, 1 implicit solve, 2 2-D table, 3 CHIMES table).
+// *iters receives the bisection iterations of an implicit solve (0 otherwise).
 __device__ __forceinline__ double cool_particle(const Soa &s, int64_t i, double u, double dt,
                                                 int use_cool, int cool_on, const CoolDev &cd,
                                                 const GridDev &gd, const ChimesDev &ch,
-                                                int64_t *counters) {
+                                                int64_t *counters, int *iters) {
+    *iters = 0;
     if (cool_on && use_cool == 1) {
         int it = 0;
         u = cool_one(s.rho[i], u, dt, cd, &it);
+        *iters = it;
         if (it >= cd.max_iter) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
     } else if (use_cool == 2 && gd.du) {
         u = grid_apply(s.rho[i], u, dt, gd);
@@ -298,6 +301,19 @@ __device__ __forceinline__ double cool_particle(const Soa &s, int64_t i, double 
         }
     }
     return u;
+}
+
+// Adds the bisection iterations of a wavefront's cooling solves and the
+// number of its lanes that bisected to the step counters, one atomic each per
+// wavefront; every lane of the wavefront must call it.
+__device__ __forceinline__ void count_cool_iters(int it, int64_t *counters) {
+    unsigned long long sum = (unsigned long long)it;
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o, 64);
+    unsigned long long nb = (unsigned long long)__popcll(__ballot(it > 0));
+    if ((threadIdx.x & 63) == 0 && nb) {
+        atomicAdd((unsigned long long *)&counters[CNT_COOL_ITERS], sum);
+        atomicAdd((unsigned long long *)&counters[CNT_COOL_BISECT], nb);
+    }
 }
 
 // A particle whose state stopped being finite is counted (the step goes on;
@@ -313,19 +329,22 @@ __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, Coo
                          int cool_on, GridDev gd, ChimesDev ch, int64_t *counters) {
 #pragma clang fp contract(off)
     int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
-    if (i >= n) return;
-    const RunParams &r = rp[s.run[i]];
-    double dt = r.dt;
-    double half = 0.5 * dt;
-    s.vx[i] = s.vx[i] + s.ax[i] * half;
-    s.vy[i] = s.vy[i] + s.ay[i] * half;
-    s.vz[i] = s.vz[i] + s.az[i] * half;
-    double du = s.du[i];
-    double u = s.u[i] + half * (du + s.ut_prev[i]);
-    s.ut_prev[i] = du;
-    u = cool_particle(s, i, u, dt, r.use_cool, cool_on, cd, gd, ch, counters);
-    s.u[i] = u;
-    count_nonfinite(s, i, u, counters);
+    int it = 0;
+    if (i < n) {
+        const RunParams &r = rp[s.run[i]];
+        double dt = r.dt;
+        double half = 0.5 * dt;
+        s.vx[i] = s.vx[i] + s.ax[i] * half;
+        s.vy[i] = s.vy[i] + s.ay[i] * half;
+        s.vz[i] = s.vz[i] + s.az[i] * half;
+        double du = s.du[i];
+        double u = s.u[i] + half * (du + s.ut_prev[i]);
+        s.ut_prev[i] = du;
+        u = cool_particle(s, i, u, dt, r.use_cool, cool_on, cd, gd, ch, counters, &it);
+        s.u[i] = u;
+        count_nonfinite(s, i, u, counters);
+    }
+    count_cool_iters(it, counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -394,15 +413,14 @@ __global__ void k_blk_drift(Soa s, int64_t n, double d) {
 // End of substep t1: the active particles take the closing half kick, the u
 // trapezoid and cooling over their step, a new bin (a shorter step at once, a
 // longer one only one level at a time and only where t1 is a multiple of the
-// longer step), then the opening half kick of their next step.
-__global__ void k_blk_close(Soa s, const RunParams *__restrict__ rp, int64_t n, double dt_max,
-                            int B, int64_t t1, double c_cfl, CoolDev cd, int cool_on, GridDev gd,
-                            ChimesDev ch, int64_t *counters) {
+// longer step), then the opening half kick of their next step.  This is one
+// active particle i of k_blk_close.
+__device__ __forceinline__ void blk_close_one(Soa &s, const RunParams *__restrict__ rp, int64_t i,
+                                              double dt_max, int B, int64_t t1, double c_cfl,
+                                              const CoolDev &cd, int cool_on, const GridDev &gd,
+                                              const ChimesDev &ch, int64_t *counters, int *it) {
 #pragma clang fp contract(off)
-    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
-    if (i >= n) return;
     int b = s.bin[i];
-    if (!blk_active(b, B, t1)) return;
     double dto = ldexp(dt_max, -b);
     double half = 0.5 * dto;
     s.vx[i] = s.vx[i] + s.ax[i] * half;
@@ -411,7 +429,7 @@ __global__ void k_blk_close(Soa s, const RunParams *__restrict__ rp, int64_t n, 
     double du = s.du[i];
     double u = s.u[i] + half * (du + s.ut_prev[i]);
     s.ut_prev[i] = du;
-    u = cool_particle(s, i, u, dto, rp[s.run[i]].use_cool, cool_on, cd, gd, ch, counters);
+    u = cool_particle(s, i, u, dto, rp[s.run[i]].use_cool, cool_on, cd, gd, ch, counters, it);
     s.u[i] = u;
     bool fl = false;
     int bt = blk_target(blk_dt_crit(s, i, c_cfl), dt_max, B, &fl);
@@ -425,6 +443,17 @@ __global__ void k_blk_close(Soa s, const RunParams *__restrict__ rp, int64_t n, 
     s.vz[i] = s.vz[i] + s.az[i] * h2;
     if (fl) atomicAdd((unsigned long long *)&counters[CNT_BIN_FLOOR], 1ull);
     count_nonfinite(s, i, u, counters);
+}
+
+__global__ void k_blk_close(Soa s, const RunParams *__restrict__ rp, int64_t n, double dt_max,
+                            int B, int64_t t1, double c_cfl, CoolDev cd, int cool_on, GridDev gd,
+                            ChimesDev ch, int64_t *counters) {
+#pragma clang fp contract(off)
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    int it = 0;
+    if (i < n && blk_active(s.bin[i], B, t1))
+        blk_close_one(s, rp, i, dt_max, B, t1, c_cfl, cd, cool_on, gd, ch, counters, &it);
+    count_cool_iters(it, counters);
 }
 
 __global__ void k_copy_d(const double *__restrict__ a, double *__restrict__ b, int64_t p0,

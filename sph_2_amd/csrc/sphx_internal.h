@@ -165,7 +165,9 @@ enum {
     CNT_BH_OVERFLOW,
     CNT_NONFINITE,
     CNT_BIN_FLOOR,
-    CNT_SLOTS = 8
+    CNT_COOL_ITERS,   // bisection iterations of the implicit cooling solve
+    CNT_COOL_BISECT,  // particles whose cooling solve bisected (not an early exit)
+    CNT_SLOTS = 10
 };
 
 // Step phases timed with HIP events when Engine::timing is on.

@@ -12,7 +12,8 @@ Workloads (11_Dec_2022/ paths of hassanfv/SPH_2 for the reference setups):
   plummer         Plummer sphere r = a / sqrt(U^(-2/3) - 1), mass fraction <= 0.999.
   turbulent_cloud uniform sphere with a Gaussian random velocity field,
                   P(k) ~ k^-4 on a 64^3 grid, trilinear interpolation, rms
-                  Mach 2 at T = 1e4 K (unit template GPU_sph/IC/step_1.py:11-66).
+                  Mach 2 at T = 1e4 K, 1e4 Msun in 20 pc (unit template
+                  GPU_sph/IC/step_1.py:11-66).
   cooling_table   200 log-spaced T nodes on 10..1e8 K of a seeded piecewise
                   power-law Lambda(T).
 """
@@ -109,7 +110,13 @@ def plummer(n: int, seed: int = 0, a: float = 1.0, eps: float = 0.01) -> IC:
               eps=np.full(n, eps))
 
 
-def cloud_units(mass_msun: float = 1.0e4, radius_pc: float = 1.0) -> dict:
+def cloud_units(mass_msun: float = 1.0e4, radius_pc: float = 20.0) -> dict:
+    """Code units of the cooling cloud: M = 1e4 Msun in R = 20 pc (mean n_H ~ 9
+    cm^-3).  At this density the 200-node Lambda(T) table cools 1e4 K gas on a
+    time scale of a few to tens of 1e-4 code-time steps (1 code time = 13.4
+    Myr) toward the ~100 K balance with the 2e-26 erg/s heating, so every step
+    solves the implicit energy equation by bracketing and bisection instead of
+    stopping at the table's 10 K floor."""
     ul = radius_pc * PC
     um = mass_msun * MSUN
     ut = np.sqrt(ul ** 3 / (G_CGS * um))

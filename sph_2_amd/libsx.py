@@ -425,9 +425,10 @@ def smoothing_fast_new_mpi2(nbeg, nend, pos, h, *, mode="mixed", device=0, **pkw
     return _smoothing(pos, h, None, None, 0.0, nbeg, nend, mode, device, **pkw)[0]
 
 
-def smoothing_h(pos, h, *, mode="mixed", device=0, **pkw):
-    """All rows; returns (h, k, iterations)."""
-    return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)
+def smoothing_h(pos, h, *, mode="mixed", device=0, nbeg=None, nend=None, **pkw):
+    """Rows [nbeg, nend) (default all), iteration started at 2h; returns (h, k,
+    iterations)."""
+    return _smoothing(pos, h, None, None, 0.0, nbeg, nend, mode, device, **pkw)
 
 
 def do_smoothingX(poz, *, kth=64, device=0):
@@ -449,9 +450,7 @@ def do_smoothingX(poz, *, kth=64, device=0):
     t = _t()
     if r1 == r0:
         return np.zeros(0)
-    lo, hi = pos.min(0), pos.max(0)
-    vol = float(np.prod(np.maximum(hi - lo, 1e-300)))
-    h0 = 0.5 * (vol * (kth + 1) / (n * 4.18879020478639)) ** (1.0 / 3.0)
+    h0 = _knn_start(pos, kth)
     dp = dev(pos)
     dh = dev(np.full(n, h0))
     out = t.empty(r1 - r0, dtype=t.float64, device="cuda")
@@ -460,6 +459,28 @@ def do_smoothingX(poz, *, kth=64, device=0):
     if rc == _lib.CAPPED:
         warnings.warn("kNN search did not resolve some rows (NaN)", RuntimeWarning)
     return out.cpu().numpy()
+
+
+def _knn_start(pos, kth):
+    """First search radius / 2 of the kNN walk: the radius that holds kth + 1
+    particles at the mean density over the extents that are not degenerate
+    (volume for 3-D input, area for planar, length for collinear input), so a
+    flat or collinear set does not start at a vanishing radius.  Only a seed:
+    the walk brackets and selects the exact value from any positive start."""
+    n = len(pos)
+    ext = pos.max(0) - pos.min(0)
+    big = float(ext.max())
+    if not big > 0.0:
+        return 1.0                      # every particle on one point: h = 0 is found exactly
+    e = ext[ext > 1e-12 * big]
+    q = (kth + 1) / n
+    if len(e) == 3:
+        r = (float(np.prod(e)) * q / 4.18879020478639) ** (1.0 / 3.0)
+    elif len(e) == 2:
+        r = (float(np.prod(e)) * q / np.pi) ** 0.5
+    else:
+        r = 0.5 * float(e[0]) * q
+    return 0.5 * r
 
 
 def _row_offset(pos, sub):

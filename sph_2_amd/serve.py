@@ -238,8 +238,18 @@ class Server:
             self.sym_budget = int(hbm_budget_bytes * share)
             self.batch.set_gravity_symmetric(symmetric[0], self.sym_budget)
 
-    def run(self, requests: list[Request], keep_results: bool = False, log=None) -> dict:
+    def run(self, requests: list[Request], keep_results: bool = False, log=None,
+            max_in_flight: int = 2) -> dict:
+        """Serve `requests`; returns the latency and throughput report.
+
+        The host does not wait for each step: up to `max_in_flight` batch steps
+        are queued on the stream (each followed by an event) while the host
+        admits arrivals and does its bookkeeping; it waits only for the oldest
+        step's event when more are queued, and on the synchronous copy-out of a
+        finished run, whose completion time is taken after that copy."""
         torch = _lib.torch()
+        if max_in_flight < 1:
+            raise ValueError("max_in_flight must be at least 1")
         for r in requests:
             if r.payload is None:
                 r.payload = build_payload(r)
@@ -256,6 +266,7 @@ class Server:
         loops = 0
         psteps = 0
         t_admit = t_step = t_retire = 0.0
+        inflight = collections.deque()      # events of queued steps
         while pending or live:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
@@ -273,7 +284,11 @@ class Server:
                 continue
             tb = time.perf_counter()
             self.batch.step(1)
-            torch.cuda.synchronize()
+            ev = torch.cuda.Event()
+            ev.record()
+            inflight.append(ev)
+            while len(inflight) >= max_in_flight:
+                inflight.popleft().synchronize()
             t_step += time.perf_counter() - tb
             loops += 1
             finished = []
@@ -283,6 +298,8 @@ class Server:
                 live[rid] = (req, ta, s)
                 if s >= req.steps:
                     finished.append(rid)
+            if finished:
+                inflight.clear()                  # the copy-out below waits for every step
             for rid in finished:
                 req, ta, s = live.pop(rid)
                 tr0 = time.perf_counter()
@@ -308,6 +325,7 @@ class Server:
                "p50_queue_s": float(np.percentile(queue, 50)),
                "p99_queue_s": float(np.percentile(queue, 99)),
                "capacity_particles": self.capacity,
+               "max_steps_in_flight": max_in_flight,
                "host_s": {"admit": t_admit, "step": t_step, "retire": t_retire}}
         if keep_results:
             rep["results"] = results

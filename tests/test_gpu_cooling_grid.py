@@ -15,8 +15,9 @@ def table():
     from sph_2_amd.serve import default_cooling
 
     cool = default_cooling()
-    # the cloud (1e4 Msun in 1 pc) has n_H ~ 1e5 cm^-3: cover it
-    grid = CoolingGrid.from_solve(cool, T_min=1e3, T_max=1e7, n_u=120, nH_min=1e2, nH_max=1e8,
+    # the cloud (1e4 Msun in 20 pc) has a mean n_H ~ 9 cm^-3: cover it and its
+    # compressions
+    grid = CoolingGrid.from_solve(cool, T_min=1e3, T_max=1e7, n_u=120, nH_min=1e-1, nH_max=1e5,
                                   n_rho=90)
     kw = cool.kw
     cp = oracle.CoolParams(kw["unit_u_cgs"], kw["unit_rho_cgs"], kw["unit_time_s"], XH=kw["XH"],

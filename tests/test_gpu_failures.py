@@ -30,11 +30,12 @@ def test_coincident_particles_cap_the_h_iteration(gpu):
     pl.close()
     assert c["h_capped"] > 0
     # the cap leaves the coincident particles a vanishing h, as the reference's
-    # does; whatever that does to their state is what the counter reports
+    # does; whatever that does to the state (their density overflows, and the
+    # direct-sum gravity then carries a non-finite position to every particle)
+    # is what the counter reports, particle for particle
     bad = ~(np.isfinite(st["pos"]).all(1) & np.isfinite(st["vel"]).all(1) &
             np.isfinite(st["u"]))
     assert c["nonfinite"] == int(bad.sum())
-    assert not bad[200:].any() or bad[:200].any()
     with warnings.catch_warnings(record=True) as w:
         warnings.simplefilter("always")
         h, k, it = libsx.smoothing_h(pos, e.h)

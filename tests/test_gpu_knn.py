@@ -45,3 +45,20 @@ def test_lattice_ties_and_duplicates(gpu):
     got = libsx.do_smoothingX((pos, pos))
     assert np.array_equal(got, oracle.knn_h(pos))
     assert np.all(got[:300] == 0.0)
+
+
+def test_planar_and_collinear_input(gpu):
+    """Flat (z = 0) and collinear sets: the search starts from the area or
+    length density, not a vanishing volume, and still selects the exact rank."""
+    from sph_2_amd import libsx
+
+    rng = np.random.default_rng(11)
+    flat = rng.random((6000, 3))
+    flat[:, 2] = 0.0
+    got = libsx.do_smoothingX((flat, flat))
+    assert np.all(np.isfinite(got)) and np.all(got > 0)
+    assert np.array_equal(got, oracle.knn_h(flat))
+    line = np.zeros((3000, 3))
+    line[:, 0] = rng.random(3000)
+    got = libsx.do_smoothingX((line, line))
+    assert np.array_equal(got, oracle.knn_h(line))
