@@ -285,6 +285,18 @@ int sphx_global_dt(sphx_ctx *ctx, int64_t n, const double *h, const double *c,
                    const double *acc, const double *acc_g, const double *divV, double c_cfl,
                    double dt_min, double dt_max, double *out, void *stream);
 
+/* sphx_global_dt without a context and without any allocation, for callers
+ * that cannot allocate device memory (code under hipGraph capture, serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * Same kernels and result as sphx_global_dt; their per-bucket extrema go to a
+ * fixed 5 MiB device array of the library (one per device, held until the
+ * process ends), so n is limited to 8,388,608 particles (SPHX_ERR_ARG beyond).
+ * `device` is made current for the call.  Calls that overlap on one device
+ * share that array and race: serialise them (one stream, or events). */
+int sphx_global_dt_noalloc(int device, int64_t n, const double *h, const double *c,
+                           const double *acc, const double *acc_g, const double *divV,
+                           double c_cfl, double dt_min, double dt_max, double *out, void *stream);
+
 /* Morton keys (21 bits per axis, 63-bit key) over the bounding box of pos,
  * then a stable LSD radix sort; keys_out[i] and perm_out[i] = original index of
  * the i-th smallest key.  Synchronous. */

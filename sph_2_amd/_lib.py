@@ -123,6 +123,8 @@ SIGNATURES = {
     "sphx_cool": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V, _V]),
     "sphx_diagnostics": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _V, _V]),
     "sphx_global_dt": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _D, _D, _V, _V]),
+    "sphx_global_dt_noalloc": (C.c_int, [C.c_int, _I64, _V, _V, _V, _V, _V, _D, _D, _D, _V,
+                                         _V]),
     "sphx_knn_h": (C.c_int, [_V, _I64, _V, _V, _I32, _I64, _I64, _V, _V]),
     "sphx_morton_sort": (C.c_int, [_V, _I64, _V, _V, _V, _V]),
     "sphx_plan_create": (C.c_int, [C.c_int, C.c_int, _V, _V, _I64, _V]),

@@ -8,6 +8,8 @@
 // getPE (GPU_sph/IC/libsx2_2t.py:579-629).
 #include "sphx_internal.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int IT = 256;
@@ -807,4 +809,79 @@ int sphx_global_dt_launch(const double *h, const double *c, const double *acc, i
     k_dt_one<<<1, 256, 0, st>>>(scratch, nb, c_cfl, dt_min, dt_max, out);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free global time step: the kernels of sphx_global_dt with their
+// per-bucket extrema in a fixed device array instead of a context's buffer.
+// Each device holds its own copy of the array until the process ends; its
+// address is fetched once per device with the device current, and only that
+// address is handed to kernels.  Nothing in the array needs clearing: every
+// slot a call reads was written by the same call.  Calls that overlap on one
+// device share the array and race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int64_t DT_NOALLOC_MAX_N = int64_t(1) << 23;                  // 8,388,608 particles
+constexpr int64_t DT_NOALLOC_SLOTS = 5 * ((DT_NOALLOC_MAX_N + 63) / 64);  // 5 MiB of doubles
+constexpr int DT_NOALLOC_MAX_DEVICES = 64;
+
+static __device__ __align__(256) double scratch[DT_NOALLOC_SLOTS];
+
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[DT_NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on `device` (current), looked up on the first
+// call per device.
+int scratch_on(int device, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            sphx_set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return SPHX_ERR_HIP;
+        }
+        g_scratch_ptr[device] = (double *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return SPHX_OK;
+}
+
+}  // namespace
+
+extern "C" int sphx_global_dt_noalloc(int device, int64_t n, const double *h, const double *c,
+                                      const double *acc, const double *acc_g, const double *divV,
+                                      double c_cfl, double dt_min, double dt_max, double *out,
+                                      void *stream) {
+    if (n <= 0 || !h || !c || !acc || !acc_g || !divV || !out) {
+        sphx_set_error("bad arguments");
+        return SPHX_ERR_ARG;
+    }
+    if (n > DT_NOALLOC_MAX_N) {
+        sphx_set_error("sphx_global_dt_noalloc: n = " + std::to_string(n) +
+                       " exceeds its fixed scratch (at most " +
+                       std::to_string(DT_NOALLOC_MAX_N) + " particles); use sphx_global_dt");
+        return SPHX_ERR_ARG;
+    }
+    if (!(c_cfl > 0.0 && dt_min > 0.0 && dt_max >= dt_min)) {
+        sphx_set_error("global dt needs c_cfl > 0 and 0 < dt_min <= dt_max");
+        return SPHX_ERR_ARG;
+    }
+    int ndev = 0;
+    SPHX_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev || device >= DT_NOALLOC_MAX_DEVICES) {
+        sphx_set_error("sphx_global_dt_noalloc: no such device " + std::to_string(device));
+        return SPHX_ERR_ARG;
+    }
+    int prev = 0;
+    SPHX_HIP(hipGetDevice(&prev));
+    if (prev != device) SPHX_HIP(hipSetDevice(device));
+    double *buf = nullptr;
+    int rc = scratch_on(device, &buf);
+    if (rc == SPHX_OK)
+        rc = sphx_global_dt_launch(h, c, acc, 3, acc_g, 3, divV, n, c_cfl, dt_min, dt_max, buf,
+                                   DT_NOALLOC_SLOTS, out, (hipStream_t)stream);
+    if (prev != device) hipSetDevice(prev);
+    return rc;
 }
