@@ -86,7 +86,7 @@ def main(argv=None) -> int:
     ap.add_argument("--bh-theta", type=float, default=0.5,
                     help="also time the step with Barnes-Hut gravity at this opening angle "
                          "(0: skip); reported beside the direct-sum headline")
-    ap.add_argument("--serve-gravity", default="4,0.5",
+    ap.add_argument("--serve-gravity", default="4,0.1",
                     help="serve: 'S,share' for the symmetric gravity sum (S block pairs per "
                          "wave, share of the budget for its partials) or 'tiled'")
     ap.add_argument("--serve-runs", type=int, default=512,
@@ -580,7 +580,7 @@ def serve_stream(a, runs: int, verbose: bool = False) -> dict:
     srv.batch.close()
     keep = ("runs", "wall_s", "particle_steps", "particle_steps_per_s", "p50_latency_s",
             "p99_latency_s", "p50_queue_s", "p99_queue_s", "capacity_particles", "host_s",
-            "loop_steps", "max_steps_in_flight")
+            "loop_steps", "max_steps_in_flight", "gpu_step_s", "gpu_step_frac")
     out = {k: rep[k] for k in keep if k in rep}
     out["config"] = {
         "workload": f"{runs} Evrard/cooling-cloud runs (half each), N = floor({a.nmin} * "

@@ -131,6 +131,8 @@ typedef struct sphx_counters {
                                over the particles of the last step */
     int64_t cool_bisected;  /* particles whose cooling solve bracketed and bisected a root in
                                the last step (the others left at the table floor or unchanged) */
+    int64_t nbr_overflow;   /* buckets whose neighbour lists did not fit in the last step
+                               (their div/curl and hydro sums came from the candidate walk) */
 } sphx_counters;
 
 typedef struct sphx_diag {
@@ -314,6 +316,18 @@ int sphx_plan_create(int device, int mode, const sphx_params *p, const sphx_cool
 int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, const double *vel,
                    const double *u, const double *m, const double *h, const double *eps,
                    void *stream);
+/* sphx_plan_load with the particle buffers left in the caller's ownership
+ * (the driver's own device arrays, DRV2:150-189): pos, vel (n,3), u and h (n,)
+ * become the run's state between calls.  Every sphx_plan_init /
+ * sphx_plan_step reads them at its start (edits made between calls take
+ * effect) and writes the state back into them at its end, in the caller's
+ * particle order; m and eps are read once here.  The plan keeps its own
+ * Morton-ordered working copy, because the sort reorders the state every step:
+ * binding costs a gather and a scatter of 8 doubles per particle per call
+ * (about 0.1 ms at N = 1,048,576), not per step.  sphx_plan_load or
+ * sphx_plan_restore ends the binding. */
+int sphx_plan_bind(sphx_plan *plan, int64_t n, double *pos, double *vel, double *u,
+                   const double *m, double *h, const double *eps, void *stream);
 /* Forces at t = 0 (DRV2:311-392 without its Euler u update). */
 int sphx_plan_init(sphx_plan *plan, void *stream);
 /* n_steps KDK steps at fixed dt.  use_cooling: 0 none, 1 the implicit solve
@@ -392,6 +406,15 @@ int sphx_plan_chimes_ions(sphx_plan *plan, float *out, void *stream);
  * results do not change.  No counterpart in the reference (brute-force pair
  * loops, LIBV2:27-38). */
 int sphx_plan_set_cell_level(sphx_plan *plan, int32_t bias);
+/* Neighbour lists (on by default): the density pass records each particle's
+ * accepted neighbours in the order its candidate walk meets them, and the
+ * div/curl and hydro passes iterate those lists instead of walking the
+ * candidates again.  The same pairs in the same order: results are
+ * bit-identical either way; off runs the walks (a tuning and test knob).  A
+ * particle with more than 128 such neighbours sends its bucket back to the
+ * walk (counted in nbr_overflow).  No counterpart in the reference, whose
+ * kernels test every pair (LIBV2:247-417). */
+int sphx_plan_set_neighbor_lists(sphx_plan *plan, int32_t on);
 int sphx_batch_set_cell_level(sphx_batch *b, int32_t bias);
 
 /* Adaptive global time step inside the step loop (DRV2:517-561): after each
@@ -443,15 +466,20 @@ int sphx_batch_runs(sphx_batch *b, int64_t *ids, int64_t *sizes, int64_t *steps,
 int sphx_batch_set_cooling_grid(sphx_batch *b, const sphx_cool_grid *grid);
 /* Gravity of the batch's runs from the symmetric direct sum (each pair
  * evaluated once for both particles; block_pairs = 512-particle block pairs
- * per wavefront) instead of the fixed tiled shape.  Its partial sums take
- * sphx_gravity_symmetric_bytes(n, block_pairs) per run of n particles
- * (about 12 n^2 / 1024 bytes); budget_bytes of device memory are reserved
- * here and admissions that would exceed them are refused (SPHX_REFUSED).
- * Only before the first admission.  A run's results do not depend on the
- * other runs either way.  No counterpart in the reference (one simulation per
- * process, acc_g of LIBV2:119-168 over all pairs). */
+ * per wavefront) instead of the fixed tiled shape.  Its partial sums total
+ * sphx_gravity_symmetric_bytes(n, block_pairs) per run of n particles (about
+ * 12 n^2 / 1024 bytes); budget_bytes of device memory are reserved here and
+ * the triangle of 512-particle block pairs is summed in bands of rows whose
+ * partial sums fit them, each band's folded into FP64 accumulators before the
+ * next (the sums do not depend on the banding).  A run whose largest row
+ * does not fit, sphx_gravity_symmetric_min_bytes(n, block_pairs), is refused
+ * (SPHX_REFUSED).  Only before the first admission.  A run's results do not
+ * depend on the other runs either way.  No counterpart in the reference (one
+ * simulation per process, acc_g of LIBV2:119-168 over all pairs).  Plans and
+ * stateless calls band the same way with 16 GiB per band. */
 int sphx_batch_set_gravity_symmetric(sphx_batch *b, int32_t block_pairs, int64_t budget_bytes);
 int64_t sphx_gravity_symmetric_bytes(int64_t n, int32_t block_pairs);
+int64_t sphx_gravity_symmetric_min_bytes(int64_t n, int32_t block_pairs);
 /* As sphx_plan_set_chimes_table, for runs admitted with use_cooling = 3 (the
  * energy lookup only: batches keep no species fractions). */
 int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_table *table);

@@ -51,7 +51,8 @@ class Counters(C.Structure):
                 ("h_iters_sum", C.c_int64), ("h_capped", C.c_int64), ("cool_capped", C.c_int64),
                 ("cand_buckets", C.c_int64), ("buckets", C.c_int64), ("bh_overflow", C.c_int64),
                 ("nonfinite", C.c_int64), ("bin_floor", C.c_int64),
-                ("cool_iters", C.c_int64), ("cool_bisected", C.c_int64)]
+                ("cool_iters", C.c_int64), ("cool_bisected", C.c_int64),
+                ("nbr_overflow", C.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -129,6 +130,7 @@ SIGNATURES = {
     "sphx_morton_sort": (C.c_int, [_V, _I64, _V, _V, _V, _V]),
     "sphx_plan_create": (C.c_int, [C.c_int, C.c_int, _V, _V, _I64, _V]),
     "sphx_plan_load": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _V]),
+    "sphx_plan_bind": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _V, _V]),
     "sphx_plan_init": (C.c_int, [_V, _V]),
     "sphx_plan_step": (C.c_int, [_V, _I32, _D, _I32, _V]),
     "sphx_plan_store": (C.c_int, [_V, _V, _V, _V, _V, _V, _V, _V, _V, _V, _V]),
@@ -138,6 +140,7 @@ SIGNATURES = {
     "sphx_plan_set_gravity_config": (C.c_int, [_V, _I32, _I32, _I32]),
     "sphx_ctx_set_gravity_config": (C.c_int, [_V, _I32, _I32, _I32]),
     "sphx_plan_set_timing": (C.c_int, [_V, _I32]),
+    "sphx_plan_set_neighbor_lists": (C.c_int, [_V, _I32]),
     "sphx_plan_set_adaptive_dt": (C.c_int, [_V, _I32, _D, _D, _D]),
     "sphx_plan_set_gravity_bh": (C.c_int, [_V, _D]),
     "sphx_plan_set_cooling_grid": (C.c_int, [_V, _V]),
@@ -149,6 +152,7 @@ SIGNATURES = {
     "sphx_batch_set_chimes_table": (C.c_int, [_V, _V]),
     "sphx_batch_set_gravity_symmetric": (C.c_int, [_V, _I32, _I64]),
     "sphx_gravity_symmetric_bytes": (C.c_int64, [_I64, _I32]),
+    "sphx_gravity_symmetric_min_bytes": (C.c_int64, [_I64, _I32]),
     "sphx_plan_set_cell_level": (C.c_int, [_V, _I32]),
     "sphx_plan_set_block_steps": (C.c_int, [_V, _I32, _D]),
     "sphx_plan_block_bins": (C.c_int, [_V, _V, _V]),

@@ -15,9 +15,10 @@ import time
 CUS = 256
 CANDIDATES = [(1, 256), (1, 512), (2, 256), (2, 512), (2, 1024), (4, 256), (4, 512)]
 # Symmetric sum (ipl 8, blocks of 512): js = block pairs per wave.  Its
-# partial sums take 12 B x n^2 / 1024, so it is offered up to 2M particles.
+# partial sums (12 B x n^2 / 1024 in all) are summed in row bands of bounded
+# memory, so it is offered at every size the tuner sweeps (up to 8M).
 SYM_SEGMENTS = (1, 2, 4, 8, 16, 32, 64)
-SYM_MAX_N = 1 << 21
+SYM_MAX_N = 1 << 23
 _CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gravity_tuning.json")
 
 

@@ -70,6 +70,17 @@ __global__ void k_gather_by_id(const double *__restrict__ src, const int32_t *__
     int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
     if (i < n) dst[i] = src[id[i]];
 }
+// (n, 3) rows in caller order -> SoA in engine order: slot k takes row id[k].
+__global__ void k_gather_aos_by_id(const double *__restrict__ aos, const int32_t *__restrict__ id,
+                                   double *__restrict__ x, double *__restrict__ y,
+                                   double *__restrict__ z, int64_t n) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int64_t q = 3 * (int64_t)id[k];
+    x[k] = aos[q];
+    y[k] = aos[q + 1];
+    z[k] = aos[q + 2];
+}
 __global__ void k_gather_by_id_i(const int32_t *__restrict__ src, const int32_t *__restrict__ id,
                                  int32_t *__restrict__ dst, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * UT + threadIdx.x;
@@ -152,6 +163,7 @@ int Engine::reserve(int64_t capacity) {
     bbox = nullptr; bbox_cap = 0;
     d_run = nullptr;
     dtpart = nullptr; dtpart_cap = 0;
+    nbr = nullptr; nbr_ovf = nullptr; nbr_bcap = 0; prec = nullptr; fbal = nullptr;
     cap = 0;
     n = 0;
     R = 0;
@@ -171,6 +183,15 @@ int Engine::reserve(int64_t capacity) {
     SPHX_CHECK(alloc((void **)&hist, sizeof(uint32_t) * hist_cap));
     SPHX_CHECK(alloc((void **)&counters, sizeof(int64_t) * CNT_SLOTS));
     SPHX_HIP(hipMemset(counters, 0, sizeof(int64_t) * CNT_SLOTS));
+    if (want_lists && mode == SPHX_MIXED) {
+        // every run has at most one partial bucket
+        int64_t bc = capacity / SPHX_BUCKET + std::max(max_runs, 1) + 1;
+        SPHX_CHECK(alloc((void **)&nbr, sizeof(int32_t) * (size_t)bc * SPHX_NBR_MAX * 64));
+        SPHX_CHECK(alloc((void **)&nbr_ovf, sizeof(int32_t) * (size_t)bc));
+        SPHX_CHECK(alloc((void **)&prec, sizeof(PairRec) * (size_t)capacity));
+        SPHX_CHECK(alloc((void **)&fbal, sizeof(float) * (size_t)capacity));
+        nbr_bcap = bc;
+    }
     cap = capacity;
     return SPHX_OK;
 }
@@ -330,9 +351,12 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
         SPHX_CHECK(smoothing(st, predictor));
     }
     SPHX_CHECK(tmark(st, PH_SMOOTH, 1));
+    // the density pass writes neighbour lists that the div/curl and hydro
+    // passes of this step iterate (positions and h do not change in between)
+    const bool lists = lists_ready();
     SPHX_CHECK(tmark(st, PH_DENS, 0));
     SPHX_CHECK(build_boxes(st, true));
-    SPHX_CHECK(density_eos(st));
+    SPHX_CHECK(density_eos(st, lists));
     SPHX_CHECK(tmark(st, PH_DENS, 1));
     SPHX_CHECK(tmark(st, PH_GRAV, 0));
     if (bh_theta > 0.0) {
@@ -343,10 +367,10 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
     }
     SPHX_CHECK(tmark(st, PH_GRAV, 1));
     SPHX_CHECK(tmark(st, PH_DIVCURL, 0));
-    SPHX_CHECK(divcurl(st));
+    SPHX_CHECK(divcurl(st, lists));
     SPHX_CHECK(tmark(st, PH_DIVCURL, 1));
     SPHX_CHECK(tmark(st, PH_HYDRO, 0));
-    SPHX_CHECK(hydro(st));
+    SPHX_CHECK(hydro(st, lists));
     SPHX_CHECK(tmark(st, PH_HYDRO, 1));
     SPHX_CHECK(sphx_combine(*this, st));
     return SPHX_OK;
@@ -544,6 +568,7 @@ int Engine::read_counters(sphx_counters *out) {
     out->bin_floor = c[CNT_BIN_FLOOR];
     out->cool_iters = c[CNT_COOL_ITERS];
     out->cool_bisected = c[CNT_COOL_BISECT];
+    out->nbr_overflow = c[CNT_NBR_OVF];
     return SPHX_OK;
 }
 
@@ -1272,7 +1297,34 @@ struct sphx_plan {
     int32_t adaptive = 0;
     double c_cfl = 0.25, dt_min = 1e-5, dt_max = 1e-3;
     bool seed_pending = false;
+    // caller-owned state (sphx_plan_bind): read at the start of every init /
+    // step call, written back at its end, in the caller's particle order
+    bool bound = false;
+    double *bpos = nullptr, *bvel = nullptr, *bu = nullptr, *bh = nullptr;
 };
+
+// Bound buffers -> the plan's working arrays (engine order).
+static int plan_pull_bound(sphx_plan *plan, hipStream_t st) {
+    Engine &E = plan->E;
+    if (!plan->bound || E.n == 0) return SPHX_OK;
+    k_gather_aos_by_id<<<gu(E.n), UT, 0, st>>>(plan->bpos, E.s.id, E.s.x, E.s.y, E.s.z, E.n);
+    k_gather_aos_by_id<<<gu(E.n), UT, 0, st>>>(plan->bvel, E.s.id, E.s.vx, E.s.vy, E.s.vz, E.n);
+    k_gather_by_id<<<gu(E.n), UT, 0, st>>>(plan->bu, E.s.id, E.s.u, E.n);
+    k_gather_by_id<<<gu(E.n), UT, 0, st>>>(plan->bh, E.s.id, E.s.h, E.n);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+// The plan's working arrays -> bound buffers (caller order).
+static int plan_push_bound(sphx_plan *plan, hipStream_t st) {
+    Engine &E = plan->E;
+    if (!plan->bound || E.n == 0) return SPHX_OK;
+    SPHX_CHECK(scatter_out_aos(E, E.s.x, E.s.y, E.s.z, 0, E.n, plan->bpos, st));
+    SPHX_CHECK(scatter_out_aos(E, E.s.vx, E.s.vy, E.s.vz, 0, E.n, plan->bvel, st));
+    SPHX_CHECK(scatter_out(E, E.s.u, 0, E.n, plan->bu, st));
+    SPHX_CHECK(scatter_out(E, E.s.h, 0, E.n, plan->bh, st));
+    return SPHX_OK;
+}
 
 #define PLAN_PROLOGUE(plan)                        \
     if (!plan) {                                   \
@@ -1300,6 +1352,7 @@ extern "C" int sphx_plan_create(int device, int mode, const sphx_params *p,
     pl->E.device = device;
     pl->E.mode = mode;
     pl->E.p = *p;
+    pl->E.want_lists = true;
     int rc = pl->E.reserve(std::max<int64_t>(capacity, 64));
     if (rc >= 0) rc = sphx_copy_cool(pl->E, cool);
     if (rc < 0) {
@@ -1337,6 +1390,18 @@ extern "C" int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, con
     E.steps = 0;
     plan->loaded = true;
     plan->inited = false;
+    plan->bound = false;
+    return SPHX_OK;
+}
+
+extern "C" int sphx_plan_bind(sphx_plan *plan, int64_t n, double *pos, double *vel, double *u,
+                              const double *m, double *h, const double *eps, void *stream) {
+    SPHX_CHECK(sphx_plan_load(plan, n, pos, vel, u, m, h, eps, stream));
+    plan->bound = true;
+    plan->bpos = pos;
+    plan->bvel = vel;
+    plan->bu = u;
+    plan->bh = h;
     return SPHX_OK;
 }
 
@@ -1353,6 +1418,7 @@ extern "C" int sphx_plan_init(sphx_plan *plan, void *stream) {
     E.run_params.assign(1, rp0);
     SPHX_CHECK(E.upload_run_params(st));
     SPHX_CHECK(sphx_zero_counters(E.counters, st));
+    SPHX_CHECK(plan_pull_bound(plan, st));
     if (E.mode == SPHX_MIXED) {
         SPHX_CHECK(E.sort_and_permute(st, true));
         SPHX_CHECK(E.forces(st, false, false));
@@ -1360,6 +1426,7 @@ extern "C" int sphx_plan_init(sphx_plan *plan, void *stream) {
         SPHX_CHECK(E.plain_forces(st, false, false));
     }
     SPHX_CHECK(E.bootstrap_finish(st));
+    SPHX_CHECK(plan_push_bound(plan, st));
     E.blk_started = false;
     E.blk_tick = 0;
     plan->inited = true;
@@ -1401,9 +1468,11 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
     // cooling is gated by the run's use_cool flag (device memory), so a
     // captured step graph stays valid when the flag changes
     // the Barnes-Hut build sizes its levels on the host, so those steps run eagerly
-    return (E.use_graph && !E.timing && E.bh_theta == 0.0 && E.blk_B == 0)
-               ? E.step_graphs(st, n_steps)
-               : E.step(st, n_steps);
+    SPHX_CHECK(plan_pull_bound(plan, st));
+    SPHX_CHECK((E.use_graph && !E.timing && E.bh_theta == 0.0 && E.blk_B == 0)
+                   ? E.step_graphs(st, n_steps)
+                   : E.step(st, n_steps));
+    return plan_push_bound(plan, st);
 }
 
 extern "C" int sphx_plan_set_adaptive_dt(sphx_plan *plan, int32_t on, double c_cfl,
@@ -1536,6 +1605,7 @@ extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream
     }
     plan->seed_pending = h.seed_pending != 0;
     plan->loaded = plan->inited = true;
+    plan->bound = false;  // the restored run replaces a bound one
     E.drop_graph();
     SPHX_HIP(hipStreamSynchronize(st));
     return SPHX_OK;
@@ -1681,6 +1751,15 @@ extern "C" int sphx_plan_set_cell_level(sphx_plan *plan, int32_t bias) {
     PLAN_PROLOGUE(plan);
     (void)st;
     return sphx_set_cell_level(plan->E, bias);
+}
+
+extern "C" int sphx_plan_set_neighbor_lists(sphx_plan *plan, int32_t on) {
+    if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
+    if ((on != 0) != plan->E.use_lists) {
+        plan->E.use_lists = on != 0;
+        plan->E.drop_graph();  // the captured steps launch the other kernels
+    }
+    return SPHX_OK;
 }
 
 extern "C" int sphx_plan_set_timing(sphx_plan *plan, int32_t on) {

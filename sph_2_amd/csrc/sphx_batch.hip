@@ -167,6 +167,7 @@ extern "C" int sphx_batch_create(int device, int mode, const sphx_params *p,
     b->E.p = *p;
     b->E.max_runs = max_runs;
     b->E.gcfg = GravConfig{2, 512, 1};  // fixed shape: a run's sums do not depend on the mix
+    b->E.want_lists = true;
     int rc = b->E.reserve(std::max<int64_t>(capacity, 64));
     if (rc >= 0 && cool) rc = sphx_copy_cool(b->E, cool);
     if (rc >= 0) rc = b->E.set_runs({});
@@ -212,7 +213,8 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
         std::vector<int64_t> with_new = E.run_n;
         with_new.push_back(n);
         if (!sphx_sym_fits(E, with_new, E.gcfg.js)) {
-            sphx_set_error("admission refused: gravity partial-sum memory exhausted");
+            sphx_set_error("admission refused: a row of the run's gravity partial sums exceeds "
+                           "the reserved memory");
             return SPHX_REFUSED;
         }
     }
@@ -358,6 +360,11 @@ extern "C" int64_t sphx_gravity_symmetric_bytes(int64_t n, int32_t block_pairs) 
     return sphx_sym_run_bytes(n, block_pairs);
 }
 
+extern "C" int64_t sphx_gravity_symmetric_min_bytes(int64_t n, int32_t block_pairs) {
+    if (n <= 0 || block_pairs < 1) return 0;
+    return sphx_sym_min_bytes(n, block_pairs);
+}
+
 extern "C" int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_table *table) {
     void *stream = nullptr;
     BATCH_PROLOGUE(b);
@@ -461,7 +468,8 @@ extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *
         std::vector<int64_t> with_new = E.run_n;
         with_new.push_back(h.n);
         if (!sphx_sym_fits(E, with_new, E.gcfg.js)) {
-            sphx_set_error("admission refused: gravity partial-sum memory exhausted");
+            sphx_set_error("admission refused: a row of the run's gravity partial sums exceeds "
+                           "the reserved memory");
             return SPHX_REFUSED;
         }
     }
@@ -517,5 +525,7 @@ extern "C" int sphx_batch_query(sphx_batch *b, sphx_counters *out) {
 
 extern "C" int64_t sphx_batch_bytes_per_particle(void) {
     // 25 FP64 arrays + 7 int32 arrays + 2 uint64 keys + tables (< 1 B amortised)
-    return 25 * 8 + 7 * 4 + 2 * 8 + 1;
+    // + the neighbour lists of the pair passes (SPHX_NBR_MAX int32 entries, an
+    // 80-byte record and a Balsara factor per particle)
+    return 25 * 8 + 7 * 4 + 2 * 8 + 1 + SPHX_NBR_MAX * 4 + (int64_t)sizeof(PairRec) + 4;
 }

@@ -370,9 +370,11 @@ struct SymTabs {
     const int32_t *row_nseg;       // per block: segments of its row
     const int32_t *row_seg0;       // per block: first (global) segment of its row
     const double *bbox;            // per block: box(6), max eps, common mass or -1
-    float *px, *py, *pz;           // j-side partials
-    double *qx, *qy, *qz;          // i-side partials, SYM_B slots per segment
+    float *px, *py, *pz;           // j-side partials of the current row band
+    double *qx, *qy, *qz;          // i-side partials of the band, SYM_B slots per segment
     int32_t seg0;                  // global index of the launch's first segment
+    int32_t q_seg0;                // global index of the band's first segment
+    int64_t p_base;                // row_off of the band's first row
     const int32_t *act;            // block time steps: blocks holding an active particle
                                    // (nullptr: every block pair is computed)
     const int32_t *run;            // run of each particle
@@ -539,7 +541,7 @@ __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *
         S.ax[p] = S.ay[p] = S.az[p] = 0.0f;
         kx[p].init(); ky[p].init(); kz[p].init();
     }
-    const int64_t roff = T.row_off[I];
+    const int64_t roff = T.row_off[I] - T.p_base;
     const int rj0 = T.row_j0[I];
 
     for (int J = J0; J < J1; ++J) {
@@ -576,7 +578,7 @@ __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *
     for (int p = 0; p < 8; ++p) {
         int li = lane + 64 * p;
         if (li < cnt) {
-            int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
+            int64_t o = (int64_t)(T.seg0 + sg - T.q_seg0) * SYM_B + li;
             if (add) {
                 T.qx[o] += kx[p].value();
                 T.qy[o] += ky[p].value();
@@ -616,7 +618,7 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
     if (!any) return false;
     const double ox = 0.5 * (bI[0] + bI[1]), oy = 0.5 * (bI[2] + bI[3]), oz = 0.5 * (bI[4] + bI[5]);
     const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
-    const int64_t roff = T.row_off[I];
+    const int64_t roff = T.row_off[I] - T.p_base;
     const int rj0 = T.row_j0[I];
 
     for (int h = 0; 256 * h < cnt; ++h) {
@@ -712,7 +714,7 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
         for (int p = 0; p < 4; ++p) {
             int li = lane + 64 * (4 * h + p);
             if (li < cnt) {
-                int64_t o = (int64_t)(T.seg0 + sg) * SYM_B + li;
+                int64_t o = (int64_t)(T.seg0 + sg - T.q_seg0) * SYM_B + li;
                 T.qx[o] = kx[p].value();
                 T.qy[o] = ky[p].value();
                 T.qz[o] = kz[p].value();
@@ -734,33 +736,52 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 }
 
 
-// a_i = G (sum over the segments of i's row of their i-side sums, in order,
-// + sum over the earlier rows of the run of their j-side sums for i, in order).
-__global__ void k_grav_sym_reduce(SymTabs T, double G, int64_t p0, int64_t p1, double *gx,
-                                  double *gy, double *gz) {
+// The block-pair triangle is summed in bands of rows [lo, hi), sized so that a
+// band's partial sums fit a memory budget.  For particle i of block I (row I):
+//   a_i = G (sum over the earlier rows of the run of their j-side sums for i,
+//            in row order, + sum over the segments of row I of their i-side
+//            sums, in order),
+// one FP64 accumulator per component, held in g between bands (zeroed before
+// the first band by k_grav_sym_zero).  Each band adds the terms of its rows in
+// that order and finishes the particles whose own row it holds, so the sums do
+// not depend on how the rows are banded.
+__global__ void k_grav_sym_zero(int64_t p0, int64_t p1, double *gx, double *gy, double *gz) {
+    int64_t i = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < p1) gx[i] = gy[i] = gz[i] = 0.0;
+}
+
+__global__ void k_grav_sym_reduce(SymTabs T, double G, int64_t p0, int64_t p1, int lo, int hi,
+                                  double *gx, double *gy, double *gz) {
     int64_t i = p0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p1) return;
     int r = T.run[i];
     int b0 = T.run_b0[r];
     int I = b0 + (int)((i - T.r_pbeg[r]) / SYM_B);
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    int ns = T.row_nseg[I], sq = T.row_seg0[I];
-    int64_t li = i - T.b_beg[I];
+    if (I < lo) return;                            // finished by an earlier band
+    const int a0 = b0 > lo ? b0 : lo, a1 = I < hi ? I : hi;
+    const bool own = I < hi;
+    if (a1 <= a0 && !own) return;                  // no row of this band reaches i
+    double sx = gx[i], sy = gy[i], sz = gz[i];
     // unrolled so that the loads of several slots are in flight at once (the
     // additions keep their order)
 #pragma unroll 8
-    for (int k = 0; k < ns; ++k) {
-        int64_t o = (int64_t)(sq + k) * SYM_B + li;
-        sx += T.qx[o]; sy += T.qy[o]; sz += T.qz[o];
-    }
-#pragma unroll 8
-    for (int I2 = b0; I2 < I; ++I2) {
-        int64_t o = T.row_off[I2] + (i - T.row_j0[I2]);
+    for (int I2 = a0; I2 < a1; ++I2) {
+        int64_t o = T.row_off[I2] - T.p_base + (i - T.row_j0[I2]);
         sx += (double)T.px[o]; sy += (double)T.py[o]; sz += (double)T.pz[o];
     }
-    gx[i] = G * sx;
-    gy[i] = G * sy;
-    gz[i] = G * sz;
+    if (own) {
+        int ns = T.row_nseg[I], sq = T.row_seg0[I] - T.q_seg0;
+        int64_t li = i - T.b_beg[I];
+#pragma unroll 8
+        for (int k = 0; k < ns; ++k) {
+            int64_t o = (int64_t)(sq + k) * SYM_B + li;
+            sx += T.qx[o]; sy += T.qy[o]; sz += T.qz[o];
+        }
+        sx = G * sx; sy = G * sy; sz = G * sz;
+    }
+    gx[i] = sx;
+    gy[i] = sy;
+    gz[i] = sz;
 }
 
 // Issue-rate reference for the far block pairs of k_grav_sym: the same
@@ -837,8 +858,15 @@ struct SymHost {
     int64_t q_cap = 0;
     int32_t *act = nullptr;         // per block: holds an active particle (block time steps)
     int64_t act_cap = 0;
-    int64_t budget = -1;            // bytes the partial sums may take (-1: no limit)
+    int64_t budget = -1;            // bytes reserved for a band's partial sums (-1: none)
     double *pool = nullptr;         // reserved memory of a budget (i-side, then j-side sums)
+    // row bands of the block-pair triangle whose partial sums fit the budget
+    // (or SYM_BAND_BYTES without one)
+    struct Band {
+        int row_lo, row_hi, seg_lo, seg_hi;
+        int64_t p_base, slots;
+    };
+    std::vector<Band> bands;
     int o_bbeg = 0, o_bcnt = 0, o_runb0 = 0, o_si = 0, o_sj0 = 0, o_sj1 = 0, o_rj0 = 0, o_rns = 0,
         o_rs0 = 0;
 };
@@ -847,20 +875,35 @@ void sphx_sym_host_free(SymHost *h) { delete h; }
 
 static int64_t sym_blocks(int64_t n) { return (n + SYM_B - 1) / SYM_B; }
 
-// Bytes of partial sums one run of n particles needs at segment length S:
-// 12 per j-side slot (row a of nb blocks holds the particles after block a)
-// and 24 x SYM_B per segment.
+// Partial-sum memory of one band when no budget is reserved (plans and
+// stateless calls): 16 GiB holds N = 1,048,576 in one band; larger runs are
+// summed in several.
+constexpr int64_t SYM_BAND_BYTES = int64_t(16) << 30;
+
+// Bytes of partial sums of one row of the triangle: 12 per j-side slot (row a
+// of nb blocks holds the particles after block a) and 24 x SYM_B per segment.
+static int64_t sym_row_bytes(int64_t slots, int64_t segs) { return 12 * slots + 24 * SYM_B * segs; }
+
+// Bytes of partial sums one run of n particles needs at segment length S, all
+// rows together.
 int64_t sphx_sym_run_bytes(int64_t n, int S) {
-    int64_t nb = sym_blocks(n), slots = 0, segs = 0;
-    for (int64_t a = 0; a < nb; ++a) {
-        slots += n - std::min(n, (a + 1) * SYM_B);
-        segs += (nb - a + S - 1) / S;
-    }
-    return 12 * slots + 24 * SYM_B * segs;
+    int64_t nb = sym_blocks(n), bytes = 0;
+    for (int64_t a = 0; a < nb; ++a)
+        bytes += sym_row_bytes(n - std::min(n, (a + 1) * SYM_B), (nb - a + S - 1) / S);
+    return bytes;
+}
+
+// The least band memory with which a run of n particles can be summed: its
+// first (largest) row.
+int64_t sphx_sym_min_bytes(int64_t n, int S) {
+    if (n <= 0) return 0;
+    int64_t nb = sym_blocks(n);
+    return sym_row_bytes(n - std::min<int64_t>(n, SYM_B), (nb + S - 1) / S);
 }
 
 // Fix the partial-sum memory of an engine (served batches): allocated once;
-// runs whose partials would not fit are refused at admission.
+// the rows of every run are summed in bands that fit it, and runs whose
+// largest row would not fit are refused at admission.
 int sphx_sym_reserve(Engine &E, int S, int64_t budget_bytes) {
     if (!E.grav.sym) E.grav.sym = new SymHost();
     SymHost &H = *E.grav.sym;
@@ -876,9 +919,9 @@ int sphx_sym_reserve(Engine &E, int S, int64_t budget_bytes) {
 // length S (always true without a reservation).
 bool sphx_sym_fits(const Engine &E, const std::vector<int64_t> &sizes, int S) {
     if (!E.grav.sym || E.grav.sym->budget < 0) return true;
-    int64_t bytes = 0;
-    for (int64_t n : sizes) bytes += sphx_sym_run_bytes(n, S);
-    return bytes <= E.grav.sym->budget;
+    for (int64_t n : sizes)
+        if (sphx_sym_min_bytes(n, S) > E.grav.sym->budget) return false;
+    return true;
 }
 
 static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
@@ -919,20 +962,35 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     runb0.push_back((int32_t)bbeg.size());
     run_seg0.push_back((int32_t)si.size());
     int nb = (int)bbeg.size(), ns = (int)si.size();
-    int64_t need_p = 3 * plen, need_q = 3 * (int64_t)SYM_B * ns;
-    if (H.budget >= 0) {
-        if (8 * need_q + 4 * need_p > H.budget) {
-            sphx_set_error("runs exceed the reserved memory of the symmetric gravity sum");
-            return SPHX_ERR_STATE;
+    // row bands: consecutive rows while their partial sums fit the budget
+    const int64_t band_bytes = H.budget >= 0 ? H.budget : SYM_BAND_BYTES;
+    std::vector<SymHost::Band> bands;
+    int64_t max_slots = 0, max_segs = 0;
+    for (int I = 0; I < nb;) {
+        SymHost::Band bd{I, I, rs0[I], rs0[I], roff[I], 0};
+        int64_t segs = 0;
+        while (I < nb) {
+            int64_t rsl = (I + 1 < nb ? roff[I + 1] : plen) - roff[I];
+            if (sym_row_bytes(bd.slots + rsl, segs + rns[I]) > band_bytes) {
+                if (I == bd.row_lo) {
+                    sphx_set_error("a row of the symmetric gravity sum (" +
+                                   std::to_string(sym_row_bytes(rsl, rns[I])) +
+                                   " bytes of partial sums) exceeds its reserved memory");
+                    return SPHX_ERR_STATE;
+                }
+                break;
+            }
+            bd.slots += rsl;
+            segs += rns[I];
+            ++I;
         }
-    } else {
-        double need_gb = (4.0 * need_p + 8.0 * need_q) / 1e9;
-        if (need_gb > 96.0) {
-            sphx_set_error("symmetric gravity would need " + std::to_string((int)need_gb) +
-                           " GB of partial sums for these runs; use a tiled configuration");
-            return SPHX_ERR_ARG;
-        }
+        bd.row_hi = I;
+        bd.seg_hi = I < nb ? rs0[I] : ns;
+        max_slots = std::max(max_slots, bd.slots);
+        max_segs = std::max(max_segs, segs);
+        bands.push_back(bd);
     }
+    int64_t need_p = 3 * max_slots, need_q = 3 * (int64_t)SYM_B * max_segs;
     std::vector<int32_t> all;
     auto put = [&](const std::vector<int32_t> &v) {
         int o = (int)all.size();
@@ -954,8 +1012,8 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
         H.bbox_cap = 8 * (int64_t)nb + 64;
     }
     if (H.budget >= 0) {
-        H.q = H.pool;                                // carved from the reservation
-        H.part = reinterpret_cast<float *>(H.pool + need_q);
+        H.q = H.pool;  // carved from the reservation band by band (sym_run)
+        H.part = nullptr;
     } else {
         if (need_p > H.part_cap) {
             SPHX_CHECK(E.alloc((void **)&H.part, sizeof(float) * (need_p + 64)));
@@ -979,6 +1037,7 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     H.nblocks = nb;
     H.nseg = ns;
     H.part_len = plen;
+    H.bands = bands;
     return SPHX_OK;
 }
 
@@ -1001,20 +1060,35 @@ static int sym_run(Engine &E, hipStream_t st) {
     }
     k_tile_boxes<<<H.nblocks, 64, 0, st>>>(E.t, d + H.o_runb0, H.nblocks, SYM_B, E.s.m, H.bbox,
                                            blk ? E.s.bin : nullptr, E.blk_B, E.blk_t1, H.act);
+    const unsigned pg = (unsigned)((p1 - p0 + 255) / 256);
+    k_grav_sym_zero<<<pg, 256, 0, st>>>(p0, p1, E.s.gx, E.s.gy, E.s.gz);
     SPHX_HIP(hipGetLastError());
-    int64_t pl = H.part_len, ql = (int64_t)SYM_B * H.nseg;
-    SymTabs T{d + H.o_bbeg, d + H.o_bcnt, d + H.o_runb0, d + H.o_si + s0, d + H.o_sj0 + s0,
-              d + H.o_sj1 + s0, H.row_off, d + H.o_rj0, d + H.o_rns, d + H.o_rs0, H.bbox,
-              H.part, H.part + pl, H.part + 2 * pl, H.q, H.q + ql, H.q + 2 * ql, s0,
-              blk ? H.act : nullptr, E.s.run, E.t.r_pbeg, E.n};
     const double *eps = E.p.eps_from_h ? E.s.h : E.s.eps;
-    SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
-    k_grav_sym<<<(unsigned)(s1 - s0), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
-    SPHX_HIP(hipGetLastError());
-    SPHX_CHECK(E.tmark(st, PH_GRAVK, 1));
-    k_grav_sym_reduce<<<(unsigned)((p1 - p0 + 255) / 256), 256, 0, st>>>(T, E.p.G, p0, p1, E.s.gx,
-                                                                         E.s.gy, E.s.gz);
-    SPHX_HIP(hipGetLastError());
+    // the kernel-time phase spans the pair launches (and, with several bands,
+    // the reductions between them)
+    bool marked = false;
+    int left = 0;
+    for (const SymHost::Band &bd : H.bands) left += std::min(bd.seg_hi, s1) > std::max(bd.seg_lo, s0);
+    for (const SymHost::Band &bd : H.bands) {
+        const int a = std::max(bd.seg_lo, s0), b = std::min(bd.seg_hi, s1);
+        if (b <= a) continue;
+        const int64_t segs = bd.seg_hi - bd.seg_lo, ql = (int64_t)SYM_B * segs;
+        double *q = H.q;
+        float *part = H.budget >= 0 ? reinterpret_cast<float *>(H.pool + 3 * ql) : H.part;
+        const int64_t pl = bd.slots;
+        SymTabs T{d + H.o_bbeg, d + H.o_bcnt, d + H.o_runb0, d + H.o_si + a, d + H.o_sj0 + a,
+                  d + H.o_sj1 + a, H.row_off, d + H.o_rj0, d + H.o_rns, d + H.o_rs0, H.bbox,
+                  part, part + pl, part + 2 * pl, q, q + ql, q + 2 * ql, a, bd.seg_lo, bd.p_base,
+                  blk ? H.act : nullptr, E.s.run, E.t.r_pbeg, E.n};
+        if (!marked) SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
+        marked = true;
+        k_grav_sym<<<(unsigned)(b - a), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
+        SPHX_HIP(hipGetLastError());
+        if (--left == 0) SPHX_CHECK(E.tmark(st, PH_GRAVK, 1));
+        k_grav_sym_reduce<<<pg, 256, 0, st>>>(T, E.p.G, p0, p1, bd.row_lo, bd.row_hi, E.s.gx,
+                                               E.s.gy, E.s.gz);
+        SPHX_HIP(hipGetLastError());
+    }
     return SPHX_OK;
 }
 

@@ -12,6 +12,8 @@
 // getPressure_Adiabatic/getCsound_Adiabatic 766-789.
 #include "sphx_internal.h"
 
+#include <algorithm>
+
 namespace {
 
 // Walk the candidate buckets, in increasing order, of a group of particles of
@@ -297,15 +299,86 @@ __device__ __forceinline__ PairGeom pair_geom(double dx, double dy, double dz, d
     return g;
 }
 
+// The pair terms of the div/curl and hydro passes, shared by the candidate
+// walks and the neighbour-list passes so that both evaluate every pair with
+// the same instructions (the list passes are then bit-identical to the walks).
+__device__ __forceinline__ void divcurl_pair(double vxi, double vyi, double vzi, double vxj,
+                                             double vyj, double vzj, float mj, double dx,
+                                             double dy, double dz, double r2, double tt,
+                                             KahanF &kd, KahanF &kx, KahanF &ky, KahanF &kz) {
+    PairGeom g = pair_geom(dx, dy, dz, r2, tt);
+    float h5 = g.hinv * g.hinv;
+    h5 = h5 * h5 * g.hinv;
+    float F = SPHX_SIGMA_F * h5 * f_shape(g.q) * mj;
+    float vx = (float)(vxi - vxj), vy = (float)(vyi - vyj), vz = (float)(vzi - vzj);
+    kd.add(F * (vx * g.dx + vy * g.dy + vz * g.dz));
+    kx.add(F * (vy * g.dz - vz * g.dy));
+    ky.add(F * (vz * g.dx - vx * g.dz));
+    kz.add(F * (vx * g.dy - vy * g.dx));
+}
+
+struct HydroArgs {
+    float alpha, beta, eta2, w_eps;
+    int32_t av_form, balsara;
+};
+
+// Particle i's side of the hydro pair terms.
+struct HydroI {
+    double vx, vy, vz;
+    float rho, pr, c, f;  // rho, P / rho^2, sound speed, Balsara factor
+};
+
+__device__ __forceinline__ void hydro_pair(const HydroArgs &a, const HydroI &I, double vxj,
+                                           double vyj, double vzj, float mj, float rhoj,
+                                           float prj, float cj, float fj, double dx, double dy,
+                                           double dz, double r2, double tt, KahanF &kx,
+                                           KahanF &ky, KahanF &kz, KahanF &kdu) {
+    PairGeom g = pair_geom(dx, dy, dz, r2, tt);
+    float h5 = g.hinv * g.hinv;
+    h5 = h5 * h5 * g.hinv;
+    float F = SPHX_SIGMA_F * h5 * f_shape(g.q);
+    float vx = (float)(I.vx - vxj), vy = (float)(I.vy - vyj), vz = (float)(I.vz - vzj);
+    float vr = vx * g.dx + vy * g.dy + vz * g.dz;
+    float pij = 0.0f;
+    if (vr <= 0.0f) {
+        float rhob = 0.5f * (I.rho + rhoj);
+        float hij = 0.5f * (float)tt;
+        if (a.av_form == 0) {
+            float mu = hij * vr / (g.r2 + hij * hij * a.eta2);
+            pij = (-a.alpha * 0.5f * (I.c + cj) * mu + a.beta * mu * mu) / rhob;
+        } else {
+            float w = vr / (__builtin_amdgcn_sqrtf(g.r2) + a.w_eps);
+            pij = -0.5f * a.alpha * (I.c + cj - 3.0f * w) * w / rhob;
+        }
+        pij *= 0.5f * (I.f + fj);
+    }
+    float mF = mj * F;
+    float sc_ = mF * (I.pr + prj + pij);
+    kx.add(-sc_ * g.dx);
+    ky.add(-sc_ * g.dy);
+    kz.add(-sc_ * g.dz);
+    kdu.add(mF * (I.pr + 0.5f * pij) * vr);
+}
+
+__device__ __forceinline__ double balsara_f(double dv, double cv, double c, double h) {
+    double a = fabs(dv);
+    return a / (a + cv + 0.0001 * c / h);
+}
+
 struct DensArgs {
     double gamma;
     int32_t *nsym;        // symmetric neighbour count, nullable
     int64_t *counters;
+    // neighbour lists for the div/curl and hydro passes (all null: none written)
+    int32_t *lst;         // [bucket][entry < SPHX_NBR_MAX][lane]: accepted j in walk order
+    PairRec *rec;         // per particle: the j-side values of the pair passes
+    int32_t *ovf;         // per bucket: 1 when a lane's list did not fit
 };
 
 __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     __shared__ double sx[64], sy[64], sz[64], sh[64];
     __shared__ float sm[64];
+    __shared__ int32_t sj[64];
     int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
@@ -319,6 +392,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     acc.init();
     int nn = 0;
     int ncand = 0;
+    int32_t *lst = a.lst ? a.lst + (int64_t)b * SPHX_NBR_MAX * 64 + lane : nullptr;
     const int r = t.b_run[b];
     const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
     for (int g0 = 0; g0 < cnt;) {
@@ -355,6 +429,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                 sz[k.slot] = zj;
                 sh[k.slot] = hj;
                 sm[k.slot] = (float)s.m[j0 + lane];
+                sj[k.slot] = j0 + lane;
             }
             __syncthreads();
             // candidates four at a time, their LDS loads issued before the tests
@@ -363,6 +438,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h3 = g.hinv * g.hinv * g.hinv;
                     acc.add(sm[q] * (SPHX_SIGMA_F * h3 * w_shape(g.q)));
+                    if (lst && nn < SPHX_NBR_MAX) lst[(int64_t)nn * 64] = sj[q];
                     ++nn;
                 };
             int q0 = 0;
@@ -391,19 +467,52 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     if (ok) {
         double rho = acc.value();
         double u = s.u[i];
+        double P = (a.gamma - 1.0) * rho * u;
+        double c = sqrt(a.gamma * (a.gamma - 1.0) * u);
         s.rho[i] = rho;
-        s.P[i] = (a.gamma - 1.0) * rho * u;
-        s.c[i] = sqrt(a.gamma * (a.gamma - 1.0) * u);
+        s.P[i] = P;
+        s.c[i] = c;
         if (a.nsym) a.nsym[i] = nn;
+        if (a.rec) {
+            // the values the pair passes stage for a neighbour j, as they stage them
+            PairRec q;
+            q.x = xi; q.y = yi; q.z = zi; q.h = hi;
+            q.vx = s.vx[i]; q.vy = s.vy[i]; q.vz = s.vz[i];
+            q.m = (float)s.m[i];
+            q.rho = (float)rho;
+            q.pr = (float)(P / (rho * rho));
+            q.c = (float)c;
+            q.len = nn;
+            q.pad = 0;
+            a.rec[i] = q;
+        }
+    }
+    if (a.ovf) {
+        uint64_t o = __ballot(ok && nn > SPHX_NBR_MAX);
+        if (lane == 0) {
+            a.ovf[b] = o ? 1 : 0;
+            if (o && a.counters) atomicAdd((unsigned long long *)&a.counters[CNT_NBR_OVF], 1ull);
+        }
     }
     if (lane == 0 && a.counters)
         atomicAdd((unsigned long long *)&a.counters[CNT_CAND], (unsigned long long)ncand);
 }
 
-__global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
+__device__ __forceinline__ void divcurl_store(const Soa &s, int i, bool ok, const KahanF &kd,
+                                              const KahanF &kx, const KahanF &ky,
+                                              const KahanF &kz) {
+    if (ok) {
+        double ir = 1.0 / s.rho[i];
+        s.divv[i] = -kd.value() * ir;
+        double cx = kx.value() * ir, cy = ky.value() * ir, cz = kz.value() * ir;
+        s.curlv[i] = sqrt(cx * cx + cy * cy + cz * cz);
+    }
+}
+
+// div/curl of the particles of bucket b by the candidate walk.
+__device__ __forceinline__ void divcurl_bucket(const Tables &t, const Soa &s, int b) {
     __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
     __shared__ float sm[64];
-    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
@@ -452,16 +561,8 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
             // candidates four at a time, their LDS loads issued before the tests
             // (each pair is met in the same order as one at a time)
             auto pair = [&](int q, double dx, double dy, double dz, double r2, double tt) {
-                    PairGeom g = pair_geom(dx, dy, dz, r2, tt);
-                    float h5 = g.hinv * g.hinv;
-                    h5 = h5 * h5 * g.hinv;
-                    float F = SPHX_SIGMA_F * h5 * f_shape(g.q) * sm[q];
-                    float vx = (float)(vxi - svx[q]), vy = (float)(vyi - svy[q]),
-                          vz = (float)(vzi - svz[q]);
-                    kd.add(F * (vx * g.dx + vy * g.dy + vz * g.dz));
-                    kx.add(F * (vy * g.dz - vz * g.dy));
-                    ky.add(F * (vz * g.dx - vx * g.dz));
-                    kz.add(F * (vx * g.dy - vy * g.dx));
+                    divcurl_pair(vxi, vyi, vzi, svx[q], svy[q], svz[q], sm[q], dx, dy, dz, r2,
+                                 tt, kd, kx, ky, kz);
                 };
             int q0 = 0;
             for (; q0 + 4 <= k.n; q0 += 4) {
@@ -486,39 +587,32 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
             }
         });
     }
-    if (ok) {
-        double ir = 1.0 / s.rho[i];
-        s.divv[i] = -kd.value() * ir;
-        double cx = kx.value() * ir, cy = ky.value() * ir, cz = kz.value() * ir;
-        s.curlv[i] = sqrt(cx * cx + cy * cy + cz * cz);
-    }
+    divcurl_store(s, i, ok, kd, kx, ky, kz);
 }
 
-struct HydroArgs {
-    float alpha, beta, eta2, w_eps;
-    int32_t av_form, balsara;
-};
-
-__device__ __forceinline__ double balsara_f(double dv, double cv, double c, double h) {
-    double a = fabs(dv);
-    return a / (a + cv + 0.0001 * c / h);
+__global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
+    divcurl_bucket(t, s, t.boff + xcd_block(blockIdx.x, gridDim.x));
 }
 
-__global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
+// Hydro acceleration and du/dt of the particles of bucket b by the candidate walk.
+__device__ __forceinline__ void hydro_bucket(const Tables &t, const Soa &s, const HydroArgs &a,
+                                             int b) {
     __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
     __shared__ float sm[64], srho[64], spr[64], sc[64], sf[64];
-    int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
     int lane = threadIdx.x;
     int cnt = t.b_cnt[b];
     int i = t.b_beg[b] + lane;
     bool ok = lane < cnt;
     double xi = ok ? s.x[i] : 0.0, yi = ok ? s.y[i] : 0.0, zi = ok ? s.z[i] : 0.0;
     double hi = ok ? s.h[i] : 0.0;
-    double vxi = ok ? s.vx[i] : 0.0, vyi = ok ? s.vy[i] : 0.0, vzi = ok ? s.vz[i] : 0.0;
-    float rhoi = ok ? (float)s.rho[i] : 1.0f;
-    float pri = ok ? (float)(s.P[i] / (s.rho[i] * s.rho[i])) : 0.0f;
-    float ci = ok ? (float)s.c[i] : 0.0f;
-    float fi = (ok && a.balsara) ? (float)balsara_f(s.divv[i], s.curlv[i], s.c[i], s.h[i]) : 1.0f;
+    HydroI I;
+    I.vx = ok ? s.vx[i] : 0.0;
+    I.vy = ok ? s.vy[i] : 0.0;
+    I.vz = ok ? s.vz[i] : 0.0;
+    I.rho = ok ? (float)s.rho[i] : 1.0f;
+    I.pr = ok ? (float)(s.P[i] / (s.rho[i] * s.rho[i])) : 0.0f;
+    I.c = ok ? (float)s.c[i] : 0.0f;
+    I.f = (ok && a.balsara) ? (float)balsara_f(s.divv[i], s.curlv[i], s.c[i], s.h[i]) : 1.0f;
     double hI = t.bhmax[b];
     const double ib[6] = {t.bxl[b], t.bxh[b], t.byl[b], t.byh[b], t.bzl[b], t.bzh[b]};
     KahanF kx, ky, kz, kdu;
@@ -565,32 +659,8 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
             // candidates four at a time, their LDS loads issued before the tests
             // (each pair is met in the same order as one at a time)
             auto pair = [&](int q, double dx, double dy, double dz, double r2, double tt) {
-                    PairGeom g = pair_geom(dx, dy, dz, r2, tt);
-                    float h5 = g.hinv * g.hinv;
-                    h5 = h5 * h5 * g.hinv;
-                    float F = SPHX_SIGMA_F * h5 * f_shape(g.q);
-                    float vx = (float)(vxi - svx[q]), vy = (float)(vyi - svy[q]),
-                          vz = (float)(vzi - svz[q]);
-                    float vr = vx * g.dx + vy * g.dy + vz * g.dz;
-                    float pij = 0.0f;
-                    if (vr <= 0.0f) {
-                        float rhob = 0.5f * (rhoi + srho[q]);
-                        float hij = 0.5f * (float)tt;
-                        if (a.av_form == 0) {
-                            float mu = hij * vr / (g.r2 + hij * hij * a.eta2);
-                            pij = (-a.alpha * 0.5f * (ci + sc[q]) * mu + a.beta * mu * mu) / rhob;
-                        } else {
-                            float w = vr / (__builtin_amdgcn_sqrtf(g.r2) + a.w_eps);
-                            pij = -0.5f * a.alpha * (ci + sc[q] - 3.0f * w) * w / rhob;
-                        }
-                        pij *= 0.5f * (fi + sf[q]);
-                    }
-                    float mF = sm[q] * F;
-                    float sc_ = mF * (pri + spr[q] + pij);
-                    kx.add(-sc_ * g.dx);
-                    ky.add(-sc_ * g.dy);
-                    kz.add(-sc_ * g.dz);
-                    kdu.add(mF * (pri + 0.5f * pij) * vr);
+                    hydro_pair(a, I, svx[q], svy[q], svz[q], sm[q], srho[q], spr[q], sc[q],
+                               sf[q], dx, dy, dz, r2, tt, kx, ky, kz, kdu);
                 };
             int q0 = 0;
             for (; q0 + 4 <= k.n; q0 += 4) {
@@ -621,6 +691,164 @@ __global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
         s.az[i] = kz.value();
         s.du[i] = kdu.value();
     }
+}
+
+__global__ __launch_bounds__(64) void k_hydro(Tables t, Soa s, HydroArgs a) {
+    hydro_bucket(t, s, a, t.boff + xcd_block(blockIdx.x, gridDim.x));
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour-list passes.  The density walk records, for every particle, the
+// indices of the particles it accepted (r_ij^2 <= (h_i + h_j)^2) in the order
+// it met them, and a record of the values the pair passes stage for a
+// neighbour.  Positions and h do not change between the density pass and the
+// hydro pass of a step, and div/curl and hydro accept exactly the pairs density
+// accepts, so iterating a particle's list visits the same pairs in the same
+// order as the candidate walk: with the shared pair terms the sums are
+// bit-identical, without the walk's candidate tests or its wave-wide pair
+// evaluations for candidates that only some lanes accept.  Buckets where a list
+// overflowed (more than SPHX_NBR_MAX neighbours) are left to the walk.
+// ---------------------------------------------------------------------------
+struct ListArgs {
+    const int32_t *lst;
+    const PairRec *rec;
+    const int32_t *ovf;
+    const float *fbal;   // Balsara factor per particle (hydro with the switch on)
+};
+
+__device__ __forceinline__ double rec_dbl(uint32_t lo, uint32_t hi) {
+    return __hiloint2double((int)hi, (int)lo);
+}
+
+// The position, h, velocity and mass of record j (four 16-byte loads).
+struct RecA {
+    double x, y, z, h, vx, vy, vz;
+    float m, rho;
+};
+__device__ __forceinline__ RecA load_rec_a(const PairRec *rec, int j) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(rec + j);
+    uint4 a = q[0], b = q[1], c = q[2], d = q[3];
+    RecA o;
+    o.x = rec_dbl(a.x, a.y); o.y = rec_dbl(a.z, a.w);
+    o.z = rec_dbl(b.x, b.y); o.h = rec_dbl(b.z, b.w);
+    o.vx = rec_dbl(c.x, c.y); o.vy = rec_dbl(c.z, c.w);
+    o.vz = rec_dbl(d.x, d.y);
+    o.m = __uint_as_float(d.z); o.rho = __uint_as_float(d.w);
+    return o;
+}
+// P / rho^2, sound speed and list length of record j (the fifth 16 bytes).
+__device__ __forceinline__ uint4 load_rec_b(const PairRec *rec, int j) {
+    return reinterpret_cast<const uint4 *>(rec + j)[4];
+}
+
+__global__ __launch_bounds__(64) void k_divcurl_list(Tables t, Soa s, ListArgs L) {
+    const int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
+    if (L.ovf[b]) return;  // the walk launch takes this bucket
+    const int lane = threadIdx.x;
+    const int cnt = t.b_cnt[b];
+    const int i = t.b_beg[b] + lane;
+    const bool ok = lane < cnt;
+    RecA me{};
+    int len = 0;
+    if (ok) {
+        me = load_rec_a(L.rec, i);
+        len = (int)load_rec_b(L.rec, i).z;
+    }
+    const int maxlen = wave_max_i(len);
+    const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64 + lane;
+    KahanF kd, kx, ky, kz;
+    kd.init(); kx.init(); ky.init(); kz.init();
+    for (int k = 0; k < maxlen; ++k) {
+        if (k < len) {
+            const RecA o = load_rec_a(L.rec, lp[(int64_t)k * 64]);
+            double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+            double r2 = sphx_r2(dx, dy, dz);
+            double tt = me.h + o.h;
+            divcurl_pair(me.vx, me.vy, me.vz, o.vx, o.vy, o.vz, o.m, dx, dy, dz, r2, tt, kd, kx,
+                         ky, kz);
+        }
+    }
+    divcurl_store(s, i, ok, kd, kx, ky, kz);
+}
+
+__global__ __launch_bounds__(64) void k_hydro_list(Tables t, Soa s, HydroArgs a, ListArgs L) {
+    const int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
+    if (L.ovf[b]) return;  // the walk launch takes this bucket
+    const int lane = threadIdx.x;
+    const int cnt = t.b_cnt[b];
+    const int i = t.b_beg[b] + lane;
+    const bool ok = lane < cnt;
+    RecA me{};
+    HydroI I;
+    I.vx = I.vy = I.vz = 0.0;
+    I.rho = 1.0f; I.pr = 0.0f; I.c = 0.0f; I.f = 1.0f;
+    int len = 0;
+    if (ok) {
+        me = load_rec_a(L.rec, i);
+        uint4 e = load_rec_b(L.rec, i);
+        I.vx = me.vx; I.vy = me.vy; I.vz = me.vz;
+        I.rho = me.rho;
+        I.pr = __uint_as_float(e.x);
+        I.c = __uint_as_float(e.y);
+        if (a.balsara) I.f = L.fbal[i];
+        len = (int)e.z;
+    }
+    const int maxlen = wave_max_i(len);
+    const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64 + lane;
+    KahanF kx, ky, kz, kdu;
+    kx.init(); ky.init(); kz.init(); kdu.init();
+    for (int k = 0; k < maxlen; ++k) {
+        if (k < len) {
+            const int j = lp[(int64_t)k * 64];
+            const RecA o = load_rec_a(L.rec, j);
+            const uint4 e = load_rec_b(L.rec, j);
+            const float fj = a.balsara ? L.fbal[j] : 1.0f;
+            double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+            double r2 = sphx_r2(dx, dy, dz);
+            double tt = me.h + o.h;
+            hydro_pair(a, I, o.vx, o.vy, o.vz, o.m, o.rho, __uint_as_float(e.x),
+                       __uint_as_float(e.y), fj, dx, dy, dz, r2, tt, kx, ky, kz, kdu);
+        }
+    }
+    if (ok) {
+        s.ax[i] = kx.value();
+        s.ay[i] = ky.value();
+        s.az[i] = kz.value();
+        s.du[i] = kdu.value();
+    }
+}
+
+// The walk for the buckets whose lists overflowed: each wave scans 64 flags
+// at a time and walks the flagged buckets of its share.
+__global__ __launch_bounds__(64) void k_divcurl_ovf(Tables t, Soa s, const int32_t *ovf, int nb) {
+    const int lane = threadIdx.x;
+    for (int base = blockIdx.x * 64; base < nb; base += gridDim.x * 64) {
+        uint64_t m = __ballot(base + lane < nb && ovf[t.boff + base + lane] != 0);
+        while (m) {
+            int q = __builtin_ctzll(m);
+            m &= m - 1;
+            divcurl_bucket(t, s, t.boff + base + q);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_hydro_ovf(Tables t, Soa s, HydroArgs a,
+                                                  const int32_t *ovf, int nb) {
+    const int lane = threadIdx.x;
+    for (int base = blockIdx.x * 64; base < nb; base += gridDim.x * 64) {
+        uint64_t m = __ballot(base + lane < nb && ovf[t.boff + base + lane] != 0);
+        while (m) {
+            int q = __builtin_ctzll(m);
+            m &= m - 1;
+            hydro_bucket(t, s, a, t.boff + base + q);
+        }
+    }
+}
+
+// Balsara factor of every particle of [p0, p1), as the walk computes it.
+__global__ void k_fbal(Soa s, int64_t p0, int64_t p1, float *__restrict__ fbal) {
+    int64_t i = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < p1) fbal[i] = (float)balsara_f(s.divv[i], s.curlv[i], s.c[i], s.h[i]);
 }
 
 // kNN smoothing length (do_smoothingX, LIBSX:762-783): h = 0.5 sqrt(v) with v
@@ -839,26 +1067,62 @@ int Engine::smoothing(hipStream_t st, bool predictor) {
     return SPHX_OK;
 }
 
-int Engine::density_eos(hipStream_t st) {
+bool Engine::lists_ready() const {
+    return use_lists && nbr && prec && fbal && nbr_ovf && nbr_bcap >= t.nb;
+}
+
+int Engine::density_eos(hipStream_t st, bool lists) {
     if (n == 0 || nb_launch() == 0) return SPHX_OK;
-    DensArgs a{p.gamma, tmpi, counters};
+    DensArgs a{p.gamma, tmpi, counters, nullptr, nullptr, nullptr};
+    if (lists) {
+        if (!lists_ready()) {
+            sphx_set_error("neighbour lists requested but not allocated");
+            return SPHX_ERR_STATE;
+        }
+        a.lst = nbr;
+        a.rec = prec;
+        a.ovf = nbr_ovf;
+    }
     k_density<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
 
-int Engine::divcurl(hipStream_t st) {
+// Launch shape of the overflow walks: enough waves to spread the flag scan,
+// never more than one per 64 buckets.
+static unsigned ovf_grid(unsigned nb) { return std::min(256u, (nb + 63) / 64); }
+
+int Engine::divcurl(hipStream_t st, bool lists) {
     if (n == 0 || nb_launch() == 0) return SPHX_OK;
-    k_divcurl<<<nb_launch(), 64, 0, st>>>(tview(), s);
+    if (lists) {
+        ListArgs L{nbr, prec, nbr_ovf, fbal};
+        k_divcurl_list<<<nb_launch(), 64, 0, st>>>(tview(), s, L);
+        k_divcurl_ovf<<<ovf_grid(nb_launch()), 64, 0, st>>>(tview(), s, nbr_ovf,
+                                                           (int)nb_launch());
+        if (p.balsara) {
+            int64_t p0 = rrun >= 0 ? rp0 : 0, p1 = rrun >= 0 ? rp1 : n;
+            if (p1 > p0)
+                k_fbal<<<(unsigned)((p1 - p0 + 255) / 256), 256, 0, st>>>(s, p0, p1, fbal);
+        }
+    } else {
+        k_divcurl<<<nb_launch(), 64, 0, st>>>(tview(), s);
+    }
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
 
-int Engine::hydro(hipStream_t st) {
+int Engine::hydro(hipStream_t st, bool lists) {
     if (n == 0 || nb_launch() == 0) return SPHX_OK;
     HydroArgs a{(float)p.alpha, (float)p.beta, (float)(p.eta * p.eta), (float)p.av_w_eps,
                 p.av_form, p.balsara};
-    k_hydro<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
+    if (lists) {
+        ListArgs L{nbr, prec, nbr_ovf, fbal};
+        k_hydro_list<<<nb_launch(), 64, 0, st>>>(tview(), s, a, L);
+        k_hydro_ovf<<<ovf_grid(nb_launch()), 64, 0, st>>>(tview(), s, a, nbr_ovf,
+                                                         (int)nb_launch());
+    } else {
+        k_hydro<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
+    }
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

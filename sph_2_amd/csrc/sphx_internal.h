@@ -23,6 +23,20 @@
 
 #define SPHX_BUCKET 64
 #define SPHX_SUPER 64
+#define SPHX_NBR_MAX 128  // entries of a particle's neighbour list (more: its bucket walks)
+
+// What the pair passes stage for a neighbour j, written by the density pass
+// for the neighbour-list passes: FP64 position, h and velocity, and the FP32
+// values the walks derive from the FP64 state (m, rho, P / rho^2, c), plus the
+// length of j's own list.  80 bytes, read as five 16-byte loads.
+struct alignas(16) PairRec {
+    double x, y, z, h;
+    double vx, vy, vz;
+    float m, rho;
+    float pr, c;
+    int32_t len, pad;
+};
+static_assert(sizeof(PairRec) == 80, "PairRec layout");
 
 // ---------------------------------------------------------------------------
 // Error plumbing
@@ -167,7 +181,8 @@ enum {
     CNT_BIN_FLOOR,
     CNT_COOL_ITERS,   // bisection iterations of the implicit cooling solve
     CNT_COOL_BISECT,  // particles whose cooling solve bisected (not an early exit)
-    CNT_SLOTS = 10
+    CNT_NBR_OVF,      // buckets whose neighbour list overflowed (walked instead)
+    CNT_SLOTS = 11
 };
 
 // Step phases timed with HIP events when Engine::timing is on.
@@ -255,6 +270,7 @@ struct GravHost {
 };
 void sphx_sym_host_free(struct SymHost *h);
 int64_t sphx_sym_run_bytes(int64_t n, int S);
+int64_t sphx_sym_min_bytes(int64_t n, int S);
 int sphx_set_cell_level(struct Engine &E, int32_t bias);
 int sphx_pair_probe_launch(int64_t blocks, int iters, float *out, hipStream_t st);
 int sphx_sym_reserve(struct Engine &E, int S, int64_t budget_bytes);
@@ -307,6 +323,16 @@ struct Engine {
     double *tab_d = nullptr;     // box arrays
     int64_t tab_d_cap = 0;
     Tables t{};
+    // neighbour lists of the density pass for the div/curl and hydro passes
+    // (plans and batches; allocated with the capacity when want_lists is set)
+    bool want_lists = false;
+    bool use_lists = true;      // tuning knob: the walks give the same bits
+    int32_t *nbr = nullptr;     // nbr_bcap buckets x SPHX_NBR_MAX x 64
+    int32_t *nbr_ovf = nullptr; // per bucket
+    int64_t nbr_bcap = 0;
+    PairRec *prec = nullptr;    // per particle of capacity
+    float *fbal = nullptr;      // per particle of capacity
+    bool lists_ready() const;
     int32_t *cand_off = nullptr, *cand_cnt = nullptr, *cand = nullptr;
     int64_t cand_cap = 0;
     int64_t cand_total = 0;
@@ -389,10 +415,10 @@ struct Engine {
     int build_boxes(hipStream_t st, bool with_h);
     int smoothing(hipStream_t st, bool predictor);
     int build_candidates(hipStream_t st);
-    int density_eos(hipStream_t st);
+    int density_eos(hipStream_t st, bool lists = false);
     int gravity(hipStream_t st, bool with_pot);
-    int divcurl(hipStream_t st);
-    int hydro(hipStream_t st);
+    int divcurl(hipStream_t st, bool lists = false);
+    int hydro(hipStream_t st, bool lists = false);
     int forces(hipStream_t st, bool predictor, bool with_pot);
 
     // plain FP64 baseline pieces

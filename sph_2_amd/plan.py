@@ -67,6 +67,11 @@ class SphPlan:
         check(_lib.load().sphx_plan_set_cell_level(self._h, int(bias)), "set_cell_level")
         self._cell_set = True
 
+    def set_neighbor_lists(self, on: bool = True):
+        """Neighbour lists for the div/curl and hydro passes (default on); the
+        candidate walks when off.  Results are the same bits either way."""
+        check(_lib.load().sphx_plan_set_neighbor_lists(self._h, int(on)), "set_neighbor_lists")
+
     def load(self, pos, vel, u, m, h, eps=None):
         t = _lib.torch()
         _lib.check_run_arrays(pos, vel, u, m, h, eps)
@@ -84,6 +89,32 @@ class SphPlan:
         check(_lib.load().sphx_plan_load(self._h, self.n, *[ptr(b) for b in self._bufs], ptr(de),
                                          _lib.stream_handle()), "sphx_plan_load")
         self._eps = de
+
+    def bind(self, pos, vel, u, m, h, eps=None):
+        """Load from caller-owned torch tensors on the device (float64,
+        contiguous; pos and vel (n, 3), u and h (n,)) and keep them as the
+        run's state: every init / step reads them first and writes the state
+        back into them (caller order) at its end."""
+        t = _lib.torch()
+        arrs = {"pos": pos, "vel": vel, "u": u, "m": m, "h": h}
+        for k, a in arrs.items():
+            if not (isinstance(a, t.Tensor) and a.is_cuda and a.dtype == t.float64
+                    and a.is_contiguous()):
+                raise TypeError(f"bind needs contiguous float64 device tensors ({k})")
+        _lib.check_run_arrays(pos, vel, u, m, h, eps)
+        if self.mode == _lib.MIXED and not getattr(self, "_gcfg_set", False):
+            from .autotune import choose_gravity_config
+            self.set_gravity_config(*choose_gravity_config(len(m)))
+        if self.mode == _lib.MIXED and not getattr(self, "_cell_set", False):
+            from .autotune import choose_cell_level
+            check(_lib.load().sphx_plan_set_cell_level(self._h, choose_cell_level(len(m))),
+                  "set_cell_level")
+        de = dev(np.asarray(eps, np.float64), t.float64) if eps is not None else None
+        self._bufs = [pos, vel, u, m, h]
+        self._eps = de
+        self.n = int(m.shape[0])
+        check(_lib.load().sphx_plan_bind(self._h, self.n, *[ptr(b) for b in self._bufs], ptr(de),
+                                         _lib.stream_handle()), "sphx_plan_bind")
 
     def init(self):
         check(_lib.load().sphx_plan_init(self._h, _lib.stream_handle()), "sphx_plan_init")

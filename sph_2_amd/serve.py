@@ -86,7 +86,13 @@ class Batch:
 
     @staticmethod
     def symmetric_bytes(n: int, block_pairs: int) -> int:
+        """All partial sums of one run of n particles (summed in row bands)."""
         return int(_lib.load().sphx_gravity_symmetric_bytes(int(n), int(block_pairs)))
+
+    @staticmethod
+    def symmetric_min_bytes(n: int, block_pairs: int) -> int:
+        """The least reserved memory with which a run of n particles is admitted."""
+        return int(_lib.load().sphx_gravity_symmetric_min_bytes(int(n), int(block_pairs)))
 
     def set_chimes_table(self, table):
         """Attach a libsx.ChimesTable for runs admitted with cooling="chimes"."""
@@ -226,9 +232,10 @@ class Server:
     """Admit by HBM budget, step every live run together, retire when done."""
 
     def __init__(self, hbm_budget_bytes: int, max_runs: int = 512, graph: bool = True,
-                 cooling: CoolingTable | None = None, symmetric: tuple | None = (4, 0.5)):
-        """symmetric: (block pairs per wave, share of the budget for the
-        symmetric gravity sum's partials), or None for the tiled sum."""
+                 cooling: CoolingTable | None = None, symmetric: tuple | None = (4, 0.1)):
+        """symmetric: (block pairs per wave, share of the budget reserved for
+        the symmetric gravity sum's partials, which it streams in row bands),
+        or None for the tiled sum."""
         share = symmetric[1] if symmetric else 0.0
         self.capacity = int(hbm_budget_bytes * (1.0 - share) // Batch.bytes_per_particle())
         self.batch = Batch(self.capacity, max_runs, cooling=cooling or default_cooling(),
@@ -253,7 +260,7 @@ class Server:
         for r in requests:
             if r.payload is None:
                 r.payload = build_payload(r)
-            if r.n > self.capacity or (self.symmetric and Batch.symmetric_bytes(
+            if r.n > self.capacity or (self.symmetric and Batch.symmetric_min_bytes(
                     r.n, self.symmetric[0]) > self.sym_budget):
                 raise ValueError(f"request {r.rid} (N={r.n}) exceeds the whole budget")
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
@@ -267,6 +274,7 @@ class Server:
         psteps = 0
         t_admit = t_step = t_retire = 0.0
         inflight = collections.deque()      # events of queued steps
+        step_events = []                    # (start, end) of every batch step on the stream
         while pending or live:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
@@ -283,10 +291,13 @@ class Server:
                 time.sleep(max(0.0, min(0.01, nxt - now)))
                 continue
             tb = time.perf_counter()
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
             self.batch.step(1)
-            ev = torch.cuda.Event()
+            ev = torch.cuda.Event(enable_timing=True)
             ev.record()
             inflight.append(ev)
+            step_events.append((ev0, ev))
             while len(inflight) >= max_in_flight:
                 inflight.popleft().synchronize()
             t_step += time.perf_counter() - tb
@@ -316,6 +327,7 @@ class Server:
             busy += time.perf_counter() - tb
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        gpu_step_s = 1e-3 * sum(a.elapsed_time(b) for a, b in step_events)
         lat = np.array([d["completed"] - d["arrival"] for d in done.values()])
         queue = np.array([d["admitted"] - d["arrival"] for d in done.values()])
         rep = {"runs": len(done), "wall_s": wall, "loop_steps": loops,
@@ -326,6 +338,7 @@ class Server:
                "p99_queue_s": float(np.percentile(queue, 99)),
                "capacity_particles": self.capacity,
                "max_steps_in_flight": max_in_flight,
+               "gpu_step_s": gpu_step_s, "gpu_step_frac": gpu_step_s / wall,
                "host_s": {"admit": t_admit, "step": t_step, "retire": t_retire}}
         if keep_results:
             rep["results"] = results

@@ -114,10 +114,10 @@ def test_plan_graph_matches_eager(gpu):
 
 
 def test_serving_loop_small(gpu):
-    from sph_2_amd.serve import Server, make_stream
+    from sph_2_amd.serve import Batch, Server, make_stream
 
     reqs = make_stream(12, n_min=2048, n_max=16384, rate=50.0, steps=20, seed=3)
-    srv = Server(hbm_budget_bytes=40000 * 300, max_runs=8, graph=True)
+    srv = Server(hbm_budget_bytes=40000 * Batch.bytes_per_particle(), max_runs=8, graph=True)
     rep = srv.run(reqs, keep_results=True)
     assert rep["runs"] == 12
     assert rep["p99_latency_s"] >= rep["p50_latency_s"] > 0
@@ -136,12 +136,15 @@ def test_serving_loop_small(gpu):
 
 def test_symmetric_gravity_alone_equals_in_the_mix(gpu, runs):
     """With the symmetric gravity sum a run's result still does not depend on
-    the other runs; admissions beyond the reserved partial-sum memory are
+    the other runs; its partial sums stream through the reserved memory in row
+    bands, so the result does not depend on the reservation either (one band
+    or one row per band), and only a run whose largest row does not fit is
     refused."""
     from sph_2_amd.serve import Batch
 
     a, b, c, cool = runs
-    need = Batch.symmetric_bytes(a.n, 2)
+    need = Batch.symmetric_min_bytes(a.n, 2)
+    assert 0 < need < Batch.symmetric_bytes(a.n, 2)
     outs = []
     for mixed in (False, True):
         bt = Batch(60000, 8, cooling=cool)
@@ -158,8 +161,18 @@ def test_symmetric_gravity_alone_equals_in_the_mix(gpu, runs):
         outs.append(bt.retire(ra))
         bt.close()
     _same(outs[0], outs[1])
-    small = Batch(60000, 8)
-    small.set_gravity_symmetric(2, int(need * 1.5))
-    assert small.admit(a, 1e-4) is not None
-    assert small.admit(a, 1e-4) is None            # the second copy does not fit
-    small.close()
+    tiny = Batch(60000, 8)
+    tiny.set_gravity_symmetric(2, need - 1)
+    assert tiny.admit(a, 1e-4) is None             # its first row does not fit
+    tiny.close()
+    banded = []
+    for budget in (need, 64 * 2 ** 20):            # a band per row / one band
+        bt = Batch(60000, 8)
+        bt.set_gravity_symmetric(2, budget)
+        ra = bt.admit(a, 2e-4)
+        assert ra is not None
+        assert bt.admit(c, 1e-4) is not None       # another run streams through too
+        bt.step(3)
+        banded.append(bt.retire(ra))
+        bt.close()
+    _same(banded[0], banded[1])

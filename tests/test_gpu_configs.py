@@ -294,3 +294,29 @@ def test_sph_passes_1m_rows(gpu, evr1m):
     print("SPH 1M rows: divcurl", rel_rms(dv, dvo), "acc", rel_rms(acc, acc_o),
           "du", rel_rms(du, du_o))
     assert rel_rms(acc, acc_o) < 1e-5 and rel_rms(du, du_o) < 1e-5
+
+
+def test_direct_gravity_4m_in_row_bands(gpu):
+    """The symmetric direct sum at N = 4,194,304 (the Plummer sphere of the
+    Barnes-Hut configuration): its 206 GB of partial sums stream through 16 GiB
+    row bands.  Against the FP64 oracle on 512 sampled rows, and Newton's third
+    law over all particles (sum_i m_i a_i = 0 to round-off)."""
+    from sph_2_amd import _lib, libsx
+    from sph_2_amd.autotune import _default
+
+    n = 1 << 22
+    e = ic.plummer(n, seed=0, eps=0.01)
+    rows = np.sort(np.random.default_rng(3).choice(n, 512, replace=False))
+    ref = oracle.gravity(e.pos, e.m, e.eps, 1.0, rows=rows)
+    ctx = libsx._context(0)
+    try:
+        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 8, 512, 16), "config")
+        acc = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.eps)
+    finally:
+        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(4096)), "config")
+    err = rel_rms(acc[rows], ref)
+    scale = float((e.m[:, None] * np.abs(acc)).sum())
+    net = float(np.abs((e.m[:, None] * acc).sum(0)).max())
+    print(f"direct gravity 4M: rel RMS {err:.2e} on {len(rows)} rows, net force {net / scale:.2e}")
+    assert err < 1e-5
+    assert net < 1e-6 * scale

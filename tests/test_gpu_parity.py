@@ -316,6 +316,40 @@ def test_plan_one_step(gpu):
         pl.close()
 
 
+def test_plan_bind_keeps_the_callers_buffers(gpu):
+    """sphx_plan_bind: the caller's device tensors hold the state after every
+    call, in the caller's order; the run is bit-identical to a loaded plan's;
+    an edit of a bound buffer between calls is what the next step starts from."""
+    torch = gpu
+    from sph_2_amd.plan import SphPlan
+
+    e = ic.evrard(6000, seed=8)
+    bufs = {k: torch.tensor(getattr(e, k), dtype=torch.float64, device="cuda")
+            for k in ("pos", "vel", "u", "m", "h")}
+    pb = SphPlan(e.n)
+    pb.bind(bufs["pos"], bufs["vel"], bufs["u"], bufs["m"], bufs["h"])
+    pb.init()
+    assert np.array_equal(bufs["h"].cpu().numpy(), pb.state(("h",))["h"])
+    pb.step(2, 1e-4)
+    pl = SphPlan(e.n)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h)
+    pl.init()
+    pl.step(2, 1e-4)
+    a, b = pb.state(), pl.state()
+    for f in a:
+        assert np.array_equal(a[f], b[f]), f
+    for f in ("pos", "vel", "u", "h"):
+        assert np.array_equal(bufs[f].cpu().numpy(), a[f]), f
+    bufs["u"][:100] *= 2.0                       # edited between calls
+    pb.step(1, 1e-4)
+    pl.step(1, 1e-4)
+    a2, b2 = pb.state(), pl.state()
+    assert np.all(a2["u"][:100] > 1.5 * b2["u"][:100])
+    assert np.array_equal(bufs["u"].cpu().numpy(), a2["u"])
+    pb.close()
+    pl.close()
+
+
 def test_diagnostics(gpu, evr):
     from sph_2_amd import libsx
 
@@ -362,6 +396,45 @@ def test_cell_level_does_not_change_results(gpu):
     pl.close()
     for f in ref:
         assert np.array_equal(ref[f], later[f]), f
+
+
+@pytest.mark.parametrize("case", ["evrard", "cloud_balsara", "overflow"])
+def test_neighbor_lists_do_not_change_results(gpu, case):
+    """div/curl and hydro over the density pass's neighbour lists give the same
+    bits as the candidate walks: Evrard; a turbulent cloud with the Gadget
+    viscosity and the Balsara switch (per-particle factors for the lists); and
+    buckets whose lists overflow (h held at three times its converged value for
+    the particles with x > 0.3 by an iteration cap of one count, so their
+    neighbour counts pass 128): those buckets walk, the others iterate their
+    lists, inside the same step."""
+    from sph_2_amd.plan import SphPlan
+
+    kw = {}
+    if case == "evrard":
+        e = ic.evrard(32768, seed=4)
+    elif case == "cloud_balsara":
+        e = ic.turbulent_cloud(16384, seed=2, ng=32)
+        kw = dict(av_form=1, balsara=1)
+    else:
+        e = ic.evrard(16384, seed=6)
+        e.h = np.where(e.pos[:, 0] > 0.3, 3.0 * e.h, e.h)
+        kw = dict(max_iter=1)
+    outs, cnts = [], []
+    for lists in (True, False):
+        pl = SphPlan(e.n, **kw)
+        pl.set_neighbor_lists(lists)
+        pl.load(e.pos, e.vel, e.u, e.m, e.h)
+        pl.init()
+        pl.step(3, 1e-4)
+        outs.append(pl.state())
+        cnts.append(pl.counters())
+        pl.close()
+    for f in outs[0]:
+        assert np.array_equal(outs[0][f], outs[1][f]), f
+    if case == "overflow":
+        assert 0 < cnts[0]["nbr_overflow"] < cnts[0]["buckets"], cnts[0]
+    else:
+        assert cnts[0]["nbr_overflow"] == 0, cnts[0]
 
 
 def test_symmetric_gravity_conserves_momentum_at_scale(gpu):
