@@ -114,6 +114,26 @@ typedef struct sphx_chimes_table {
     double unit_length_cm, unit_rho_cgs, unit_u_cgs, unit_time_s;
 } sphx_chimes_table;
 
+/* The analytic H/He cooling network of GPU_sph/Cooling_v1/photolibs3.py:
+ * 140-543 (photoionisation equilibrium of H and He under a UV background,
+ * Cen 1992 collisional rates, the T(u) fixed point, and DoCooling_h's
+ * backward-Euler bracket x/÷1.1 and bisection), in FP64.  The background
+ * J_nu = J21 1e-21 (nu_H0 / nu) erg s^-1 cm^-2 Hz^-1 sr^-1 (the reference's
+ * Jcoeff: 1e-5 in photolibs3.py, 50 in the GPU drivers' myPhotoLibsGPU.h:
+ * 166-296) is integrated over a frequency grid from each ionisation edge to
+ * 6e16 Hz: n_grid 0 as photolibs3.py (np.arange with step 2e14 Hz), n_grid
+ * > 0 that many points per edge (200 in myPhotoLibsGPU.h). */
+typedef struct sphx_network {
+    double XH;                                  /* hydrogen mass fraction, 0.76 */
+    double J21;                                 /* background strength (Jcoeff) */
+    double unit_u_cgs, unit_rho_cgs, unit_time_s;  /* code -> cgs (1 for cgs inputs) */
+    double tol;                                 /* bisection ends at |du/u| <= tol, 1e-4 */
+    int32_t n_grid;                             /* frequency grid, see above */
+    int32_t max_iter;                           /* bisection cap, 100 (the reference's) */
+    int32_t max_bracket;                        /* bracket-step cap (the reference has none) */
+    int32_t pad;
+} sphx_network;
+
 typedef struct sphx_counters {
     int64_t n;              /* particles resident */
     int64_t steps;          /* steps taken since load */
@@ -238,6 +258,18 @@ int sphx_neighbors(sphx_ctx *ctx, int64_t n, const double *pos, const double *h,
 int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, double dt,
               const sphx_cooling *cool, double *u_out, int32_t *iters, void *stream);
 
+/* The six rates of the network's background (RadiationField,
+ * photolibs3.py:230-329): photoionisation rates of H0, He0, He+ (s^-1) and
+ * their heating rates (erg s^-1), into out[6].  Host only. */
+int sphx_network_rates(const sphx_network *net, double *out);
+/* DoCooling_h (photolibs3.py:489-543) on device arrays in the code units of
+ * net: u_out = the backward-Euler u after dt, with the net rate of
+ * coolingRateFromU_h (:470-486, T(u) clamped to 1e4..1e8 K).  iters
+ * (nullable) receives the bisection iterations.  u_out may alias u.  Returns
+ * SPHX_CAPPED when a solve hit max_iter or max_bracket.  Synchronous. */
+int sphx_cool_network(sphx_ctx *ctx, int64_t n, const double *rho, const double *u, double dt,
+                      const sphx_network *net, double *u_out, int32_t *iters, void *stream);
+
 /* applyCooling (LIBV3 myCppSPHLibs_v3.h:794-859) on device arrays: u, rho to
  * cgs, clamped inside the table, bin = first j with g[j] <= x <= g[j+1] (the
  * reference's linear scans, found here by binary search), u_new = u_nodes[j]
@@ -333,7 +365,8 @@ int sphx_plan_init(sphx_plan *plan, void *stream);
 /* n_steps KDK steps at fixed dt.  use_cooling: 0 none, 1 the implicit solve
  * on the plan's Lambda(T) table, 2 the 2-D (u, rho) table of
  * sphx_plan_set_cooling_grid, 3 the 5-D CHIMES table of
- * sphx_plan_set_chimes_table. */
+ * sphx_plan_set_chimes_table, 4 the analytic H/He network of
+ * sphx_plan_set_cooling_network. */
 int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32_t use_cooling,
                    void *stream);
 /* Copy state out in the original particle order; any pointer may be NULL.
@@ -398,6 +431,11 @@ int sphx_plan_set_cooling_grid(sphx_plan *plan, const sphx_cool_grid *grid);
  * species fractions of its last lookup; sphx_plan_chimes_ions copies them out
  * (n x n_m float32, original particle order; zero before the first lookup). */
 int sphx_plan_set_chimes_table(sphx_plan *plan, const sphx_chimes_table *table);
+/* Attach the analytic H/He network (copied, rates evaluated here); runs
+ * stepped with use_cooling = 4 solve DoCooling_h after the u update, as the
+ * GPU drivers apply their cooling after the u update
+ * (With_cooling/Model_A/hfv_sph_gpu_v3.cu:600-609). */
+int sphx_plan_set_cooling_network(sphx_plan *plan, const sphx_network *net);
 int sphx_plan_chimes_ions(sphx_plan *plan, float *out, void *stream);
 
 /* Shift (-3..3) of the Morton level at which 64-particle buckets are cut into
@@ -483,6 +521,8 @@ int64_t sphx_gravity_symmetric_min_bytes(int64_t n, int32_t block_pairs);
 /* As sphx_plan_set_chimes_table, for runs admitted with use_cooling = 3 (the
  * energy lookup only: batches keep no species fractions). */
 int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_table *table);
+/* As sphx_plan_set_cooling_network, for runs admitted with use_cooling = 4. */
+int sphx_batch_set_cooling_network(sphx_batch *b, const sphx_network *net);
 int sphx_batch_set_adaptive_dt(sphx_batch *b, int64_t run_id, int32_t on, double c_cfl,
                                double dt_min, double dt_max, void *stream);
 int sphx_batch_time(sphx_batch *b, int64_t run_id, double *t, double *dt, double *dt_parts,

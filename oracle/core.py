@@ -65,6 +65,21 @@ def lib():
         fp = C.POINTER(C.c_float)
         L.orc_chimes_apply.argtypes = [i64, dp, dp, dp, C.c_double, C.c_void_p, fp, fp, fp, fp,
                                        fp, fp, dp, fp, dp]
+        L.orc_net_rates.argtypes = [C.c_void_p, dp]
+        L.orc_net_ci.argtypes = [C.c_double, dp]
+        L.orc_net_abundance.argtypes = [C.c_double, C.c_double, dp, dp, ip]
+        L.orc_net_heat_cool.argtypes = [C.c_double, C.c_double, dp, dp]
+        L.orc_net_u_to_temp.restype = C.c_double
+        L.orc_net_u_to_temp.argtypes = [C.c_double, C.c_double, C.c_double, dp, ip]
+        L.orc_net_temp_to_u.restype = C.c_double
+        L.orc_net_temp_to_u.argtypes = [C.c_double, C.c_double, C.c_double, dp]
+        L.orc_net_rate.restype = C.c_double
+        L.orc_net_rate.argtypes = [C.c_double, C.c_double, C.c_double, dp]
+        L.orc_net_cool_one.restype = C.c_double
+        L.orc_net_cool_one.argtypes = [C.c_double, C.c_double, C.c_double, C.c_void_p, dp, ip, ip]
+        L.orc_net_cool.argtypes = [i64, dp, dp, C.c_double, C.c_void_p, dp, dp, ip, ip]
+        L.orc_net_sigma.restype = C.c_double
+        L.orc_net_sigma.argtypes = [C.c_double] * 8
         L.orc_global_dt.restype = C.c_double
         L.orc_global_dt.argtypes = [i64, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         _LIB = L
@@ -130,6 +145,104 @@ class CoolParams:
         return _CoolParams(self.unit_u_cgs, self.unit_rho_cgs, self.unit_time_s, self.XH, self.mu,
                            self.gamma, self.heat, self.tol, self.max_iter, self.max_bracket,
                            len(self.T), 0)
+
+
+class _NetParams(C.Structure):
+    _fields_ = [("XH", C.c_double), ("J21", C.c_double), ("unit_u_cgs", C.c_double),
+                ("unit_rho_cgs", C.c_double), ("unit_time_s", C.c_double), ("tol", C.c_double),
+                ("n_grid", C.c_int32), ("max_iter", C.c_int32), ("max_bracket", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+@dataclass
+class NetParams:
+    """The analytic H/He network of photolibs3.py (GPU_sph/Cooling_v1/photolibs3.py:140-543):
+    hydrogen mass fraction, radiation-field strength (its Jcoeff; 1e-5 there, 50 in
+    myPhotoLibsGPU.h), frequency grid (0: np.arange with step 0.02e16 Hz as photolibs3.py;
+    n > 0: n points per edge as myPhotoLibsGPU.h), code -> cgs unit factors, and the solve's
+    relative bracket width (1e-4) and caps."""
+    XH: float = 0.76
+    J21: float = 1e-5
+    unit_u_cgs: float = 1.0
+    unit_rho_cgs: float = 1.0
+    unit_time_s: float = 1.0
+    tol: float = 1e-4
+    n_grid: int = 0
+    max_iter: int = 100
+    max_bracket: int = 400
+
+    def c(self) -> _NetParams:
+        return _NetParams(self.XH, self.J21, self.unit_u_cgs, self.unit_rho_cgs,
+                          self.unit_time_s, self.tol, self.n_grid, self.max_iter,
+                          self.max_bracket, 0)
+
+
+def net_rates(p: NetParams):
+    """RadiationField (photolibs3.py:230-329): (gJH0, gJHe0, gJHep, HRate_H0, HRate_He0,
+    HRate_Hep)."""
+    out = np.zeros(6)
+    c = p.c()
+    lib().orc_net_rates(C.byref(c), _d(out)[1])
+    return out
+
+
+def net_ci(T):
+    """RandCIRates (photolibs3.py:140-166)."""
+    out = np.zeros(7)
+    lib().orc_net_ci(float(T), _d(out)[1])
+    return out
+
+
+def net_abundance(T, nH, rf):
+    """Abundance_hX (photolibs3.py:169-211): (nH0, nHe0, nHp, ne, nHep, nHepp), iterations."""
+    out = np.zeros(6)
+    it = np.zeros(1, np.int32)
+    rf_, rfp = _d(rf)
+    lib().orc_net_abundance(float(T), float(nH), rfp, _d(out)[1], _i(it)[1])
+    return out, int(it[0])
+
+
+def net_heat_cool(T, nH, rf):
+    """coolingHeatingRates (photolibs3.py:332-376): (heating, Lambda) per nH^2."""
+    out = np.zeros(2)
+    rf_, rfp = _d(rf)
+    lib().orc_net_heat_cool(float(T), float(nH), rfp, _d(out)[1])
+    return out
+
+
+def net_u_to_temp(u, nH, XH, rf):
+    """convert_u_to_temp_h (photolibs3.py:379-422)."""
+    rf_, rfp = _d(rf)
+    return lib().orc_net_u_to_temp(float(u), float(nH), float(XH), rfp, C.POINTER(C.c_int32)())
+
+
+def net_temp_to_u(T, nH, XH, rf):
+    """convert_Temp_to_u (photolibs3.py:425-467)."""
+    rf_, rfp = _d(rf)
+    return lib().orc_net_temp_to_u(float(T), float(nH), float(XH), rfp)
+
+
+def net_rate(u, nH, XH, rf):
+    """coolingRateFromU_h (photolibs3.py:470-486): heating - Lambda per nH^2 at T(u)
+    clamped to [1e4, 1e8] K."""
+    rf_, rfp = _d(rf)
+    return lib().orc_net_rate(float(u), float(nH), float(XH), rfp)
+
+
+def net_cool(rho, u, dt, p: NetParams, rf=None):
+    """DoCooling_h (photolibs3.py:489-543) on arrays in the code units of p:
+    returns (u_new, bisection iterations, status 0 ok / 1 bisection cap / 2 bracket cap)."""
+    rho, rp = _d(np.atleast_1d(rho))
+    u, up = _d(np.atleast_1d(u))
+    rf = net_rates(p) if rf is None else rf
+    rf_, rfp = _d(rf)
+    n = len(u)
+    out = np.zeros(n)
+    it = np.zeros(n, np.int32)
+    st = np.zeros(n, np.int32)
+    c = p.c()
+    lib().orc_net_cool(n, rp, up, float(dt), C.byref(c), rfp, _d(out)[1], _i(it)[1], _i(st)[1])
+    return out, it, st
 
 
 def _d(a):
@@ -494,12 +607,14 @@ def bootstrap(s: State, p: Params) -> State:
 
 
 def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None,
-         grid: dict | None = None, chimes: "ChimesTable | None" = None) -> State:
+         grid: dict | None = None, chimes: "ChimesTable | None" = None,
+         network: NetParams | None = None) -> State:
     """One KDK step (hfv_sph_gpu_v2_evrard.cu:410-482).  grid (dict of
     cool_grid_apply's table arguments) cools by the 2-D table after the u
     update, as hfv_sph_gpu_v3.cu:600-609 does; chimes looks the energy up in
     the 5-D table after the u update, as hfvGpuSphTyp_v7.cu:665-673 does
-    (s.ion receives the species fractions)."""
+    (s.ion receives the species fractions); network solves the analytic H/He
+    network's implicit step (DoCooling_h) after the u update."""
     s = s.copy()
     s.vel = s.vel + s.acc * (0.5 * dt)
     s.pos = s.pos + s.vel * dt
@@ -513,6 +628,8 @@ def step(s: State, dt: float, p: Params, cool_p: CoolParams | None = None,
         s.u = cool_grid_apply(s.rho, s.u, dt, **grid)
     elif chimes is not None:
         s.u, s.ion, _ = chimes_apply(s.pos, s.rho, s.u, dt, chimes)
+    elif network is not None:
+        s.u, _, _ = net_cool(s.rho, s.u, dt, network)
     s.t = s.t + dt
     return s
 

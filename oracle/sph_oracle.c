@@ -759,3 +759,288 @@ void orc_chimes_apply(int64_t n, const double *pos, const double *rho, const dou
         }
     }
 }
+
+/* ---------------------------------------------------------------------------
+ * Analytic H/He cooling network of PHOTOPY (the FP64 v3 functions the
+ * reference's DoCooling_h calls), restated function by function:
+ *   orc_net_sigma        phCrossSectionX   PHOTOPY:120-134 (Verner et al. 1996)
+ *   orc_net_rates        RadiationField    PHOTOPY:230-329 (J_neu_func :215-227)
+ *   orc_net_ci           RandCIRates       PHOTOPY:140-166 (Cen 1992)
+ *   orc_net_abundance    Abundance_hX      PHOTOPY:169-211
+ *   orc_net_heat_cool    coolingHeatingRates PHOTOPY:332-376
+ *   orc_net_u_to_temp    convert_u_to_temp_h PHOTOPY:379-422
+ *   orc_net_temp_to_u    convert_Temp_to_u PHOTOPY:425-467
+ *   orc_net_rate         coolingRateFromU_h PHOTOPY:470-486
+ *   orc_net_cool_one     DoCooling_h       PHOTOPY:489-543
+ * The radiation field is parameterised: J21 (the reference's Jcoeff: 1e-5 in
+ * PHOTOPY, 50 in GPU_sph/With_cooling/Model_A/myPhotoLibsGPU.h:166-296) and
+ * the frequency grid (n_grid 0: np.arange(nu_edge, 6e16, 0.02e16) as PHOTOPY;
+ * n_grid > 0: n_grid points of (6e16 - nu_edge) / n_grid as the GPU header).
+ * The bracket loops of DoCooling_h have no cap in the reference; here at most
+ * max_bracket steps (counted as capped).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    double XH, J21;
+    double unit_u_cgs, unit_rho_cgs, unit_time_s;
+    double tol;
+    int32_t n_grid, max_iter, max_bracket, pad;
+} orc_net_params;
+
+typedef struct {
+    double gJH0, gJHe0, gJHep, HH0, HHe0, HHep;
+} orc_net_rf;
+
+#define ORC_H_EV 4.135667696e-15
+#define ORC_CLIGHT 29979245800.0
+#define ORC_NET_KB 1.3807e-16
+#define ORC_NET_MH 1.6726e-24
+
+static double orc_net_neu_to_eV(double neu) { return ORC_H_EV * neu; }
+static double orc_net_eV_to_neu(double eV) { return eV / ORC_H_EV; }
+static double orc_net_lamb_to_neu(double lamb) { return ORC_CLIGHT / (lamb * 1e-8); }
+static double orc_net_neu_to_lamb(double neu) { return ORC_CLIGHT / neu * 1e8; }
+
+double orc_net_sigma(double neu, double E0, double sig0, double ya, double P, double yw,
+                     double y0, double y1) {
+    double E = orc_net_neu_to_eV(neu);
+    double x = E / E0 - y0;
+    double y = sqrt(x * x + y1 * y1);
+    double Fy = (((x - 1.0) * (x - 1.0) + yw * yw) * pow(y, 0.5 * P - 5.5)) *
+                pow(1.0 + sqrt(y / ya), -P);
+    return sig0 * Fy * 1e-18;
+}
+
+/* One edge of RadiationField: photoionisation rate g and heating rate H for
+ * the species with threshold E (eV) and cross-section parameters c[7]. */
+static void orc_net_edge(const orc_net_params *p, double E, const double *c, double *g,
+                         double *H) {
+    const double hplanck = 6.626196e-27;
+    double start = orc_net_eV_to_neu(E) / 1e16;
+    int64_t n;
+    double step;
+    if (p->n_grid > 0) {
+        n = p->n_grid;
+        step = (6.0 - start) / (double)n;
+    } else {
+        /* np.arange(start, 6.0, 0.02): ceil((stop - start) / step) values
+         * start + i * delta with delta = (start + step) - start */
+        n = (int64_t)ceil((6.0 - start) / 0.02);
+        step = (start + 0.02) - start;
+    }
+    double nu_edge = orc_net_lamb_to_neu(orc_net_neu_to_lamb(orc_net_eV_to_neu(E)));
+    double nu0 = orc_net_eV_to_neu(E);
+    double *nu = (double *)malloc(sizeof(double) * (size_t)n);
+    double *fg = (double *)malloc(sizeof(double) * (size_t)n);
+    double *fh = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int64_t i = 0; i < n; ++i) nu[i] = (start + (double)i * step) * 1e16;
+    for (int64_t i = 0; i < n; ++i) {
+        double sig = orc_net_sigma(nu[i], c[0], c[1], c[2], c[3], c[4], c[5], c[6]);
+        double J = p->J21 * 1e-21 / (nu[i] / nu_edge);
+        fg[i] = 4.0 * M_PI * J * sig / hplanck / nu[i];
+        fh[i] = 4.0 * M_PI * J * sig * (nu[i] - nu0) / nu[i];
+    }
+    double d = n > 1 ? nu[1] - nu[0] : 0.0;
+    double sg = 0.0, sh = 0.0;
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        sg += d * (fg[i] + fg[i + 1]) / 2.0;
+        sh += d * (fh[i] + fh[i + 1]) / 2.0;
+    }
+    free(nu); free(fg); free(fh);
+    *g = sg;
+    *H = sh;
+}
+
+void orc_net_rates(const orc_net_params *p, double *out6) {
+    static const double cH0[7] = {4.298e-1, 5.475e4, 3.288e1, 2.963, 0.0, 0.0, 0.0};
+    static const double cHe0[7] = {1.361e1, 9.492e2, 1.469e0, 3.188, 2.039, 4.434e-1, 2.136};
+    static const double cHep[7] = {1.720e0, 1.369e4, 3.288e1, 2.963, 0.0, 0.0, 0.0};
+    orc_net_edge(p, 13.60, cH0, &out6[0], &out6[3]);
+    orc_net_edge(p, 24.59, cHe0, &out6[1], &out6[4]);
+    orc_net_edge(p, 54.42, cHep, &out6[2], &out6[5]);
+}
+
+/* aHp, aHep, aHepp, ad, geH0, geHe0, geHep */
+void orc_net_ci(double T, double *r7) {
+    double Tfact = 1.0 / (1.0 + sqrt(T / 1e5));
+    double aHp = 8.41e-11 * pow(T / 1000.0, -0.2) / (1.0 + pow(T / 1e6, 0.7)) / sqrt(T);
+    r7[0] = aHp;
+    r7[1] = 1.5e-10 * pow(T, -0.6353);
+    r7[2] = 4.0 * aHp;
+    r7[3] = 1.9e-3 * pow(T, -1.5) * exp(-470000.0 / T) * (1.0 + 0.3 * exp(-94000.0 / T));
+    r7[4] = 5.85e-11 * sqrt(T) * exp(-157809.1 / T) * Tfact;
+    r7[5] = 2.38e-11 * sqrt(T) * exp(-285335.4 / T) * Tfact;
+    r7[6] = 5.68e-12 * sqrt(T) * exp(-631515.0 / T) * Tfact;
+}
+
+/* nH0, nHe0, nHp, ne, nHep, nHepp relative to nH; iters (nullable) */
+void orc_net_abundance(double T, double nH, const double *rf, double *ab6, int32_t *iters) {
+    double r[7];
+    orc_net_ci(T, r);
+    double aHp = r[0], aHep = r[1], aHepp = r[2], ad = r[3], geH0 = r[4], geHe0 = r[5],
+           geHep = r[6];
+    double gJH0 = rf[0], gJHe0 = rf[1], gJHep = rf[2];
+    double Y = 0.24;
+    double y = Y / (4.0 - 4.0 * Y);
+    double ne = 1.0, ne_old;
+    double nH0 = 0, nHp = 0, nHep = 0, nHe0 = 0, nHepp = 0;
+    int niter = 1;
+    double dne = 1.0;
+    while (dne > 1e-4 && niter < 100) {
+        ne_old = ne;
+        nH0 = aHp / (aHp + geH0 + gJH0 / (ne * nH));
+        nHp = 1.0 - nH0;
+        nHep = y / (1.0 + (aHep + ad) / (geHe0 + gJHe0 / (ne * nH)) +
+                    (geHep + gJHep / (ne * nH)) / aHepp);
+        nHe0 = nHep * (aHep + ad) / (geHe0 + gJHe0 / (ne * nH));
+        nHepp = nHep * (geHep + gJHep / (ne * nH)) / aHepp;
+        ne = nHp + nHep + 2.0 * nHepp;
+        ne = 0.5 * (ne + ne_old);
+        dne = fabs(ne - ne_old);
+        ++niter;
+    }
+    ab6[0] = nH0; ab6[1] = nHe0; ab6[2] = nHp; ab6[3] = ne; ab6[4] = nHep; ab6[5] = nHepp;
+    if (iters) *iters = niter;
+}
+
+/* heating and cooling per nH^2: out2 = {heat, lambda} */
+void orc_net_heat_cool(double T, double nH, const double *rf, double *out2) {
+    double r[7], ab[6];
+    orc_net_ci(T, r);
+    orc_net_abundance(T, nH, rf, ab, NULL);
+    double aHp = r[0], aHep = r[1], aHepp = r[2], ad = r[3], geH0 = r[4], geHe0 = r[5],
+           geHep = r[6];
+    double nH0 = ab[0], nHe0 = ab[1], nHp = ab[2], ne = ab[3], nHep = ab[4], nHepp = ab[5];
+    double heat = nH0 * rf[3] / nH + nHe0 * rf[4] / nH + nHep * rf[5] / nH;
+    double Tfact = 1.0 / (1.0 + sqrt(T / 1e5));
+    double BetaH0 = 7.5e-19 * exp(-118348.0 / T) * Tfact;
+    double BetaHep = 5.54e-17 * pow(T, -0.397) * exp(-473638.0 / T) * Tfact;
+    double lt = 5.5 - log10(T);
+    double Betaff = 1.43e-27 * sqrt(T) * (1.0 + 0.34 * exp(-(lt * lt) / 3.0));
+    double exc = BetaH0 * ne * nH0 + BetaHep * ne * nHep;
+    double ion = 2.18e-11 * geH0 * ne * nH0 + 3.94e-11 * geHe0 * ne * nHe0 +
+                 8.72e-11 * geHep * ne * nHep;
+    double rec = 1.036e-16 * T * ne * (aHp * nHp) + 1.036e-16 * T * ne * (aHep * nHep) +
+                 1.036e-16 * T * ne * (aHepp * nHepp) + 6.526e-11 * ad * ne * nHep;
+    double ff = Betaff * (nHp + nHep + 4.0 * nHepp) * ne;
+    out2[0] = heat;
+    out2[1] = exc + ion + rec + ff;
+}
+
+double orc_net_u_to_temp(double u, double nH, double XH, const double *rf, int32_t *iters) {
+    const double gamma = 5.0 / 3.0;
+    double yHe = (1.0 - XH) / (4.0 * XH);
+    double ne_guess = 1.0;
+    double mu = (1.0 + 4.0 * yHe) / (1.0 + yHe + ne_guess);
+    double temp = (gamma - 1.0) * ORC_NET_MH / ORC_NET_KB * mu * u;
+    double temp_old;
+    int niter = 1;
+    double dT = temp;
+    double ab[6];
+    while (dT / temp > 1e-4 && niter < 100) {
+        temp_old = temp;
+        orc_net_abundance(temp_old, nH, rf, ab, NULL);
+        ne_guess = ab[3];
+        mu = (1.0 + 4.0 * yHe) / (1.0 + yHe + ne_guess);
+        temp = (gamma - 1.0) * ORC_NET_MH / ORC_NET_KB * mu * u;
+        temp = 0.5 * (temp + temp_old);
+        dT = fabs(temp - temp_old);
+        ++niter;
+    }
+    if (iters) *iters = niter;
+    return temp;
+}
+
+double orc_net_temp_to_u(double temp, double nH, double XH, const double *rf) {
+    const double gamma = 5.0 / 3.0;
+    double yHe = (1.0 - XH) / (4.0 * XH);
+    double ne_guess = 1.0;
+    double mu = (1.0 + 4.0 * yHe) / (1.0 + yHe + ne_guess);
+    double u = ORC_NET_KB / ORC_NET_MH / (gamma - 1.0) / mu * temp;
+    double u_old;
+    int niter = 1;
+    double du = u;
+    double ab[6];
+    while (du / u > 1e-4 && niter < 100) {
+        u_old = u;
+        double t_old = orc_net_u_to_temp(u_old, nH, XH, rf, NULL);
+        orc_net_abundance(t_old, nH, rf, ab, NULL);
+        ne_guess = ab[3];
+        mu = (1.0 + 4.0 * yHe) / (1.0 + yHe + ne_guess);
+        u = ORC_NET_KB / ORC_NET_MH / (gamma - 1.0) / mu * temp;
+        u = 0.5 * (u + u_old);
+        du = fabs(u - u_old);
+        ++niter;
+    }
+    return u;
+}
+
+double orc_net_rate(double u, double nH, double XH, const double *rf) {
+    double T = orc_net_u_to_temp(u, nH, XH, rf, NULL);
+    if (T < 1e4) T = 1e4;
+    if (T > 1e8) T = 1e8;
+    double hc[2];
+    orc_net_heat_cool(T, nH, rf, hc);
+    return hc[0] - hc[1];
+}
+
+/* DoCooling_h in cgs (rho g/cm^3, u erg/g, dt s); *status: 0 converged,
+ * 1 bisection cap, 2 bracket cap. */
+double orc_net_cool_one(double rho, double u_old, double dt, const orc_net_params *p,
+                        const double *rf, int32_t *iters, int32_t *status) {
+    double XH = p->XH;
+    double nH = XH * rho / ORC_NET_MH;
+    double ratefact = nH * nH / rho;
+    double u = u_old, u_upper = u, u_lower = u;
+    int st = 0;
+    double G = orc_net_rate(u, nH, XH, rf);
+    double g0 = u - u_old - ratefact * G * dt;
+    if (iters) *iters = 0;
+    if (g0 == 0.0) {
+        if (status) *status = 0;
+        return u_old;
+    }
+    int nb = 0;
+    if (g0 > 0.0) {
+        while (u_upper - u_old - ratefact * orc_net_rate(u_upper, nH, XH, rf) * dt > 0.0) {
+            if (nb++ >= p->max_bracket) { st = 2; break; }
+            u_upper /= 1.1;
+        }
+        u_lower = u_upper;
+        u_upper = u_lower * 1.1;
+    } else {
+        while (u_upper - u_old - ratefact * orc_net_rate(u_upper, nH, XH, rf) * dt < 0.0) {
+            if (nb++ >= p->max_bracket) { st = 2; break; }
+            u_upper *= 1.1;
+        }
+        u_lower = u_upper / 1.1;
+    }
+    int niter = 1;
+    double du = u;
+    while (fabs(du / u) > p->tol && niter < p->max_iter) {
+        u = 0.5 * (u_lower + u_upper);
+        G = orc_net_rate(u, nH, XH, rf);
+        if (u - u_old - ratefact * G * dt > 0.0) u_upper = u;
+        else u_lower = u;
+        du = fabs(u_upper - u_lower);
+        ++niter;
+    }
+    if (niter >= p->max_iter && st == 0) st = 1;
+    if (iters) *iters = niter - 1;
+    if (status) *status = st;
+    return u;
+}
+
+/* Arrays in code units (unit factors of p); rf from orc_net_rates. */
+void orc_net_cool(int64_t n, const double *rho, const double *u, double dt,
+                  const orc_net_params *p, const double *rf, double *u_out, int32_t *it_out,
+                  int32_t *st_out) {
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t i = 0; i < n; ++i) {
+        int32_t it = 0, st = 0;
+        double r = orc_net_cool_one(rho[i] * p->unit_rho_cgs, u[i] * p->unit_u_cgs,
+                                    dt * p->unit_time_s, p, rf, &it, &st);
+        u_out[i] = r / p->unit_u_cgs;
+        if (it_out) it_out[i] = it;
+        if (st_out) st_out[i] = st;
+    }
+}

@@ -46,6 +46,14 @@ class Cooling(C.Structure):
                 ("pad", C.c_int32), ("T", C.POINTER(C.c_double)), ("L", C.POINTER(C.c_double))]
 
 
+class Network(C.Structure):
+    """sphx_network (include/sphx.h)."""
+    _fields_ = [("XH", C.c_double), ("J21", C.c_double), ("unit_u_cgs", C.c_double),
+                ("unit_rho_cgs", C.c_double), ("unit_time_s", C.c_double), ("tol", C.c_double),
+                ("n_grid", C.c_int32), ("max_iter", C.c_int32), ("max_bracket", C.c_int32),
+                ("pad", C.c_int32)]
+
+
 class Counters(C.Structure):
     _fields_ = [("n", C.c_int64), ("steps", C.c_int64), ("h_iters_max", C.c_int64),
                 ("h_iters_sum", C.c_int64), ("h_capped", C.c_int64), ("cool_capped", C.c_int64),
@@ -122,6 +130,10 @@ SIGNATURES = {
     "sphx_bh_gravity": (C.c_int, [_V, _I64, _V, _V, _V, _D, _D, _I64, _I64, _V, _V, _V, _V]),
     "sphx_neighbors": (C.c_int, [_V, _I64, _V, _V, _I64, _I64, _V, _V, _I64, _V, _V]),
     "sphx_cool": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V, _V]),
+    "sphx_network_rates": (C.c_int, [_V, _V]),
+    "sphx_cool_network": (C.c_int, [_V, _I64, _V, _V, _D, _V, _V, _V, _V]),
+    "sphx_plan_set_cooling_network": (C.c_int, [_V, _V]),
+    "sphx_batch_set_cooling_network": (C.c_int, [_V, _V]),
     "sphx_diagnostics": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _V, _V]),
     "sphx_global_dt": (C.c_int, [_V, _I64, _V, _V, _V, _V, _V, _D, _D, _D, _V, _V]),
     "sphx_global_dt_noalloc": (C.c_int, [C.c_int, _I64, _V, _V, _V, _V, _V, _D, _D, _D, _V,
@@ -224,7 +236,9 @@ def check(rc: int, what: str = "") -> int:
 
 def cooling_mode(cooling) -> int:
     """False/0: none; True/1/"solve": implicit solve; 2/"table": 2-D table;
-    3/"chimes": 5-D CHIMES table."""
+    3/"chimes": 5-D CHIMES table; 4/"network": analytic H/He network."""
+    if cooling in ("network", 4):
+        return 4
     if cooling in ("chimes", 3):
         return 3
     if cooling in ("table", 2):

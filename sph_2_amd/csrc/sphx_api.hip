@@ -1212,6 +1212,25 @@ extern "C" int sphx_cool(sphx_ctx *ctx, int64_t n, const double *rho, const doub
     return c.cool_capped > 0 ? SPHX_CAPPED : SPHX_OK;
 }
 
+extern "C" int sphx_cool_network(sphx_ctx *ctx, int64_t n, const double *rho, const double *u,
+                                 double dt, const sphx_network *net, double *u_out,
+                                 int32_t *iters, void *stream) {
+    CTX_PROLOGUE(ctx);
+    if (n < 0 || (n > 0 && (!rho || !u || !u_out))) { sphx_set_error("bad arguments"); return SPHX_ERR_ARG; }
+    if (!(dt >= 0.0)) { sphx_set_error("dt must be >= 0"); return SPHX_ERR_ARG; }
+    NetDev nd;
+    SPHX_CHECK(sphx_network_dev(net, &nd));
+    Engine &E = ctx->E;
+    SPHX_CHECK(E.reserve(64));
+    if (n == 0) return SPHX_OK;
+    SPHX_CHECK(sphx_zero_counters(E.counters, st));
+    SPHX_CHECK(sphx_net_launch(nd, rho, u, dt, n, u_out, iters, E.counters, st));
+    SPHX_HIP(hipStreamSynchronize(st));
+    sphx_counters c{};
+    SPHX_CHECK(E.read_counters(&c));
+    return c.cool_capped > 0 ? SPHX_CAPPED : SPHX_OK;
+}
+
 extern "C" int sphx_cool_grid_apply(sphx_ctx *ctx, int64_t n, const double *rho,
                                     const double *u_in, double dt, const sphx_cool_grid *grid,
                                     double *u_out, void *stream) {
@@ -1439,7 +1458,8 @@ extern "C" int sphx_plan_step(sphx_plan *plan, int32_t n_steps, double dt, int32
     if (!plan->inited) { sphx_set_error("call sphx_plan_init first"); return SPHX_ERR_STATE; }
     if (n_steps < 0 || !(dt > 0.0)) { sphx_set_error("bad step count or dt"); return SPHX_ERR_ARG; }
     Engine &E = plan->E;
-    if (use_cooling < 0 || use_cooling > 3) { sphx_set_error("use_cooling is 0, 1 (solve), 2 (2-D table) or 3 (CHIMES table)"); return SPHX_ERR_ARG; }
+    if (use_cooling < 0 || use_cooling > 4) { sphx_set_error("use_cooling is 0, 1 (solve), 2 (2-D table), 3 (CHIMES table) or 4 (H/He network)"); return SPHX_ERR_ARG; }
+    if (use_cooling == 4 && !E.net.on) { sphx_set_error("plan has no cooling network"); return SPHX_ERR_ARG; }
     if (use_cooling == 1 && !E.has_cool) { sphx_set_error("plan has no cooling table"); return SPHX_ERR_ARG; }
     if (use_cooling == 2 && !E.grid.du) { sphx_set_error("plan has no 2-D cooling table"); return SPHX_ERR_ARG; }
     if (use_cooling == 3 && !E.chimes.u_evol) { sphx_set_error("plan has no CHIMES table"); return SPHX_ERR_ARG; }
@@ -1582,7 +1602,7 @@ extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream
     if (h.mode != E.mode) { sphx_set_error("checkpoint precision mode differs from the plan's"); return SPHX_ERR_ARG; }
     if (h.n <= 0 || h.n > E.cap) { sphx_set_error("checkpoint particle count exceeds the plan capacity"); return SPHX_ERR_ARG; }
     if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du) ||
-        (h.rp.use_cool == 3 && !E.chimes.u_evol)) {
+        (h.rp.use_cool == 3 && !E.chimes.u_evol) || (h.rp.use_cool == 4 && !E.net.on)) {
         sphx_set_error("checkpoint run cools with a table the plan does not have");
         return SPHX_ERR_ARG;
     }
@@ -1621,6 +1641,15 @@ extern "C" int sphx_plan_set_chimes_table(sphx_plan *plan, const sphx_chimes_tab
     void *stream = nullptr;
     PLAN_PROLOGUE(plan);
     return plan->E.set_chimes(table, true, st);
+}
+
+extern "C" int sphx_plan_set_cooling_network(sphx_plan *plan, const sphx_network *net) {
+    if (!plan) { sphx_set_error("null plan"); return SPHX_ERR_ARG; }
+    NetDev nd;
+    SPHX_CHECK(sphx_network_dev(net, &nd));
+    plan->E.net = nd;
+    plan->E.drop_graph();  // the network's parameters are captured by value
+    return SPHX_OK;
 }
 
 extern "C" int sphx_plan_chimes_ions(sphx_plan *plan, float *out, void *stream) {

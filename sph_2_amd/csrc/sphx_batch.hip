@@ -201,7 +201,8 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
         return SPHX_ERR_ARG;
     }
     if (!E.p.eps_from_h && !eps) { sphx_set_error("eps required when eps_from_h == 0"); return SPHX_ERR_ARG; }
-    if (use_cooling < 0 || use_cooling > 3) { sphx_set_error("use_cooling is 0, 1 (solve), 2 (2-D table) or 3 (CHIMES table)"); return SPHX_ERR_ARG; }
+    if (use_cooling < 0 || use_cooling > 4) { sphx_set_error("use_cooling is 0, 1 (solve), 2 (2-D table), 3 (CHIMES table) or 4 (H/He network)"); return SPHX_ERR_ARG; }
+    if (use_cooling == 4 && !E.net.on) { sphx_set_error("batch has no cooling network"); return SPHX_ERR_ARG; }
     if (use_cooling == 1 && !E.has_cool) { sphx_set_error("batch has no cooling table"); return SPHX_ERR_ARG; }
     if (use_cooling == 2 && !E.grid.du) { sphx_set_error("batch has no 2-D cooling table"); return SPHX_ERR_ARG; }
     if (use_cooling == 3 && !E.chimes.u_evol) { sphx_set_error("batch has no CHIMES table"); return SPHX_ERR_ARG; }
@@ -371,6 +372,15 @@ extern "C" int sphx_batch_set_chimes_table(sphx_batch *b, const sphx_chimes_tabl
     return b->E.set_chimes(table, false, st);
 }
 
+extern "C" int sphx_batch_set_cooling_network(sphx_batch *b, const sphx_network *net) {
+    if (!b) { sphx_set_error("null batch"); return SPHX_ERR_ARG; }
+    NetDev nd;
+    SPHX_CHECK(sphx_network_dev(net, &nd));
+    b->E.net = nd;
+    b->E.drop_graph();
+    return SPHX_OK;
+}
+
 extern "C" int sphx_batch_set_cooling_grid(sphx_batch *b, const sphx_cool_grid *grid) {
     void *stream = nullptr;
     BATCH_PROLOGUE(b);
@@ -455,7 +465,7 @@ extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *
         return SPHX_ERR_ARG;
     }
     if ((h.rp.use_cool == 1 && !E.has_cool) || (h.rp.use_cool == 2 && !E.grid.du) ||
-        (h.rp.use_cool == 3 && !E.chimes.u_evol)) {
+        (h.rp.use_cool == 3 && !E.chimes.u_evol) || (h.rp.use_cool == 4 && !E.net.on)) {
         sphx_set_error("checkpoint run cools with a table the batch does not have");
         return SPHX_ERR_ARG;
     }

@@ -365,6 +365,15 @@ __device__ __forceinline__ double balsara_f(double dv, double cv, double c, doub
     return a / (a + cv + 0.0001 * c / h);
 }
 
+// Position of entry k of lane L in its bucket's list block: lanes hold their
+// entries in runs of 8 (32 bytes), the runs of the 64 lanes side by side, so
+// that a lane fills its own 32-byte piece as it accepts neighbours (four lanes
+// complete a 128-byte line together) and iteration k of a list pass reads
+// 64 runs that iterations k + 1 .. k + 7 read again.
+__device__ __forceinline__ int64_t nbr_slot(int k, int lane) {
+    return ((int64_t)(k >> 3) * 64 + lane) * 8 + (k & 7);
+}
+
 struct DensArgs {
     double gamma;
     int32_t *nsym;        // symmetric neighbour count, nullable
@@ -392,7 +401,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     acc.init();
     int nn = 0;
     int ncand = 0;
-    int32_t *lst = a.lst ? a.lst + (int64_t)b * SPHX_NBR_MAX * 64 + lane : nullptr;
+    int32_t *lst = a.lst ? a.lst + (int64_t)b * SPHX_NBR_MAX * 64 : nullptr;
     const int r = t.b_run[b];
     const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
     for (int g0 = 0; g0 < cnt;) {
@@ -438,7 +447,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h3 = g.hinv * g.hinv * g.hinv;
                     acc.add(sm[q] * (SPHX_SIGMA_F * h3 * w_shape(g.q)));
-                    if (lst && nn < SPHX_NBR_MAX) lst[(int64_t)nn * 64] = sj[q];
+                    if (lst && nn < SPHX_NBR_MAX) lst[nbr_slot(nn, lane)] = sj[q];
                     ++nn;
                 };
             int q0 = 0;
@@ -755,12 +764,12 @@ __global__ __launch_bounds__(64) void k_divcurl_list(Tables t, Soa s, ListArgs L
         len = (int)load_rec_b(L.rec, i).z;
     }
     const int maxlen = wave_max_i(len);
-    const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64 + lane;
+    const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64;
     KahanF kd, kx, ky, kz;
     kd.init(); kx.init(); ky.init(); kz.init();
     for (int k = 0; k < maxlen; ++k) {
         if (k < len) {
-            const RecA o = load_rec_a(L.rec, lp[(int64_t)k * 64]);
+            const RecA o = load_rec_a(L.rec, lp[nbr_slot(k, lane)]);
             double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
             double r2 = sphx_r2(dx, dy, dz);
             double tt = me.h + o.h;
@@ -794,12 +803,12 @@ __global__ __launch_bounds__(64) void k_hydro_list(Tables t, Soa s, HydroArgs a,
         len = (int)e.z;
     }
     const int maxlen = wave_max_i(len);
-    const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64 + lane;
+    const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64;
     KahanF kx, ky, kz, kdu;
     kx.init(); ky.init(); kz.init(); kdu.init();
     for (int k = 0; k < maxlen; ++k) {
         if (k < len) {
-            const int j = lp[(int64_t)k * 64];
+            const int j = lp[nbr_slot(k, lane)];
             const RecA o = load_rec_a(L.rec, j);
             const uint4 e = load_rec_b(L.rec, j);
             const float fj = a.balsara ? L.fbal[j] : 1.0f;

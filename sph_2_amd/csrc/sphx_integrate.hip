@@ -9,6 +9,7 @@
 #include "sphx_internal.h"
 
 #include <mutex>
+#include <vector>
 
 namespace {
 
@@ -122,6 +123,164 @@ __device__ double cool_one(double rho, double u_code, double dt, const CoolDev &
     }
     *iters = it;
     return 0.5 * (lo + hi) / cd.unit_u_cgs;
+}
+
+// ---------------------------------------------------------------------------
+// Analytic H/He network (GPU_sph/Cooling_v1/photolibs3.py:140-543), FP64:
+// Cen (1992) recombination and collisional rates, photoionisation equilibrium
+// of H and He under the background rates nd.rf (a damped fixed point in the
+// electron fraction), T(u) through the mean molecular weight (a damped fixed
+// point), the net rate at T clamped to 1e4..1e8 K, and DoCooling_h's
+// backward-Euler solve: bracket the root by factors of 1.1 from u_old, then
+// bisect until the bracket is within tol of the midpoint.
+// ---------------------------------------------------------------------------
+#define NET_KB 1.3807e-16
+#define NET_MH 1.6726e-24
+
+struct NetCI {
+    double aHp, aHep, aHepp, ad, geH0, geHe0, geHep;
+};
+
+__device__ __forceinline__ NetCI net_ci(double T) {
+    NetCI r;
+    double Tfact = 1.0 / (1.0 + sqrt(T / 1e5));
+    r.aHp = 8.41e-11 * pow(T / 1000.0, -0.2) / (1.0 + pow(T / 1e6, 0.7)) / sqrt(T);
+    r.aHep = 1.5e-10 * pow(T, -0.6353);
+    r.aHepp = 4.0 * r.aHp;
+    r.ad = 1.9e-3 * pow(T, -1.5) * exp(-470000.0 / T) * (1.0 + 0.3 * exp(-94000.0 / T));
+    r.geH0 = 5.85e-11 * sqrt(T) * exp(-157809.1 / T) * Tfact;
+    r.geHe0 = 2.38e-11 * sqrt(T) * exp(-285335.4 / T) * Tfact;
+    r.geHep = 5.68e-12 * sqrt(T) * exp(-631515.0 / T) * Tfact;
+    return r;
+}
+
+// Abundances relative to nH: nH0, nHe0, nHp, ne, nHep, nHepp (helium by mass
+// fixed at Y = 0.24, as the reference).
+__device__ __forceinline__ void net_abundance(const NetCI &r, double nH, const double *rf,
+                                              double *ab) {
+    const double y = 0.24 / (4.0 - 4.0 * 0.24);
+    double ne = 1.0, ne_old;
+    double nH0 = 0, nHp = 0, nHep = 0, nHe0 = 0, nHepp = 0;
+    int niter = 1;
+    double dne = 1.0;
+    while (dne > 1e-4 && niter < 100) {
+        ne_old = ne;
+        double ph = ne * nH;
+        nH0 = r.aHp / (r.aHp + r.geH0 + rf[0] / ph);
+        nHp = 1.0 - nH0;
+        double a = (r.aHep + r.ad) / (r.geHe0 + rf[1] / ph);
+        double b = (r.geHep + rf[2] / ph) / r.aHepp;
+        nHep = y / (1.0 + a + b);
+        nHe0 = nHep * (r.aHep + r.ad) / (r.geHe0 + rf[1] / ph);
+        nHepp = nHep * (r.geHep + rf[2] / ph) / r.aHepp;
+        ne = nHp + nHep + 2.0 * nHepp;
+        ne = 0.5 * (ne + ne_old);
+        dne = fabs(ne - ne_old);
+        ++niter;
+    }
+    ab[0] = nH0; ab[1] = nHe0; ab[2] = nHp; ab[3] = ne; ab[4] = nHep; ab[5] = nHepp;
+}
+
+__device__ double net_u_to_temp(double u, double nH, double XH, const double *rf) {
+    const double gamma = 5.0 / 3.0;
+    double yHe = (1.0 - XH) / (4.0 * XH);
+    double mu = (1.0 + 4.0 * yHe) / (1.0 + yHe + 1.0);
+    double temp = (gamma - 1.0) * NET_MH / NET_KB * mu * u;
+    int niter = 1;
+    double dT = temp;
+    double ab[6];
+    while (dT / temp > 1e-4 && niter < 100) {
+        double temp_old = temp;
+        net_abundance(net_ci(temp_old), nH, rf, ab);
+        mu = (1.0 + 4.0 * yHe) / (1.0 + yHe + ab[3]);
+        temp = (gamma - 1.0) * NET_MH / NET_KB * mu * u;
+        temp = 0.5 * (temp + temp_old);
+        dT = fabs(temp - temp_old);
+        ++niter;
+    }
+    return temp;
+}
+
+// Heating minus cooling per nH^2 at the temperature of u (clamped to 1e4..1e8 K).
+__device__ double net_rate(double u, double nH, double XH, const double *rf) {
+    double T = net_u_to_temp(u, nH, XH, rf);
+    T = T < 1e4 ? 1e4 : (T > 1e8 ? 1e8 : T);
+    NetCI r = net_ci(T);
+    double ab[6];
+    net_abundance(r, nH, rf, ab);
+    double nH0 = ab[0], nHe0 = ab[1], nHp = ab[2], ne = ab[3], nHep = ab[4], nHepp = ab[5];
+    double heat = nH0 * rf[3] / nH + nHe0 * rf[4] / nH + nHep * rf[5] / nH;
+    double Tfact = 1.0 / (1.0 + sqrt(T / 1e5));
+    double BetaH0 = 7.5e-19 * exp(-118348.0 / T) * Tfact;
+    double BetaHep = 5.54e-17 * pow(T, -0.397) * exp(-473638.0 / T) * Tfact;
+    double lt = 5.5 - log10(T);
+    double Betaff = 1.43e-27 * sqrt(T) * (1.0 + 0.34 * exp(-(lt * lt) / 3.0));
+    double exc = BetaH0 * ne * nH0 + BetaHep * ne * nHep;
+    double ion = 2.18e-11 * r.geH0 * ne * nH0 + 3.94e-11 * r.geHe0 * ne * nHe0 +
+                 8.72e-11 * r.geHep * ne * nHep;
+    double rec = 1.036e-16 * T * ne * (r.aHp * nHp) + 1.036e-16 * T * ne * (r.aHep * nHep) +
+                 1.036e-16 * T * ne * (r.aHepp * nHepp) + 6.526e-11 * r.ad * ne * nHep;
+    double ff = Betaff * (nHp + nHep + 4.0 * nHepp) * ne;
+    return heat - (exc + ion + rec + ff);
+}
+
+// DoCooling_h in cgs; *iters = bisection iterations, *capped = 1 when a cap
+// ended the bracket or the bisection.
+__device__ double net_cool_one(double rho, double u_old, double dt, const NetDev &nd, int *iters,
+                               int *capped) {
+    const double nH = nd.XH * rho / NET_MH;
+    const double ratefact = nH * nH / rho;
+    double u = u_old, u_upper = u, u_lower = u;
+    *iters = 0;
+    *capped = 0;
+    double g0 = u - u_old - ratefact * net_rate(u, nH, nd.XH, nd.rf) * dt;
+    if (g0 == 0.0) return u_old;
+    int nb = 0;
+    if (g0 > 0.0) {
+        while (u_upper - u_old - ratefact * net_rate(u_upper, nH, nd.XH, nd.rf) * dt > 0.0) {
+            if (nb++ >= nd.max_bracket) { *capped = 1; break; }
+            u_upper /= 1.1;
+        }
+        u_lower = u_upper;
+        u_upper = u_lower * 1.1;
+    } else {
+        while (u_upper - u_old - ratefact * net_rate(u_upper, nH, nd.XH, nd.rf) * dt < 0.0) {
+            if (nb++ >= nd.max_bracket) { *capped = 1; break; }
+            u_upper *= 1.1;
+        }
+        u_lower = u_upper / 1.1;
+    }
+    int niter = 1;
+    double du = u;
+    while (fabs(du / u) > nd.tol && niter < nd.max_iter) {
+        u = 0.5 * (u_lower + u_upper);
+        double G = net_rate(u, nH, nd.XH, nd.rf);
+        if (u - u_old - ratefact * G * dt > 0.0) u_upper = u;
+        else u_lower = u;
+        du = fabs(u_upper - u_lower);
+        ++niter;
+    }
+    if (niter >= nd.max_iter) *capped = 1;
+    *iters = niter - 1;
+    return u;
+}
+
+// The network step of a particle in code units.
+__device__ __forceinline__ double net_particle(double rho, double u, double dt, const NetDev &nd,
+                                               int *iters, int *capped) {
+    double un = net_cool_one(rho * nd.unit_rho_cgs, u * nd.unit_u_cgs, dt * nd.unit_time_s, nd,
+                             iters, capped);
+    return un / nd.unit_u_cgs;
+}
+
+__global__ void k_net(NetDev nd, const double *__restrict__ rho, const double *u_in, double dt,
+                      int64_t n, double *u_out, int32_t *__restrict__ iters, int64_t *counters) {
+    int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
+    if (i >= n) return;
+    int it = 0, cap = 0;
+    u_out[i] = net_particle(rho[i], u_in[i], dt, nd, &it, &cap);
+    if (iters) iters[i] = it;
+    if (cap && counters) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
 }
 
 __global__ void k_cool(CoolDev cd, const double *__restrict__ rho, const double *__restrict__ u_in,
@@ -285,13 +444,18 @@ __global__ void k_chimes_apply(ChimesDev c, const double *__restrict__ pos,
 __device__ __forceinline__ double cool_particle(const Soa &s, int64_t i, double u, double dt,
                                                 int use_cool, int cool_on, const CoolDev &cd,
                                                 const GridDev &gd, const ChimesDev &ch,
-                                                int64_t *counters, int *iters) {
+                                                const NetDev &nd, int64_t *counters, int *iters) {
     *iters = 0;
     if (cool_on && use_cool == 1) {
         int it = 0;
         u = cool_one(s.rho[i], u, dt, cd, &it);
         *iters = it;
         if (it >= cd.max_iter) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
+    } else if (use_cool == 4 && nd.on) {
+        int it = 0, cap = 0;
+        u = net_particle(s.rho[i], u, dt, nd, &it, &cap);
+        *iters = it;
+        if (cap) atomicAdd((unsigned long long *)&counters[CNT_COOL_CAPPED], 1ull);
     } else if (use_cool == 2 && gd.du) {
         u = grid_apply(s.rho[i], u, dt, gd);
     } else if (use_cool == 3 && ch.u_evol) {
@@ -328,7 +492,7 @@ __device__ __forceinline__ void count_nonfinite(const Soa &s, int64_t i, double 
 }
 
 __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, CoolDev cd,
-                         int cool_on, GridDev gd, ChimesDev ch, int64_t *counters) {
+                         int cool_on, GridDev gd, ChimesDev ch, NetDev nd, int64_t *counters) {
 #pragma clang fp contract(off)
     int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
     int it = 0;
@@ -342,7 +506,7 @@ __global__ void k_finish(Soa s, const RunParams *__restrict__ rp, int64_t n, Coo
         double du = s.du[i];
         double u = s.u[i] + half * (du + s.ut_prev[i]);
         s.ut_prev[i] = du;
-        u = cool_particle(s, i, u, dt, r.use_cool, cool_on, cd, gd, ch, counters, &it);
+        u = cool_particle(s, i, u, dt, r.use_cool, cool_on, cd, gd, ch, nd, counters, &it);
         s.u[i] = u;
         count_nonfinite(s, i, u, counters);
     }
@@ -420,7 +584,8 @@ __global__ void k_blk_drift(Soa s, int64_t n, double d) {
 __device__ __forceinline__ void blk_close_one(Soa &s, const RunParams *__restrict__ rp, int64_t i,
                                               double dt_max, int B, int64_t t1, double c_cfl,
                                               const CoolDev &cd, int cool_on, const GridDev &gd,
-                                              const ChimesDev &ch, int64_t *counters, int *it) {
+                                              const ChimesDev &ch, const NetDev &nd,
+                                              int64_t *counters, int *it) {
 #pragma clang fp contract(off)
     int b = s.bin[i];
     double dto = ldexp(dt_max, -b);
@@ -431,7 +596,7 @@ __device__ __forceinline__ void blk_close_one(Soa &s, const RunParams *__restric
     double du = s.du[i];
     double u = s.u[i] + half * (du + s.ut_prev[i]);
     s.ut_prev[i] = du;
-    u = cool_particle(s, i, u, dto, rp[s.run[i]].use_cool, cool_on, cd, gd, ch, counters, it);
+    u = cool_particle(s, i, u, dto, rp[s.run[i]].use_cool, cool_on, cd, gd, ch, nd, counters, it);
     s.u[i] = u;
     bool fl = false;
     int bt = blk_target(blk_dt_crit(s, i, c_cfl), dt_max, B, &fl);
@@ -449,12 +614,12 @@ __device__ __forceinline__ void blk_close_one(Soa &s, const RunParams *__restric
 
 __global__ void k_blk_close(Soa s, const RunParams *__restrict__ rp, int64_t n, double dt_max,
                             int B, int64_t t1, double c_cfl, CoolDev cd, int cool_on, GridDev gd,
-                            ChimesDev ch, int64_t *counters) {
+                            ChimesDev ch, NetDev nd, int64_t *counters) {
 #pragma clang fp contract(off)
     int64_t i = (int64_t)blockIdx.x * IT + threadIdx.x;
     int it = 0;
     if (i < n && blk_active(s.bin[i], B, t1))
-        blk_close_one(s, rp, i, dt_max, B, t1, c_cfl, cd, cool_on, gd, ch, counters, &it);
+        blk_close_one(s, rp, i, dt_max, B, t1, c_cfl, cd, cool_on, gd, ch, nd, counters, &it);
     count_cool_iters(it, counters);
 }
 
@@ -589,7 +754,7 @@ CoolDev sphx_engine_cool(const Engine &E) { return make_cool(E); }
 int Engine::finish_step(hipStream_t st) {
     if (n == 0) return SPHX_OK;
     k_finish<<<g1(n), IT, 0, st>>>(s, d_run, n, make_cool(*this), has_cool ? 1 : 0, grid,
-                                   chimes, counters);
+                                   chimes, net, counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
@@ -611,7 +776,8 @@ int Engine::block_drift(double d, hipStream_t st) {
 int Engine::block_close(int64_t t1, hipStream_t st) {
     if (n == 0) return SPHX_OK;
     k_blk_close<<<g1(n), IT, 0, st>>>(s, d_run, n, blk_dtmax, blk_B, t1, blk_c_cfl,
-                                      make_cool(*this), has_cool ? 1 : 0, grid, chimes, counters);
+                                      make_cool(*this), has_cool ? 1 : 0, grid, chimes, net,
+                                      counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
@@ -807,6 +973,109 @@ int sphx_global_dt_launch(const double *h, const double *c, const double *acc, i
     k_dt_part<<<(unsigned)nb, 64, 0, st>>>(h, c, acc, acc + 1, acc + 2, as, acc_g, acc_g + 1,
                                            acc_g + 2, gs, divv, nullptr, nullptr, n, scratch);
     k_dt_one<<<1, 256, 0, st>>>(scratch, nb, c_cfl, dt_min, dt_max, out);
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host side of the network: the background rates (RadiationField,
+// photolibs3.py:230-329): for each ionisation edge (H0 13.60 eV, He0 24.59 eV,
+// He+ 54.42 eV) the trapezoid sums over the frequency grid of
+// 4 pi J sigma / (h nu) (photoionisation) and 4 pi J sigma (nu - nu_edge) / nu
+// (heating), with the cross sections of Verner et al. (1996).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr double NET_H_EV = 4.135667696e-15;  // Planck constant in eV s
+constexpr double NET_CLIGHT = 29979245800.0;  // cm/s
+
+double net_sigma(double nu, const double *c) {
+    double E = NET_H_EV * nu;
+    double x = E / c[0] - c[5];
+    double y = sqrt(x * x + c[6] * c[6]);
+    double Fy = (((x - 1.0) * (x - 1.0) + c[4] * c[4]) * pow(y, 0.5 * c[3] - 5.5)) *
+                pow(1.0 + sqrt(y / c[2]), -c[3]);
+    return c[1] * Fy * 1e-18;
+}
+
+void net_edge(const sphx_network &p, double E, const double *c, double *g, double *H) {
+    const double hplanck = 6.626196e-27;
+    const double nu0 = E / NET_H_EV;
+    const double start = nu0 / 1e16;
+    int64_t n;
+    double step;
+    if (p.n_grid > 0) {
+        n = p.n_grid;
+        step = (6.0 - start) / (double)n;
+    } else {  // the values of np.arange(start, 6.0, 0.02)
+        n = (int64_t)ceil((6.0 - start) / 0.02);
+        step = (start + 0.02) - start;
+    }
+    // the edge frequency through wavelength, as the reference's J_neu_func
+    const double lamb = NET_CLIGHT / nu0 * 1e8;
+    const double nu_edge = NET_CLIGHT / (lamb * 1e-8);
+    std::vector<double> fg(n), fh(n), nu(n);
+    for (int64_t i = 0; i < n; ++i) {
+        nu[i] = (start + (double)i * step) * 1e16;
+        double sig = net_sigma(nu[i], c);
+        double J = p.J21 * 1e-21 / (nu[i] / nu_edge);
+        fg[i] = 4.0 * M_PI * J * sig / hplanck / nu[i];
+        fh[i] = 4.0 * M_PI * J * sig * (nu[i] - nu0) / nu[i];
+    }
+    double d = n > 1 ? nu[1] - nu[0] : 0.0;
+    double sg = 0.0, sh = 0.0;
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        sg += d * (fg[i] + fg[i + 1]) / 2.0;
+        sh += d * (fh[i] + fh[i + 1]) / 2.0;
+    }
+    *g = sg;
+    *H = sh;
+}
+
+}  // namespace
+
+extern "C" int sphx_network_rates(const sphx_network *net, double *out) {
+    if (!net || !out) { sphx_set_error("null argument"); return SPHX_ERR_ARG; }
+    if (!(net->J21 >= 0.0) || net->n_grid < 0 || net->n_grid > 1000000) {
+        sphx_set_error("sphx_network: J21 >= 0 and 0 <= n_grid <= 1e6");
+        return SPHX_ERR_ARG;
+    }
+    // Verner et al. (1996) parameters E0, sigma0, ya, P, yw, y0, y1
+    static const double cH0[7] = {4.298e-1, 5.475e4, 3.288e1, 2.963, 0.0, 0.0, 0.0};
+    static const double cHe0[7] = {1.361e1, 9.492e2, 1.469e0, 3.188, 2.039, 4.434e-1, 2.136};
+    static const double cHep[7] = {1.720e0, 1.369e4, 3.288e1, 2.963, 0.0, 0.0, 0.0};
+    net_edge(*net, 13.60, cH0, &out[0], &out[3]);
+    net_edge(*net, 24.59, cHe0, &out[1], &out[4]);
+    net_edge(*net, 54.42, cHep, &out[2], &out[5]);
+    return SPHX_OK;
+}
+
+int sphx_network_dev(const sphx_network *net, NetDev *out) {
+    if (!net || !out) { sphx_set_error("null argument"); return SPHX_ERR_ARG; }
+    if (!(net->XH > 0.0 && net->XH <= 1.0) || !(net->unit_u_cgs > 0) ||
+        !(net->unit_rho_cgs > 0) || !(net->unit_time_s > 0) || !(net->tol > 0) ||
+        net->max_iter <= 0 || net->max_bracket < 0) {
+        sphx_set_error("invalid sphx_network (XH in (0, 1], positive units and tol, max_iter > 0)");
+        return SPHX_ERR_ARG;
+    }
+    NetDev d;
+    SPHX_CHECK(sphx_network_rates(net, d.rf));
+    d.XH = net->XH;
+    d.tol = net->tol;
+    d.unit_u_cgs = net->unit_u_cgs;
+    d.unit_rho_cgs = net->unit_rho_cgs;
+    d.unit_time_s = net->unit_time_s;
+    d.max_iter = net->max_iter;
+    d.max_bracket = net->max_bracket;
+    d.on = 1;
+    *out = d;
+    return SPHX_OK;
+}
+
+int sphx_net_launch(const NetDev &nd, const double *rho, const double *u_in, double dt, int64_t n,
+                    double *u_out, int32_t *iters, int64_t *counters, hipStream_t st) {
+    if (n <= 0) return SPHX_OK;
+    k_net<<<g1(n), IT, 0, st>>>(nd, rho, u_in, dt, n, u_out, iters, counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -234,6 +234,14 @@ struct GridDev {
     double dt_ref_s = 0, unit_u_cgs = 0, unit_rho_cgs = 0, unit_time_s = 0;
 };
 
+// The analytic H/He network on the device: its parameters and the six
+// background rates (passed by value).
+struct NetDev {
+    double XH = 0, tol = 0, unit_u_cgs = 0, unit_rho_cgs = 0, unit_time_s = 0;
+    double rf[6] = {};
+    int32_t max_iter = 0, max_bracket = 0, on = 0;
+};
+
 // Device view of a 5-D CHIMES table (sphx_chimes_table).  ion: per-particle
 // species fractions indexed by the particle's caller id (plans only; null in
 // batches and stateless calls use their own output).
@@ -287,6 +295,7 @@ struct Engine {
     double *grid_buf = nullptr;
     int64_t grid_cap = 0;
     int set_cool_grid(const sphx_cool_grid *g, hipStream_t st);
+    NetDev net{};              // optional analytic H/He network (runs with use_cool == 4)
     ChimesDev chimes{};        // optional 5-D CHIMES table (runs with use_cool == 3)
     float *chimes_buf = nullptr;
     int64_t chimes_cap = 0;
@@ -519,6 +528,9 @@ struct CoolDev {
     int32_t max_iter, max_bracket, ntab;
     const double *T, *L;
 };
+int sphx_network_dev(const sphx_network *net, NetDev *out);
+int sphx_net_launch(const NetDev &nd, const double *rho, const double *u_in, double dt,
+                    int64_t n, double *u_out, int32_t *iters, int64_t *counters, hipStream_t st);
 int sphx_grid_apply_launch(const GridDev &g, const double *rho, const double *u_in, double dt,
                            int64_t n, double *u_out, hipStream_t st);
 int sphx_grid_build_launch(const GridDev &g, const CoolDev &cd, double *du, hipStream_t st);

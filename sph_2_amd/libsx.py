@@ -774,14 +774,60 @@ def hcooler(r_p, u_p, nH_p, dt_sec, table: ChimesTable, *, device=0):
     return float(un[0] * kw["unit_u_cgs"]), float(mu[0])
 
 
-def DoCooling_h(rho, u_old, dt, XH, table: CoolingTable | None = None):
-    """Scalar cgs form of DoCooling_h(rho, u_old, dt, XH) (PHOTOPY:489): returns
-    u in cgs after an implicit step of dt seconds over the tabulated Lambda(T)."""
-    if table is None:
-        from .ic import cooling_table
-        T, L = cooling_table()
-        table = CoolingTable(T, L, XH=XH)
-    else:
-        table = CoolingTable(table.T, table.L, **{**table.kw, "XH": XH, "unit_u_cgs": 1.0,
-                                                   "unit_rho_cgs": 1.0, "unit_time_s": 1.0})
-    return float(DoCooling(np.array([rho]), np.array([u_old]), dt, table)[0])
+class CoolingNetwork:
+    """The analytic H/He cooling network of photolibs3.py:140-543 (sphx_network).
+
+    J21 is the reference's Jcoeff (the UV background at the H0 edge in units of
+    1e-21 erg/s/cm^2/Hz/sr): 1e-5 in photolibs3.py; `gpu_drivers()` gives the
+    background of the GPU drivers' myPhotoLibsGPU.h (J21 = 50 on 200 frequency
+    points per edge).  Unit factors convert code units to cgs (1 for cgs)."""
+
+    def __init__(self, XH=0.76, J21=1e-5, unit_u_cgs=1.0, unit_rho_cgs=1.0, unit_time_s=1.0,
+                 tol=1e-4, n_grid=0, max_iter=100, max_bracket=400):
+        self.kw = dict(XH=XH, J21=J21, unit_u_cgs=unit_u_cgs, unit_rho_cgs=unit_rho_cgs,
+                       unit_time_s=unit_time_s, tol=tol, n_grid=n_grid, max_iter=max_iter,
+                       max_bracket=max_bracket)
+
+    @classmethod
+    def gpu_drivers(cls, **kw):
+        return cls(**{"J21": 50.0, "n_grid": 200, **kw})
+
+    def c(self) -> _lib.Network:
+        return _lib.Network(**self.kw, pad=0)
+
+    def rates(self):
+        """RadiationField (photolibs3.py:230-329): gJH0, gJHe0, gJHep (s^-1) and the
+        heating rates HRate_H0, HRate_He0, HRate_Hep (erg/s)."""
+        out = (C.c_double * 6)()
+        c = self.c()
+        check(_lib.load().sphx_network_rates(C.byref(c), out), "network rates")
+        return np.array(out[:])
+
+
+def cool_network(rho, u, dt, network: CoolingNetwork, *, device=0, return_iters=False):
+    """DoCooling_h of the network on arrays in the network's code units."""
+    t = _t()
+    dr, du = _rho_u(rho, u)
+    n = len(dr)
+    out = t.empty(n, dtype=t.float64, device="cuda")
+    it = t.empty(n, dtype=t.int32, device="cuda")
+    c = network.c()
+    rc = check(_lib.load().sphx_cool_network(_context(device), n, ptr(dr), ptr(du), float(dt),
+                                            C.byref(c), ptr(out), ptr(it), _lib.stream_handle()),
+               "cool_network")
+    if rc == _lib.CAPPED:
+        warnings.warn("network cooling solve capped for some particles", RuntimeWarning)
+    res = out.cpu().numpy()
+    return (res, it.cpu().numpy()) if return_iters else res
+
+
+def DoCooling_h(rho, u_old, dt, XH, network: CoolingNetwork | None = None):
+    """DoCooling_h(rho, u_old, dt, XH) (photolibs3.py:489-543) in cgs: the u after an
+    implicit step of dt seconds of the analytic H/He network (photolibs3.py's
+    background unless `network` is given).  Scalars in, a float out, as the
+    reference; arrays give arrays."""
+    net = CoolingNetwork(**{**(network.kw if network is not None else {}), "XH": XH,
+                            "unit_u_cgs": 1.0, "unit_rho_cgs": 1.0, "unit_time_s": 1.0})
+    scalar = np.ndim(rho) == 0 and np.ndim(u_old) == 0
+    out = cool_network(np.atleast_1d(rho), np.atleast_1d(u_old), dt, net)
+    return float(out[0]) if scalar else out

@@ -142,6 +142,12 @@ class SphPlan:
         check(_lib.load().sphx_plan_set_chimes_table(self._h, C.byref(c)), "set_chimes_table")
         self._n_species = table.n_species
 
+    def set_cooling_network(self, network):
+        """Attach a libsx.CoolingNetwork; step(..., cooling="network") uses it."""
+        c = network.c()
+        check(_lib.load().sphx_plan_set_cooling_network(self._h, C.byref(c)),
+              "set_cooling_network")
+
     def chimes_ion_fractions(self):
         """Species fractions of every particle at its last CHIMES lookup,
         (n, n_species) float32 in the original particle order (the ionFrac
@@ -154,7 +160,8 @@ class SphPlan:
 
     def step(self, n_steps: int, dt: float, cooling=False) -> int:
         """cooling: False, True / "solve" (implicit Lambda(T) solve), "table"
-        (2-D table) or "chimes" (5-D CHIMES table)."""
+        (2-D table), "chimes" (5-D CHIMES table) or "network" (the analytic H/He
+        network)."""
         return check(_lib.load().sphx_plan_step(self._h, int(n_steps), float(dt),
                                                 _lib.cooling_mode(cooling), _lib.stream_handle()),
                      "sphx_plan_step")

@@ -69,6 +69,12 @@ class Batch:
         g = grid.c()
         check(_lib.load().sphx_batch_set_cooling_grid(self._h, C.byref(g)), "set_cooling_grid")
 
+    def set_cooling_network(self, network):
+        """Attach a libsx.CoolingNetwork for runs admitted with cooling="network"."""
+        c = network.c()
+        check(_lib.load().sphx_batch_set_cooling_network(self._h, C.byref(c)),
+              "set_cooling_network")
+
     def counters(self) -> dict:
         """Counters of the last step over all live runs (h-iteration caps,
         cooling caps, non-finite particles, ...)."""
