@@ -316,18 +316,21 @@ struct BhArgs {
     int64_t *ovf;    // engine counter of overflowing groups (nullable)
 };
 
+// One list entry against lane i, its geometry already formed: (dx, dy, dz)
+// from i to the entry, r2 = |d|^2 and t = eps_i + eps_j, twice the pair's
+// softening e = (eps_i + eps_j) / 2.  Newtonian where r2 >= t^2 (that is
+// r >= 2e: (4e) e and t t round the same product), the cubic-spline softened
+// kernel inside.
 template <bool POT>
-__device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, float4 j, float ej,
+__device__ __forceinline__ void bh_eval(float dx, float dy, float dz, float r2, float t, float mj,
                                         float &ax, float &ay, float &az, float &ph) {
-    float dx = j.x - xi, dy = j.y - yi, dz = j.z - zi;
-    float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-    float e = 0.5f * (ei + ej);
     float f, p;
-    if (r2 >= 4.0f * e * e) {
+    if (r2 >= t * t) {
         float ri = __builtin_amdgcn_rsqf(r2);
         f = ri * ri * ri;
         p = ri;
     } else {
+        float e = 0.5f * t;
         float r = r2 > 0.0f ? __builtin_amdgcn_sqrtf(r2) : 0.0f;
         float ie = __builtin_amdgcn_rcpf(e);
         float q = r * ie;
@@ -347,11 +350,33 @@ __device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, 
                     : 0.0f;
         }
     }
-    float s = j.w * f;
+    float s = mj * f;
     ax = __builtin_fmaf(s, dx, ax);
     ay = __builtin_fmaf(s, dy, ay);
     az = __builtin_fmaf(s, dz, az);
-    if (POT) ph = __builtin_fmaf(j.w, p, ph);
+    if (POT) ph = __builtin_fmaf(mj, p, ph);
+}
+
+// The Newtonian branch of bh_eval alone, for entries every lane sees at
+// r >= 2e: the same operations, so the same bits.
+template <bool POT>
+__device__ __forceinline__ void bh_far(float dx, float dy, float dz, float r2, float mj, float &ax,
+                                       float &ay, float &az, float &ph) {
+    float ri = __builtin_amdgcn_rsqf(r2);
+    float f = ri * ri * ri;
+    float s = mj * f;
+    ax = __builtin_fmaf(s, dx, ax);
+    ay = __builtin_fmaf(s, dy, ay);
+    az = __builtin_fmaf(s, dz, az);
+    if (POT) ph = __builtin_fmaf(mj, ri, ph);
+}
+
+template <bool POT>
+__device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, float4 j, float ej,
+                                        float &ax, float &ay, float &az, float &ph) {
+    float dx = j.x - xi, dy = j.y - yi, dz = j.z - zi;
+    float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+    bh_eval<POT>(dx, dy, dz, r2, ei + ej, j.w, ax, ay, az, ph);
 }
 
 template <bool POT>
@@ -390,8 +415,26 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
             float e[8];
 #pragma unroll
             for (int k = 0; k < 8; ++k) { j[k] = il[q + k]; e[k] = ie[q + k]; }
+            float dx[8], dy[8], dz[8], r2[8], t[8];
+            bool far = true;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) bh_pair<POT>(xi, yi, zi, ei, j[k], e[k], ax, ay, az, ph);
+            for (int k = 0; k < 8; ++k) {
+                dx[k] = j[k].x - xi;
+                dy[k] = j[k].y - yi;
+                dz[k] = j[k].z - zi;
+                r2[k] = __builtin_fmaf(dx[k], dx[k], __builtin_fmaf(dy[k], dy[k], dz[k] * dz[k]));
+                t[k] = ei + e[k];
+                far = far && r2[k] >= t[k] * t[k];
+            }
+            // lanes past the group's end take part in the vote as far
+            if (__all(far || !ok)) {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) bh_far<POT>(dx[k], dy[k], dz[k], r2[k], j[k].w, ax, ay, az, ph);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    bh_eval<POT>(dx[k], dy[k], dz[k], r2[k], t[k], j[k].w, ax, ay, az, ph);
+            }
         }
         for (; q < icnt; ++q) bh_pair<POT>(xi, yi, zi, ei, il[q], ie[q], ax, ay, az, ph);
         kx.add(ax); ky.add(ay); kz.add(az); kp.add(ph);

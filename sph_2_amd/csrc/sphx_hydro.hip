@@ -374,6 +374,27 @@ __device__ __forceinline__ int64_t nbr_slot(int k, int lane) {
     return ((int64_t)(k >> 3) * 64 + lane) * 8 + (k & 7);
 }
 
+// Calls f(j) for entries 0 .. len-1 of this lane's list, every lane on its
+// own entry k at once (maxlen: the wave's longest list); each 8-entry run
+// comes in with two 16-byte loads.
+template <class F>
+__device__ __forceinline__ void for_list(const int32_t *lp, int lane, int len, int maxlen, F &&f) {
+    for (int k0 = 0; k0 < maxlen; k0 += 8) {
+        int4 a = make_int4(0, 0, 0, 0), c = a;
+        if (k0 < len) {
+            const int4 *p = reinterpret_cast<const int4 *>(lp + nbr_slot(k0, lane));
+            a = p[0];
+            c = p[1];
+        }
+#pragma unroll 1
+        for (int q = 0; q < 8 && k0 + q < maxlen; ++q) {
+            if (k0 + q < len) f(a.x);
+            a.x = a.y; a.y = a.z; a.z = a.w; a.w = c.x;
+            c.x = c.y; c.y = c.z; c.z = c.w;
+        }
+    }
+}
+
 struct DensArgs {
     double gamma;
     int32_t *nsym;        // symmetric neighbour count, nullable
@@ -767,16 +788,14 @@ __global__ __launch_bounds__(64) void k_divcurl_list(Tables t, Soa s, ListArgs L
     const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64;
     KahanF kd, kx, ky, kz;
     kd.init(); kx.init(); ky.init(); kz.init();
-    for (int k = 0; k < maxlen; ++k) {
-        if (k < len) {
-            const RecA o = load_rec_a(L.rec, lp[nbr_slot(k, lane)]);
-            double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-            double r2 = sphx_r2(dx, dy, dz);
-            double tt = me.h + o.h;
-            divcurl_pair(me.vx, me.vy, me.vz, o.vx, o.vy, o.vz, o.m, dx, dy, dz, r2, tt, kd, kx,
-                         ky, kz);
-        }
-    }
+    for_list(lp, lane, len, maxlen, [&](int j) {
+        const RecA o = load_rec_a(L.rec, j);
+        double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+        double r2 = sphx_r2(dx, dy, dz);
+        double tt = me.h + o.h;
+        divcurl_pair(me.vx, me.vy, me.vz, o.vx, o.vy, o.vz, o.m, dx, dy, dz, r2, tt, kd, kx, ky,
+                     kz);
+    });
     divcurl_store(s, i, ok, kd, kx, ky, kz);
 }
 
@@ -806,19 +825,16 @@ __global__ __launch_bounds__(64) void k_hydro_list(Tables t, Soa s, HydroArgs a,
     const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64;
     KahanF kx, ky, kz, kdu;
     kx.init(); ky.init(); kz.init(); kdu.init();
-    for (int k = 0; k < maxlen; ++k) {
-        if (k < len) {
-            const int j = lp[nbr_slot(k, lane)];
-            const RecA o = load_rec_a(L.rec, j);
-            const uint4 e = load_rec_b(L.rec, j);
-            const float fj = a.balsara ? L.fbal[j] : 1.0f;
-            double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
-            double r2 = sphx_r2(dx, dy, dz);
-            double tt = me.h + o.h;
-            hydro_pair(a, I, o.vx, o.vy, o.vz, o.m, o.rho, __uint_as_float(e.x),
-                       __uint_as_float(e.y), fj, dx, dy, dz, r2, tt, kx, ky, kz, kdu);
-        }
-    }
+    for_list(lp, lane, len, maxlen, [&](int j) {
+        const RecA o = load_rec_a(L.rec, j);
+        const uint4 e = load_rec_b(L.rec, j);
+        const float fj = a.balsara ? L.fbal[j] : 1.0f;
+        double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+        double r2 = sphx_r2(dx, dy, dz);
+        double tt = me.h + o.h;
+        hydro_pair(a, I, o.vx, o.vy, o.vz, o.m, o.rho, __uint_as_float(e.x), __uint_as_float(e.y),
+                   fj, dx, dy, dz, r2, tt, kx, ky, kz, kdu);
+    });
     if (ok) {
         s.ax[i] = kx.value();
         s.ay[i] = ky.value();
