@@ -162,6 +162,9 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     while (true) {
         bool active = ok && !done;
         if (!__any(active)) break;
+        // once some lanes have settled, the walk culls against the box of the
+        // lanes still counting: only their counts are read
+        const bool all_active = !__any(ok && !active);
         double s2 = sphx_sq(sr);
         int kc = 0;
         for (int g0 = 0; g0 < cnt;) {
@@ -169,9 +172,9 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
             bool mine = lane >= g0 && lane < g1;
             double R = wave_max_d((active && mine) ? sr : 0.0);
             double gb[6];
-            if (cuts) {
+            if (cuts || !all_active) {
                 double hdummy;
-                seg_box(ok && mine, xi, yi, zi, 0.0, gb, &hdummy);
+                seg_box(active && mine, xi, yi, zi, 0.0, gb, &hdummy);
             } else {
                 for (int k = 0; k < 6; ++k) gb[k] = ib[k];
             }
