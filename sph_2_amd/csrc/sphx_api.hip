@@ -366,11 +366,14 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
         SPHX_CHECK(sphx_gravity_run(*this, gcfg, with_pot, st));
     }
     SPHX_CHECK(tmark(st, PH_GRAV, 1));
+    // without the Balsara switch hydro reads nothing of div/curl's output, and
+    // one pass over the lists gives both (timed as the div/curl phase)
+    const bool fused = lists && !p.balsara;
     SPHX_CHECK(tmark(st, PH_DIVCURL, 0));
-    SPHX_CHECK(divcurl(st, lists));
+    SPHX_CHECK(fused ? divcurl_hydro_lists(st) : divcurl(st, lists));
     SPHX_CHECK(tmark(st, PH_DIVCURL, 1));
     SPHX_CHECK(tmark(st, PH_HYDRO, 0));
-    SPHX_CHECK(hydro(st, lists));
+    if (!fused) SPHX_CHECK(hydro(st, lists));
     SPHX_CHECK(tmark(st, PH_HYDRO, 1));
     SPHX_CHECK(sphx_combine(*this, st));
     return SPHX_OK;

@@ -55,6 +55,59 @@ __device__ __forceinline__ void for_candidates(const Tables &t, int r, const dou
     }
 }
 
+// for_candidates with the candidate loads run ahead: `load(J, beg, cnt)`
+// issues the reads `body(J, v)` needs of bucket J (lane q: particle beg + q,
+// q < cnt) and is called for the next surviving bucket before `body` runs on
+// the current one, so a walk no longer waits on each bucket's loads.  The
+// buckets' first particle and count come with the bucket culling, one bucket
+// per lane, and reach the loads by readlane.  Buckets are met in the same
+// order as by for_candidates.
+template <class ReachS, class ReachB, class Load, class Body>
+__device__ __forceinline__ void for_candidates_pf(const Tables &t, int r, const double *ib, int lane,
+                                                  ReachS reach_s, ReachB reach_b, Load load,
+                                                  Body body) {
+    double axl = ib[0], axh = ib[1], ayl = ib[2], ayh = ib[3], azl = ib[4], azh = ib[5];
+    int s0 = t.r_sbeg[r], s1 = t.r_send[r];
+    decltype(load(0, 0, 0)) cur{};
+    int Jc = -1;
+    for (int sb = s0; sb < s1; sb += 64) {
+        int S = sb + lane;
+        bool sp = false;
+        if (S < s1) {
+            double d2 = sphx_box_d2(axl, axh, ayl, ayh, azl, azh, t.sxl[S], t.sxh[S], t.syl[S],
+                                    t.syh[S], t.szl[S], t.szh[S]);
+            sp = sphx_within(d2, reach_s(S));
+        }
+        uint64_t smask = __ballot(sp);
+        while (smask) {
+            int S2 = sb + __builtin_ctzll(smask);
+            smask &= smask - 1;
+            int j0 = t.s_bbeg[S2], j1 = t.s_bend[S2];
+            int J = j0 + lane;
+            bool bp = false;
+            int vbeg = 0, vcnt = 0;
+            if (J < j1) {
+                double d2 = sphx_box_d2(axl, axh, ayl, ayh, azl, azh, t.bxl[J], t.bxh[J],
+                                        t.byl[J], t.byh[J], t.bzl[J], t.bzh[J]);
+                bp = sphx_within(d2, reach_b(J));
+                vbeg = t.b_beg[J];
+                vcnt = t.b_cnt[J];
+            }
+            uint64_t bmask = __ballot(bp);
+            while (bmask) {
+                int q = __builtin_ctzll(bmask);
+                bmask &= bmask - 1;
+                auto nxt = load(j0 + q, __builtin_amdgcn_readlane(vbeg, q),
+                                __builtin_amdgcn_readlane(vcnt, q));
+                if (Jc >= 0) body(Jc, cur);
+                cur = nxt;
+                Jc = j0 + q;
+            }
+        }
+    }
+    if (Jc >= 0) body(Jc, cur);
+}
+
 // Segments of a bucket.  64 consecutive Morton-sorted particles usually fill
 // one compact cell, but where the curve jumps between distant cells their
 // bounding box spans much of the run and would make every bucket of that
@@ -180,18 +233,25 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
             }
             g0 = g1;
             if (!(R > 0.0)) continue;  // no active lane in this segment
-            for_candidates(
+            struct Cand {
+                double x, y, z;
+                int n;
+            };
+            for_candidates_pf(
                 t, r, gb, lane, [&](int) { return R; }, [&](int) { return R; },
-                [&](int J) {
-                    int jc = t.b_cnt[J], j0 = t.b_beg[J];
-                    double xj = 0.0, yj = 0.0, zj = 0.0;
-                    bool keep = false;
+                [&](int, int jb, int jc) {
+                    Cand c{0.0, 0.0, 0.0, jc};
                     if (lane < jc) {
-                        xj = s.x[j0 + lane];
-                        yj = s.y[j0 + lane];
-                        zj = s.z[j0 + lane];
-                        keep = near_box(gb, xj, yj, zj, R);
+                        c.x = s.x[jb + lane];
+                        c.y = s.y[jb + lane];
+                        c.z = s.z[jb + lane];
                     }
+                    return c;
+                },
+                [&](int, const Cand &c) {
+                    double xj = c.x, yj = c.y, zj = c.z;
+                    bool keep = false;
+                    if (lane < c.n) keep = near_box(gb, xj, yj, zj, R);
                     Kept k = keep_mask(keep, lane);
                     if (k.n == 0) return;
                     __syncthreads();
@@ -439,20 +499,28 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
           hS = hI;
       }
       g0 = g1;
-      for_candidates(
+      struct Cand {
+          double x, y, z, h, m;
+          int beg, n;
+      };
+      for_candidates_pf(
         t, r, gb, lane, [&](int S) { return hS + t.shmax[S]; }, [&](int J) { return hS + t.bhmax[J]; },
-        [&](int J) {
-            int jc = t.b_cnt[J], j0 = t.b_beg[J];
-            ++ncand;
-            double xj = 0.0, yj = 0.0, zj = 0.0, hj = 0.0;
-            bool keep = false;
+        [&](int, int jb, int jc) {
+            Cand c{0.0, 0.0, 0.0, 0.0, 0.0, jb, jc};
             if (lane < jc) {
-                xj = s.x[j0 + lane];
-                yj = s.y[j0 + lane];
-                zj = s.z[j0 + lane];
-                hj = s.h[j0 + lane];
-                keep = near_box(gb, xj, yj, zj, hS + hj);
+                c.x = s.x[jb + lane];
+                c.y = s.y[jb + lane];
+                c.z = s.z[jb + lane];
+                c.h = s.h[jb + lane];
+                c.m = s.m[jb + lane];
             }
+            return c;
+        },
+        [&](int, const Cand &c) {
+            ++ncand;
+            double xj = c.x, yj = c.y, zj = c.z, hj = c.h;
+            bool keep = false;
+            if (lane < c.n) keep = near_box(gb, xj, yj, zj, hS + hj);
             Kept k = keep_mask(keep, lane);
             if (k.n == 0) return;
             __syncthreads();
@@ -461,8 +529,8 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                 sy[k.slot] = yj;
                 sz[k.slot] = zj;
                 sh[k.slot] = hj;
-                sm[k.slot] = (float)s.m[j0 + lane];
-                sj[k.slot] = j0 + lane;
+                sm[k.slot] = (float)c.m;
+                sj[k.slot] = c.beg + lane;
             }
             __syncthreads();
             // candidates four at a time, their LDS loads issued before the tests
@@ -846,6 +914,56 @@ __global__ __launch_bounds__(64) void k_hydro_list(Tables t, Soa s, HydroArgs a,
     }
 }
 
+// div/curl and hydro in one list pass, for steps without the Balsara switch
+// (hydro then reads nothing div/curl produces): each neighbour's record is
+// gathered once for both, and each of the eight sums meets its pairs in the
+// list order, as the two separate passes do, so the results are the same.
+__global__ __launch_bounds__(64) void k_dchydro_list(Tables t, Soa s, HydroArgs a, ListArgs L) {
+    const int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
+    if (L.ovf[b]) return;  // the walk launches take this bucket
+    const int lane = threadIdx.x;
+    const int cnt = t.b_cnt[b];
+    const int i = t.b_beg[b] + lane;
+    const bool ok = lane < cnt;
+    RecA me{};
+    HydroI I;
+    I.vx = I.vy = I.vz = 0.0;
+    I.rho = 1.0f; I.pr = 0.0f; I.c = 0.0f; I.f = 1.0f;
+    int len = 0;
+    if (ok) {
+        me = load_rec_a(L.rec, i);
+        uint4 e = load_rec_b(L.rec, i);
+        I.vx = me.vx; I.vy = me.vy; I.vz = me.vz;
+        I.rho = me.rho;
+        I.pr = __uint_as_float(e.x);
+        I.c = __uint_as_float(e.y);
+        len = (int)e.z;
+    }
+    const int maxlen = wave_max_i(len);
+    const int32_t *lp = L.lst + (int64_t)b * SPHX_NBR_MAX * 64;
+    KahanF kd, kx, ky, kz, hx, hy, hz, hdu;
+    kd.init(); kx.init(); ky.init(); kz.init();
+    hx.init(); hy.init(); hz.init(); hdu.init();
+    for_list(lp, lane, len, maxlen, [&](int j) {
+        const RecA o = load_rec_a(L.rec, j);
+        const uint4 e = load_rec_b(L.rec, j);
+        double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+        double r2 = sphx_r2(dx, dy, dz);
+        double tt = me.h + o.h;
+        divcurl_pair(me.vx, me.vy, me.vz, o.vx, o.vy, o.vz, o.m, dx, dy, dz, r2, tt, kd, kx, ky,
+                     kz);
+        hydro_pair(a, I, o.vx, o.vy, o.vz, o.m, o.rho, __uint_as_float(e.x), __uint_as_float(e.y),
+                   1.0f, dx, dy, dz, r2, tt, hx, hy, hz, hdu);
+    });
+    divcurl_store(s, i, ok, kd, kx, ky, kz);
+    if (ok) {
+        s.ax[i] = hx.value();
+        s.ay[i] = hy.value();
+        s.az[i] = hz.value();
+        s.du[i] = hdu.value();
+    }
+}
+
 // The walk for the buckets whose lists overflowed: each wave scans 64 flags
 // at a time and walks the flagged buckets of its share.
 __global__ __launch_bounds__(64) void k_divcurl_ovf(Tables t, Soa s, const int32_t *ovf, int nb) {
@@ -1135,6 +1253,22 @@ int Engine::divcurl(hipStream_t st, bool lists) {
     } else {
         k_divcurl<<<nb_launch(), 64, 0, st>>>(tview(), s);
     }
+    SPHX_HIP(hipGetLastError());
+    return SPHX_OK;
+}
+
+int Engine::divcurl_hydro_lists(hipStream_t st) {
+    if (n == 0 || nb_launch() == 0) return SPHX_OK;
+    if (p.balsara || !lists_ready()) {
+        sphx_set_error("the fused list pass needs the lists and no Balsara switch");
+        return SPHX_ERR_STATE;
+    }
+    HydroArgs a{(float)p.alpha, (float)p.beta, (float)(p.eta * p.eta), (float)p.av_w_eps,
+                p.av_form, p.balsara};
+    ListArgs L{nbr, prec, nbr_ovf, fbal};
+    k_dchydro_list<<<nb_launch(), 64, 0, st>>>(tview(), s, a, L);
+    k_divcurl_ovf<<<ovf_grid(nb_launch()), 64, 0, st>>>(tview(), s, nbr_ovf, (int)nb_launch());
+    k_hydro_ovf<<<ovf_grid(nb_launch()), 64, 0, st>>>(tview(), s, a, nbr_ovf, (int)nb_launch());
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -428,6 +428,8 @@ struct Engine {
     int gravity(hipStream_t st, bool with_pot);
     int divcurl(hipStream_t st, bool lists = false);
     int hydro(hipStream_t st, bool lists = false);
+    // div/curl and hydro in one pass over the lists (lists on, no Balsara)
+    int divcurl_hydro_lists(hipStream_t st);
     int forces(hipStream_t st, bool predictor, bool with_pot);
 
     // plain FP64 baseline pieces
