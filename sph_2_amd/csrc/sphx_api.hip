@@ -367,7 +367,8 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
     }
     SPHX_CHECK(tmark(st, PH_GRAV, 1));
     // without the Balsara switch hydro reads nothing of div/curl's output, and
-    // one pass over the lists gives both (timed as the div/curl phase)
+    // one pass over the lists gives both and the total acceleration (timed as
+    // the div/curl phase)
     const bool fused = lists && !p.balsara;
     SPHX_CHECK(tmark(st, PH_DIVCURL, 0));
     SPHX_CHECK(fused ? divcurl_hydro_lists(st) : divcurl(st, lists));
@@ -375,7 +376,8 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
     SPHX_CHECK(tmark(st, PH_HYDRO, 0));
     if (!fused) SPHX_CHECK(hydro(st, lists));
     SPHX_CHECK(tmark(st, PH_HYDRO, 1));
-    SPHX_CHECK(sphx_combine(*this, st));
+    // the fused pass adds gravity to the accelerations as it stores them
+    if (!fused) SPHX_CHECK(sphx_combine(*this, st));
     return SPHX_OK;
 }
 

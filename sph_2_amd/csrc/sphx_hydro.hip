@@ -697,7 +697,7 @@ __global__ __launch_bounds__(64) void k_divcurl(Tables t, Soa s) {
 
 // Hydro acceleration and du/dt of the particles of bucket b by the candidate walk.
 __device__ __forceinline__ void hydro_bucket(const Tables &t, const Soa &s, const HydroArgs &a,
-                                             int b) {
+                                             int b, bool add_g = false) {
     __shared__ double sx[64], sy[64], sz[64], sh[64], svx[64], svy[64], svz[64];
     __shared__ float sm[64], srho[64], spr[64], sc[64], sf[64];
     int lane = threadIdx.x;
@@ -787,9 +787,10 @@ __device__ __forceinline__ void hydro_bucket(const Tables &t, const Soa &s, cons
         });
     }
     if (ok) {
-        s.ax[i] = kx.value();
-        s.ay[i] = ky.value();
-        s.az[i] = kz.value();
+        // add_g: the total acceleration, gravity + a_sph, as sphx_combine adds it
+        s.ax[i] = add_g ? s.gx[i] + kx.value() : kx.value();
+        s.ay[i] = add_g ? s.gy[i] + ky.value() : ky.value();
+        s.az[i] = add_g ? s.gz[i] + kz.value() : kz.value();
         s.du[i] = kdu.value();
     }
 }
@@ -914,12 +915,14 @@ __global__ __launch_bounds__(64) void k_hydro_list(Tables t, Soa s, HydroArgs a,
     }
 }
 
-// div/curl and hydro in one list pass, for steps without the Balsara switch
-// (hydro then reads nothing div/curl produces): each neighbour's record is
-// gathered once for both, and each of the eight sums meets its pairs in the
-// list order, as the two separate passes do, so the results are the same.
-__global__ __launch_bounds__(64) void k_dchydro_list(Tables t, Soa s, HydroArgs a, ListArgs L) {
-    const int b = t.boff + xcd_block(blockIdx.x, gridDim.x);
+// div/curl and hydro of bucket b in one list pass, for steps without the
+// Balsara switch (hydro then reads nothing div/curl produces): each
+// neighbour's record is gathered once for both, and each of the eight sums
+// meets its pairs in the list order, as the two separate passes do, so the
+// results are the same.  The stored acceleration is the total, gravity +
+// a_sph, added as sphx_combine adds it.
+__device__ __forceinline__ void dchydro_bucket(const Tables &t, const Soa &s, const HydroArgs &a,
+                                               const ListArgs &L, int b) {
     if (L.ovf[b]) return;  // the walk launches take this bucket
     const int lane = threadIdx.x;
     const int cnt = t.b_cnt[b];
@@ -957,11 +960,15 @@ __global__ __launch_bounds__(64) void k_dchydro_list(Tables t, Soa s, HydroArgs 
     });
     divcurl_store(s, i, ok, kd, kx, ky, kz);
     if (ok) {
-        s.ax[i] = hx.value();
-        s.ay[i] = hy.value();
-        s.az[i] = hz.value();
+        s.ax[i] = s.gx[i] + hx.value();
+        s.ay[i] = s.gy[i] + hy.value();
+        s.az[i] = s.gz[i] + hz.value();
         s.du[i] = hdu.value();
     }
+}
+
+__global__ __launch_bounds__(64) void k_dchydro_list(Tables t, Soa s, HydroArgs a, ListArgs L) {
+    dchydro_bucket(t, s, a, L, t.boff + xcd_block(blockIdx.x, gridDim.x));
 }
 
 // The walk for the buckets whose lists overflowed: each wave scans 64 flags
@@ -979,14 +986,14 @@ __global__ __launch_bounds__(64) void k_divcurl_ovf(Tables t, Soa s, const int32
 }
 
 __global__ __launch_bounds__(64) void k_hydro_ovf(Tables t, Soa s, HydroArgs a,
-                                                  const int32_t *ovf, int nb) {
+                                                  const int32_t *ovf, int nb, int add_g) {
     const int lane = threadIdx.x;
     for (int base = blockIdx.x * 64; base < nb; base += gridDim.x * 64) {
         uint64_t m = __ballot(base + lane < nb && ovf[t.boff + base + lane] != 0);
         while (m) {
             int q = __builtin_ctzll(m);
             m &= m - 1;
-            hydro_bucket(t, s, a, t.boff + base + q);
+            hydro_bucket(t, s, a, t.boff + base + q, add_g != 0);
         }
     }
 }
@@ -1266,9 +1273,10 @@ int Engine::divcurl_hydro_lists(hipStream_t st) {
     HydroArgs a{(float)p.alpha, (float)p.beta, (float)(p.eta * p.eta), (float)p.av_w_eps,
                 p.av_form, p.balsara};
     ListArgs L{nbr, prec, nbr_ovf, fbal};
-    k_dchydro_list<<<nb_launch(), 64, 0, st>>>(tview(), s, a, L);
-    k_divcurl_ovf<<<ovf_grid(nb_launch()), 64, 0, st>>>(tview(), s, nbr_ovf, (int)nb_launch());
-    k_hydro_ovf<<<ovf_grid(nb_launch()), 64, 0, st>>>(tview(), s, a, nbr_ovf, (int)nb_launch());
+    const int nb = (int)nb_launch();
+    k_dchydro_list<<<nb, 64, 0, st>>>(tview(), s, a, L);
+    k_divcurl_ovf<<<ovf_grid(nb), 64, 0, st>>>(tview(), s, nbr_ovf, nb);
+    k_hydro_ovf<<<ovf_grid(nb), 64, 0, st>>>(tview(), s, a, nbr_ovf, nb, 1);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }
@@ -1281,7 +1289,7 @@ int Engine::hydro(hipStream_t st, bool lists) {
         ListArgs L{nbr, prec, nbr_ovf, fbal};
         k_hydro_list<<<nb_launch(), 64, 0, st>>>(tview(), s, a, L);
         k_hydro_ovf<<<ovf_grid(nb_launch()), 64, 0, st>>>(tview(), s, a, nbr_ovf,
-                                                         (int)nb_launch());
+                                                         (int)nb_launch(), 0);
     } else {
         k_hydro<<<nb_launch(), 64, 0, st>>>(tview(), s, a);
     }

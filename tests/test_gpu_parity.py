@@ -400,8 +400,9 @@ def test_cell_level_does_not_change_results(gpu):
 
 @pytest.mark.parametrize("case", ["evrard", "cloud_balsara", "overflow"])
 def test_neighbor_lists_do_not_change_results(gpu, case):
-    """div/curl and hydro over the density pass's neighbour lists give the same
-    bits as the candidate walks: Evrard; a turbulent cloud with the Gadget
+    """div/curl and hydro over the density pass's neighbour lists (one fused
+    pass that also adds gravity, without the Balsara switch; two passes with
+    it) give the same bits as the candidate walks: Evrard; a turbulent cloud with the Gadget
     viscosity and the Balsara switch (per-particle factors for the lists); and
     buckets whose lists overflow (h held at three times its converged value for
     the particles with x > 0.3 by an iteration cap of one count, so their
