@@ -334,9 +334,12 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
 
 // Cubic spline W / (sigma / h^3) and grad-W factor F / (sigma / h^5) in FP32.
 __device__ __forceinline__ float w_shape(float q) {
-    if (q <= 1.0f) return 1.0f - 1.5f * q * q + 0.75f * q * q * q;
+    // both pieces, then a select: the lanes of a wave straddle q = 1, so a
+    // branch would run both anyway and add the exec-mask bookkeeping
+    float w1 = 1.0f - 1.5f * q * q + 0.75f * q * q * q;
     float w = fmaxf(2.0f - q, 0.0f);
-    return 0.25f * w * w * w;
+    float w2 = 0.25f * w * w * w;
+    return q <= 1.0f ? w1 : w2;
 }
 __device__ __forceinline__ float f_shape(float q) {
     if (q <= 1.0f) return -3.0f + 2.25f * q;
@@ -486,6 +489,9 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
     int nn = 0;
     int ncand = 0;
     int32_t *lst = a.lst ? a.lst + (int64_t)b * SPHX_NBR_MAX * 64 : nullptr;
+    // this lane's first run of the block: entry k at ((k / 8) * 64 * 8) + k mod 8
+    // from here (nbr_slot(k, lane) - nbr_slot(0, lane))
+    int32_t *lane_lst = lst ? lst + lane * 8 : nullptr;
     const int r = t.b_run[b];
     const uint64_t cuts = bucket_cuts(t, r, i, ok, lane);
     for (int g0 = 0; g0 < cnt;) {
@@ -539,7 +545,7 @@ __global__ __launch_bounds__(64) void k_density(Tables t, Soa s, DensArgs a) {
                     PairGeom g = pair_geom(dx, dy, dz, r2, tt);
                     float h3 = g.hinv * g.hinv * g.hinv;
                     acc.add(sm[q] * (SPHX_SIGMA_F * h3 * w_shape(g.q)));
-                    if (lst && nn < SPHX_NBR_MAX) lst[nbr_slot(nn, lane)] = sj[q];
+                    if (lst && nn < SPHX_NBR_MAX) lane_lst[((nn >> 3) << 9) | (nn & 7)] = sj[q];
                     ++nn;
                 };
             int q0 = 0;
