@@ -380,7 +380,18 @@ struct SymTabs {
     const int32_t *run;            // run of each particle
     const int32_t *r_pbeg;         // first particle of each run
     int64_t n;
+    const int32_t *order;          // launch order: group g takes segment order[seg0 + g] (global)
 };
+
+// The segment a group takes.  Groups start roughly in index order and the
+// last ones set the end of the launch, so the host orders each run's
+// segments of a band with the ones nearest the diagonal of the block-pair
+// triangle first: those hold the diagonal block and the block pairs within
+// softening reach, the longest work, and the short far segments fill the
+// tail.  A segment's sums go to its own slots, so the order changes no result.
+__device__ __forceinline__ int sym_seg(const SymTabs &T) {
+    return T.order[T.seg0 + blockIdx.x] - T.seg0;
+}
 
 __device__ __forceinline__ float sym_rot(float v, int addr) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
@@ -484,6 +495,46 @@ __device__ __forceinline__ void sym_rotate(SymJ &J, int addr) {
     }
 }
 
+// Near-path pairs with a far mask: bit p set means every pair of i slot p
+// with this j set is beyond the softening reach, so the spline's r^-3 branch
+// (the same operations) is taken without its test.  far is wave-uniform.
+template <int P>
+__device__ __forceinline__ void sym_pairs_masked(SymIP<P> &S, SymJ &J, unsigned far) {
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        float dx = J.x - S.x[p];
+        float dy = J.y - S.y[p];
+        float dz = J.z - S.z[p];
+        float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        float f;
+        if (far & (1u << p)) {
+            float ri = __builtin_amdgcn_rsqf(r2);
+            f = (ri * ri) * ri;
+        } else {
+            f = sym_spline_f(r2, 0.5f * (S.e[p] + J.e));
+        }
+        float si = J.m * f;
+        S.ax[p] = __builtin_fmaf(si, dx, S.ax[p]);
+        S.ay[p] = __builtin_fmaf(si, dy, S.ay[p]);
+        S.az[p] = __builtin_fmaf(si, dz, S.az[p]);
+        float sj = S.m[p] * f;
+        J.fx = __builtin_fmaf(-sj, dx, J.fx);
+        J.fy = __builtin_fmaf(-sj, dy, J.fy);
+        J.fz = __builtin_fmaf(-sj, dz, J.fz);
+    }
+}
+
+template <int P>
+__device__ __forceinline__ void sym_chunk_masked(SymIP<P> &S, SymJ &A, SymJ &B, int addr,
+                                                 unsigned farA, unsigned farB) {
+    for (int s = 0; s < 64; ++s) {
+        sym_pairs_masked(S, A, farA);
+        sym_rotate<2>(A, addr);
+        sym_pairs_masked(S, B, farB);
+        sym_rotate<2>(B, addr);
+    }
+}
+
 template <int MODE, int P = 8>
 __device__ __forceinline__ void sym_chunk(SymIP<P> &S, SymJ &A, SymJ &B, int addr) {
     for (int s = 0; s < 64; ++s) {
@@ -515,7 +566,7 @@ __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *
                                                 const double *__restrict__ m,
                                                 const double *__restrict__ eps, bool add) {
     const int lane = threadIdx.x;
-    const int sg = blockIdx.x;
+    const int sg = sym_seg(T);
     const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
     const int beg = T.b_beg[I], cnt = T.b_cnt[I];
     const double *bI = T.bbox + 8 * (int64_t)I;
@@ -594,9 +645,10 @@ __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *
 
 // The same segment restricted to its near and diagonal block pairs.  The
 // block's 512 i are taken as two halves of 256 (4 per lane), one after the
-// other: the registers of this path then stay within the far path's budget,
-// and each half tests the 128-particle j chunks against its own box, so that
-// more of them take the r^-3 path.  The j sets rotate once per half; the
+// other: the registers of this path then stay within the far path's budget.
+// Each of a half's four i slots (64 particles, Morton-contiguous) is tested
+// against each 64-particle j set of a chunk, so that the pairs beyond the
+// softening reach take the r^-3 path.  The j sets rotate once per half; the
 // first half stores their j-side sums and the second adds to them (lane L
 // owns the same j slots in both).  Stores the segment's i-side sums and
 // returns true, or returns false at once when the segment has no near pair.
@@ -606,7 +658,7 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
                                                  const double *__restrict__ m,
                                                  const double *__restrict__ eps) {
     const int lane = threadIdx.x;
-    const int sg = blockIdx.x;
+    const int sg = sym_seg(T);
     const int I = T.seg_i[sg], J0 = T.seg_j0[sg], J1 = T.seg_j1[sg];
     const int beg = T.b_beg[I], cnt = T.b_cnt[I];
     const double *bI = T.bbox + 8 * (int64_t)I;
@@ -620,12 +672,11 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
     const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
     const int64_t roff = T.row_off[I] - T.p_base;
     const int rj0 = T.row_j0[I];
+    __shared__ float slotbox[4][7];
 
     for (int h = 0; 256 * h < cnt; ++h) {
         SymIP<4> S;
         KahanF kx[4], ky[4], kz[4];
-        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        double emax = 0.0;
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             int li = lane + 64 * (4 * h + p);
@@ -638,17 +689,20 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
             S.e[p] = ok ? (float)eps[i] : 1.0f;
             S.ax[p] = S.ay[p] = S.az[p] = 0.0f;
             kx[p].init(); ky[p].init(); kz[p].init();
-            if (ok) {
-                lo[0] = fmin(lo[0], (double)S.x[p]); hi[0] = fmax(hi[0], (double)S.x[p]);
-                lo[1] = fmin(lo[1], (double)S.y[p]); hi[1] = fmax(hi[1], (double)S.y[p]);
-                lo[2] = fmin(lo[2], (double)S.z[p]); hi[2] = fmax(hi[2], (double)S.z[p]);
-                emax = fmax(emax, (double)S.e[p]);
-            }
         }
-        // the half's box (block-relative FP32 coordinates) and largest softening
-        const double hb[6] = {wave_min_d(lo[0]), wave_max_d(hi[0]), wave_min_d(lo[1]),
-                              wave_max_d(hi[1]), wave_min_d(lo[2]), wave_max_d(hi[2])};
-        const double he = wave_max_d(emax);
+        // the box (block-relative FP32 coordinates: min and max are exact) and
+        // largest softening of each slot p, the 64 i particles lane + 64 (4h + p),
+        // kept in LDS for the chunk tests
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            bool ok = lane + 64 * (4 * h + p) < cnt;
+            float v[7] = {wave_min_f(ok ? S.x[p] : INFINITY), wave_max_f(ok ? S.x[p] : -INFINITY),
+                          wave_min_f(ok ? S.y[p] : INFINITY), wave_max_f(ok ? S.y[p] : -INFINITY),
+                          wave_min_f(ok ? S.z[p] : INFINITY), wave_max_f(ok ? S.z[p] : -INFINITY),
+                          wave_max_f(ok ? S.e[p] : 0.0f)};
+            if (lane < 7) slotbox[p][lane] = v[lane];
+        }
+        __syncthreads();
 
         for (int J = J0; J < J1; ++J) {
             const int jb = T.b_beg[J], jc = T.b_cnt[J];
@@ -660,28 +714,43 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
                 SymJ A = sym_load(x, y, z, m, eps, jb, jc, c + lane, ox, oy, oz);
                 SymJ B = sym_load(x, y, z, m, eps, jb, jc, c + 64 + lane, ox, oy, oz);
                 int cm = mode;
+                unsigned farA = 0, farB = 0;
                 if (mode == 2) {
-                    // A near block pair can still hold far (half, chunk) pairs: test the
-                    // chunk's box against the half's (both from the FP32 relative
+                    // A near block pair still holds many far pairs: test each i slot's
+                    // box against each j set's (both from the FP32 relative
                     // coordinates the pairs use, with a margin for their rounding).
-                    bool va = c + lane < jc, vb = c + 64 + lane < jc;
-                    double cb[6] = {
-                        wave_min_d(fmin(va ? (double)A.x : INFINITY, vb ? (double)B.x : INFINITY)),
-                        wave_max_d(fmax(va ? (double)A.x : -INFINITY, vb ? (double)B.x : -INFINITY)),
-                        wave_min_d(fmin(va ? (double)A.y : INFINITY, vb ? (double)B.y : INFINITY)),
-                        wave_max_d(fmax(va ? (double)A.y : -INFINITY, vb ? (double)B.y : -INFINITY)),
-                        wave_min_d(fmin(va ? (double)A.z : INFINITY, vb ? (double)B.z : INFINITY)),
-                        wave_max_d(fmax(va ? (double)A.z : -INFINITY, vb ? (double)B.z : -INFINITY))};
-                    double ce = wave_max_d(fmax(va ? (double)A.e : 0.0, vb ? (double)B.e : 0.0));
-                    double big = 1.0;
-                    for (int k = 0; k < 6; ++k) big = fmax(big, fmax(fabs(cb[k]), fabs(hb[k])));
-                    double d2 = sphx_box_d2(hb[0], hb[1], hb[2], hb[3], hb[4], hb[5], cb[0], cb[1],
-                                            cb[2], cb[3], cb[4], cb[5]);
-                    double reach = (he + ce) * (1.0 + 1e-5) + 4e-6 * big;
-                    if (d2 > reach * reach) cm = 1;
+                    // A (slot, set) beyond reach takes r^-3 without the spline's test;
+                    // the forces are the same, as that test would pass for all its pairs.
+                    // lane q < 4 tests slot q against set A, lane 4 + q against set B
+                    const bool va = c + lane < jc, vb = c + 64 + lane < jc;
+                    float sv[14] = {wave_min_f(va ? A.x : INFINITY), wave_max_f(va ? A.x : -INFINITY),
+                                    wave_min_f(va ? A.y : INFINITY), wave_max_f(va ? A.y : -INFINITY),
+                                    wave_min_f(va ? A.z : INFINITY), wave_max_f(va ? A.z : -INFINITY),
+                                    wave_max_f(va ? A.e : 0.0f),
+                                    wave_min_f(vb ? B.x : INFINITY), wave_max_f(vb ? B.x : -INFINITY),
+                                    wave_min_f(vb ? B.y : INFINITY), wave_max_f(vb ? B.y : -INFINITY),
+                                    wave_min_f(vb ? B.z : INFINITY), wave_max_f(vb ? B.z : -INFINITY),
+                                    wave_max_f(vb ? B.e : 0.0f)};
+                    bool far_q = false;
+                    if (lane < 8) {
+                        const float *jb7 = lane < 4 ? sv : sv + 7;
+                        const float *ib7 = slotbox[lane & 3];
+                        double big = 1.0;
+                        for (int k = 0; k < 6; ++k)
+                            big = fmax(big, fmax(fabs((double)jb7[k]), fabs((double)ib7[k])));
+                        double d2 = sphx_box_d2(ib7[0], ib7[1], ib7[2], ib7[3], ib7[4], ib7[5],
+                                                jb7[0], jb7[1], jb7[2], jb7[3], jb7[4], jb7[5]);
+                        double reach = ((double)ib7[6] + (double)jb7[6]) * (1.0 + 1e-5) + 4e-6 * big;
+                        far_q = d2 > reach * reach;
+                    }
+                    uint64_t fm = __ballot(far_q);
+                    farA = (unsigned)(fm & 15u);
+                    farB = (unsigned)((fm >> 4) & 15u);
+                    if (farA == 15u && farB == 15u) cm = 1;
                 }
                 if (cm == 1) sym_chunk<1, 4>(S, A, B, addr);
-                else if (cm == 2) sym_chunk<2, 4>(S, A, B, addr);
+                else if (cm == 2 && (farA | farB) == 0) sym_chunk<2, 4>(S, A, B, addr);
+                else if (cm == 2) sym_chunk_masked<4>(S, A, B, addr, farA, farB);
                 else sym_chunk<3, 4>(S, A, B, addr);
                 if (mode != 3) {
                     int qa = c + lane, qb = c + 64 + lane;
@@ -868,7 +937,7 @@ struct SymHost {
     };
     std::vector<Band> bands;
     int o_bbeg = 0, o_bcnt = 0, o_runb0 = 0, o_si = 0, o_sj0 = 0, o_sj1 = 0, o_rj0 = 0, o_rns = 0,
-        o_rs0 = 0;
+        o_rs0 = 0, o_ord = 0;
 };
 
 void sphx_sym_host_free(SymHost *h) { delete h; }
@@ -991,6 +1060,20 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
         bands.push_back(bd);
     }
     int64_t need_p = 3 * max_slots, need_q = 3 * (int64_t)SYM_B * max_segs;
+    // launch order (sym_seg): within each run's part of each band, segments
+    // by their distance from the diagonal (first block pair J0 - I), nearest
+    // first, ties in index order
+    std::vector<int32_t> ord(ns);
+    for (int g = 0; g < ns; ++g) ord[g] = g;
+    for (const SymHost::Band &bd : bands) {
+        for (int r = 0; r < E.R; ++r) {
+            const int lo = std::max(bd.seg_lo, run_seg0[r]), hi = std::min(bd.seg_hi, run_seg0[r + 1]);
+            if (hi - lo < 2) continue;
+            std::stable_sort(ord.begin() + lo, ord.begin() + hi, [&](int32_t x, int32_t y) {
+                return sj0[x] - si[x] < sj0[y] - si[y];
+            });
+        }
+    }
     std::vector<int32_t> all;
     auto put = [&](const std::vector<int32_t> &v) {
         int o = (int)all.size();
@@ -998,7 +1081,8 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
         return o;
     };
     int o_bbeg = put(bbeg), o_bcnt = put(bcnt), o_runb0 = put(runb0), o_si = put(si),
-        o_sj0 = put(sj0), o_sj1 = put(sj1), o_rj0 = put(rj0), o_rns = put(rns), o_rs0 = put(rs0);
+        o_sj0 = put(sj0), o_sj1 = put(sj1), o_rj0 = put(rj0), o_rns = put(rns), o_rs0 = put(rs0),
+        o_ord = put(ord);
     if ((int64_t)all.size() > H.dev_i_cap) {
         SPHX_CHECK(E.alloc((void **)&H.dev_i, sizeof(int32_t) * (all.size() + 1024)));
         H.dev_i_cap = (int64_t)all.size() + 1024;
@@ -1030,7 +1114,7 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
         SPHX_HIP(hipMemcpyAsync(H.row_off, roff.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, st));
     SPHX_HIP(hipStreamSynchronize(st));
     H.o_bbeg = o_bbeg; H.o_bcnt = o_bcnt; H.o_runb0 = o_runb0; H.o_si = o_si; H.o_sj0 = o_sj0;
-    H.o_sj1 = o_sj1; H.o_rj0 = o_rj0; H.o_rns = o_rns; H.o_rs0 = o_rs0;
+    H.o_sj1 = o_sj1; H.o_rj0 = o_rj0; H.o_rns = o_rns; H.o_rs0 = o_rs0; H.o_ord = o_ord;
     H.run_seg0 = run_seg0;
     H.seg_len = S;
     H.sizes = E.run_n;
@@ -1079,7 +1163,7 @@ static int sym_run(Engine &E, hipStream_t st) {
         SymTabs T{d + H.o_bbeg, d + H.o_bcnt, d + H.o_runb0, d + H.o_si + a, d + H.o_sj0 + a,
                   d + H.o_sj1 + a, H.row_off, d + H.o_rj0, d + H.o_rns, d + H.o_rs0, H.bbox,
                   part, part + pl, part + 2 * pl, q, q + ql, q + 2 * ql, a, bd.seg_lo, bd.p_base,
-                  blk ? H.act : nullptr, E.s.run, E.t.r_pbeg, E.n};
+                  blk ? H.act : nullptr, E.s.run, E.t.r_pbeg, E.n, d + H.o_ord};
         if (!marked) SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
         marked = true;
         k_grav_sym<<<(unsigned)(b - a), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);
