@@ -1061,18 +1061,26 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     }
     int64_t need_p = 3 * max_slots, need_q = 3 * (int64_t)SYM_B * max_segs;
     // launch order (sym_seg): within each run's part of each band, segments
-    // by their distance from the diagonal (first block pair J0 - I), nearest
-    // first, ties in index order
-    std::vector<int32_t> ord(ns);
-    for (int g = 0; g < ns; ++g) ord[g] = g;
+    // by their distance from the diagonal, nearest first: the k-th segment of
+    // every row of the part (first block pair J0 = I + k S), for k = 0, 1, ...,
+    // rows in order.  The rows of a part are consecutive and their segments
+    // contiguous, so this is a permutation of the part's range, built in one
+    // pass (the tables are rebuilt whenever the run set changes).
+    std::vector<int32_t> ord;
+    ord.reserve(ns);
     for (const SymHost::Band &bd : bands) {
         for (int r = 0; r < E.R; ++r) {
-            const int lo = std::max(bd.seg_lo, run_seg0[r]), hi = std::min(bd.seg_hi, run_seg0[r + 1]);
-            if (hi - lo < 2) continue;
-            std::stable_sort(ord.begin() + lo, ord.begin() + hi, [&](int32_t x, int32_t y) {
-                return sj0[x] - si[x] < sj0[y] - si[y];
-            });
+            const int r0 = std::max(bd.row_lo, runb0[r]), r1 = std::min(bd.row_hi, runb0[r + 1]);
+            int kmax = 0;
+            for (int I = r0; I < r1; ++I) kmax = std::max(kmax, rns[I]);
+            for (int k = 0; k < kmax; ++k)
+                for (int I = r0; I < r1; ++I)
+                    if (k < rns[I]) ord.push_back(rs0[I] + k);
         }
+    }
+    if ((int)ord.size() != ns) {
+        sphx_set_error("symmetric gravity: segment order does not cover the segments");
+        return SPHX_ERR_STATE;
     }
     std::vector<int32_t> all;
     auto put = [&](const std::vector<int32_t> &v) {
