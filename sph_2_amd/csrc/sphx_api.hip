@@ -179,7 +179,7 @@ int Engine::reserve(int64_t capacity) {
     for (int32_t **q : iptr) SPHX_CHECK(alloc((void **)q, I));
     SPHX_CHECK(alloc((void **)&keys, sizeof(uint64_t) * capacity));
     SPHX_CHECK(alloc((void **)&keys2, sizeof(uint64_t) * capacity));
-    hist_cap = 256 * ((capacity + 4095) / 4096 + 1);
+    hist_cap = 256 * ((capacity + SPHX_RS_TILE - 1) / SPHX_RS_TILE + 1);
     SPHX_CHECK(alloc((void **)&hist, sizeof(uint32_t) * hist_cap));
     SPHX_CHECK(alloc((void **)&counters, sizeof(int64_t) * CNT_SLOTS));
     SPHX_HIP(hipMemset(counters, 0, sizeof(int64_t) * CNT_SLOTS));

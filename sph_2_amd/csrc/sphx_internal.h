@@ -22,6 +22,8 @@
 #include "../../include/sphx.h"
 
 #define SPHX_BUCKET 64
+// keys per workgroup tile of the radix sort (its histograms: 256 per tile)
+#define SPHX_RS_TILE 1024
 #define SPHX_SUPER 64
 #define SPHX_NBR_MAX 128  // entries of a particle's neighbour list (more: its bucket walks)
 

@@ -12,8 +12,9 @@
 namespace {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_TILE = SPHX_RS_TILE;
+constexpr int RS_ITEMS = RS_TILE / RS_THREADS;
+static_assert(RS_ITEMS * RS_THREADS == RS_TILE, "a tile is whole rounds of the workgroup");
 
 __device__ __forceinline__ uint64_t spread3(uint64_t v) {
     // Spread the low 21 bits of v so bit b lands at bit 3b.
