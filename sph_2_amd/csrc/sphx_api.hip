@@ -187,7 +187,7 @@ int Engine::reserve(int64_t capacity) {
         // every run has at most one partial bucket
         int64_t bc = capacity / SPHX_BUCKET + std::max(max_runs, 1) + 1;
         SPHX_CHECK(alloc((void **)&nbr, sizeof(int32_t) * (size_t)bc * SPHX_NBR_MAX * 64));
-        SPHX_CHECK(alloc((void **)&nbr_ovf, sizeof(int32_t) * (size_t)bc));
+        SPHX_CHECK(alloc((void **)&nbr_ovf, sizeof(int32_t) * 2 * (size_t)bc));
         SPHX_CHECK(alloc((void **)&prec, sizeof(PairRec) * (size_t)capacity));
         SPHX_CHECK(alloc((void **)&fbal, sizeof(float) * (size_t)capacity));
         nbr_bcap = bc;

@@ -347,7 +347,7 @@ struct Engine {
     bool want_lists = false;
     bool use_lists = true;      // tuning knob: the walks give the same bits
     int32_t *nbr = nullptr;     // nbr_bcap buckets x SPHX_NBR_MAX x 64
-    int32_t *nbr_ovf = nullptr; // per bucket
+    int32_t *nbr_ovf = nullptr; // two per bucket (one per half of a half-bucket walk)
     int64_t nbr_bcap = 0;
     PairRec *prec = nullptr;    // per particle of capacity
     float *fbal = nullptr;      // per particle of capacity
@@ -435,6 +435,8 @@ struct Engine {
     int smoothing(hipStream_t st, bool predictor);
     int build_candidates(hipStream_t st);
     int density_eos(hipStream_t st, bool lists = false);
+    // groups per bucket of the density walk (1 or 2)
+    int walk_halves() const;
     int gravity(hipStream_t st, bool with_pot);
     int divcurl(hipStream_t st, bool lists = false);
     int hydro(hipStream_t st, bool lists = false);
