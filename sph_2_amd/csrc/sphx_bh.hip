@@ -325,13 +325,16 @@ template <bool POT>
 __device__ __forceinline__ void bh_eval(float dx, float dy, float dz, float r2, float t, float mj,
                                         float &ax, float &ay, float &az, float &ph) {
     float f, p;
+    // r^-1 once for both branches; inside the softening length it also gives
+    // r = r^2 r^-1 (a transcendental costs several FP32 instructions' issue
+    // time, and a wave whose lanes split between the branches runs both)
+    const float ri = __builtin_amdgcn_rsqf(r2);
     if (r2 >= t * t) {
-        float ri = __builtin_amdgcn_rsqf(r2);
         f = ri * ri * ri;
         p = ri;
     } else {
         float e = 0.5f * t;
-        float r = r2 > 0.0f ? __builtin_amdgcn_sqrtf(r2) : 0.0f;
+        float r = r2 > 0.0f ? r2 * ri : 0.0f;
         float ie = __builtin_amdgcn_rcpf(e);
         float q = r * ie;
         if (q <= 1.0f) {
@@ -342,7 +345,6 @@ __device__ __forceinline__ void bh_eval(float dx, float dy, float dz, float r2, 
         } else {
             float w = 2.0f - q;
             float w2 = w * w;
-            float ri = __builtin_amdgcn_rcpf(r);
             f = ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
             float q2 = q * q;
             p = POT ? -ri * (1.0f / 15.0f) -

@@ -139,12 +139,13 @@ __device__ __forceinline__ void near_pair(IState<IPL> &S, float4 j, float ej, in
         float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
         float e = 0.5f * (S.e[p] + ej);
         float f, ph;
+        // r^-1 once for both branches (as in sym_spline_f)
+        const float ri = __builtin_amdgcn_rsqf(r2);
         if (r2 >= 4.0f * e * e) {
-            float ri = __builtin_amdgcn_rsqf(r2);
             f = ri * ri * ri;
             ph = ri;
         } else {
-            float r = r2 > 0.0f ? __builtin_amdgcn_sqrtf(r2) : 0.0f;
+            float r = r2 > 0.0f ? r2 * ri : 0.0f;
             float ie = __builtin_amdgcn_rcpf(e);
             float q = r * ie;
             if (q <= 1.0f) {
@@ -154,7 +155,6 @@ __device__ __forceinline__ void near_pair(IState<IPL> &S, float4 j, float ej, in
             } else {
                 float w = 2.0f - q;
                 float w2 = w * w;
-                float ri = __builtin_amdgcn_rcpf(r);
                 f = ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
                 float q2 = q * q;
                 ph = -ri * (1.0f / 15.0f) -
@@ -398,21 +398,22 @@ __device__ __forceinline__ float sym_rot(float v, int addr) {
 }
 
 // The spline-softened force factor of near_pair, without mass or potential.
+// r^-1 is taken once, before the softening test, and also gives r = r^2 r^-1
+// inside the softening length: a transcendental costs several FP32
+// instructions' issue time, and a wave whose lanes split between the two
+// branches runs both.
 __device__ __forceinline__ float sym_spline_f(float r2, float e) {
-    if (r2 >= 4.0f * e * e) {
-        float ri = __builtin_amdgcn_rsqf(r2);
-        return ri * ri * ri;
-    }
-    float r = r2 > 0.0f ? __builtin_amdgcn_sqrtf(r2) : 0.0f;
-    float ie = __builtin_amdgcn_rcpf(e);
-    float q = r * ie;
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    if (r2 >= 4.0f * e * e) return ri * ri * ri;
+    const float r = r2 > 0.0f ? r2 * ri : 0.0f;
+    const float ie = __builtin_amdgcn_rcpf(e);
+    const float q = r * ie;
     if (q <= 1.0f) {
         float q2 = q * q;
         return ie * ie * ie * (4.0f / 3.0f - 1.2f * q2 + 0.5f * q2 * q);
     }
     float w = 2.0f - q;
     float w2 = w * w;
-    float ri = __builtin_amdgcn_rcpf(r);
     return ri * ri * ri * (1.0f - w2 * w2 * (1.0f - 0.8f * w + w2 * (1.0f / 6.0f)));
 }
 
