@@ -1581,7 +1581,7 @@ static void ckpt_header(CkptHeader &h, const Engine &E) {
     h.p = E.p;
     h.g = E.gcfg;
     h.bh_theta = E.bh_theta;
-    h.morton_bits = E.morton_bits;
+    h.morton_bits = E.morton_bits();
 }
 
 extern "C" int sphx_plan_save(sphx_plan *plan, const char *path, void *stream) {
@@ -1611,7 +1611,7 @@ extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream
         sphx_set_error("checkpoint run cools with a table the plan does not have");
         return SPHX_ERR_ARG;
     }
-    if (h.morton_bits != E.morton_bits) { sphx_set_error("checkpoint from an engine with another key layout"); return SPHX_ERR_ARG; }
+    if (h.morton_bits != E.morton_bits()) { sphx_set_error("checkpoint from an engine with another key layout"); return SPHX_ERR_ARG; }
     E.p = h.p;
     SPHX_CHECK(E.set_runs({h.n}));
     SPHX_CHECK(sphx_ckpt_upload(path, E, 0, h.n, st));

@@ -84,7 +84,6 @@ __global__ void kb_scatter_aos(const double *__restrict__ a, const double *__res
 }  // namespace
 
 int sphx_launch_morton(Engine &E, hipStream_t st);
-int sphx_morton_bits(int max_runs);
 int sphx_copy_cool(Engine &E, const sphx_cooling *cool);
 
 struct BatchRun {
@@ -440,7 +439,7 @@ extern "C" int sphx_batch_save_run(sphx_batch *b, int64_t run_id, const char *pa
     h.mode = E.mode;
     h.p = E.p;
     h.g = E.gcfg;
-    h.morton_bits = E.morton_bits;
+    h.morton_bits = E.morton_bits();
     h.n = E.hr_pend[r] - E.hr_pbeg[r];
     h.steps = b->runs[r].steps;
     h.rp = E.run_params[r];
@@ -460,7 +459,7 @@ extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *
         sphx_set_error("checkpoint physics parameters differ from the batch's");
         return SPHX_ERR_ARG;
     }
-    if (h.morton_bits != E.morton_bits) {
+    if (h.morton_bits != E.morton_bits()) {
         sphx_set_error("checkpoint from an engine with another key layout");
         return SPHX_ERR_ARG;
     }

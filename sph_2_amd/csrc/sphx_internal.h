@@ -294,6 +294,9 @@ int sphx_pair_probe_launch(int64_t blocks, int iters, float *out, hipStream_t st
 int sphx_sym_reserve(struct Engine &E, int S, int64_t budget_bytes);
 bool sphx_sym_fits(const struct Engine &E, const std::vector<int64_t> &sizes, int S);
 
+// Morton bits per axis for an engine holding up to max_runs runs (sphx_sort.hip).
+int sphx_morton_bits(int max_runs);
+
 struct Engine {
     int device = 0;
     int mode = SPHX_MIXED;
@@ -385,12 +388,14 @@ struct Engine {
         Tables v = t;
         v.boff = rrun >= 0 ? rb0 : 0;
         v.keys = keys;
-        v.mbits = morton_bits;
+        v.mbits = morton_bits();
         return v;
     }
     unsigned nb_launch() const { return (unsigned)(rrun >= 0 ? rb1 - rb0 : t.nb); }
-    int morton_bits = 21;      // bits per axis of the Morton key
     int max_runs = 1;          // run id bits are reserved for this many runs
+    // bits per axis of the Morton keys: 21 for a lone run, fewer when run ids
+    // share the key (the layout k_morton writes and the bucket cuts read)
+    int morton_bits() const { return sphx_morton_bits(max_runs); }
 
     // Step graphs: one captured hipGraph of one step per pointer parity (the
     // sort swaps state and spare arrays, so consecutive steps alternate

@@ -176,3 +176,31 @@ def test_symmetric_gravity_alone_equals_in_the_mix(gpu, runs):
         banded.append(bt.retire(ra))
         bt.close()
     _same(banded[0], banded[1])
+
+
+@pytest.mark.parametrize("slots", [8, 512])
+def test_batch_cuts_buckets_like_a_plan(gpu, slots):
+    """A batch's keys give each axis fewer Morton bits than a plan's (the run
+    id takes the top bits: 20 per axis for 8 run slots, 18 for 512), and the
+    bucket cuts must read the level of a key difference in that layout: then
+    a run's neighbour walks visit the same candidate buckets in a batch as in
+    a plan at the same cut level.  A uniform cloud (no two particles share a
+    cell at 18 bits) and the bootstrap's walks on the initial positions, so
+    both sort alike."""
+    from sph_2_amd.plan import SphPlan
+    from sph_2_amd.serve import Batch
+
+    e = ic.turbulent_cloud(16384, seed=3, ng=32)
+    pl = SphPlan(e.n)
+    pl.set_cell_level(0)                           # a batch's level (no tuning)
+    pl.load(e.pos, e.vel, e.u, e.m, e.h)
+    pl.init()
+    cp = pl.counters()
+    pl.close()
+    bt = Batch(2 * e.n, slots)
+    assert bt.admit(e, 1e-4) is not None
+    cb = bt.counters()
+    bt.close()
+    assert cb["buckets"] == cp["buckets"] == 256
+    assert cb["h_iters_sum"] == cp["h_iters_sum"]
+    assert cb["cand_buckets"] == cp["cand_buckets"]
