@@ -196,7 +196,7 @@ int Engine::reserve(int64_t capacity) {
     return SPHX_OK;
 }
 
-int Engine::set_runs(const std::vector<int64_t> &sizes) {
+int Engine::set_runs(const std::vector<int64_t> &sizes, hipStream_t st) {
     int64_t total = 0;
     for (int64_t v : sizes) total += v;
     if (total > cap) {
@@ -249,8 +249,11 @@ int Engine::set_runs(const std::vector<int64_t> &sizes) {
     for (auto *v : {&hb_beg, &hb_cnt, &hb_run, &hs_bbeg, &hs_bend, &hr_sbeg, &hr_send, &hr_pbeg,
                     &hr_pend, &hr_lcut})
         all.insert(all.end(), v->begin(), v->end());
-    if (!all.empty())
-        SPHX_HIP(hipMemcpy(tab_i32, all.data(), sizeof(int32_t) * all.size(), hipMemcpyHostToDevice));
+    if (!all.empty()) {
+        SPHX_HIP(hipMemcpyAsync(tab_i32, all.data(), sizeof(int32_t) * all.size(),
+                                hipMemcpyHostToDevice, st));
+        SPHX_HIP(hipStreamSynchronize(st));  // `all` is a host temporary
+    }
     int64_t nd = 8 * (int64_t)nb + 7 * (int64_t)ns;
     if (nd > tab_d_cap) {
         SPHX_CHECK(alloc((void **)&tab_d, sizeof(double) * (nd + 1024)));
@@ -646,7 +649,7 @@ static int ctx_load(sphx_ctx *ctx, int64_t n, const double *pos, const double *m
                     const double *v, bool sort, hipStream_t st) {
     Engine &E = ctx->E;
     SPHX_CHECK(E.reserve(std::max<int64_t>(n, 64)));
-    SPHX_CHECK(E.set_runs({n}));
+    SPHX_CHECK(E.set_runs({n}, st));
     if (n == 0) return SPHX_OK;
     SPHX_CHECK(sphx_aos_to_soa(pos, n, E.s.x, E.s.y, E.s.z, st));
     if (v) SPHX_CHECK(sphx_aos_to_soa(v, n, E.s.vx, E.s.vy, E.s.vz, st));
@@ -1395,7 +1398,7 @@ extern "C" int sphx_plan_load(sphx_plan *plan, int64_t n, const double *pos, con
     if (n <= 0 || n > E.cap) { sphx_set_error("particle count must be 1 .. the plan capacity"); return SPHX_ERR_ARG; }
     if (!pos || !vel || !u || !m || !h) { sphx_set_error("null array"); return SPHX_ERR_ARG; }
     if (!E.p.eps_from_h && !eps) { sphx_set_error("eps required when eps_from_h == 0"); return SPHX_ERR_ARG; }
-    SPHX_CHECK(E.set_runs({n}));
+    SPHX_CHECK(E.set_runs({n}, st));
     SPHX_CHECK(sphx_aos_to_soa(pos, n, E.s.x, E.s.y, E.s.z, st));
     SPHX_CHECK(sphx_aos_to_soa(vel, n, E.s.vx, E.s.vy, E.s.vz, st));
     k_copy_d<<<gu(n), UT, 0, st>>>(u, E.s.u, n);
@@ -1613,7 +1616,7 @@ extern "C" int sphx_plan_restore(sphx_plan *plan, const char *path, void *stream
     }
     if (h.morton_bits != E.morton_bits()) { sphx_set_error("checkpoint from an engine with another key layout"); return SPHX_ERR_ARG; }
     E.p = h.p;
-    SPHX_CHECK(E.set_runs({h.n}));
+    SPHX_CHECK(E.set_runs({h.n}, st));
     SPHX_CHECK(sphx_ckpt_upload(path, E, 0, h.n, st));
     k_fill_i<<<gu(h.n), UT, 0, st>>>(E.s.run, 0, h.n);
     SPHX_HIP(hipGetLastError());

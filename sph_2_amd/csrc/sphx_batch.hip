@@ -235,7 +235,7 @@ extern "C" int sphx_batch_admit(sphx_batch *b, int64_t n, const double *pos, con
     SPHX_CHECK(E.pull_run_params(st));   // device dt (adaptive runs) and t
     std::vector<int64_t> sizes = E.run_n;
     sizes.push_back(n);
-    SPHX_CHECK(E.set_runs(sizes));
+    SPHX_CHECK(E.set_runs(sizes, st));
     E.run_params.resize(E.R);
     E.run_params[r] = RunParams{dt, use_cooling, E.p.eps_from_h};
     SPHX_CHECK(E.upload_run_params(st));
@@ -314,7 +314,7 @@ extern "C" int sphx_batch_retire(sphx_batch *b, int64_t run_id, double *pos, dou
     sizes.erase(sizes.begin() + r);
     E.run_params.erase(E.run_params.begin() + r);
     b->runs.erase(b->runs.begin() + r);
-    SPHX_CHECK(E.set_runs(sizes));
+    SPHX_CHECK(E.set_runs(sizes, st));
     SPHX_CHECK(E.upload_run_params(st));
     E.drop_graph();
     return SPHX_OK;
@@ -490,7 +490,7 @@ extern "C" int sphx_batch_restore_run(sphx_batch *b, const char *path, int64_t *
     SPHX_CHECK(E.pull_run_params(st));
     std::vector<int64_t> sizes = E.run_n;
     sizes.push_back(n);
-    SPHX_CHECK(E.set_runs(sizes));
+    SPHX_CHECK(E.set_runs(sizes, st));
     E.run_params.resize(E.R);
     E.run_params[r] = h.rp;
     SPHX_CHECK(E.upload_run_params(st));

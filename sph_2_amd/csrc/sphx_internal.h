@@ -430,7 +430,9 @@ struct Engine {
     ~Engine();
     int alloc(void **ptr, size_t bytes);
     int reserve(int64_t capacity);
-    int set_runs(const std::vector<int64_t> &sizes);
+    // Rebuild the run tables; the upload is ordered after the work already
+    // queued on st (which may still read the old tables), then waited for.
+    int set_runs(const std::vector<int64_t> &sizes, hipStream_t st = nullptr);
     int upload_run_params(hipStream_t st);
     int pull_run_params(hipStream_t st);  // device dt, t, dt_parts -> host copy
 

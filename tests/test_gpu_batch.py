@@ -204,3 +204,31 @@ def test_batch_cuts_buckets_like_a_plan(gpu, slots):
     assert cb["buckets"] == cp["buckets"] == 256
     assert cb["h_iters_sum"] == cp["h_iters_sum"]
     assert cb["cand_buckets"] == cp["cand_buckets"]
+
+
+def test_batch_on_a_side_stream(gpu, runs):
+    """The batch calls on a non-default stream (the caller's current stream):
+    admitting and retiring runs rewrites the run tables while steps queued on
+    that stream may still read them, so the upload is ordered on the caller's
+    stream.  The results equal the same sequence on the default stream."""
+    import torch
+
+    from sph_2_amd.serve import Batch
+
+    a, b, c, cool = runs
+    outs = []
+    for side in (False, True):
+        s = torch.cuda.Stream() if side else torch.cuda.current_stream()
+        with torch.cuda.stream(s):
+            bt = Batch(20000, 8, cooling=cool, graph=True)
+            ra = bt.admit(a, 1e-4)
+            bt.step(2)                             # queued, not waited for
+            rb = bt.admit(b, 2e-4, cooling=True)   # new tables behind the queued steps
+            bt.step(2)
+            bt.retire(rb)
+            rc = bt.admit(c, 1e-4)
+            bt.step(2)
+            outs.append(bt.retire(ra))
+            bt.close()
+        torch.cuda.synchronize()
+    _same(outs[0], outs[1])
