@@ -137,14 +137,15 @@ __device__ __forceinline__ void near_pair(IState<IPL> &S, float4 j, float ej, in
         float dy = j.y - S.y[p];
         float dz = j.z - S.z[p];
         float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-        float e = 0.5f * (S.e[p] + ej);
+        const float t = S.e[p] + ej;  // 2e; r^2 >= t^2 is r^2 >= (4e) e
         float f, ph;
         // r^-1 once for both branches (as in sym_spline_f)
         const float ri = __builtin_amdgcn_rsqf(r2);
-        if (r2 >= 4.0f * e * e) {
+        if (r2 >= t * t) {
             f = ri * ri * ri;
             ph = ri;
         } else {
+            const float e = 0.5f * t;
             float r = r2 > 0.0f ? r2 * ri : 0.0f;
             float ie = __builtin_amdgcn_rcpf(e);
             float q = r * ie;
@@ -397,14 +398,17 @@ __device__ __forceinline__ float sym_rot(float v, int addr) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
 }
 
-// The spline-softened force factor of near_pair, without mass or potential.
+// The spline-softened force factor of near_pair, without mass or potential;
+// t = eps_i + eps_j is twice the pair's softening e.  Newtonian where
+// r^2 >= t^2, the same test as r^2 >= (4e) e (both products round alike).
 // r^-1 is taken once, before the softening test, and also gives r = r^2 r^-1
 // inside the softening length: a transcendental costs several FP32
 // instructions' issue time, and a wave whose lanes split between the two
 // branches runs both.
-__device__ __forceinline__ float sym_spline_f(float r2, float e) {
+__device__ __forceinline__ float sym_spline_f(float r2, float t) {
     const float ri = __builtin_amdgcn_rsqf(r2);
-    if (r2 >= 4.0f * e * e) return ri * ri * ri;
+    if (r2 >= t * t) return ri * ri * ri;
+    const float e = 0.5f * t;
     const float r = r2 > 0.0f ? r2 * ri : 0.0f;
     const float ie = __builtin_amdgcn_rcpf(e);
     const float q = r * ie;
@@ -467,7 +471,7 @@ __device__ __forceinline__ void sym_pairs(SymIP<P> &S, SymJ &J) {
             float ri = __builtin_amdgcn_rsqf(r2);
             f = (ri * ri) * ri;
         } else {
-            f = sym_spline_f(r2, 0.5f * (S.e[p] + J.e));
+            f = sym_spline_f(r2, S.e[p] + J.e);
         }
         float si = MODE == 0 ? f : J.m * f;
         S.ax[p] = __builtin_fmaf(si, dx, S.ax[p]);
@@ -512,7 +516,7 @@ __device__ __forceinline__ void sym_pairs_masked(SymIP<P> &S, SymJ &J, unsigned 
             float ri = __builtin_amdgcn_rsqf(r2);
             f = (ri * ri) * ri;
         } else {
-            f = sym_spline_f(r2, 0.5f * (S.e[p] + J.e));
+            f = sym_spline_f(r2, S.e[p] + J.e);
         }
         float si = J.m * f;
         S.ax[p] = __builtin_fmaf(si, dx, S.ax[p]);
