@@ -380,7 +380,9 @@ int sphx_plan_diagnostics(sphx_plan *plan, int32_t with_pe, sphx_diag *out, void
 int sphx_plan_destroy(sphx_plan *plan);
 
 /* Gravity launch shape of a plan or context: ipl particles per lane (1, 2, 4),
- * j-tile tj (256, 512, 1024), j split js >= 1 (more workgroups for small N).
+ * j-tile tj (256, 512, 1024), j split js >= 1 (more workgroups for small N);
+ * or ipl = 8, the symmetric sum: (8, 512, S) with S block pairs per wave, or
+ * (8, 256, 1) / (8, 128, 1) with each block pair split into parts of tj j.
  * No reference counterpart: the reference launches acc_g with a fixed
  * <<<ceil(N/256), 256>>> (DRV2:280-281); the autotuner sets this. */
 int sphx_plan_set_gravity_config(sphx_plan *plan, int32_t ipl, int32_t tj, int32_t js);

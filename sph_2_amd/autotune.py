@@ -2,7 +2,8 @@
 
 choose_gravity_config(n) returns (ipl, tj, js): particles per lane, j-tile
 size and j split of the tiled sum, or (8, 512, block pairs per wave) for the
-symmetric sum.  The defaults below keep >= 4 workgroups per CU on the
+symmetric sum, or (8, tj, 1) for the symmetric sum with every block pair split
+into parts of tj j particles (tj = 256 or 128: more, shorter waves).  The defaults below keep >= 4 workgroups per CU on the
 256-CU MI355X; ``tune_gravity`` measures the candidates on the device and
 caches the winner per size class in a JSON file.
 """
@@ -18,15 +19,19 @@ CANDIDATES = [(1, 256), (1, 512), (2, 256), (2, 512), (2, 1024), (4, 256), (4, 5
 # partial sums (12 B x n^2 / 1024 in all) are summed in row bands of bounded
 # memory, so it is offered at every size the tuner sweeps (up to 8M).
 SYM_SEGMENTS = (1, 2, 4, 8, 16, 32, 64)
+# ... and block pairs split into parts of tj j particles (one part per wave)
+SYM_SPLITS = (256, 128)
 SYM_MAX_N = 1 << 23
 _CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gravity_tuning.json")
 
 
 def _size_class(n: int) -> str:
+    """The power of two nearest to n in ratio (the tuned size whose shape a run
+    of n particles takes): n < sqrt(2) k for the largest power of two k <= n."""
     k = 1
-    while k < n:
+    while 2 * k <= n:
         k <<= 1
-    return str(k)
+    return str(k if n * n < 2 * k * k else 2 * k)
 
 
 def _default(n: int):
@@ -94,13 +99,14 @@ def tune_cells(n: int, seed: int = 0, steps: int = 3):
 def pick(res: dict):
     """The fastest shape, except that among shapes within 1 % of it the one
     needing the least extra memory wins: a tiled shape (none), else the
-    symmetric sum with the most block pairs per wave (fewest partial sums)."""
+    symmetric sum with the most block pairs per wave (fewest partial sums),
+    whole block pairs before split ones."""
     t_best = min(res.values())
     near = [k for k, v in res.items() if v <= 1.01 * t_best]
     tiled = [k for k in near if k[0] != 8]
     if tiled:
         return min(tiled, key=res.get)
-    return max(near, key=lambda k: k[2])
+    return max(near, key=lambda k: (k[2], k[1]))
 
 
 def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
@@ -137,6 +143,15 @@ def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
                 libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
             torch.cuda.synchronize()
             res[(8, 512, S)] = (time.perf_counter() - t0) / reps
+        for tj in SYM_SPLITS:
+            check(load().sphx_ctx_set_gravity_config(ctx, 8, tj, 1), "config")
+            libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+            torch.cuda.synchronize()
+            res[(8, tj, 1)] = (time.perf_counter() - t0) / reps
     best = pick(res)
     check(load().sphx_ctx_set_gravity_config(ctx, *_default(n)), "config")
     if write:

@@ -1750,7 +1750,9 @@ extern "C" int sphx_plan_destroy(sphx_plan *plan) {
 // splits (1..64).  Symmetric sum: ipl 8, tj 512, js = block pairs per wave
 // (1..4096).
 static bool gravity_config_ok(int32_t ipl, int32_t tj, int32_t js) {
-    if (ipl == 8) return tj == 512 && js >= 1 && js <= 4096;
+    // symmetric sum: js block pairs per wave, or (js = 1) a block pair split
+    // into 512 / tj parts of tj j particles
+    if (ipl == 8) return (tj == 512 && js >= 1 && js <= 4096) || ((tj == 256 || tj == 128) && js == 1);
     return (ipl == 1 || ipl == 2 || ipl == 4) && (tj == 256 || tj == 512 || tj == 1024) &&
            js >= 1 && js <= 64 && !(ipl == 1 && tj == 1024) && !(ipl == 4 && tj == 1024);
 }

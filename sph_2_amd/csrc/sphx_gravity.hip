@@ -356,7 +356,9 @@ int launch_grav(const GravTabs &gt, const GravIO &io, int nblocks, bool pot, hip
 // lane with its sum over the 512 i.  The j-side sums of row I go to their own
 // slots of a partial buffer and the i-side sums of each segment to theirs;
 // k_grav_sym_reduce adds them in a fixed order, so results do not depend on
-// scheduling.  The diagonal block pair (J = I) is summed one-sided.  Far block
+// scheduling.  With tj < SYM_B a segment is a part of one block pair (its
+// j particles [c0, c0 + tj)), so that mid-size runs launch several waves per
+// block pair.  The diagonal block pair (J = I) is summed one-sided.  Far block
 // pairs whose blocks each have one common mass use no mass in the pair and
 // scale the chunk sums (rsq + 14 VALU per pair for two interactions).
 // ---------------------------------------------------------------------------
@@ -382,6 +384,9 @@ struct SymTabs {
     const int32_t *r_pbeg;         // first particle of each run
     int64_t n;
     const int32_t *order;          // launch order: group g takes segment order[seg0 + g] (global)
+    const int32_t *seg_c0;         // per segment: first j of each of its blocks (0, or a
+                                   // multiple of jspan when a block pair is split)
+    int32_t jspan;                 // j particles a segment takes of each block (SYM_B, 256, 128)
 };
 
 // The segment a group takes.  Groups start roughly in index order and the
@@ -599,6 +604,7 @@ __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *
     }
     const int64_t roff = T.row_off[I] - T.p_base;
     const int rj0 = T.row_j0[I];
+    const int c0 = T.seg_c0[sg];
 
     for (int J = J0; J < J1; ++J) {
         const int jb = T.b_beg[J], jc = T.b_cnt[J];
@@ -608,7 +614,8 @@ __device__ __forceinline__ void sym_segment_far(const SymTabs &T, const double *
         if (mode > 1) continue;
         const float si = mode == 0 ? (float)bJ[7] : 1.0f;  // i-side scale: the j block's mass
         const float sj = mode == 0 ? (float)mI : 1.0f;     // j-side scale: the i block's mass
-        for (int c = 0; c < jc; c += 128) {
+        const int ce = jc < c0 + T.jspan ? jc : c0 + T.jspan;
+        for (int c = c0; c < ce; c += 128) {
             SymJ A = sym_load(x, y, z, m, eps, jb, jc, c + lane, ox, oy, oz);
             SymJ B = sym_load(x, y, z, m, eps, jb, jc, c + 64 + lane, ox, oy, oz);
             if (mode == 0) sym_chunk<0>(S, A, B, addr);
@@ -677,6 +684,7 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
     const int addr = ((lane + 1) & 63) << 2;  // lane L takes the value of lane L + 1
     const int64_t roff = T.row_off[I] - T.p_base;
     const int rj0 = T.row_j0[I];
+    const int c0 = T.seg_c0[sg];
     __shared__ float slotbox[4][7];
 
     for (int h = 0; 256 * h < cnt; ++h) {
@@ -715,7 +723,8 @@ __device__ __forceinline__ bool sym_segment_near(const SymTabs &T, const double 
             if (!actI && !T.act[J]) continue;
             const int mode = sym_mode(bI, bJ, J == I);
             if (mode <= 1) continue;
-            for (int c = 0; c < jc; c += 128) {
+            const int ce = jc < c0 + T.jspan ? jc : c0 + T.jspan;
+            for (int c = c0; c < ce; c += 128) {
                 SymJ A = sym_load(x, y, z, m, eps, jb, jc, c + lane, ox, oy, oz);
                 SymJ B = sym_load(x, y, z, m, eps, jb, jc, c + 64 + lane, ox, oy, oz);
                 int cm = mode;
@@ -917,6 +926,7 @@ __global__ __launch_bounds__(256) void k_pair_probe(float *out, int iters, float
 // length change.
 struct SymHost {
     int seg_len = 0;
+    int seg_tj = 0;                 // j particles per segment and block (SYM_B: whole blocks)
     std::vector<int64_t> sizes;
     int nblocks = 0, nseg = 0;
     std::vector<int32_t> run_seg0;  // first segment of each run (+ total)
@@ -942,7 +952,7 @@ struct SymHost {
     };
     std::vector<Band> bands;
     int o_bbeg = 0, o_bcnt = 0, o_runb0 = 0, o_si = 0, o_sj0 = 0, o_sj1 = 0, o_rj0 = 0, o_rns = 0,
-        o_rs0 = 0, o_ord = 0;
+        o_rs0 = 0, o_ord = 0, o_c0 = 0;
 };
 
 void sphx_sym_host_free(SymHost *h) { delete h; }
@@ -1002,8 +1012,11 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     if (!E.grav.sym) E.grav.sym = new SymHost();
     SymHost &H = *E.grav.sym;
     const int S = cfg.js;
-    if (H.seg_len == S && H.sizes == E.run_n) return SPHX_OK;
-    std::vector<int32_t> bbeg, bcnt, runb0, si, sj0, sj1, rj0, rns, rs0;
+    // tj < SYM_B (with one block pair per segment) splits every block pair
+    // into SYM_B / tj segments of tj j particles each: more, shorter waves
+    const int span = cfg.tj < SYM_B ? cfg.tj : SYM_B;
+    if (H.seg_len == S && H.seg_tj == span && H.sizes == E.run_n) return SPHX_OK;
+    std::vector<int32_t> bbeg, bcnt, runb0, si, sj0, sj1, sc0, rj0, rns, rs0;
     std::vector<int64_t> roff;
     std::vector<int32_t> run_seg0;
     int64_t off = 0, plen = 0;
@@ -1024,10 +1037,13 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
             plen += (off + nr) - j0;
             rs0.push_back((int32_t)si.size());
             int k = 0;
-            for (int J = I; J < b1; J += S, ++k) {
-                si.push_back(I);
-                sj0.push_back(J);
-                sj1.push_back(std::min(b1, J + S));
+            for (int J = I; J < b1; J += S) {
+                for (int c = 0; c < bcnt[J]; c += span, ++k) {
+                    si.push_back(I);
+                    sj0.push_back(J);
+                    sj1.push_back(std::min(b1, J + S));
+                    sc0.push_back(c);
+                }
             }
             rns.push_back(k);
         }
@@ -1067,7 +1083,8 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     int64_t need_p = 3 * max_slots, need_q = 3 * (int64_t)SYM_B * max_segs;
     // launch order (sym_seg): within each run's part of each band, segments
     // by their distance from the diagonal, nearest first: the k-th segment of
-    // every row of the part (first block pair J0 = I + k S), for k = 0, 1, ...,
+    // every row of the part (first block pair J0 = I + k S; with split block
+    // pairs, part k mod (SYM_B / tj) of block I + k / (SYM_B / tj)), for k = 0, 1, ...,
     // rows in order.  The rows of a part are consecutive and their segments
     // contiguous, so this is a permutation of the part's range, built in one
     // pass (the tables are rebuilt whenever the run set changes).
@@ -1095,7 +1112,7 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     };
     int o_bbeg = put(bbeg), o_bcnt = put(bcnt), o_runb0 = put(runb0), o_si = put(si),
         o_sj0 = put(sj0), o_sj1 = put(sj1), o_rj0 = put(rj0), o_rns = put(rns), o_rs0 = put(rs0),
-        o_ord = put(ord);
+        o_ord = put(ord), o_c0 = put(sc0);
     if ((int64_t)all.size() > H.dev_i_cap) {
         SPHX_CHECK(E.alloc((void **)&H.dev_i, sizeof(int32_t) * (all.size() + 1024)));
         H.dev_i_cap = (int64_t)all.size() + 1024;
@@ -1128,8 +1145,10 @@ static int sym_prepare(Engine &E, const GravConfig &cfg, hipStream_t st) {
     SPHX_HIP(hipStreamSynchronize(st));
     H.o_bbeg = o_bbeg; H.o_bcnt = o_bcnt; H.o_runb0 = o_runb0; H.o_si = o_si; H.o_sj0 = o_sj0;
     H.o_sj1 = o_sj1; H.o_rj0 = o_rj0; H.o_rns = o_rns; H.o_rs0 = o_rs0; H.o_ord = o_ord;
+    H.o_c0 = o_c0;
     H.run_seg0 = run_seg0;
     H.seg_len = S;
+    H.seg_tj = span;
     H.sizes = E.run_n;
     H.nblocks = nb;
     H.nseg = ns;
@@ -1176,7 +1195,8 @@ static int sym_run(Engine &E, hipStream_t st) {
         SymTabs T{d + H.o_bbeg, d + H.o_bcnt, d + H.o_runb0, d + H.o_si + a, d + H.o_sj0 + a,
                   d + H.o_sj1 + a, H.row_off, d + H.o_rj0, d + H.o_rns, d + H.o_rs0, H.bbox,
                   part, part + pl, part + 2 * pl, q, q + ql, q + 2 * ql, a, bd.seg_lo, bd.p_base,
-                  blk ? H.act : nullptr, E.s.run, E.t.r_pbeg, E.n, d + H.o_ord};
+                  blk ? H.act : nullptr, E.s.run, E.t.r_pbeg, E.n, d + H.o_ord, d + H.o_c0 + a,
+                  H.seg_tj};
         if (!marked) SPHX_CHECK(E.tmark(st, PH_GRAVK, 0));
         marked = true;
         k_grav_sym<<<(unsigned)(b - a), 64, 0, st>>>(T, E.s.x, E.s.y, E.s.z, E.s.m, eps);

@@ -133,3 +133,15 @@ def test_turbulent_cloud_mach():
     cs = np.sqrt(ic.KB * 1e4 / (1.22 * ic.MH)) / units["unit_velocity_cms"]
     assert abs(np.sqrt((c.vel ** 2).sum(1).mean()) / cs - 2.0) < 1e-9
     assert np.abs(c.vel.mean(0)).max() < 1e-12
+
+
+def test_autotune_pick_prefers_least_partial_memory():
+    """Among shapes within 1 % of the fastest the tuner keeps the one with the
+    least extra memory: a tiled shape, else the most block pairs per wave,
+    whole block pairs before split ones; a clearly faster shape still wins."""
+    from sph_2_amd.autotune import pick
+
+    assert pick({(8, 512, 1): 1.0, (8, 256, 1): 0.995}) == (8, 512, 1)
+    assert pick({(8, 512, 1): 1.0, (8, 256, 1): 0.95}) == (8, 256, 1)
+    assert pick({(8, 512, 1): 1.0, (8, 512, 4): 1.005, (8, 128, 1): 0.999}) == (8, 512, 4)
+    assert pick({(8, 512, 16): 1.0, (2, 512, 1): 1.008}) == (2, 512, 1)

@@ -178,7 +178,8 @@ def test_gravity_symmetric(gpu, evr, n, masses, eps):
     the oracle: far block pairs with one common mass, with mixed masses, near
     and diagonal block pairs, partial blocks (n = 3001: a last block of 441,
     whose second half of 256 is partial; 2200: a last block of 152, with no
-    second half; 300: one partial block), segment lengths 1, 3 and 64."""
+    second half; 300: one partial block), segment lengths 1, 3 and 64, and
+    block pairs split into parts of 256 and 128 j particles."""
     from sph_2_amd import _lib, libsx
     from sph_2_amd.autotune import _default
 
@@ -189,12 +190,12 @@ def test_gravity_symmetric(gpu, evr, n, masses, eps):
     acc_o = oracle.gravity(pos, m, e, 1.0)
     ctx = libsx._context(0)
     try:
-        for S in (1, 3, 64):
-            _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 8, 512, S), "config")
+        for tj, S in ((512, 1), (512, 3), (512, 64), (256, 1), (128, 1)):
+            _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 8, tj, S), "config")
             acc = libsx.getAcc_g_smth(pos, m, 1.0, e)
-            assert rel_rms(acc, acc_o) < 1e-5, S
+            assert rel_rms(acc, acc_o) < 1e-5, (tj, S)
             assert np.max(np.linalg.norm(acc - acc_o, axis=1) /
-                          np.linalg.norm(acc_o, axis=1)) < 1e-4, S
+                          np.linalg.norm(acc_o, axis=1)) < 1e-4, (tj, S)
             # the potential call falls back to the tiled sum
             acc2, pot = libsx.getAcc_g_smth(pos, m, 1.0, e, return_potential=True)
             assert rel_rms(acc2, acc_o) < 1e-5
