@@ -35,6 +35,12 @@ def _size_class(n: int) -> str:
 
 
 def _default(n: int):
+    """The shape of a size the table holds no entry for: from 1M particles the
+    symmetric sum with 16 block pairs per wave (its partial sums are summed in
+    row bands of bounded memory), below that the tiled kernel with enough j
+    split for four workgroups per CU."""
+    if (1 << 20) <= n <= SYM_MAX_N:
+        return 8, 512, 16
     ipl, tj = 2, 512
     blocks = max(1, (n + 256 * ipl - 1) // (256 * ipl))
     js = 1
@@ -109,51 +115,45 @@ def pick(res: dict):
     return max(near, key=lambda k: (k[2], k[1]))
 
 
-def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
-    """Time every candidate shape on an Evrard sphere of n particles."""
-    import numpy as np
+def _sweep(libsx, e, n: int, reps: int, res: dict):
+    """Time every candidate shape (stateless call, mean of reps after one
+    warm-up) into res."""
     import torch
 
-    from . import ic, libsx
-    from ._lib import check, load
+    def timed(cfg):
+        libsx.set_gravity_config(*cfg)
+        libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
+        torch.cuda.synchronize()
+        res[cfg] = (time.perf_counter() - t0) / reps
 
-    e = ic.evrard(n, seed=seed)
-    ctx = libsx._context(0)
-    res = {}
     for ipl, tj in CANDIDATES:
         blocks = (n + 256 * ipl - 1) // (256 * ipl)
         for js in (1, 2, 4, 8, 16):
             if js > 1 and blocks * js > 16 * CUS:
                 break
-            check(load().sphx_ctx_set_gravity_config(ctx, ipl, tj, js), "config")
-            libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
-            torch.cuda.synchronize()
-            res[(ipl, tj, js)] = (time.perf_counter() - t0) / reps
+            timed((ipl, tj, js))
     if n <= SYM_MAX_N:
         for S in SYM_SEGMENTS:
-            check(load().sphx_ctx_set_gravity_config(ctx, 8, 512, S), "config")
-            libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
-            torch.cuda.synchronize()
-            res[(8, 512, S)] = (time.perf_counter() - t0) / reps
+            timed((8, 512, S))
         for tj in SYM_SPLITS:
-            check(load().sphx_ctx_set_gravity_config(ctx, 8, tj, 1), "config")
-            libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
-            torch.cuda.synchronize()
-            res[(8, tj, 1)] = (time.perf_counter() - t0) / reps
+            timed((8, tj, 1))
+
+
+def tune_gravity(n: int, seed: int = 0, reps: int = 2, write: bool = True):
+    """Time every candidate shape on an Evrard sphere of n particles."""
+    from . import ic, libsx
+
+    e = ic.evrard(n, seed=seed)
+    res = {}
+    try:
+        _sweep(libsx, e, n, reps, res)
+    finally:
+        libsx.set_gravity_config()  # back to the tuned shapes
     best = pick(res)
-    check(load().sphx_ctx_set_gravity_config(ctx, *_default(n)), "config")
     if write:
         try:
             with open(_CACHE) as f:

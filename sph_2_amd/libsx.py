@@ -41,6 +41,33 @@ def _context(device: int = 0):
     return c
 
 
+_gcfg_fixed = {}  # device -> gravity launch shape fixed by set_gravity_config
+
+
+def set_gravity_config(ipl=None, tj=512, js=1, *, device=0):
+    """Fix the launch shape of the stateless gravity calls on a device
+    ((ipl, tj, js) as sphx_ctx_set_gravity_config: ipl 1/2/4 the tiled sum,
+    8 the symmetric one), or, called with no shape, return to the tuned shape
+    for each call's particle count (sph_2_amd/autotune.py), the default."""
+    if ipl is None:
+        _gcfg_fixed.pop(device, None)
+        return
+    cfg = (int(ipl), int(tj), int(js))
+    check(_lib.load().sphx_ctx_set_gravity_config(_context(device), *cfg), "set_gravity_config")
+    _gcfg_fixed[device] = cfg
+
+
+def _gravity_shape(device: int, n: int):
+    """Give the device's context the tuned shape for n particles unless one
+    is fixed."""
+    if device not in _gcfg_fixed:
+        from .autotune import choose_gravity_config
+
+        check(_lib.load().sphx_ctx_set_gravity_config(_context(device),
+                                                      *choose_gravity_config(n)),
+              "set_gravity_config")
+
+
 def _mode(mode: str) -> int:
     try:
         return _MODES[mode]
@@ -261,6 +288,7 @@ def getAcc_g_smth(pos, mass, G, epsilon, *, mode="mixed", device=0, nbeg=None, n
     dp, dm, de = dev(pos), dev(mass), dev(np.ascontiguousarray(eps))
     acc = t.empty((r1 - r0, 3), dtype=t.float64, device="cuda")
     pot = t.empty(r1 - r0, dtype=t.float64, device="cuda") if return_potential else None
+    _gravity_shape(device, n)
     check(_lib.load().sphx_getAcc_g_smth(_context(device), _mode(mode), n, ptr(dp), ptr(dm),
                                          ptr(de), float(G), r0, r1, ptr(acc), ptr(pot),
                                          _lib.stream_handle()), "getAcc_g_smth")

@@ -244,20 +244,19 @@ def test_gravity_1m_autotuned_shape(gpu, evr1m):
     """The symmetric sum with the launch shape the tuner picks at 1M, (8, 512,
     16), against the oracle on 1024 sampled rows: at this size 3 % of the block
     pairs take the near path, against 20-38 % at 4096 particles."""
-    from sph_2_amd import _lib, libsx
-    from sph_2_amd.autotune import _default, choose_gravity_config
+    from sph_2_amd import libsx
+    from sph_2_amd.autotune import choose_gravity_config
 
     e = evr1m
     cfg = tuple(choose_gravity_config(e.n))
     assert cfg[0] == 8, cfg
     rows = np.sort(np.random.default_rng(7).choice(e.n, 1024, replace=False))
     ref = oracle.gravity(e.pos, e.m, e.h, 1.0, rows=rows)
-    ctx = libsx._context(0)
     try:
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *cfg), "config")
+        libsx.set_gravity_config(*cfg)
         acc = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
     finally:
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(4096)), "config")
+        libsx.set_gravity_config()
     err = rel_rms(acc[rows], ref)
     worst = float(np.max(np.linalg.norm(acc[rows] - ref, axis=1) / np.linalg.norm(ref, axis=1)))
     print(f"gravity 1M {cfg}: rel RMS {err:.2e}, worst row {worst:.2e}")
@@ -301,19 +300,17 @@ def test_direct_gravity_4m_in_row_bands(gpu):
     Barnes-Hut configuration): its 206 GB of partial sums stream through 16 GiB
     row bands.  Against the FP64 oracle on 512 sampled rows, and Newton's third
     law over all particles (sum_i m_i a_i = 0 to round-off)."""
-    from sph_2_amd import _lib, libsx
-    from sph_2_amd.autotune import _default
+    from sph_2_amd import libsx
 
     n = 1 << 22
     e = ic.plummer(n, seed=0, eps=0.01)
     rows = np.sort(np.random.default_rng(3).choice(n, 512, replace=False))
     ref = oracle.gravity(e.pos, e.m, e.eps, 1.0, rows=rows)
-    ctx = libsx._context(0)
     try:
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 8, 512, 16), "config")
+        libsx.set_gravity_config(8, 512, 16)
         acc = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.eps)
     finally:
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(4096)), "config")
+        libsx.set_gravity_config()
     err = rel_rms(acc[rows], ref)
     scale = float((e.m[:, None] * np.abs(acc)).sum())
     net = float(np.abs((e.m[:, None] * acc).sum(0)).max())

@@ -156,16 +156,14 @@ def test_gravity(gpu, evr, mode):
 def test_gravity_launch_shapes(gpu, evr, cfg):
     """Every launch shape the tuner may pick (particles per lane, j-tile, j
     split with partial sums) gives the same forces and potential."""
-    from sph_2_amd import _lib, libsx
+    from sph_2_amd import libsx
 
     acc_o, pot_o = oracle.gravity(evr["pos"], evr["m"], evr["h"], 1.0, potential=True)
-    ctx = libsx._context(0)
-    _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *cfg), "config")
+    libsx.set_gravity_config(*cfg)
     try:
         acc, pot = libsx.getAcc_g_smth(evr["pos"], evr["m"], 1.0, evr["h"], return_potential=True)
     finally:
-        from sph_2_amd.autotune import _default
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(N)), "config")
+        libsx.set_gravity_config()
     assert rel_rms(acc, acc_o) < 1e-5
     assert rel_rms(pot, pot_o) < 1e-5
 
@@ -180,18 +178,16 @@ def test_gravity_symmetric(gpu, evr, n, masses, eps):
     whose second half of 256 is partial; 2200: a last block of 152, with no
     second half; 300: one partial block), segment lengths 1, 3 and 64, and
     block pairs split into parts of 256 and 128 j particles."""
-    from sph_2_amd import _lib, libsx
-    from sph_2_amd.autotune import _default
+    from sph_2_amd import libsx
 
     rng = np.random.default_rng(n)
     pos = evr["pos"][:n]
     m = evr["m"][:n] if masses == "equal" else evr["m"][:n] * rng.uniform(0.5, 1.5, n)
     e = evr["h"][:n] if eps == "h" else np.full(n, eps)
     acc_o = oracle.gravity(pos, m, e, 1.0)
-    ctx = libsx._context(0)
     try:
         for tj, S in ((512, 1), (512, 3), (512, 64), (256, 1), (128, 1)):
-            _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 8, tj, S), "config")
+            libsx.set_gravity_config(8, tj, S)
             acc = libsx.getAcc_g_smth(pos, m, 1.0, e)
             assert rel_rms(acc, acc_o) < 1e-5, (tj, S)
             assert np.max(np.linalg.norm(acc - acc_o, axis=1) /
@@ -200,7 +196,7 @@ def test_gravity_symmetric(gpu, evr, n, masses, eps):
             acc2, pot = libsx.getAcc_g_smth(pos, m, 1.0, e, return_potential=True)
             assert rel_rms(acc2, acc_o) < 1e-5
     finally:
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(N)), "config")
+        libsx.set_gravity_config()
 
 
 def test_plan_step_symmetric_gravity(gpu):
@@ -444,19 +440,17 @@ def test_symmetric_gravity_conserves_momentum_at_scale(gpu):
     symmetric sum applies each pair's force to both particles, so the total
     gravitational force sum_i m_i a_i vanishes to round-off, and the result
     agrees with the tiled sum on the same input."""
-    from sph_2_amd import _lib, libsx
-    from sph_2_amd.autotune import _default
+    from sph_2_amd import libsx
 
     n = 262144
     e = ic.evrard(n, seed=9)
-    ctx = libsx._context(0)
     try:
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 8, 512, 2), "config")
+        libsx.set_gravity_config(8, 512, 2)
         a_sym = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, 2, 512, 4), "config")
+        libsx.set_gravity_config(2, 512, 4)
         a_til = libsx.getAcc_g_smth(e.pos, e.m, 1.0, e.h)
     finally:
-        _lib.check(_lib.load().sphx_ctx_set_gravity_config(ctx, *_default(N)), "config")
+        libsx.set_gravity_config()
     scale = float((e.m[:, None] * np.abs(a_sym)).sum())
     net = np.abs((e.m[:, None] * a_sym).sum(0)).max()
     assert net / scale < 1e-6
