@@ -33,7 +33,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector (packed) peak
 # HBM bytes per gravity-kernel launch at N = 1,048,576: 2 x FETCH_SIZE + WRITE_SIZE of the
 # step's dispatch in two rocprofv3 --pmc runs of `bench.py --workload kernels` (FETCH_SIZE reads
 # half of the bytes of that workload's 256 MiB calibration copy, hence the factor 2).
-GRAV_TRAFFIC_BYTES_1M = {"k_grav": 11.982e9, "k_grav_sym": 44.733e9}
+GRAV_TRAFFIC_BYTES_1M = {"k_grav": 11.982e9, "k_grav_sym": 52.163e9}
 PEAK_HBM_GBS = 8000.0
 
 
