@@ -65,7 +65,11 @@ typedef struct sphx_params {
     int32_t h_predictor; /* 1: Gadget-2 eq. 31 start of the h iteration (LIBSX3:960) */
     int32_t h_mode;      /* 0: neighbour-count band iteration; 1: kNN h = half the distance
                             of rank knn_kth, as do_smoothingX (LIBSX:762-783) each step of
-                            augsphx6.5.py:510 (mixed precision only) */
+                            augsphx6.5.py:510 (mixed precision only); 2: the reference's
+                            own rule, steps of 0.002 h_prev from 2 h_prev (or the eq. 31
+                            start) until ndown <= k <= nup, the largest radius after
+                            max_iter + 1 passes (LIBSX3:947-1006, LIBV2:7-64); 3: the same
+                            with steps of 0.02 h_prev (do_smoothingX4) */
     int32_t knn_kth;     /* rank for h_mode 1 (64, self included at rank 0) */
     double av_w_eps;     /* Gadget AV: w = v_ij.r_ij / (r + av_w_eps); 1e-5 in LIBSX:158-169
                             and LIBV2, 1e-15 in augsphx6.5.py PI_ij */

@@ -43,6 +43,8 @@ def lib():
         L.orc_count_within.argtypes = [i64, dp, dp, lp, i64, ip]
         L.orc_smoothing.argtypes = [i64, dp, dp, ip, dp, C.c_double, i32, i32, i32, i32, lp, i64,
                                     dp, ip, ip, ip]
+        L.orc_smoothing_steps.argtypes = [i64, dp, dp, ip, dp, C.c_double, i32, i32, i32, i32,
+                                          C.c_double, lp, i64, dp, ip, ip, ip]
         L.orc_neighbors.argtypes = [i64, dp, dp, lp, i64, lp, lp]
         L.orc_density.argtypes = [i64, dp, dp, dp, lp, i64, dp, ip]
         L.orc_eos.argtypes = [i64, dp, dp, C.c_double, dp, dp]
@@ -117,7 +119,9 @@ class Params:
     balsara: int = 0
     eps_from_h: bool = True   # softening eps_i = h_i
     h_predictor: bool = True  # Gadget-2 eq. 31 start for the h iteration
-    h_mode: int = 0           # 0 count-band iteration, 1 kNN (do_smoothingX)
+    h_mode: int = 0           # 0 count-band iteration, 1 kNN (do_smoothingX), 2 the
+                              # reference's fixed-step rule (LIBSX3:947-1006), 3 the
+                              # same with ten times the step (do_smoothingX4)
     knn_kth: int = 64
     av_w_eps: float = 1e-5    # Gadget AV: w = v.r / (r + av_w_eps)
 
@@ -278,9 +282,13 @@ def count_within(pos, radius, rows=None):
 
 
 def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(), rows=None):
-    """Smoothing-length iteration (LIBSX3:946-1010, LIBV7:255-330).
+    """Smoothing-length iteration (LIBSX3:946-1010, LIBV7:255-330): the
+    count-band iteration, or with p.h_mode == 2 (3) the reference's own rule
+    of fixed steps of 0.002 (0.02) h_prev (LIBSX3:947-1006, LIBV2:7-64;
+    do_smoothingX4).
 
-    Returns (h, k, iterations, status) per row; status 1 = not converged."""
+    Returns (h, k, iterations, status) per row; status 1 = not converged (the
+    step rule: the pass limit reached)."""
     n = len(pos)
     pos, pp = _d(pos)
     hp, hpp = _d(h_prev)
@@ -296,8 +304,13 @@ def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(),
     k = np.zeros(nr, np.int32)
     it = np.zeros(nr, np.int32)
     st = np.zeros(nr, np.int32)
-    lib().orc_smoothing(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter, rp, nr,
-                        _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
+    if p.h_mode in (2, 3):
+        lib().orc_smoothing_steps(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter,
+                                  0.02 if p.h_mode == 3 else 0.002, rp, nr,
+                                  _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
+    else:
+        lib().orc_smoothing(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter, rp, nr,
+                            _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
     return h, k, it, st
 
 

@@ -189,6 +189,83 @@ void orc_smoothing(int64_t n, const double *pos, const double *h_prev, const int
     }
 }
 
+/*
+ * The reference's own smoothing-length rule (LIBSX3:947-1006
+ * smoothing_fast_new_mpi3, LIBSX:940-990 smoothing_fast_new_mpi2; the same
+ * loop as the CUDA smoothing_h of LIBV2:7-64, there in FP32).  Support
+ * radius s (= 2h): from 2h_prev, or with the Gadget-2 eq. 31 start
+ * 2 (0.5 h (1 + F) + (1/3) (h divV) dt) (LIBSX3:961, in that order of
+ * operations; F = cbrt(nngb/k_prev) by orc_h_factor, where the reference takes
+ * (Nngb/Nngb_previous)**(1/3)).  Each pass counts k = #{j : sqrt(r2_ij) <= s}
+ * (self included); k < ndown: s += hstep h_prev; k > nup: s -= hstep h_prev
+ * (hstep = coeff * 2: 0.002 with LIBSX3's coeff = 0.001, 0.02 with
+ * do_smoothingX4's 0.01); the largest s is kept, and after pass
+ * max_iter + 1 (N_iter > 100) s becomes that largest value whatever the
+ * count.  The loop ends at the first pass whose count lies in [ndown, nup].
+ * k_out is the last pass's count (the reference's Nngbx); status 1 = the
+ * pass limit was reached.
+ */
+static int64_t count_within_sqrt(const double *pos, int64_t n, int64_t i, double s) {
+    int64_t k = 0;
+    for (int64_t j = 0; j < n; ++j)
+        if (sqrt(pair_r2(pos, i, j)) <= s) ++k;
+    return k;
+}
+
+static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_prev,
+                             int32_t k_prev, int use_pred, double divv, double dt, int32_t nngb,
+                             int32_t ndown, int32_t nup, int32_t max_iter, double hstep,
+                             double *h_out, int32_t *k_out, int32_t *it_out, int32_t *st_out) {
+    double s = 2.0 * h_prev;
+    if (use_pred) {
+        double f = orc_h_factor(nngb, k_prev);
+        double t1 = 0.5 * h_prev;
+        double t2 = 1.0 + f;
+        double t3 = t1 * t2;
+        double t4 = h_prev * divv;
+        double t5 = (1.0 / 3.0) * t4;
+        double t6 = t5 * dt;
+        double sp = 2.0 * (t3 + t6);
+        if (sp > 0.0 && sp < INFINITY) s = sp;
+    }
+    const double step = hstep * h_prev;
+    double smax = s;
+    int64_t k = 0;
+    int32_t it = 0, st = 0;
+    for (;;) {
+        k = count_within_sqrt(pos, n, i, s);
+        const int in_band = k >= ndown && k <= nup;
+        if (k < ndown) s = s + step;
+        if (k > nup) s = s - step;
+        if (s > smax) smax = s;
+        ++it;
+        if (it > max_iter) {
+            s = smax;
+            st = 1;
+            break;
+        }
+        if (in_band) break;
+    }
+    *h_out = 0.5 * s;
+    *k_out = (int32_t)k;
+    *it_out = it;
+    *st_out = st;
+}
+
+void orc_smoothing_steps(int64_t n, const double *pos, const double *h_prev, const int32_t *k_prev,
+                         const double *divv, double dt, int32_t nngb, int32_t ndown, int32_t nup,
+                         int32_t max_iter, double hstep, const int64_t *rows, int64_t nrows,
+                         double *h_out, int32_t *k_out, int32_t *it_out, int32_t *st_out) {
+    int use_pred = (k_prev != NULL && divv != NULL);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t r = 0; r < nrows; ++r) {
+        int64_t i = row_of(rows, r);
+        smooth_steps_one(pos, n, i, h_prev[i], use_pred ? k_prev[i] : 0, use_pred,
+                         use_pred ? divv[i] : 0.0, dt, nngb, ndown, nup, max_iter, hstep,
+                         &h_out[r], &k_out[r], &it_out[r], &st_out[r]);
+    }
+}
+
 /* Neighbour lists under the symmetric pair test r2 <= (h_i+h_j)^2 (the gate of
  * LIBV2:89 "rr <= 2*hij").  Two-phase: with idx == NULL only offsets are
  * written (offsets[r+1] = count for row r); then call again to fill. */

@@ -385,7 +385,7 @@ int Engine::forces(hipStream_t st, bool predictor, bool with_pot) {
 }
 
 int Engine::plain_forces(hipStream_t st, bool predictor, bool with_pot) {
-    if (p.h_mode != 0) {
+    if (p.h_mode == 1) {
         sphx_set_error("kNN smoothing lengths (h_mode 1) are a mixed-precision option");
         return SPHX_ERR_ARG;
     }
@@ -617,7 +617,7 @@ static int check_mode(int mode) {
 static int check_params(const sphx_params *p) {
     if (!p || p->nngb <= 0 || p->ndown > p->nngb || p->nup < p->nngb || p->max_iter <= 0 ||
         !(p->gamma > 1.0) || (p->av_form != 0 && p->av_form != 1) ||
-        (p->h_mode != 0 && p->h_mode != 1) || p->knn_kth < 0 || p->knn_kth > 4096 ||
+        p->h_mode < 0 || p->h_mode > 3 || p->knn_kth < 0 || p->knn_kth > 4096 ||
         !(p->av_w_eps >= 0.0)) {
         sphx_set_error("invalid sphx_params");
         return SPHX_ERR_ARG;
@@ -1366,7 +1366,7 @@ extern "C" int sphx_plan_create(int device, int mode, const sphx_params *p,
     if (!out) { sphx_set_error("null output pointer"); return SPHX_ERR_ARG; }
     SPHX_CHECK(check_mode(mode));
     SPHX_CHECK(check_params(p));
-    if (mode == SPHX_FP64 && p->h_mode != 0) {
+    if (mode == SPHX_FP64 && p->h_mode == 1) {
         sphx_set_error("kNN smoothing lengths (h_mode 1) are a mixed-precision option");
         return SPHX_ERR_ARG;
     }

@@ -209,8 +209,16 @@ struct SmoothArgs {
     int32_t *h_iters;  // per particle, nullable
     int64_t *counters;
     int32_t halves;    // 1: a group per bucket; 2: a group per half bucket
+    double hstep;      // RULE 2: the step as a fraction of h_prev (2 x the reference's coeff)
 };
 
+// RULE 0: the count-band iteration (a log-space Newton step, kept inside the
+// bracket by bisection).  RULE 2: the reference's own rule (LIBSX3:947-1006,
+// LIBV2:7-64; do_smoothingX4 with ten times the step): fixed steps of
+// a.hstep h_prev (0.002 or 0.02) from 2 h_prev (or the eq. 31
+// start, in the reference's order of operations), counts by sqrt(r^2) <= s as
+// the reference tests, the largest radius taken after max_iter + 1 passes.
+template <int RULE>
 __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a) {
     __shared__ double sx[64], sy[64], sz[64];
     const WalkSlot ws = walk_slot(t, a.halves);
@@ -236,8 +244,8 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
             double t2 = 1.0 + f;
             double t3 = t1 * t2;
             double t4 = hprev * s.divv[i];
-            double t5 = t4 * dt;
-            double t6 = t5 / 3.0;
+            double t5 = RULE == 2 ? (1.0 / 3.0) * t4 : t4 * dt;
+            double t6 = RULE == 2 ? t5 * dt : t5 / 3.0;
             double sp = 2.0 * (t3 + t6);
             if (sp > 0.0 && sp < INFINITY) sr = sp;
         }
@@ -246,6 +254,8 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     int klo = 0, khi = 0, kfin = 0;
     bool have_lo = false, have_hi = false, done = !ok, conv = false;
     int it = 0;
+    const double step = a.hstep * hprev;  // RULE 2: (coeff * 2) * h_prev
+    double smax = sr;                    // RULE 2: the largest radius tried
     while (true) {
         bool active = ok && !done;
         if (!__any(active)) break;
@@ -306,19 +316,32 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             double r2 = sphx_r2(xi - cx[u], yi - cy[u], zi - cz[u]);
-                            kc += (mine && r2 <= s2) ? 1 : 0;
+                            kc += (mine && (RULE == 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
                         }
                     }
                     for (; q < k.n; ++q) {
                         double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
-                        kc += (mine && r2 <= s2) ? 1 : 0;
+                        kc += (mine && (RULE == 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
                     }
                 });
         }
         if (active) {
 #pragma clang fp contract(off)
             ++it;
-            if (kc >= a.ndown && kc <= a.nup) {
+            if (RULE == 2) {
+                const bool in_band = kc >= a.ndown && kc <= a.nup;
+                if (kc < a.ndown) sr = sr + step;
+                if (kc > a.nup) sr = sr - step;
+                if (sr > smax) smax = sr;
+                kfin = kc;
+                if (it > a.max_iter) {
+                    sr = smax;  // the pass limit: the largest radius, whatever the count
+                    done = true;
+                } else if (in_band) {
+                    done = true;
+                    conv = true;
+                }
+            } else if (kc >= a.ndown && kc <= a.nup) {
                 done = true;
                 conv = true;
                 kfin = kc;
@@ -345,8 +368,10 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     }
     if (ok) {
         if (!conv) {
-            if (have_hi) { sr = hi; kfin = khi; }
-            else { sr = lo; kfin = klo; }
+            if (RULE == 0) {
+                if (have_hi) { sr = hi; kfin = khi; }
+                else { sr = lo; kfin = klo; }
+            }
             atomicAdd((unsigned long long *)&a.counters[CNT_H_CAPPED], 1ull);
         }
         s.h[i] = 0.5 * sr;
@@ -1271,8 +1296,9 @@ int Engine::smoothing(hipStream_t st, bool predictor) {
     if (n == 0 || nb_launch() == 0) return SPHX_OK;
     const int halves = nb_launch() <= SMOOTH_HALF_BUCKETS ? 2 : 1;
     SmoothArgs a{p.nngb, p.ndown, p.nup, p.max_iter, (predictor && p.h_predictor) ? 1 : 0,
-                 d_run, h_iters, counters, halves};
-    k_smoothing<<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
+                 d_run, h_iters, counters, halves, sphx_h_step(p.h_mode)};
+    if (p.h_mode >= 2) k_smoothing<2><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
+    else k_smoothing<0><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -81,6 +81,11 @@ __device__ __forceinline__ double sphx_sq(double t) {
     return t * t;
 }
 
+// Step of the reference's smoothing-length rule as a fraction of h_prev:
+// (coeff * 2) with coeff = 0.001 (h_mode 2: LIBSX3:952, DRV2:286) or 0.01
+// (h_mode 3: do_smoothingX4); 2 x fl(coeff) is fl(2 coeff) exactly.
+inline double sphx_h_step(int h_mode) { return h_mode == 3 ? 0.02 : 0.002; }
+
 // cbrt(nngb/k) clamped to [1/2, 2] by a fixed-count Newton iteration with
 // separately rounded operations; identical on any IEEE machine.
 __device__ __forceinline__ double sphx_h_factor(int nngb, int k) {

@@ -14,12 +14,15 @@ constexpr int PT = 256;
 
 inline unsigned grid_rows(int64_t r0, int64_t r1) { return (unsigned)((r1 - r0 + PT - 1) / PT); }
 
-// smoothing_h (myCppSPHLibs_v2.h:7-64) with the canonical update rule.
+// smoothing_h (myCppSPHLibs_v2.h:7-64) with the canonical update rule, or
+// (RULE 2) its own rule of fixed steps of hstep h_prev (also LIBSX3:947-1006;
+// the same decisions as k_smoothing<2>).
+template <int RULE>
 __global__ __launch_bounds__(PT) void kp_smoothing(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     const double *__restrict__ h_prev, const int32_t *__restrict__ k_prev,
     const double *__restrict__ divv, const double *__restrict__ dt_p, double dt, int32_t nngb,
-    int32_t ndown, int32_t nup, int32_t max_iter, int64_t n, int64_t r0, int64_t r1,
+    int32_t ndown, int32_t nup, int32_t max_iter, double hstep, int64_t n, int64_t r0, int64_t r1,
     double *__restrict__ h_out, int32_t *__restrict__ k_out, int32_t *__restrict__ it_out,
     int64_t *counters) {
 #pragma clang fp contract(off)
@@ -35,8 +38,8 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
         double t2 = 1.0 + f;
         double t3 = t1 * t2;
         double t4 = hp * divv[i];
-        double t5 = t4 * dt;
-        double t6 = t5 / 3.0;
+        double t5 = RULE == 2 ? (1.0 / 3.0) * t4 : t4 * dt;
+        double t6 = RULE == 2 ? t5 * dt : t5 / 3.0;
         double sp = 2.0 * (t3 + t6);
         if (sp > 0.0 && sp < INFINITY) s = sp;
     }
@@ -44,7 +47,31 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
     int64_t klo = 0, khi = 0, k = 0;
     bool have_lo = false, have_hi = false, conv = false;
     int it = 0;
-    while (it < max_iter) {
+    if (RULE == 2) {
+        const double step = hstep * hp;  // (coeff * 2) * h_prev
+        double smax = s;
+        for (;;) {
+            k = 0;
+            for (int64_t j = 0; j < n; ++j) {
+                double r2 = sphx_r2(xi - x[j], yi - y[j], zi - z[j]);
+                if (sqrt(r2) <= s) ++k;
+            }
+            const bool in_band = k >= ndown && k <= nup;
+            if (k < ndown) s = s + step;
+            if (k > nup) s = s - step;
+            if (s > smax) smax = s;
+            ++it;
+            if (it > max_iter) {
+                s = smax;
+                break;
+            }
+            if (in_band) {
+                conv = true;
+                break;
+            }
+        }
+    }
+    while (RULE == 0 && it < max_iter) {
         double s2 = s * s;
         k = 0;
         for (int64_t j = 0; j < n; ++j) {
@@ -70,8 +97,10 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
         s = snew;
     }
     if (!conv) {
-        if (have_hi) { s = hi; k = khi; }
-        else { s = lo; k = klo; }
+        if (RULE == 0) {
+            if (have_hi) { s = hi; k = khi; }
+            else { s = lo; k = klo; }
+        }
         atomicAdd((unsigned long long *)&counters[CNT_H_CAPPED], 1ull);
     }
     h_out[i - r0] = 0.5 * s;
@@ -325,9 +354,17 @@ int plain_smoothing(const double *x, const double *y, const double *z, const dou
                     double *h_out, int32_t *k_out, int32_t *it_out, int64_t *counters,
                     hipStream_t st) {
     if (r1 <= r0) return SPHX_OK;
-    kp_smoothing<<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt, p.nngb,
-                                                    p.ndown, p.nup, p.max_iter, n, r0, r1, h_out,
-                                                    k_out, it_out, counters);
+    const double hstep = sphx_h_step(p.h_mode);
+    if (p.h_mode >= 2)
+        kp_smoothing<2><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
+                                                           p.nngb, p.ndown, p.nup, p.max_iter,
+                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           counters);
+    else
+        kp_smoothing<0><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
+                                                           p.nngb, p.ndown, p.nup, p.max_iter,
+                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;
 }

@@ -441,15 +441,21 @@ def _smoothing(pos, h, k_prev, divV, dt, nbeg, nend, mode, device, **pkw):
 def smoothing_fast_new_mpi3(nbeg, nend, pos, h, Nngb_previous, divV, dt, *, mode="mixed",
                             device=0, **pkw):
     """Smoothing lengths and neighbour counts for rows [nbeg, nend)
-    (LIBSX3:946-1010): Gadget-2 eq. 31 start, then the count iteration.
-    Returns (h, Nngb) like the reference."""
+    (LIBSX3:947-1010) by the reference's own rule: the Gadget-2 eq. 31 start,
+    then steps of 0.002 h until 59..69 neighbours lie within 2h, at most 101
+    counts (the largest radius then).  Returns (h, Nngb) like the reference
+    (Nngb: the last count).  h_mode=0 gives the count-band iteration instead."""
+    pkw.setdefault("h_mode", 2)
     h_out, k, _ = _smoothing(pos, h, np.asarray(Nngb_previous).astype(np.int32), divV, dt, nbeg,
                              nend, mode, device, **pkw)
     return h_out, k.astype(np.float64)
 
 
 def smoothing_fast_new_mpi2(nbeg, nend, pos, h, *, mode="mixed", device=0, **pkw):
-    """Rows [nbeg, nend), iteration started at 2h (LIBSX:946-999)."""
+    """Rows [nbeg, nend) by the reference's own rule (LIBSX:940-990): steps of
+    0.002 h from 2h until 59..69 neighbours lie within 2h, at most 101
+    counts.  h_mode=0 gives the count-band iteration instead."""
+    pkw.setdefault("h_mode", 2)
     return _smoothing(pos, h, None, None, 0.0, nbeg, nend, mode, device, **pkw)[0]
 
 
@@ -525,9 +531,10 @@ def _row_offset(pos, sub):
 
 
 def do_smoothingX4(pos, h, *, mode="mixed", device=0, **pkw):
-    """do_smoothingX4(pos, h) (LIBSX:887-939): h in the 59..69 neighbour band,
-    started at 2h.  The reference steps the radius by +-0.01 x 2h; the band
-    and start are kept, the update is the canonical bracketed iteration."""
+    """do_smoothingX4(pos, h) (LIBSX:887-939) by the reference's own rule:
+    steps of 0.02 h from 2h until 59..69 neighbours lie within 2h, at most 101
+    counts.  h_mode=0 gives the count-band iteration instead."""
+    pkw.setdefault("h_mode", 3)
     return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)[0]
 
 

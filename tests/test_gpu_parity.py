@@ -7,6 +7,8 @@ case), plus a small random cube.  Bars:
   hydro and gravity accelerations               1e-5 relative RMS
   FP64 baseline kernels                         1e-12 relative
 """
+import warnings
+
 import numpy as np
 import pytest
 
@@ -51,17 +53,66 @@ def test_smoothing_bit_exact(gpu, evr, mode):
 
 
 @pytest.mark.parametrize("mode", ["mixed", "fp64"])
-def test_smoothing_predictor_rows(gpu, evr, mode):
+@pytest.mark.parametrize("h_mode", [2, 0])
+def test_smoothing_predictor_rows(gpu, evr, mode, h_mode):
+    """smoothing_fast_new_mpi3 on rows [100, 900) from the eq. 31 start: by
+    default the reference's own rule (steps of 0.002 h), with h_mode=0 the
+    count-band iteration; h and counts bit-exact against the oracle's rule."""
     from sph_2_amd import libsx
 
     kprev = np.clip(evr["k"] + np.arange(N) % 7 - 3, 1, None)
     dt = 1e-3
     ho, ko, _, _ = oracle.smoothing(evr["pos"], evr["h"], kprev, evr["dv"], dt,
-                                    rows=np.arange(100, 900))
+                                    p=oracle.Params(h_mode=h_mode), rows=np.arange(100, 900))
     h, kk = libsx.smoothing_fast_new_mpi3(100, 900, evr["pos"], evr["h"], kprev, evr["dv"], dt,
-                                          mode=mode)
+                                          mode=mode, h_mode=h_mode)
     assert np.array_equal(kk.astype(np.int32), ko)
     assert np.array_equal(h, ho)
+
+
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_reference_h_rule_drop_ins(gpu, evr, mode):
+    """smoothing_fast_new_mpi2 (steps of 0.002 h from 2h, LIBSX:940-990) and
+    do_smoothingX4 (steps of 0.02 h) from the IC's guess, where many rows run
+    into the reference's limit of 101 counts and take their largest radius:
+    every row bit-exact against the oracle's restatement, with the warning
+    the reference prints as a RuntimeWarning."""
+    from sph_2_amd import libsx
+
+    for fn, h_mode in ((lambda: libsx.smoothing_fast_new_mpi2(0, N, evr["pos"], evr["h0"],
+                                                              mode=mode), 2),
+                       (lambda: libsx.do_smoothingX4(evr["pos"], evr["h0"], mode=mode), 3)):
+        ho, ko, it, st = oracle.smoothing(evr["pos"], evr["h0"], p=oracle.Params(h_mode=h_mode))
+        with warnings.catch_warnings(record=True) as w:
+            warnings.simplefilter("always")
+            h = fn()
+        assert np.array_equal(h, ho), h_mode
+        assert st.any() == any(issubclass(x.category, RuntimeWarning) for x in w)
+    assert st.sum() < N  # most rows of do_smoothingX4 land in the band
+
+
+def test_plan_reference_h_rule(gpu):
+    """A plan with h_mode=2 (the reference driver's smoothing_h rule,
+    LIBV2:7-64, DRV2:419): after the bootstrap h and counts bit-exact against
+    the oracle's composition; after a step, whose eq. 31 start carries the
+    previous step's div v, within one step of 0.002 h."""
+    from sph_2_amd.plan import SphPlan
+
+    e = ic.evrard(N, seed=0)
+    p = oracle.Params(h_mode=2)
+    s0 = oracle.bootstrap(oracle.State(e.pos, e.vel, e.u, e.m, e.h), p)
+    s1 = oracle.step(s0, 1e-4, p)
+    for mode in ("mixed", "fp64"):
+        pl = SphPlan(N, mode=mode, h_mode=2)
+        pl.load(e.pos, e.vel, e.u, e.m, e.h)
+        pl.init()
+        st0 = pl.state()
+        assert np.array_equal(st0["h"], s0.h) and np.array_equal(st0["k"], s0.k)
+        pl.step(1, 1e-4)
+        st1 = pl.state()
+        assert np.mean(st1["h"] != s1.h) < 0.01
+        assert rel_max(st1["h"], s1.h) < 2.1e-3
+        pl.close()
 
 
 @pytest.mark.parametrize("mode", ["mixed", "fp64"])

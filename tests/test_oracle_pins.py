@@ -321,6 +321,83 @@ def test_smoothing_lattice_degeneracy_caps():
         assert k[c] > 69
 
 
+def _chain(n=201):
+    """Unit-spaced particles on a line: an interior particle has exactly
+    1 + 2 floor(s) neighbours within s (sqrt of a square of an integer)."""
+    pos = np.zeros((n, 3))
+    pos[:, 0] = np.arange(float(n))
+    return pos
+
+
+@pytest.mark.parametrize("h_mode,h0,h_want,k_want,passes", [
+    (2, 14.0, 14.504, 59, 37),   # up: s = 28 + 36 x 0.028 = 29.008, 57 -> 59 neighbours
+    (3, 14.0, 14.56, 59, 5),     # ten times the step: s = 28 + 4 x 0.28 = 29.12
+    (2, 18.0, 17.496, 69, 29),   # down: s = 36 - 28 x 0.036 = 34.992, 73 -> 69
+])
+def test_smoothing_step_rule_on_a_chain(h_mode, h0, h_want, k_want, passes):
+    """The reference's own rule (LIBSX3:947-1006; do_smoothingX4 with coeff
+    0.01) on a chain, where the count at each radius is known in closed form:
+    the first radius on the grid 2 h0 +- m (2 coeff h0) whose count lies in
+    59..69, reached in m + 1 counts."""
+    pos = _chain()
+    h, k, it, st = oracle.smoothing(pos, np.full(len(pos), h0), p=oracle.Params(h_mode=h_mode),
+                                    rows=[100])
+    assert abs(h[0] - h_want) < 1e-9
+    assert (k[0], it[0], st[0]) == (k_want, passes, 0)
+
+
+def test_smoothing_step_rule_eq31_start():
+    """The Gadget-2 eq. 31 start of smoothing_fast_new_mpi3 (LIBSX3:961),
+    2 (0.5 h (1 + (64/k_prev)^(1/3)) + (1/3) (h divV) dt), in exact binary
+    values: h = 16, k_prev = 64, divV = 0.1875, dt = 0.5 give s = 33, whose 67
+    chain neighbours already lie in the band: one count, h = 16.5."""
+    pos = _chain()
+    n = len(pos)
+    h, k, it, st = oracle.smoothing(pos, np.full(n, 16.0), np.full(n, 64, np.int32),
+                                    np.full(n, 0.1875), 0.5, p=oracle.Params(h_mode=2), rows=[100])
+    assert (h[0], k[0], it[0], st[0]) == (16.5, 67, 1, 0)
+
+
+def test_smoothing_step_rule_pass_limit():
+    """From h0 = 10 the chain needs 450 steps of 0.02 to reach 59 neighbours:
+    after 101 counts the rule takes the largest radius tried, 20 + 101 x 0.02,
+    and reports the last count (taken at 20 + 100 steps, accumulated by
+    repeated addition as the reference does)."""
+    pos = _chain()
+    h, k, it, st = oracle.smoothing(pos, np.full(len(pos), 10.0), p=oracle.Params(h_mode=2),
+                                    rows=[100])
+    s = 20.0
+    for _ in range(100):
+        s = s + 0.002 * 10.0
+    assert abs(h[0] - 0.5 * (s + 0.02)) < 1e-12 and abs(h[0] - 11.01) < 1e-9
+    assert (k[0], it[0], st[0]) == (1 + 2 * math.floor(s), 101, 1)
+
+
+def test_smoothing_step_rule_walks_the_grid():
+    """On an Evrard sphere from the IC guess: every row that stops inside the
+    pass limit ends on the grid 2 h0 + m 0.002 h0 (|m| <= 100) with its count
+    (sqrt(r^2) <= 2h, self included) in 59..69, and every grid radius it
+    passed on the way has a count on the side it stepped away from; rows at
+    the limit end on the largest radius they tried."""
+    e = ic.evrard(1500, seed=4)
+    rows = np.arange(0, 1500, 7)
+    h, k, it, st = oracle.smoothing(e.pos, e.h, p=oracle.Params(h_mode=2), rows=rows)
+    for r, i in enumerate(rows):
+        d = np.sqrt(((e.pos - e.pos[i]) ** 2).sum(1))
+        s0, step = 2 * e.h[i], 0.002 * e.h[i]
+        mf = (2 * h[r] - s0) / step
+        m = int(round(mf))
+        assert abs(mf - m) < 1e-6 and abs(m) <= 101
+        if st[r]:
+            assert it[r] == 101 and m >= 0
+            continue
+        assert 59 <= (d <= 2 * h[r]).sum() <= 69 and k[r] == (d <= 2 * h[r]).sum()
+        assert it[r] == abs(m) + 1
+        for j in range(abs(m)):
+            c = (d <= s0 + np.sign(m) * j * step).sum()
+            assert (c < 59) if m > 0 else (c > 69)
+
+
 def test_h_factor_cube_root():
     for k in (9, 27, 50, 64, 100, 300, 511):
         assert abs(oracle.h_factor(64, k) - (64 / k) ** (1 / 3)) < 1e-15
