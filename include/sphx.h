@@ -69,7 +69,10 @@ typedef struct sphx_params {
                             own rule, steps of 0.002 h_prev from 2 h_prev (or the eq. 31
                             start) until ndown <= k <= nup, the largest radius after
                             max_iter + 1 passes (LIBSX3:947-1006, LIBV2:7-64); 3: the same
-                            with steps of 0.02 h_prev (do_smoothingX4) */
+                            with steps of 0.02 h_prev (do_smoothingX4); 4: the LIBV7
+                            driver's rule, steps of 0.01 h_prev halved on every jump
+                            across the band, the last radius after 501 passes, counts
+                            stopping at nup + 1 (LIBV7:255-328) */
     int32_t knn_kth;     /* rank for h_mode 1 (64, self included at rank 0) */
     double av_w_eps;     /* Gadget AV: w = v_ij.r_ij / (r + av_w_eps); 1e-5 in LIBSX:158-169
                             and LIBV2, 1e-15 in augsphx6.5.py PI_ij */

@@ -44,7 +44,7 @@ def lib():
         L.orc_smoothing.argtypes = [i64, dp, dp, ip, dp, C.c_double, i32, i32, i32, i32, lp, i64,
                                     dp, ip, ip, ip]
         L.orc_smoothing_steps.argtypes = [i64, dp, dp, ip, dp, C.c_double, i32, i32, i32, i32,
-                                          C.c_double, lp, i64, dp, ip, ip, ip]
+                                          i32, lp, i64, dp, ip, ip, ip]
         L.orc_neighbors.argtypes = [i64, dp, dp, lp, i64, lp, lp]
         L.orc_density.argtypes = [i64, dp, dp, dp, lp, i64, dp, ip]
         L.orc_eos.argtypes = [i64, dp, dp, C.c_double, dp, dp]
@@ -121,7 +121,8 @@ class Params:
     h_predictor: bool = True  # Gadget-2 eq. 31 start for the h iteration
     h_mode: int = 0           # 0 count-band iteration, 1 kNN (do_smoothingX), 2 the
                               # reference's fixed-step rule (LIBSX3:947-1006), 3 the
-                              # same with ten times the step (do_smoothingX4)
+                              # same with ten times the step (do_smoothingX4), 4 the
+                              # step-halving rule of the LIBV7 driver
     knn_kth: int = 64
     av_w_eps: float = 1e-5    # Gadget AV: w = v.r / (r + av_w_eps)
 
@@ -285,7 +286,8 @@ def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(),
     """Smoothing-length iteration (LIBSX3:946-1010, LIBV7:255-330): the
     count-band iteration, or with p.h_mode == 2 (3) the reference's own rule
     of fixed steps of 0.002 (0.02) h_prev (LIBSX3:947-1006, LIBV2:7-64;
-    do_smoothingX4).
+    do_smoothingX4), or (4) the LIBV7 driver's steps of 0.01 h_prev halved on
+    each jump across the band (LIBV7:255-328).
 
     Returns (h, k, iterations, status) per row; status 1 = not converged (the
     step rule: the pass limit reached)."""
@@ -304,10 +306,9 @@ def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(),
     k = np.zeros(nr, np.int32)
     it = np.zeros(nr, np.int32)
     st = np.zeros(nr, np.int32)
-    if p.h_mode in (2, 3):
+    if p.h_mode in (2, 3, 4):
         lib().orc_smoothing_steps(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter,
-                                  0.02 if p.h_mode == 3 else 0.002, rp, nr,
-                                  _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
+                                  p.h_mode, rp, nr, _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
     else:
         lib().orc_smoothing(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter, rp, nr,
                             _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])

@@ -204,6 +204,14 @@ void orc_smoothing(int64_t n, const double *pos, const double *h_prev, const int
  * count.  The loop ends at the first pass whose count lies in [ndown, nup].
  * k_out is the last pass's count (the reference's Nngbx); status 1 = the
  * pass limit was reached.
+ *
+ * h_mode 4 is the GPU driver variant of LIBV7:255-328 (hfvGpuSphTyp_v7.cu:28,
+ * coeff 0.005): the eq. 31 start summed as ((1/3 h) divV) dt; steps of
+ * 0.01 h_prev, halved whenever the count jumps across the band from one pass
+ * to the next (k < ndown after k > nup or the reverse); no largest-radius
+ * rule: after pass 501 (N_iter > 500) s is the last updated radius; its
+ * count stops at nup + 1 once over-full (the reference breaks the j loop
+ * there), which is what k_out reports then.
  */
 static int64_t count_within_sqrt(const double *pos, int64_t n, int64_t i, double s) {
     int64_t k = 0;
@@ -214,33 +222,45 @@ static int64_t count_within_sqrt(const double *pos, int64_t n, int64_t i, double
 
 static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_prev,
                              int32_t k_prev, int use_pred, double divv, double dt, int32_t nngb,
-                             int32_t ndown, int32_t nup, int32_t max_iter, double hstep,
+                             int32_t ndown, int32_t nup, int32_t max_iter, int32_t h_mode,
                              double *h_out, int32_t *k_out, int32_t *it_out, int32_t *st_out) {
+    const int halving = h_mode == 4;
     double s = 2.0 * h_prev;
     if (use_pred) {
         double f = orc_h_factor(nngb, k_prev);
         double t1 = 0.5 * h_prev;
         double t2 = 1.0 + f;
         double t3 = t1 * t2;
-        double t4 = h_prev * divv;
-        double t5 = (1.0 / 3.0) * t4;
-        double t6 = t5 * dt;
+        double t6;
+        if (halving) {
+            double t4 = (1.0 / 3.0) * h_prev;
+            double t5 = t4 * divv;
+            t6 = t5 * dt;
+        } else {
+            double t4 = h_prev * divv;
+            double t5 = (1.0 / 3.0) * t4;
+            t6 = t5 * dt;
+        }
         double sp = 2.0 * (t3 + t6);
         if (sp > 0.0 && sp < INFINITY) s = sp;
     }
-    const double step = hstep * h_prev;
+    double coef = h_mode == 3 ? 0.02 : (halving ? 0.01 : 0.002);  /* coeff * 2 */
+    const int32_t limit = halving ? 500 : max_iter;
     double smax = s;
-    int64_t k = 0;
+    int64_t k = 0, kpre = nup;
     int32_t it = 0, st = 0;
     for (;;) {
         k = count_within_sqrt(pos, n, i, s);
+        if (halving && k > nup) k = nup + 1;  /* the reference stops counting there */
         const int in_band = k >= ndown && k <= nup;
-        if (k < ndown) s = s + step;
-        if (k > nup) s = s - step;
+        if (halving && ((k < ndown && kpre > nup) || (k > nup && kpre < ndown))) coef = 0.5 * coef;
+        if (k < ndown) s = s + coef * h_prev;
+        if (k > nup) s = s - coef * h_prev;
         if (s > smax) smax = s;
+        kpre = k;
         ++it;
-        if (it > max_iter) {
-            s = smax;
+        if (it > limit) {
+            if (!halving) s = smax;
             st = 1;
             break;
         }
@@ -254,14 +274,14 @@ static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_p
 
 void orc_smoothing_steps(int64_t n, const double *pos, const double *h_prev, const int32_t *k_prev,
                          const double *divv, double dt, int32_t nngb, int32_t ndown, int32_t nup,
-                         int32_t max_iter, double hstep, const int64_t *rows, int64_t nrows,
+                         int32_t max_iter, int32_t h_mode, const int64_t *rows, int64_t nrows,
                          double *h_out, int32_t *k_out, int32_t *it_out, int32_t *st_out) {
     int use_pred = (k_prev != NULL && divv != NULL);
 #pragma omp parallel for schedule(dynamic, 8)
     for (int64_t r = 0; r < nrows; ++r) {
         int64_t i = row_of(rows, r);
         smooth_steps_one(pos, n, i, h_prev[i], use_pred ? k_prev[i] : 0, use_pred,
-                         use_pred ? divv[i] : 0.0, dt, nngb, ndown, nup, max_iter, hstep,
+                         use_pred ? divv[i] : 0.0, dt, nngb, ndown, nup, max_iter, h_mode,
                          &h_out[r], &k_out[r], &it_out[r], &st_out[r]);
     }
 }

@@ -218,6 +218,9 @@ struct SmoothArgs {
 // a.hstep h_prev (0.002 or 0.02) from 2 h_prev (or the eq. 31
 // start, in the reference's order of operations), counts by sqrt(r^2) <= s as
 // the reference tests, the largest radius taken after max_iter + 1 passes.
+// RULE 4: the LIBV7 driver's rule (LIBV7:255-328): steps of 0.01 h_prev,
+// halved on every jump across the band, the last radius after 501 passes,
+// counts that stop at nup + 1.
 template <int RULE>
 __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a) {
     __shared__ double sx[64], sy[64], sz[64];
@@ -243,9 +246,20 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
             double t1 = 0.5 * hprev;
             double t2 = 1.0 + f;
             double t3 = t1 * t2;
-            double t4 = hprev * s.divv[i];
-            double t5 = RULE == 2 ? (1.0 / 3.0) * t4 : t4 * dt;
-            double t6 = RULE == 2 ? t5 * dt : t5 / 3.0;
+            double t6;
+            if (RULE == 4) {         // LIBV7:265-266: ((1/3 h) divV) dt
+                double u = (1.0 / 3.0) * hprev;
+                u = u * s.divv[i];
+                t6 = u * dt;
+            } else if (RULE == 2) {  // LIBSX3:961: (1/3 (h divV)) dt
+                double u = hprev * s.divv[i];
+                u = (1.0 / 3.0) * u;
+                t6 = u * dt;
+            } else {
+                double u = hprev * s.divv[i];
+                u = u * dt;
+                t6 = u / 3.0;
+            }
             double sp = 2.0 * (t3 + t6);
             if (sp > 0.0 && sp < INFINITY) sr = sp;
         }
@@ -254,8 +268,9 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     int klo = 0, khi = 0, kfin = 0;
     bool have_lo = false, have_hi = false, done = !ok, conv = false;
     int it = 0;
-    const double step = a.hstep * hprev;  // RULE 2: (coeff * 2) * h_prev
-    double smax = sr;                    // RULE 2: the largest radius tried
+    double coef = a.hstep;  // RULES 2, 4: the step over h_prev (RULE 4 halves it)
+    double smax = sr;       // RULE 2: the largest radius tried
+    int kpre = a.nup;       // RULE 4: the previous pass's count
     while (true) {
         bool active = ok && !done;
         if (!__any(active)) break;
@@ -316,26 +331,31 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             double r2 = sphx_r2(xi - cx[u], yi - cy[u], zi - cz[u]);
-                            kc += (mine && (RULE == 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
+                            kc += (mine && (RULE >= 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
                         }
                     }
                     for (; q < k.n; ++q) {
                         double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
-                        kc += (mine && (RULE == 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
+                        kc += (mine && (RULE >= 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
                     }
                 });
         }
         if (active) {
 #pragma clang fp contract(off)
             ++it;
-            if (RULE == 2) {
+            if (RULE >= 2) {
+                if (RULE == 4 && kc > a.nup) kc = a.nup + 1;  // LIBV7 stops counting there
                 const bool in_band = kc >= a.ndown && kc <= a.nup;
-                if (kc < a.ndown) sr = sr + step;
-                if (kc > a.nup) sr = sr - step;
+                if (RULE == 4 && ((kc < a.ndown && kpre > a.nup) || (kc > a.nup && kpre < a.ndown)))
+                    coef = 0.5 * coef;  // a jump across the band
+                if (kc < a.ndown) sr = sr + coef * hprev;
+                if (kc > a.nup) sr = sr - coef * hprev;
                 if (sr > smax) smax = sr;
+                kpre = kc;
                 kfin = kc;
-                if (it > a.max_iter) {
-                    sr = smax;  // the pass limit: the largest radius, whatever the count
+                if (it > (RULE == 4 ? SPHX_H4_PASSES : a.max_iter)) {
+                    // the pass limit: RULE 2 takes the largest radius, RULE 4 the last
+                    if (RULE == 2) sr = smax;
                     done = true;
                 } else if (in_band) {
                     done = true;
@@ -1297,7 +1317,8 @@ int Engine::smoothing(hipStream_t st, bool predictor) {
     const int halves = nb_launch() <= SMOOTH_HALF_BUCKETS ? 2 : 1;
     SmoothArgs a{p.nngb, p.ndown, p.nup, p.max_iter, (predictor && p.h_predictor) ? 1 : 0,
                  d_run, h_iters, counters, halves, sphx_h_step(p.h_mode)};
-    if (p.h_mode >= 2) k_smoothing<2><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
+    if (p.h_mode == 4) k_smoothing<4><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
+    else if (p.h_mode >= 2) k_smoothing<2><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     else k_smoothing<0><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;

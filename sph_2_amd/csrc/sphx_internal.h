@@ -81,10 +81,15 @@ __device__ __forceinline__ double sphx_sq(double t) {
     return t * t;
 }
 
-// Step of the reference's smoothing-length rule as a fraction of h_prev:
-// (coeff * 2) with coeff = 0.001 (h_mode 2: LIBSX3:952, DRV2:286) or 0.01
-// (h_mode 3: do_smoothingX4); 2 x fl(coeff) is fl(2 coeff) exactly.
-inline double sphx_h_step(int h_mode) { return h_mode == 3 ? 0.02 : 0.002; }
+// Step of the reference's smoothing-length rules as a fraction of h_prev:
+// (coeff * 2) with coeff = 0.001 (h_mode 2: LIBSX3:952, DRV2:286), 0.01
+// (h_mode 3: do_smoothingX4) or, at the start, 0.005 (h_mode 4: the LIBV7
+// driver, hfvGpuSphTyp_v7.cu:28); 2 x fl(coeff) is fl(2 coeff) exactly.
+inline double sphx_h_step(int h_mode) {
+    return h_mode == 3 ? 0.02 : (h_mode == 4 ? 0.01 : 0.002);
+}
+// Pass limit of the h_mode 4 rule (N_iter > 500, LIBV7:321).
+constexpr int SPHX_H4_PASSES = 500;
 
 // cbrt(nngb/k) clamped to [1/2, 2] by a fixed-count Newton iteration with
 // separately rounded operations; identical on any IEEE machine.

@@ -15,8 +15,9 @@ constexpr int PT = 256;
 inline unsigned grid_rows(int64_t r0, int64_t r1) { return (unsigned)((r1 - r0 + PT - 1) / PT); }
 
 // smoothing_h (myCppSPHLibs_v2.h:7-64) with the canonical update rule, or
-// (RULE 2) its own rule of fixed steps of hstep h_prev (also LIBSX3:947-1006;
-// the same decisions as k_smoothing<2>).
+// (RULE 2) its own rule of fixed steps of hstep h_prev (also LIBSX3:947-1006),
+// or (RULE 4) the LIBV7 driver's steps halved on each jump across the band
+// (LIBV7:255-328); the same decisions as k_smoothing<RULE>.
 template <int RULE>
 __global__ __launch_bounds__(PT) void kp_smoothing(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
@@ -37,9 +38,20 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
         double t1 = 0.5 * hp;
         double t2 = 1.0 + f;
         double t3 = t1 * t2;
-        double t4 = hp * divv[i];
-        double t5 = RULE == 2 ? (1.0 / 3.0) * t4 : t4 * dt;
-        double t6 = RULE == 2 ? t5 * dt : t5 / 3.0;
+        double t6;
+        if (RULE == 4) {         // LIBV7:265-266: ((1/3 h) divV) dt
+            double u = (1.0 / 3.0) * hp;
+            u = u * divv[i];
+            t6 = u * dt;
+        } else if (RULE == 2) {  // LIBSX3:961: (1/3 (h divV)) dt
+            double u = hp * divv[i];
+            u = (1.0 / 3.0) * u;
+            t6 = u * dt;
+        } else {
+            double u = hp * divv[i];
+            u = u * dt;
+            t6 = u / 3.0;
+        }
         double sp = 2.0 * (t3 + t6);
         if (sp > 0.0 && sp < INFINITY) s = sp;
     }
@@ -47,22 +59,27 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
     int64_t klo = 0, khi = 0, k = 0;
     bool have_lo = false, have_hi = false, conv = false;
     int it = 0;
-    if (RULE == 2) {
-        const double step = hstep * hp;  // (coeff * 2) * h_prev
+    if (RULE >= 2) {
+        double coef = hstep;  // (coeff * 2); RULE 4 halves it on each jump across the band
         double smax = s;
+        int64_t kpre = nup;
         for (;;) {
             k = 0;
             for (int64_t j = 0; j < n; ++j) {
                 double r2 = sphx_r2(xi - x[j], yi - y[j], zi - z[j]);
                 if (sqrt(r2) <= s) ++k;
             }
+            if (RULE == 4 && k > nup) k = nup + 1;  // LIBV7 stops counting there
             const bool in_band = k >= ndown && k <= nup;
-            if (k < ndown) s = s + step;
-            if (k > nup) s = s - step;
+            if (RULE == 4 && ((k < ndown && kpre > nup) || (k > nup && kpre < ndown)))
+                coef = 0.5 * coef;
+            if (k < ndown) s = s + coef * hp;
+            if (k > nup) s = s - coef * hp;
             if (s > smax) smax = s;
+            kpre = k;
             ++it;
-            if (it > max_iter) {
-                s = smax;
+            if (it > (RULE == 4 ? SPHX_H4_PASSES : max_iter)) {
+                if (RULE == 2) s = smax;
                 break;
             }
             if (in_band) {
@@ -355,7 +372,12 @@ int plain_smoothing(const double *x, const double *y, const double *z, const dou
                     hipStream_t st) {
     if (r1 <= r0) return SPHX_OK;
     const double hstep = sphx_h_step(p.h_mode);
-    if (p.h_mode >= 2)
+    if (p.h_mode == 4)
+        kp_smoothing<4><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
+                                                           p.nngb, p.ndown, p.nup, p.max_iter,
+                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           counters);
+    else if (p.h_mode >= 2)
         kp_smoothing<2><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
                                                            p.nngb, p.ndown, p.nup, p.max_iter,
                                                            hstep, n, r0, r1, h_out, k_out, it_out,

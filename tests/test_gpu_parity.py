@@ -53,11 +53,13 @@ def test_smoothing_bit_exact(gpu, evr, mode):
 
 
 @pytest.mark.parametrize("mode", ["mixed", "fp64"])
-@pytest.mark.parametrize("h_mode", [2, 0])
+@pytest.mark.parametrize("h_mode", [2, 0, 4])
 def test_smoothing_predictor_rows(gpu, evr, mode, h_mode):
     """smoothing_fast_new_mpi3 on rows [100, 900) from the eq. 31 start: by
     default the reference's own rule (steps of 0.002 h), with h_mode=0 the
-    count-band iteration; h and counts bit-exact against the oracle's rule."""
+    count-band iteration, with h_mode=4 the typed driver's rule (steps of
+    0.01 h halved on every jump across the band, LIBV7:255-328); h and counts
+    bit-exact against the oracle's rule."""
     from sph_2_amd import libsx
 
     kprev = np.clip(evr["k"] + np.arange(N) % 7 - 3, 1, None)
@@ -91,19 +93,36 @@ def test_reference_h_rule_drop_ins(gpu, evr, mode):
     assert st.sum() < N  # most rows of do_smoothingX4 land in the band
 
 
-def test_plan_reference_h_rule(gpu):
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_halving_h_rule_from_the_guess(gpu, evr, mode):
+    """The typed driver's rule (LIBV7:255-328: steps of 0.01 h from 2h, halved
+    on every jump across the band, 501 counts at most, counting stopped past
+    69) from the IC's guess: h, counts and pass counts bit-exact against the
+    oracle on every row."""
+    from sph_2_amd import libsx
+
+    ho, ko, ito, st = oracle.smoothing(evr["pos"], evr["h0"], p=oracle.Params(h_mode=4))
+    h, k, it = libsx.smoothing_h(evr["pos"], evr["h0"], mode=mode, h_mode=4)
+    assert np.array_equal(h, ho)
+    assert np.array_equal(k, ko) and np.array_equal(it, ito)
+    assert k.max() <= 70
+
+
+@pytest.mark.parametrize("h_mode,tol", [(2, 2.1e-3), (4, 1.1e-2)])
+def test_plan_reference_h_rule(gpu, h_mode, tol):
     """A plan with h_mode=2 (the reference driver's smoothing_h rule,
-    LIBV2:7-64, DRV2:419): after the bootstrap h and counts bit-exact against
-    the oracle's composition; after a step, whose eq. 31 start carries the
-    previous step's div v, within one step of 0.002 h."""
+    LIBV2:7-64, DRV2:419) or h_mode=4 (the typed driver's, LIBV7:255-328):
+    after the bootstrap h and counts bit-exact against the oracle's
+    composition; after a step, whose eq. 31 start carries the previous step's
+    div v, within two steps of the rule (0.001 h or 0.005 h)."""
     from sph_2_amd.plan import SphPlan
 
     e = ic.evrard(N, seed=0)
-    p = oracle.Params(h_mode=2)
+    p = oracle.Params(h_mode=h_mode)
     s0 = oracle.bootstrap(oracle.State(e.pos, e.vel, e.u, e.m, e.h), p)
     s1 = oracle.step(s0, 1e-4, p)
     for mode in ("mixed", "fp64"):
-        pl = SphPlan(N, mode=mode, h_mode=2)
+        pl = SphPlan(N, mode=mode, h_mode=h_mode)
         pl.load(e.pos, e.vel, e.u, e.m, e.h)
         pl.init()
         st0 = pl.state()
@@ -111,7 +130,7 @@ def test_plan_reference_h_rule(gpu):
         pl.step(1, 1e-4)
         st1 = pl.state()
         assert np.mean(st1["h"] != s1.h) < 0.01
-        assert rel_max(st1["h"], s1.h) < 2.1e-3
+        assert rel_max(st1["h"], s1.h) < tol
         pl.close()
 
 

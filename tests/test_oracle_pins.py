@@ -333,12 +333,13 @@ def _chain(n=201):
     (2, 14.0, 14.504, 59, 37),   # up: s = 28 + 36 x 0.028 = 29.008, 57 -> 59 neighbours
     (3, 14.0, 14.56, 59, 5),     # ten times the step: s = 28 + 4 x 0.28 = 29.12
     (2, 18.0, 17.496, 69, 29),   # down: s = 36 - 28 x 0.036 = 34.992, 73 -> 69
+    (4, 14.0, 14.56, 59, 9),     # the LIBV7 driver's 0.01 h steps: s = 28 + 8 x 0.14
 ])
 def test_smoothing_step_rule_on_a_chain(h_mode, h0, h_want, k_want, passes):
-    """The reference's own rule (LIBSX3:947-1006; do_smoothingX4 with coeff
-    0.01) on a chain, where the count at each radius is known in closed form:
-    the first radius on the grid 2 h0 +- m (2 coeff h0) whose count lies in
-    59..69, reached in m + 1 counts."""
+    """The reference's own rules (LIBSX3:947-1006; do_smoothingX4 with coeff
+    0.01; the LIBV7 driver's coeff 0.005) on a chain, where the count at each
+    radius is known in closed form: the first radius on the grid
+    2 h0 +- m (2 coeff h0) whose count lies in 59..69, reached in m + 1 counts."""
     pos = _chain()
     h, k, it, st = oracle.smoothing(pos, np.full(len(pos), h0), p=oracle.Params(h_mode=h_mode),
                                     rows=[100])
@@ -346,15 +347,18 @@ def test_smoothing_step_rule_on_a_chain(h_mode, h0, h_want, k_want, passes):
     assert (k[0], it[0], st[0]) == (k_want, passes, 0)
 
 
-def test_smoothing_step_rule_eq31_start():
-    """The Gadget-2 eq. 31 start of smoothing_fast_new_mpi3 (LIBSX3:961),
-    2 (0.5 h (1 + (64/k_prev)^(1/3)) + (1/3) (h divV) dt), in exact binary
-    values: h = 16, k_prev = 64, divV = 0.1875, dt = 0.5 give s = 33, whose 67
-    chain neighbours already lie in the band: one count, h = 16.5."""
+@pytest.mark.parametrize("h_mode", [2, 4])
+def test_smoothing_step_rule_eq31_start(h_mode):
+    """The Gadget-2 eq. 31 start of smoothing_fast_new_mpi3 (LIBSX3:961) and
+    of the LIBV7 driver (LIBV7:265-266), 2 (0.5 h (1 + (64/k_prev)^(1/3)) +
+    h divV dt / 3), in exact binary values: h = 16, k_prev = 64, divV =
+    0.1875, dt = 0.5 give s = 33, whose 67 chain neighbours already lie in
+    the band: one count, h = 16.5."""
     pos = _chain()
     n = len(pos)
     h, k, it, st = oracle.smoothing(pos, np.full(n, 16.0), np.full(n, 64, np.int32),
-                                    np.full(n, 0.1875), 0.5, p=oracle.Params(h_mode=2), rows=[100])
+                                    np.full(n, 0.1875), 0.5, p=oracle.Params(h_mode=h_mode),
+                                    rows=[100])
     assert (h[0], k[0], it[0], st[0]) == (16.5, 67, 1, 0)
 
 
@@ -371,6 +375,24 @@ def test_smoothing_step_rule_pass_limit():
         s = s + 0.002 * 10.0
     assert abs(h[0] - 0.5 * (s + 0.02)) < 1e-12 and abs(h[0] - 11.01) < 1e-9
     assert (k[0], it[0], st[0]) == (1 + 2 * math.floor(s), 101, 1)
+
+
+@pytest.mark.parametrize("h0", [1.2, 1.3])
+def test_smoothing_halving_rule_on_a_lattice_shell(h0):
+    """The LIBV7 driver's rule (LIBV7:255-328) where the count jumps across
+    the band: on a cubic lattice a particle has 57 neighbours within
+    [sqrt 5, sqrt 6) and 81 at sqrt 6, so the radius crosses sqrt 6 back and
+    forth; halving the step on every crossing closes in on sqrt 6, and after
+    501 counts h is sqrt(6)/2 to round-off (a fixed step would keep it a step
+    away).  The last count is 57, or 70 where the reference stops counting."""
+    g = np.arange(11.0)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    c = int(np.argmin(((pos - 5.0) ** 2).sum(1)))
+    h, k, it, st = oracle.smoothing(pos, np.full(len(pos), h0), p=oracle.Params(h_mode=4),
+                                    rows=[c])
+    assert abs(h[0] - math.sqrt(6.0) / 2) < 1e-12
+    assert k[0] in (57, 70) and (it[0], st[0]) == (501, 1)
 
 
 def test_smoothing_step_rule_walks_the_grid():
