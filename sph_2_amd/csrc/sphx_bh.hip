@@ -23,17 +23,24 @@
 namespace {
 
 constexpr int BH_LEAF = 8;        // particles per leaf at most
-// Cell stack capacity per group.  A deep walk (small opening angle) opens
-// many cells per step, and the larger stack keeps its batches of popped cells
-// large; at larger angles the smaller stack's occupancy wins (4M-particle
-// Plummer sphere: theta 0.3 walks in 62 ms with 2048 entries against 100 ms
-// with 1024, theta 0.5 in 18.9 ms with 1024 against 21.2 ms).
+// Cell stack and interaction list capacities per group (one wave).  A deep
+// walk (small opening angle) opens many cells per step, and the larger stack
+// keeps its batches of popped cells large; at larger angles the smaller
+// stack's occupancy wins.  The walk is bound by how many groups a CU holds
+// (LDS) and a SIMD holds (registers): a list of 256 entries (128 beside the
+// deep stack) and four entries evaluated together give four waves per SIMD,
+// 14-21 % faster than a list of 512 and batches of eight (4M-particle
+// Plummer sphere, theta 0.3 / 0.5 / 0.7: 48 / 15.4 / 8.5 ms against
+// 60 / 18.1 / 10.1 ms; theta 0.3 took 100 ms with the smaller stack).
 constexpr int BH_SCAP = 1024;
 constexpr int BH_SCAP_DEEP = 2048;
 constexpr double BH_DEEP_THETA = 0.35;  // the larger stack below this angle
 constexpr int BH_SLACK = 160;     // > 7 x 21 levels of pure depth-first growth
-constexpr int BH_ICAP = 512;      // interaction list capacity per group (>= 64 x BH_LEAF)
-static_assert(BH_ICAP >= 64 * BH_LEAF, "one round of leaf particles must fit the list");
+constexpr int BH_ICAP = 256;
+constexpr int BH_ICAP_DEEP = 128;
+constexpr int BH_BATCH = 4;       // list entries evaluated together
+static_assert(BH_ICAP_DEEP >= 64 && BH_ICAP_DEEP % 64 == 0 && BH_ICAP % 64 == 0,
+              "a batch of accepted cells (at most 64) must fit the list");
 static_assert(BH_SCAP > BH_SLACK + 7 * 64, "a full batch must fit above the slack");
 static_assert(BH_SCAP_DEEP > BH_SCAP, "the deep-walk stack is the larger one");
 constexpr int BHT = 256;
@@ -389,11 +396,13 @@ __device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, 
     bh_eval<POT>(dx, dy, dz, r2, ei + ej, j.w, ax, ay, az, ph);
 }
 
-template <bool POT, int SCAP>
+template <bool POT, int SCAP, int ICAP>
 __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
     __shared__ int32_t stk[SCAP];
-    __shared__ float4 il[BH_ICAP];
-    __shared__ float ie[BH_ICAP];
+    // leaf particles a lane lists per round: one round fills the list at most
+    constexpr int ROUND = ICAP / 64;
+    __shared__ float4 il[ICAP];
+    __shared__ float ie[ICAP];
     const int lane = threadIdx.x;
     const int gn = a.gcnt[blockIdx.x];
     const int64_t i = (int64_t)a.gbeg[blockIdx.x] + lane;
@@ -417,18 +426,18 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
 
     auto flush = [&]() {
         __syncthreads();
-        // entries eight at a time, their LDS loads issued before the pairs are
+        // entries BH_BATCH at a time, their LDS loads issued before the pairs are
         // evaluated (in list order, as one at a time would)
         int q = 0;
-        for (; q + 8 <= icnt; q += 8) {
-            float4 j[8];
-            float e[8];
+        for (; q + BH_BATCH <= icnt; q += BH_BATCH) {
+            float4 j[BH_BATCH];
+            float e[BH_BATCH];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) { j[k] = il[q + k]; e[k] = ie[q + k]; }
-            float dx[8], dy[8], dz[8], r2[8], t[8];
+            for (int k = 0; k < BH_BATCH; ++k) { j[k] = il[q + k]; e[k] = ie[q + k]; }
+            float dx[BH_BATCH], dy[BH_BATCH], dz[BH_BATCH], r2[BH_BATCH], t[BH_BATCH];
             bool far = true;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < BH_BATCH; ++k) {
                 dx[k] = j[k].x - xi;
                 dy[k] = j[k].y - yi;
                 dz[k] = j[k].z - zi;
@@ -439,10 +448,10 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
             // lanes past the group's end take part in the vote as far
             if (__all(far || !ok)) {
 #pragma unroll
-                for (int k = 0; k < 8; ++k) bh_far<POT>(dx[k], dy[k], dz[k], r2[k], j[k].w, ax, ay, az, ph);
+                for (int k = 0; k < BH_BATCH; ++k) bh_far<POT>(dx[k], dy[k], dz[k], r2[k], j[k].w, ax, ay, az, ph);
             } else {
 #pragma unroll
-                for (int k = 0; k < 8; ++k)
+                for (int k = 0; k < BH_BATCH; ++k)
                     bh_eval<POT>(dx[k], dy[k], dz[k], r2[k], t[k], j[k].w, ax, ay, az, ph);
             }
         }
@@ -476,8 +485,10 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
             double dx = fmax(0.0, fmax(lx - mo.cx, mo.cx - hx));
             double dy = fmax(0.0, fmax(ly - mo.cy, mo.cy - hy));
             double dz = fmax(0.0, fmax(lz - mo.cz, mo.cz - hz));
-            double d = sqrt(dx * dx + dy * dy + dz * dz);
-            if (d * a.theta > (double)lk.size + a.theta * (double)lk.delta) {
+            // d theta > s + theta delta, both sides >= 0, compared squared
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            const double rhs = (double)lk.size + a.theta * (double)lk.delta;
+            if (d2 * (a.theta * a.theta) > rhs * rhs) {
                 act = 1;
             } else if (lk.child < 0) {
                 act = 2;
@@ -492,27 +503,33 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         // accepted cells: at most one entry per lane
         int tot;
         int pos = wave_excl_scan(act == 1 ? 1 : 0, lane, &tot);
-        if (icnt + tot > BH_ICAP) flush();
+        if (icnt + tot > ICAP) flush();
         if (act == 1) {
             il[icnt + pos] = make_float4((float)(mo.cx - ox), (float)(mo.cy - oy),
                                          (float)(mo.cz - oz), mo.mass);
             ie[icnt + pos] = mo.emax;
         }
         icnt += tot;
-        // particles of opened leaves: rounds of at most BH_LEAF per lane, so
-        // one round never exceeds 64 * BH_LEAF <= BH_ICAP entries
+        // particles of opened leaves: rounds of at most ROUND per lane, so
+        // one round never exceeds 64 * ROUND = ICAP entries.  The lane
+        // that opened a leaf only writes its particles' indices into their
+        // list slots; the whole wave then loads the particles, one slot per
+        // lane (same slots, same list order)
         int rem = act == 2 ? nent : 0;
         int pb = act == 2 ? lnk : 0;
         while (__any(rem > 0)) {
-            int tk = rem < BH_LEAF ? rem : BH_LEAF;
+            int tk = rem < ROUND ? rem : ROUND;
             int lpos = wave_excl_scan(tk, lane, &tot);
-            if (icnt + tot > BH_ICAP) flush();
-            for (int q = 0; q < tk; ++q) {
-                int p = pb + q;
-                il[icnt + lpos + q] = make_float4((float)(a.x[p] - ox), (float)(a.y[p] - oy),
-                                                  (float)(a.z[p] - oz), (float)a.m[p]);
-                ie[icnt + lpos + q] = (float)a.eps[p];
+            if (icnt + tot > ICAP) flush();
+            for (int q = 0; q < tk; ++q) ie[icnt + lpos + q] = __int_as_float(pb + q);
+            __syncthreads();
+            for (int s = lane; s < tot; s += 64) {
+                const int p = __float_as_int(ie[icnt + s]);
+                il[icnt + s] = make_float4((float)(a.x[p] - ox), (float)(a.y[p] - oy),
+                                           (float)(a.z[p] - oz), (float)a.m[p]);
+                ie[icnt + s] = (float)a.eps[p];
             }
+            __syncthreads();
             icnt += tot;
             rem -= tk;
             pb += tk;
@@ -703,10 +720,11 @@ int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t s
              E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr, H.stats,
              E.counters + CNT_BH_OVERFLOW};
     const bool deep = theta < BH_DEEP_THETA;
-    if (with_pot && deep) k_bh_walk<true, BH_SCAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
-    else if (with_pot) k_bh_walk<true, BH_SCAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
-    else if (deep) k_bh_walk<false, BH_SCAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
-    else k_bh_walk<false, BH_SCAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    if (with_pot && deep)
+        k_bh_walk<true, BH_SCAP_DEEP, BH_ICAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    else if (with_pot) k_bh_walk<true, BH_SCAP, BH_ICAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    else if (deep) k_bh_walk<false, BH_SCAP_DEEP, BH_ICAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    else k_bh_walk<false, BH_SCAP, BH_ICAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
     SPHX_HIP(hipGetLastError());
     SPHX_HIP(hipEventRecord(ev[2], st));
     if (out) {
