@@ -97,3 +97,20 @@ def test_plan_tree_gravity_rejected_in_fp64_mode(gpu):
     with pytest.raises(Exception):
         pl.set_gravity_bh(0.5)
     pl.close()
+
+
+@pytest.mark.parametrize("theta,tol_pot,tol_acc", [(0.5, 3e-4, 3e-3), (0.3, 1e-4, 1e-3),
+                                                   (0.1, 5e-6, 2e-5), (1e-6, 1e-6, 1e-5)])
+def test_bh_potential(gpu, theta, tol_pot, tol_acc):
+    """The walk's potential and accelerations (the deep walk below theta
+    0.35, the shallow one above) against the FP64 direct sum, which theta ->
+    0 reproduces to FP32 round-off (measured on this input: potential 1e-4,
+    3e-5, 7e-7, 3e-8 and accelerations 8e-4, 2e-4, 3e-6, 1.4e-7 relative RMS
+    at theta 0.5, 0.3, 0.1, 1e-6)."""
+    from sph_2_amd import libsx
+
+    p = ic.plummer(6000, seed=5, eps=0.02)
+    acc, pot = libsx.getAcc_g_bh(p.pos, p.m, 1.0, p.eps, theta, return_potential=True)
+    acc_o, pot_o = oracle.gravity(p.pos, p.m, p.eps, 1.0, potential=True)
+    assert rel_rms(pot, pot_o) < tol_pot
+    assert rel_rms(acc, acc_o) < tol_acc
