@@ -311,14 +311,23 @@ __global__ void k_bh_groups(BhTree T, int32_t ncell, int32_t *__restrict__ gbeg,
     }
 }
 
-__device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total) {
-    int incl = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
+// Exclusive prefix sum over the wave of v, 0 <= v < 2^BITS, called with every
+// lane active: bit b of the lanes' values is one ballot, and the lanes below
+// this one holding it count 2^b each (mbcnt).  Exact integer sums, with no
+// LDS round trip.
+template <int BITS>
+__device__ __forceinline__ int wave_excl_scan(int v, int *total) {
+    int pos = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < BITS; ++b) {
+        const uint64_t m = __ballot((v >> b) & 1);
+        const int below = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+        pos += below << b;
+        tot += __popcll(m) << b;
     }
-    *total = __shfl(incl, 63, 64);
-    return incl - v;
+    *total = tot;
+    return pos;
 }
 
 struct BhArgs {
@@ -401,6 +410,7 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
     __shared__ int32_t stk[SCAP];
     // leaf particles a lane lists per round: one round fills the list at most
     constexpr int ROUND = ICAP / 64;
+    static_assert(ROUND < 8, "leaf particles per lane and round are scanned in three bits");
     __shared__ float4 il[ICAP];
     __shared__ float ie[ICAP];
     const int lane = threadIdx.x;
@@ -502,7 +512,7 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         }
         // accepted cells: at most one entry per lane
         int tot;
-        int pos = wave_excl_scan(act == 1 ? 1 : 0, lane, &tot);
+        int pos = wave_excl_scan<1>(act == 1 ? 1 : 0, &tot);
         if (icnt + tot > ICAP) flush();
         if (act == 1) {
             il[icnt + pos] = make_float4((float)(mo.cx - ox), (float)(mo.cy - oy),
@@ -519,7 +529,7 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         int pb = act == 2 ? lnk : 0;
         while (__any(rem > 0)) {
             int tk = rem < ROUND ? rem : ROUND;
-            int lpos = wave_excl_scan(tk, lane, &tot);
+            int lpos = wave_excl_scan<3>(tk, &tot);
             if (icnt + tot > ICAP) flush();
             for (int q = 0; q < tk; ++q) ie[icnt + lpos + q] = __int_as_float(pb + q);
             __syncthreads();
@@ -537,7 +547,7 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         maxfill = max(maxfill, icnt);
         // children of opened cells onto the stack
         int ftot;
-        int fpos = sp + wave_excl_scan(nfr, lane, &ftot);
+        int fpos = sp + wave_excl_scan<4>(nfr, &ftot);  // 0..8 children
         if (act == 3) {
             int c0 = lnk;
             for (int q = 0; q < nfr; ++q)
