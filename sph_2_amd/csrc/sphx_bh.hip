@@ -27,20 +27,22 @@ constexpr int BH_LEAF = 8;        // particles per leaf at most
 // walk (small opening angle) opens many cells per step, and the larger stack
 // keeps its batches of popped cells large; at larger angles the smaller
 // stack's occupancy wins.  The walk is bound by how many groups a CU holds
-// (LDS) and a SIMD holds (registers): a list of 256 entries (128 beside the
-// deep stack) and four entries evaluated together give four waves per SIMD,
-// 14-21 % faster than a list of 512 and batches of eight (4M-particle
-// Plummer sphere, theta 0.3 / 0.5 / 0.7: 48 / 15.4 / 8.5 ms against
-// 60 / 18.1 / 10.1 ms; theta 0.3 took 100 ms with the smaller stack).
+// (LDS) and a SIMD holds (registers): a list of 128 entries and four entries
+// evaluated together give four to five waves per SIMD, where a list of 512
+// and batches of eight gave two to three (4M-particle Plummer sphere, theta
+// 0.3 / 0.5 / 0.7: 45 / 13.8 / 7.5 ms against 60 / 18.1 / 10.1 ms; theta 0.3
+// took 100 ms with the smaller stack).
 constexpr int BH_SCAP = 1024;
 constexpr int BH_SCAP_DEEP = 2048;
 constexpr double BH_DEEP_THETA = 0.35;  // the larger stack below this angle
 constexpr int BH_SLACK = 160;     // > 7 x 21 levels of pure depth-first growth
-constexpr int BH_ICAP = 256;
-constexpr int BH_ICAP_DEEP = 128;
+constexpr int BH_ICAP = 128;
 constexpr int BH_BATCH = 4;       // list entries evaluated together
-static_assert(BH_ICAP_DEEP >= 64 && BH_ICAP_DEEP % 64 == 0 && BH_ICAP % 64 == 0,
+// leaf particles a lane lists per round: one round fills the list at most
+constexpr int BH_ROUND = BH_ICAP / 64;
+static_assert(BH_ICAP >= 64 && BH_ICAP % 64 == 0,
               "a batch of accepted cells (at most 64) must fit the list");
+static_assert(BH_ROUND < 8, "leaf particles per lane and round are scanned in three bits");
 static_assert(BH_SCAP > BH_SLACK + 7 * 64, "a full batch must fit above the slack");
 static_assert(BH_SCAP_DEEP > BH_SCAP, "the deep-walk stack is the larger one");
 constexpr int BHT = 256;
@@ -405,14 +407,11 @@ __device__ __forceinline__ void bh_pair(float xi, float yi, float zi, float ei, 
     bh_eval<POT>(dx, dy, dz, r2, ei + ej, j.w, ax, ay, az, ph);
 }
 
-template <bool POT, int SCAP, int ICAP>
+template <bool POT, int SCAP>
 __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
     __shared__ int32_t stk[SCAP];
-    // leaf particles a lane lists per round: one round fills the list at most
-    constexpr int ROUND = ICAP / 64;
-    static_assert(ROUND < 8, "leaf particles per lane and round are scanned in three bits");
-    __shared__ float4 il[ICAP];
-    __shared__ float ie[ICAP];
+    __shared__ float4 il[BH_ICAP];
+    __shared__ float ie[BH_ICAP];
     const int lane = threadIdx.x;
     const int gn = a.gcnt[blockIdx.x];
     const int64_t i = (int64_t)a.gbeg[blockIdx.x] + lane;
@@ -513,24 +512,24 @@ __global__ __launch_bounds__(64) void k_bh_walk(BhTree T, BhArgs a) {
         // accepted cells: at most one entry per lane
         int tot;
         int pos = wave_excl_scan<1>(act == 1 ? 1 : 0, &tot);
-        if (icnt + tot > ICAP) flush();
+        if (icnt + tot > BH_ICAP) flush();
         if (act == 1) {
             il[icnt + pos] = make_float4((float)(mo.cx - ox), (float)(mo.cy - oy),
                                          (float)(mo.cz - oz), mo.mass);
             ie[icnt + pos] = mo.emax;
         }
         icnt += tot;
-        // particles of opened leaves: rounds of at most ROUND per lane, so
-        // one round never exceeds 64 * ROUND = ICAP entries.  The lane
+        // particles of opened leaves: rounds of at most BH_ROUND per lane, so
+        // one round never exceeds 64 * BH_ROUND = BH_ICAP entries.  The lane
         // that opened a leaf only writes its particles' indices into their
         // list slots; the whole wave then loads the particles, one slot per
         // lane (same slots, same list order)
         int rem = act == 2 ? nent : 0;
         int pb = act == 2 ? lnk : 0;
         while (__any(rem > 0)) {
-            int tk = rem < ROUND ? rem : ROUND;
+            int tk = rem < BH_ROUND ? rem : BH_ROUND;
             int lpos = wave_excl_scan<3>(tk, &tot);
-            if (icnt + tot > ICAP) flush();
+            if (icnt + tot > BH_ICAP) flush();
             for (int q = 0; q < tk; ++q) ie[icnt + lpos + q] = __int_as_float(pb + q);
             __syncthreads();
             for (int s = lane; s < tot; s += 64) {
@@ -730,11 +729,10 @@ int sphx_bh_run(Engine &E, BhHost &H, double theta, bool with_pot, hipStream_t s
              E.s.gx, E.s.gy, E.s.gz, with_pot ? E.s.pot : nullptr, H.stats,
              E.counters + CNT_BH_OVERFLOW};
     const bool deep = theta < BH_DEEP_THETA;
-    if (with_pot && deep)
-        k_bh_walk<true, BH_SCAP_DEEP, BH_ICAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
-    else if (with_pot) k_bh_walk<true, BH_SCAP, BH_ICAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
-    else if (deep) k_bh_walk<false, BH_SCAP_DEEP, BH_ICAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
-    else k_bh_walk<false, BH_SCAP, BH_ICAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    if (with_pot && deep) k_bh_walk<true, BH_SCAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    else if (with_pot) k_bh_walk<true, BH_SCAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    else if (deep) k_bh_walk<false, BH_SCAP_DEEP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
+    else k_bh_walk<false, BH_SCAP><<<(unsigned)ngroups, 64, 0, st>>>(H.T, a);
     SPHX_HIP(hipGetLastError());
     SPHX_HIP(hipEventRecord(ev[2], st));
     if (out) {
