@@ -72,7 +72,10 @@ typedef struct sphx_params {
                             with steps of 0.02 h_prev (do_smoothingX4); 4: the LIBV7
                             driver's rule, steps of 0.01 h_prev halved on every jump
                             across the band, the last radius after 501 passes, counts
-                            stopping at nup + 1 (LIBV7:255-328) */
+                            stopping at nup + 1 (LIBV7:255-328); 5: h_smooth_fast,
+                            h -+= 0.003 h from h_prev (no predictor) on the strict
+                            count sqrt(r^2) < 2h, the last h after 10000 passes
+                            (LIBSX:1133-1184) */
     int32_t knn_kth;     /* rank for h_mode 1 (64, self included at rank 0) */
     double av_w_eps;     /* Gadget AV: w = v_ij.r_ij / (r + av_w_eps); 1e-5 in LIBSX:158-169
                             and LIBV2, 1e-15 in augsphx6.5.py PI_ij */

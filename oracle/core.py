@@ -122,7 +122,8 @@ class Params:
     h_mode: int = 0           # 0 count-band iteration, 1 kNN (do_smoothingX), 2 the
                               # reference's fixed-step rule (LIBSX3:947-1006), 3 the
                               # same with ten times the step (do_smoothingX4), 4 the
-                              # step-halving rule of the LIBV7 driver
+                              # step-halving rule of the LIBV7 driver, 5 the relative
+                              # steps of h_smooth_fast (LIBSX:1133-1184)
     knn_kth: int = 64
     av_w_eps: float = 1e-5    # Gadget AV: w = v.r / (r + av_w_eps)
 
@@ -287,7 +288,8 @@ def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(),
     count-band iteration, or with p.h_mode == 2 (3) the reference's own rule
     of fixed steps of 0.002 (0.02) h_prev (LIBSX3:947-1006, LIBV2:7-64;
     do_smoothingX4), or (4) the LIBV7 driver's steps of 0.01 h_prev halved on
-    each jump across the band (LIBV7:255-328).
+    each jump across the band (LIBV7:255-328), or (5) h_smooth_fast's steps of
+    0.003 h on the current h with a strict count (LIBSX:1133-1184).
 
     Returns (h, k, iterations, status) per row; status 1 = not converged (the
     step rule: the pass limit reached)."""
@@ -306,7 +308,7 @@ def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(),
     k = np.zeros(nr, np.int32)
     it = np.zeros(nr, np.int32)
     st = np.zeros(nr, np.int32)
-    if p.h_mode in (2, 3, 4):
+    if p.h_mode in (2, 3, 4, 5):
         lib().orc_smoothing_steps(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter,
                                   p.h_mode, rp, nr, _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
     else:

@@ -212,11 +212,22 @@ void orc_smoothing(int64_t n, const double *pos, const double *h_prev, const int
  * rule: after pass 501 (N_iter > 500) s is the last updated radius; its
  * count stops at nup + 1 once over-full (the reference breaks the j loop
  * there), which is what k_out reports then.
+ *
+ * h_mode 5 is h_smooth_fast (LIBSX:1133-1184, the band 50 +- 10 in
+ * augsphx6.5x.py:426-465): from h = h_prev (no predictor), the count is
+ * strict, #{j: sqrt(r^2) < 2h}; k > nup: h -= 0.003 h, k < ndown: h +=
+ * 0.003 h (a step on the current h, two roundings); the loop ends at the
+ * first count in [ndown, nup].  The reference has no pass limit (its counter
+ * is never acted on); here the loop stops after SMOOTH_H5_PASSES counts with
+ * status 1 and the last h.
  */
-static int64_t count_within_sqrt(const double *pos, int64_t n, int64_t i, double s) {
+#define SMOOTH_H5_PASSES 10000
+static int64_t count_within_sqrt(const double *pos, int64_t n, int64_t i, double s, int strict) {
     int64_t k = 0;
-    for (int64_t j = 0; j < n; ++j)
-        if (sqrt(pair_r2(pos, i, j)) <= s) ++k;
+    for (int64_t j = 0; j < n; ++j) {
+        const double d = sqrt(pair_r2(pos, i, j));
+        if (strict ? d < s : d <= s) ++k;
+    }
     return k;
 }
 
@@ -224,9 +235,9 @@ static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_p
                              int32_t k_prev, int use_pred, double divv, double dt, int32_t nngb,
                              int32_t ndown, int32_t nup, int32_t max_iter, int32_t h_mode,
                              double *h_out, int32_t *k_out, int32_t *it_out, int32_t *st_out) {
-    const int halving = h_mode == 4;
+    const int halving = h_mode == 4, relative = h_mode == 5;
     double s = 2.0 * h_prev;
-    if (use_pred) {
+    if (use_pred && !relative) {
         double f = orc_h_factor(nngb, k_prev);
         double t1 = 0.5 * h_prev;
         double t2 = 1.0 + f;
@@ -245,22 +256,28 @@ static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_p
         if (sp > 0.0 && sp < INFINITY) s = sp;
     }
     double coef = h_mode == 3 ? 0.02 : (halving ? 0.01 : 0.002);  /* coeff * 2 */
-    const int32_t limit = halving ? 500 : max_iter;
+    const int32_t limit = halving ? 500 : (relative ? SMOOTH_H5_PASSES : max_iter);
     double smax = s;
     int64_t k = 0, kpre = nup;
     int32_t it = 0, st = 0;
     for (;;) {
-        k = count_within_sqrt(pos, n, i, s);
+        k = count_within_sqrt(pos, n, i, s, relative);
         if (halving && k > nup) k = nup + 1;  /* the reference stops counting there */
         const int in_band = k >= ndown && k <= nup;
         if (halving && ((k < ndown && kpre > nup) || (k > nup && kpre < ndown))) coef = 0.5 * coef;
-        if (k < ndown) s = s + coef * h_prev;
-        if (k > nup) s = s - coef * h_prev;
+        if (relative) {
+            const double hc = 0.5 * s;
+            if (k < ndown) s = 2.0 * (hc + 0.003 * hc);
+            if (k > nup) s = 2.0 * (hc - 0.003 * hc);
+        } else {
+            if (k < ndown) s = s + coef * h_prev;
+            if (k > nup) s = s - coef * h_prev;
+        }
         if (s > smax) smax = s;
         kpre = k;
         ++it;
         if (it > limit) {
-            if (!halving) s = smax;
+            if (!halving && !relative) s = smax;
             st = 1;
             break;
         }

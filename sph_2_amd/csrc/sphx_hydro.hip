@@ -220,7 +220,9 @@ struct SmoothArgs {
 // the reference tests, the largest radius taken after max_iter + 1 passes.
 // RULE 4: the LIBV7 driver's rule (LIBV7:255-328): steps of 0.01 h_prev,
 // halved on every jump across the band, the last radius after 501 passes,
-// counts that stop at nup + 1.
+// counts that stop at nup + 1.  RULE 5: h_smooth_fast (LIBSX:1133-1184):
+// from h_prev with no predictor, h -+= 0.003 h on the current h, the count
+// strict (sqrt(r^2) < 2h), the last h after SPHX_H5_PASSES passes.
 template <int RULE>
 __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a) {
     __shared__ double sx[64], sy[64], sz[64];
@@ -240,7 +242,7 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
     {
 #pragma clang fp contract(off)
         sr = 2.0 * hprev;
-        if (a.predictor && ok) {
+        if (RULE != 5 && a.predictor && ok) {
             double dt = a.rp[r].dt;
             double f = sphx_h_factor(a.nngb, s.k[i]);
             double t1 = 0.5 * hprev;
@@ -331,12 +333,12 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             double r2 = sphx_r2(xi - cx[u], yi - cy[u], zi - cz[u]);
-                            kc += (mine && (RULE >= 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
+                            kc += (mine && (RULE == 5 ? sqrt(r2) < sr : (RULE >= 2 ? sqrt(r2) <= sr : r2 <= s2))) ? 1 : 0;
                         }
                     }
                     for (; q < k.n; ++q) {
                         double r2 = sphx_r2(xi - sx[q], yi - sy[q], zi - sz[q]);
-                        kc += (mine && (RULE >= 2 ? sqrt(r2) <= sr : r2 <= s2)) ? 1 : 0;
+                        kc += (mine && (RULE == 5 ? sqrt(r2) < sr : (RULE >= 2 ? sqrt(r2) <= sr : r2 <= s2))) ? 1 : 0;
                     }
                 });
         }
@@ -348,13 +350,19 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
                 const bool in_band = kc >= a.ndown && kc <= a.nup;
                 if (RULE == 4 && ((kc < a.ndown && kpre > a.nup) || (kc > a.nup && kpre < a.ndown)))
                     coef = 0.5 * coef;  // a jump across the band
-                if (kc < a.ndown) sr = sr + coef * hprev;
-                if (kc > a.nup) sr = sr - coef * hprev;
+                if (RULE == 5) {
+                    const double hc = 0.5 * sr;
+                    if (kc < a.ndown) sr = 2.0 * (hc + 0.003 * hc);
+                    if (kc > a.nup) sr = 2.0 * (hc - 0.003 * hc);
+                } else {
+                    if (kc < a.ndown) sr = sr + coef * hprev;
+                    if (kc > a.nup) sr = sr - coef * hprev;
+                }
                 if (sr > smax) smax = sr;
                 kpre = kc;
                 kfin = kc;
-                if (it > (RULE == 4 ? SPHX_H4_PASSES : a.max_iter)) {
-                    // the pass limit: RULE 2 takes the largest radius, RULE 4 the last
+                if (it > (RULE == 4 ? SPHX_H4_PASSES : (RULE == 5 ? SPHX_H5_PASSES : a.max_iter))) {
+                    // the pass limit: RULE 2 takes the largest radius, RULES 4, 5 the last
                     if (RULE == 2) sr = smax;
                     done = true;
                 } else if (in_band) {
@@ -1317,7 +1325,8 @@ int Engine::smoothing(hipStream_t st, bool predictor) {
     const int halves = nb_launch() <= SMOOTH_HALF_BUCKETS ? 2 : 1;
     SmoothArgs a{p.nngb, p.ndown, p.nup, p.max_iter, (predictor && p.h_predictor) ? 1 : 0,
                  d_run, h_iters, counters, halves, sphx_h_step(p.h_mode)};
-    if (p.h_mode == 4) k_smoothing<4><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
+    if (p.h_mode == 5) k_smoothing<5><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
+    else if (p.h_mode == 4) k_smoothing<4><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     else if (p.h_mode >= 2) k_smoothing<2><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     else k_smoothing<0><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     SPHX_HIP(hipGetLastError());

@@ -90,6 +90,11 @@ inline double sphx_h_step(int h_mode) {
 }
 // Pass limit of the h_mode 4 rule (N_iter > 500, LIBV7:321).
 constexpr int SPHX_H4_PASSES = 500;
+// Pass limit of the h_mode 5 rule (h_smooth_fast, LIBSX:1133-1184), which has
+// none in the reference: a row whose count jumps across the band at a 0.3 %
+// step would loop there for ever.  A row that converges moves h by at most
+// a factor 1.003^10000 (about e^30) first.
+constexpr int SPHX_H5_PASSES = 10000;
 
 // cbrt(nngb/k) clamped to [1/2, 2] by a fixed-count Newton iteration with
 // separately rounded operations; identical on any IEEE machine.

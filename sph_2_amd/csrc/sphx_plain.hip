@@ -17,7 +17,9 @@ inline unsigned grid_rows(int64_t r0, int64_t r1) { return (unsigned)((r1 - r0 +
 // smoothing_h (myCppSPHLibs_v2.h:7-64) with the canonical update rule, or
 // (RULE 2) its own rule of fixed steps of hstep h_prev (also LIBSX3:947-1006),
 // or (RULE 4) the LIBV7 driver's steps halved on each jump across the band
-// (LIBV7:255-328); the same decisions as k_smoothing<RULE>.
+// (LIBV7:255-328), or (RULE 5) h_smooth_fast's 0.3 % steps on the current h
+// with a strict count (LIBSX:1133-1184); the same decisions as
+// k_smoothing<RULE>.
 template <int RULE>
 __global__ __launch_bounds__(PT) void kp_smoothing(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
@@ -33,7 +35,7 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
     double xi = x[i], yi = y[i], zi = z[i];
     double hp = h_prev[i];
     double s = 2.0 * hp;
-    if (k_prev && divv) {
+    if (RULE != 5 && k_prev && divv) {
         double f = sphx_h_factor(nngb, k_prev[i]);
         double t1 = 0.5 * hp;
         double t2 = 1.0 + f;
@@ -67,18 +69,24 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
             k = 0;
             for (int64_t j = 0; j < n; ++j) {
                 double r2 = sphx_r2(xi - x[j], yi - y[j], zi - z[j]);
-                if (sqrt(r2) <= s) ++k;
+                if (RULE == 5 ? sqrt(r2) < s : sqrt(r2) <= s) ++k;
             }
             if (RULE == 4 && k > nup) k = nup + 1;  // LIBV7 stops counting there
             const bool in_band = k >= ndown && k <= nup;
             if (RULE == 4 && ((k < ndown && kpre > nup) || (k > nup && kpre < ndown)))
                 coef = 0.5 * coef;
-            if (k < ndown) s = s + coef * hp;
-            if (k > nup) s = s - coef * hp;
+            if (RULE == 5) {
+                const double hc = 0.5 * s;
+                if (k < ndown) s = 2.0 * (hc + 0.003 * hc);
+                if (k > nup) s = 2.0 * (hc - 0.003 * hc);
+            } else {
+                if (k < ndown) s = s + coef * hp;
+                if (k > nup) s = s - coef * hp;
+            }
             if (s > smax) smax = s;
             kpre = k;
             ++it;
-            if (it > (RULE == 4 ? SPHX_H4_PASSES : max_iter)) {
+            if (it > (RULE == 4 ? SPHX_H4_PASSES : (RULE == 5 ? SPHX_H5_PASSES : max_iter))) {
                 if (RULE == 2) s = smax;
                 break;
             }
@@ -372,7 +380,12 @@ int plain_smoothing(const double *x, const double *y, const double *z, const dou
                     hipStream_t st) {
     if (r1 <= r0) return SPHX_OK;
     const double hstep = sphx_h_step(p.h_mode);
-    if (p.h_mode == 4)
+    if (p.h_mode == 5)
+        kp_smoothing<5><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
+                                                           p.nngb, p.ndown, p.nup, p.max_iter,
+                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           counters);
+    else if (p.h_mode == 4)
         kp_smoothing<4><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
                                                            p.nngb, p.ndown, p.nup, p.max_iter,
                                                            hstep, n, r0, r1, h_out, k_out, it_out,

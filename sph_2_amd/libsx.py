@@ -538,11 +538,19 @@ def do_smoothingX4(pos, h, *, mode="mixed", device=0, **pkw):
     return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)[0]
 
 
-def h_smooth_fast(pos, h, *, mode="mixed", device=0, **pkw):
-    """h_smooth_fast(pos, h) (LIBSX:1131-1184): h with 59..69 neighbours within
-    2h.  The reference multiplies h by 1 -+ 0.003 per count; the band is kept,
-    the update is the canonical bracketed iteration."""
-    return _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)[0]
+def h_smooth_fast(pos, h, *, mode="mixed", device=0, nbeg=None, nend=None, **pkw):
+    """h_smooth_fast(pos, h) (LIBSX:1133-1184) by the reference's rule: from h,
+    h -+= 0.003 h until 59..69 neighbours lie strictly within 2h (nngb=50,
+    ndown=40, nup=60 give augsphx6.5x.py's band).  The reference loops without limit;
+    a row still outside the band after 10000 counts keeps its last h with a
+    RuntimeWarning.  h_mode=0 gives the count-band iteration instead."""
+    pkw.setdefault("h_mode", 5)
+    return _smoothing(pos, h, None, None, 0.0, nbeg, nend, mode, device, **pkw)[0]
+
+
+def h_smooth_fast_mpi(nbeg, nend, pos, h, *, mode="mixed", device=0, **pkw):
+    """h_smooth_fast_mpi(nbeg, nend, pos, h): rows [nbeg, nend) of h_smooth_fast."""
+    return h_smooth_fast(pos, h, mode=mode, device=device, nbeg=nbeg, nend=nend, **pkw)
 
 
 def getAcc_g_smthx(pos, mass, G, epsilon, **kw):

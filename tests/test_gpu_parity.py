@@ -108,13 +108,35 @@ def test_halving_h_rule_from_the_guess(gpu, evr, mode):
     assert k.max() <= 70
 
 
-@pytest.mark.parametrize("h_mode,tol", [(2, 2.1e-3), (4, 1.1e-2)])
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_h_smooth_fast_rule(gpu, evr, mode):
+    """h_smooth_fast (LIBSX:1133-1184: h -+= 0.003 h on a strict count, no
+    pass limit) from the IC's guess, all rows and a row range through
+    h_smooth_fast_mpi, and with augsphx6.5x.py's 50 +- 10 neighbours: h
+    bit-exact against the oracle."""
+    from sph_2_amd import libsx
+
+    ho, ko, ito, st = oracle.smoothing(evr["pos"], evr["h0"], p=oracle.Params(h_mode=5))
+    assert not st.any() and ito.max() > 20
+    assert np.array_equal(libsx.h_smooth_fast(evr["pos"], evr["h0"], mode=mode), ho)
+    h, k, it = libsx.smoothing_h(evr["pos"], evr["h0"], mode=mode, h_mode=5)
+    assert np.array_equal(k, ko) and np.array_equal(it, ito)
+    assert np.array_equal(libsx.h_smooth_fast_mpi(100, 900, evr["pos"], evr["h0"], mode=mode),
+                          ho[100:900])
+    ho, _, _, _ = oracle.smoothing(evr["pos"], evr["h0"],
+                                   p=oracle.Params(h_mode=5, nngb=50, ndown=40, nup=60))
+    assert np.array_equal(libsx.h_smooth_fast(evr["pos"], evr["h0"], mode=mode, nngb=50, ndown=40,
+                                              nup=60), ho)
+
+
+@pytest.mark.parametrize("h_mode,tol", [(2, 2.1e-3), (4, 1.1e-2), (5, 6.1e-3)])
 def test_plan_reference_h_rule(gpu, h_mode, tol):
     """A plan with h_mode=2 (the reference driver's smoothing_h rule,
-    LIBV2:7-64, DRV2:419) or h_mode=4 (the typed driver's, LIBV7:255-328):
-    after the bootstrap h and counts bit-exact against the oracle's
-    composition; after a step, whose eq. 31 start carries the previous step's
-    div v, within two steps of the rule (0.001 h or 0.005 h)."""
+    LIBV2:7-64, DRV2:419), h_mode=4 (the typed driver's, LIBV7:255-328) or
+    h_mode=5 (h_smooth_fast each step, augsphx6.5x.py:590): after the
+    bootstrap h and counts bit-exact against the oracle's composition; after
+    a step, whose start carries the previous step's state, within two steps
+    of the rule (0.001 h, 0.005 h or 0.3 %)."""
     from sph_2_amd.plan import SphPlan
 
     e = ic.evrard(N, seed=0)

@@ -395,6 +395,38 @@ def test_smoothing_halving_rule_on_a_lattice_shell(h0):
     assert k[0] in (57, 70) and (it[0], st[0]) == (501, 1)
 
 
+@pytest.mark.parametrize("h0,steps,k_want", [(14.0, 12, 59), (18.0, -10, 69)])
+def test_smoothing_relative_rule_on_a_chain(h0, steps, k_want):
+    """h_smooth_fast (LIBSX:1133-1184) on a chain, where the strict count is
+    #{j: |i - j| < s} = 2 ceil(s) - 1: from h = 14 (s = 28, 55 neighbours) h
+    grows by 0.3 % per count until 2h > 29 (59 neighbours), twelve steps since
+    ln(14.5 / 14) / ln(1.003) = 11.7; from h = 18 (71 neighbours) it shrinks
+    until 2h <= 35 (69 neighbours), ten steps.  h is h0 (1 +- 0.003)^steps to
+    round-off."""
+    pos = _chain()
+    h, k, it, st = oracle.smoothing(pos, np.full(len(pos), h0), p=oracle.Params(h_mode=5),
+                                    rows=[100])
+    assert abs(h[0] - h0 * (1.0 + math.copysign(0.003, steps)) ** abs(steps)) < 1e-12
+    assert (k[0], it[0], st[0]) == (k_want, abs(steps) + 1, 0)
+
+
+def test_smoothing_relative_rule_counts_strictly():
+    """h_smooth_fast counts dist < 2h: at h = 14.5 the particles at distance
+    29 are out (57 neighbours), so h takes one step to 14.5435, where the
+    inclusive count of the other rules already holds 59 and stops at once.
+    With augsphx6.5x.py's band 40..60 (augsphx6.5x.py:426-465), h = 10 counts
+    39 and takes one step."""
+    pos = _chain()
+    n = len(pos)
+    h, k, it, st = oracle.smoothing(pos, np.full(n, 14.5), p=oracle.Params(h_mode=5), rows=[100])
+    assert abs(h[0] - 14.5435) < 1e-12 and (k[0], it[0]) == (59, 2)
+    h, k, it, st = oracle.smoothing(pos, np.full(n, 14.5), p=oracle.Params(h_mode=2), rows=[100])
+    assert h[0] == 14.5 and (k[0], it[0]) == (59, 1)
+    h, k, it, st = oracle.smoothing(pos, np.full(n, 10.0), p=oracle.Params(h_mode=5, nngb=50, ndown=40, nup=60),
+                                    rows=[100])
+    assert abs(h[0] - 10.03) < 1e-12 and (k[0], it[0]) == (41, 2)
+
+
 def test_smoothing_step_rule_walks_the_grid():
     """On an Evrard sphere from the IC guess: every row that stops inside the
     pass limit ends on the grid 2 h0 + m 0.002 h0 (|m| <= 100) with its count
