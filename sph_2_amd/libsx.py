@@ -481,6 +481,28 @@ def do_smoothingX(poz, *, kth=64, device=0):
         if off is None:
             raise ValueError("do_smoothingX: subpos must be pos or a contiguous block of its rows")
         r0, r1 = off, off + len(sub)
+    return _knn_rows(pos, r0, r1, kth, device, "do_smoothingX")
+
+
+def smoothing_length_mpi(nbeg, nend, pos, *, kth=64, device=0):
+    """smoothing_length_mpi(nbeg, nend, pos) (LIBSX:1070-1092): rows [nbeg,
+    nend) of do_smoothingX((pos, pos)), 0.5 x np.sort(distances)[64], exact."""
+    pos = _xyz(pos)
+    r0, r1 = _rows(len(pos), nbeg, nend)
+    return _knn_rows(pos, r0, r1, kth, device, "smoothing_length_mpi")
+
+
+def smoothing_length_mpix(nbeg, nend, pos, h, *, kth=64, device=0):
+    """smoothing_length_mpix(nbeg, nend, pos, h) (LIBSX:1096-1126): the same
+    value, which the reference takes from the particles within 2.5 h only
+    (equal whenever at least 65 lie there; with fewer its loop never ends or
+    its index fails).  h is not needed for the exact search."""
+    return smoothing_length_mpi(nbeg, nend, pos, kth=kth, device=device)
+
+
+def _knn_rows(pos, r0, r1, kth, device, what):
+    """0.5 x the distance of rank kth (self at rank 0) for rows [r0, r1)."""
+    n = len(pos)
     t = _t()
     if r1 == r0:
         return np.zeros(0)
@@ -489,7 +511,7 @@ def do_smoothingX(poz, *, kth=64, device=0):
     dh = dev(np.full(n, h0))
     out = t.empty(r1 - r0, dtype=t.float64, device="cuda")
     rc = check(_lib.load().sphx_knn_h(_context(device), n, ptr(dp), ptr(dh), int(kth), r0, r1,
-                                      ptr(out), _lib.stream_handle()), "do_smoothingX")
+                                      ptr(out), _lib.stream_handle()), what)
     if rc == _lib.CAPPED:
         warnings.warn("kNN search did not resolve some rows (NaN)", RuntimeWarning)
     return out.cpu().numpy()

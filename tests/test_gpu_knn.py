@@ -31,6 +31,20 @@ def test_row_block_and_other_rank(gpu):
         libsx.do_smoothingX((pos, pos[::2].copy()))
 
 
+def test_smoothing_length_mpi_rows(gpu):
+    """smoothing_length_mpi / _mpix (LIBSX:1070-1126): rows [nbeg, nend) of
+    the exact kNN h, bit-exact against the oracle; an empty range is empty."""
+    from sph_2_amd import libsx
+
+    e = ic.plummer(6000, seed=6)
+    want = oracle.knn_h(e.pos, rows=np.arange(2500, 3100))
+    assert np.array_equal(libsx.smoothing_length_mpi(2500, 3100, e.pos), want)
+    assert np.array_equal(libsx.smoothing_length_mpix(2500, 3100, e.pos, e.h), want)
+    assert libsx.smoothing_length_mpi(7, 7, e.pos).shape == (0,)
+    with pytest.raises(ValueError):
+        libsx.smoothing_length_mpi(10, 6001, e.pos)
+
+
 def test_lattice_ties_and_duplicates(gpu):
     """Shell degeneracy on a lattice, and 300 coincident particles whose 65th
     distance is 0 (the bracket must be bisected down to the tie)."""
