@@ -276,12 +276,14 @@ static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_p
         if (s > smax) smax = s;
         kpre = k;
         ++it;
+        /* a count in the band ends the loop even on the last pass: the
+         * reference tests the band before its pass counter */
+        if (in_band) break;
         if (it > limit) {
             if (!halving && !relative) s = smax;
             st = 1;
             break;
         }
-        if (in_band) break;
     }
     *h_out = 0.5 * s;
     *k_out = (int32_t)k;

@@ -361,13 +361,13 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
                 if (sr > smax) smax = sr;
                 kpre = kc;
                 kfin = kc;
-                if (it > (RULE == 4 ? SPHX_H4_PASSES : (RULE == 5 ? SPHX_H5_PASSES : a.max_iter))) {
+                if (in_band) {  // before the pass limit, as the reference tests
+                    done = true;
+                    conv = true;
+                } else if (it > (RULE == 4 ? SPHX_H4_PASSES : (RULE == 5 ? SPHX_H5_PASSES : a.max_iter))) {
                     // the pass limit: RULE 2 takes the largest radius, RULES 4, 5 the last
                     if (RULE == 2) sr = smax;
                     done = true;
-                } else if (in_band) {
-                    done = true;
-                    conv = true;
                 }
             } else if (kc >= a.ndown && kc <= a.nup) {
                 done = true;

@@ -86,12 +86,12 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
             if (s > smax) smax = s;
             kpre = k;
             ++it;
-            if (it > (RULE == 4 ? SPHX_H4_PASSES : (RULE == 5 ? SPHX_H5_PASSES : max_iter))) {
-                if (RULE == 2) s = smax;
+            if (in_band) {  // before the pass limit, as the reference tests
+                conv = true;
                 break;
             }
-            if (in_band) {
-                conv = true;
+            if (it > (RULE == 4 ? SPHX_H4_PASSES : (RULE == 5 ? SPHX_H5_PASSES : max_iter))) {
+                if (RULE == 2) s = smax;
                 break;
             }
         }

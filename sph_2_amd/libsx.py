@@ -451,6 +451,12 @@ def smoothing_fast_new_mpi3(nbeg, nend, pos, h, Nngb_previous, divV, dt, *, mode
     return h_out, k.astype(np.float64)
 
 
+def smoothing_fast_new_mpi(nbeg, nend, pos, h, *, mode="mixed", device=0, **pkw):
+    """smoothing_fast_new_mpi (LIBSX:1006-1061): the rule of
+    smoothing_fast_new_mpi2 on rows [nbeg, nend)."""
+    return smoothing_fast_new_mpi2(nbeg, nend, pos, h, mode=mode, device=device, **pkw)
+
+
 def smoothing_fast_new_mpi2(nbeg, nend, pos, h, *, mode="mixed", device=0, **pkw):
     """Rows [nbeg, nend) by the reference's own rule (LIBSX:940-990): steps of
     0.002 h from 2h until 59..69 neighbours lie within 2h, at most 101
@@ -550,6 +556,24 @@ def _row_offset(pos, sub):
             if 0 <= off <= len(pos) - len(sub):
                 return int(off)
     return None
+
+
+def do_smoothingX2(pos, h, *, kth=64, device=0):
+    """do_smoothingX2(pos, h) (LIBSX:788-820): 0.5 x np.sort(distances within
+    3h)[64], the exact kNN h whenever at least 65 particles lie within 3h
+    (with fewer the reference's index fails); h is not needed here."""
+    return smoothing_length_mpi(None, None, pos, kth=kth, device=device)
+
+
+def do_smoothingX3(pos, h, *, mode="mixed", device=0, **pkw):
+    """do_smoothingX3(pos, h) (LIBSX:825-877): the rule of do_smoothingX4
+    (steps of 0.02 h from 2h, at most 101 counts); returns (h, N_iter) like the
+    reference, N_iter being the steps taken per row (101 where capped)."""
+    pkw.setdefault("h_mode", 3)
+    p = _lib.default_params(**pkw)
+    h_out, k, it = _smoothing(pos, h, None, None, 0.0, None, None, mode, device, **pkw)
+    inband = (k >= p.ndown) & (k <= p.nup)
+    return h_out, [int(x) for x in np.where(inband, it - 1, it)]
 
 
 def do_smoothingX4(pos, h, *, mode="mixed", device=0, **pkw):

@@ -377,6 +377,18 @@ def test_smoothing_step_rule_pass_limit():
     assert (k[0], it[0], st[0]) == (1 + 2 * math.floor(s), 101, 1)
 
 
+def test_smoothing_step_rule_converges_on_the_last_pass():
+    """A row whose 101st count is the first in the band keeps that radius, as
+    the reference tests the band before its counter: from h0 = 19.43 the chain
+    count falls from 71 to 69 exactly at the 100th step of 0.002 h0 (s = 34.974
+    < 35 <= 35.0129), so h = 17.487, not the largest radius tried (the start)."""
+    pos = _chain()
+    h, k, it, st = oracle.smoothing(pos, np.full(len(pos), 19.43), p=oracle.Params(h_mode=2),
+                                    rows=[100])
+    assert abs(h[0] - 17.487) < 1e-9
+    assert (k[0], it[0], st[0]) == (69, 101, 0)
+
+
 @pytest.mark.parametrize("h0", [1.2, 1.3])
 def test_smoothing_halving_rule_on_a_lattice_shell(h0):
     """The LIBV7 driver's rule (LIBV7:255-328) where the count jumps across
