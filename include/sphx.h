@@ -79,6 +79,9 @@ typedef struct sphx_params {
     int32_t knn_kth;     /* rank for h_mode 1 (64, self included at rank 0) */
     double av_w_eps;     /* Gadget AV: w = v_ij.r_ij / (r + av_w_eps); 1e-5 in LIBSX:158-169
                             and LIBV2, 1e-15 in augsphx6.5.py PI_ij */
+    double h_min;        /* h_mode 5 only, 0 = none: after the eleventh step an h below
+                            h_min becomes h_min and the iteration stops
+                            (h_smooth_fast_h_minimum_set, LIBSX:1192-1245) */
 } sphx_params;
 
 /* Tabulated cooling: Lambda(T) on increasing T[ntab] (K) in erg cm^3 s^-1,

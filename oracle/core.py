@@ -44,7 +44,7 @@ def lib():
         L.orc_smoothing.argtypes = [i64, dp, dp, ip, dp, C.c_double, i32, i32, i32, i32, lp, i64,
                                     dp, ip, ip, ip]
         L.orc_smoothing_steps.argtypes = [i64, dp, dp, ip, dp, C.c_double, i32, i32, i32, i32,
-                                          i32, lp, i64, dp, ip, ip, ip]
+                                          i32, C.c_double, lp, i64, dp, ip, ip, ip]
         L.orc_neighbors.argtypes = [i64, dp, dp, lp, i64, lp, lp]
         L.orc_density.argtypes = [i64, dp, dp, dp, lp, i64, dp, ip]
         L.orc_eos.argtypes = [i64, dp, dp, C.c_double, dp, dp]
@@ -126,6 +126,7 @@ class Params:
                               # steps of h_smooth_fast (LIBSX:1133-1184)
     knn_kth: int = 64
     av_w_eps: float = 1e-5    # Gadget AV: w = v.r / (r + av_w_eps)
+    h_min: float = 0.0        # h_mode 5: floor on h after eleven steps (LIBSX:1192-1245)
 
     def c(self) -> _Params:
         return _Params(self.gamma, self.alpha, self.beta, self.eta, self.G, self.nngb, self.ndown,
@@ -310,7 +311,7 @@ def smoothing(pos, h_prev, k_prev=None, divV=None, dt=0.0, p: Params = Params(),
     st = np.zeros(nr, np.int32)
     if p.h_mode in (2, 3, 4, 5):
         lib().orc_smoothing_steps(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter,
-                                  p.h_mode, rp, nr, _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
+                                  p.h_mode, p.h_min, rp, nr, _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])
     else:
         lib().orc_smoothing(n, pp, hpp, kpp, dvp, dt, p.nngb, p.ndown, p.nup, p.max_iter, rp, nr,
                             _d(h)[1], _i(k)[1], _i(it)[1], _i(st)[1])

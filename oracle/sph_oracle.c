@@ -219,7 +219,9 @@ void orc_smoothing(int64_t n, const double *pos, const double *h_prev, const int
  * 0.003 h (a step on the current h, two roundings); the loop ends at the
  * first count in [ndown, nup].  The reference has no pass limit (its counter
  * is never acted on); here the loop stops after SMOOTH_H5_PASSES counts with
- * status 1 and the last h.
+ * status 1 and the last h.  With h_min > 0 (h_smooth_fast_h_minimum_set,
+ * LIBSX:1192-1245): once eleven steps are taken, a count made at h < h_min
+ * sets h = h_min and ends the loop, in the band or not.
  */
 #define SMOOTH_H5_PASSES 10000
 static int64_t count_within_sqrt(const double *pos, int64_t n, int64_t i, double s, int strict) {
@@ -234,7 +236,8 @@ static int64_t count_within_sqrt(const double *pos, int64_t n, int64_t i, double
 static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_prev,
                              int32_t k_prev, int use_pred, double divv, double dt, int32_t nngb,
                              int32_t ndown, int32_t nup, int32_t max_iter, int32_t h_mode,
-                             double *h_out, int32_t *k_out, int32_t *it_out, int32_t *st_out) {
+                             double h_min, double *h_out, int32_t *k_out, int32_t *it_out,
+                             int32_t *st_out) {
     const int halving = h_mode == 4, relative = h_mode == 5;
     double s = 2.0 * h_prev;
     if (use_pred && !relative) {
@@ -264,6 +267,8 @@ static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_p
         k = count_within_sqrt(pos, n, i, s, relative);
         if (halving && k > nup) k = nup + 1;  /* the reference stops counting there */
         const int in_band = k >= ndown && k <= nup;
+        /* the steps taken before this count: it (the counter is raised below) */
+        const int floor_h = relative && h_min > 0.0 && it > 10 && 0.5 * s < h_min;
         if (halving && ((k < ndown && kpre > nup) || (k > nup && kpre < ndown))) coef = 0.5 * coef;
         if (relative) {
             const double hc = 0.5 * s;
@@ -278,6 +283,10 @@ static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_p
         ++it;
         /* a count in the band ends the loop even on the last pass: the
          * reference tests the band before its pass counter */
+        if (floor_h) {
+            s = 2.0 * h_min;
+            break;
+        }
         if (in_band) break;
         if (it > limit) {
             if (!halving && !relative) s = smax;
@@ -293,14 +302,15 @@ static void smooth_steps_one(const double *pos, int64_t n, int64_t i, double h_p
 
 void orc_smoothing_steps(int64_t n, const double *pos, const double *h_prev, const int32_t *k_prev,
                          const double *divv, double dt, int32_t nngb, int32_t ndown, int32_t nup,
-                         int32_t max_iter, int32_t h_mode, const int64_t *rows, int64_t nrows,
+                         int32_t max_iter, int32_t h_mode, double h_min, const int64_t *rows,
+                         int64_t nrows,
                          double *h_out, int32_t *k_out, int32_t *it_out, int32_t *st_out) {
     int use_pred = (k_prev != NULL && divv != NULL);
 #pragma omp parallel for schedule(dynamic, 8)
     for (int64_t r = 0; r < nrows; ++r) {
         int64_t i = row_of(rows, r);
         smooth_steps_one(pos, n, i, h_prev[i], use_pred ? k_prev[i] : 0, use_pred,
-                         use_pred ? divv[i] : 0.0, dt, nngb, ndown, nup, max_iter, h_mode,
+                         use_pred ? divv[i] : 0.0, dt, nngb, ndown, nup, max_iter, h_mode, h_min,
                          &h_out[r], &k_out[r], &it_out[r], &st_out[r]);
     }
 }

@@ -34,7 +34,8 @@ class Params(C.Structure):
                 ("eta", C.c_double), ("G", C.c_double), ("nngb", C.c_int32), ("ndown", C.c_int32),
                 ("nup", C.c_int32), ("max_iter", C.c_int32), ("av_form", C.c_int32),
                 ("balsara", C.c_int32), ("eps_from_h", C.c_int32), ("h_predictor", C.c_int32),
-                ("h_mode", C.c_int32), ("knn_kth", C.c_int32), ("av_w_eps", C.c_double)]
+                ("h_mode", C.c_int32), ("knn_kth", C.c_int32), ("av_w_eps", C.c_double),
+                ("h_min", C.c_double)]
 
 
 class Cooling(C.Structure):

@@ -37,6 +37,7 @@ extern "C" void sphx_default_params(sphx_params *p) {
     p->h_mode = 0;
     p->knn_kth = 64;
     p->av_w_eps = 1e-5;
+    p->h_min = 0.0;
 }
 
 namespace {
@@ -618,7 +619,7 @@ static int check_params(const sphx_params *p) {
     if (!p || p->nngb <= 0 || p->ndown > p->nngb || p->nup < p->nngb || p->max_iter <= 0 ||
         !(p->gamma > 1.0) || (p->av_form != 0 && p->av_form != 1) ||
         p->h_mode < 0 || p->h_mode > 5 || p->knn_kth < 0 || p->knn_kth > 4096 ||
-        !(p->av_w_eps >= 0.0)) {
+        !(p->av_w_eps >= 0.0) || !(p->h_min >= 0.0 && p->h_min < INFINITY)) {
         sphx_set_error("invalid sphx_params");
         return SPHX_ERR_ARG;
     }

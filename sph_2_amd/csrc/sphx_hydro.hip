@@ -210,6 +210,7 @@ struct SmoothArgs {
     int64_t *counters;
     int32_t halves;    // 1: a group per bucket; 2: a group per half bucket
     double hstep;      // RULE 2: the step as a fraction of h_prev (2 x the reference's coeff)
+    double hmin;       // RULE 5: the floor on h after eleven steps (0: none)
 };
 
 // RULE 0: the count-band iteration (a log-space Newton step, kept inside the
@@ -348,6 +349,9 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
             if (RULE >= 2) {
                 if (RULE == 4 && kc > a.nup) kc = a.nup + 1;  // LIBV7 stops counting there
                 const bool in_band = kc >= a.ndown && kc <= a.nup;
+                // LIBSX:1226-1228: after the eleventh step an h under the floor is
+                // set to it and ends the iteration, in the band or not
+                const bool floor_h = RULE == 5 && a.hmin > 0.0 && it > 11 && 0.5 * sr < a.hmin;
                 if (RULE == 4 && ((kc < a.ndown && kpre > a.nup) || (kc > a.nup && kpre < a.ndown)))
                     coef = 0.5 * coef;  // a jump across the band
                 if (RULE == 5) {
@@ -361,7 +365,12 @@ __global__ __launch_bounds__(64) void k_smoothing(Tables t, Soa s, SmoothArgs a)
                 if (sr > smax) smax = sr;
                 kpre = kc;
                 kfin = kc;
-                if (in_band) {  // before the pass limit, as the reference tests
+                if (floor_h) {
+                    sr = 2.0 * a.hmin;
+                    kfin = kc;
+                    done = true;
+                    conv = true;
+                } else if (in_band) {  // before the pass limit, as the reference tests
                     done = true;
                     conv = true;
                 } else if (it > (RULE == 4 ? SPHX_H4_PASSES : (RULE == 5 ? SPHX_H5_PASSES : a.max_iter))) {
@@ -1324,7 +1333,7 @@ int Engine::smoothing(hipStream_t st, bool predictor) {
     if (n == 0 || nb_launch() == 0) return SPHX_OK;
     const int halves = nb_launch() <= SMOOTH_HALF_BUCKETS ? 2 : 1;
     SmoothArgs a{p.nngb, p.ndown, p.nup, p.max_iter, (predictor && p.h_predictor) ? 1 : 0,
-                 d_run, h_iters, counters, halves, sphx_h_step(p.h_mode)};
+                 d_run, h_iters, counters, halves, sphx_h_step(p.h_mode), p.h_min};
     if (p.h_mode == 5) k_smoothing<5><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     else if (p.h_mode == 4) k_smoothing<4><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);
     else if (p.h_mode >= 2) k_smoothing<2><<<nb_launch() * halves, 64, 0, st>>>(tview(), s, a);

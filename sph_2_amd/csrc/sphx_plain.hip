@@ -25,7 +25,8 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
     const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
     const double *__restrict__ h_prev, const int32_t *__restrict__ k_prev,
     const double *__restrict__ divv, const double *__restrict__ dt_p, double dt, int32_t nngb,
-    int32_t ndown, int32_t nup, int32_t max_iter, double hstep, int64_t n, int64_t r0, int64_t r1,
+    int32_t ndown, int32_t nup, int32_t max_iter, double hstep, double hmin, int64_t n, int64_t r0,
+    int64_t r1,
     double *__restrict__ h_out, int32_t *__restrict__ k_out, int32_t *__restrict__ it_out,
     int64_t *counters) {
 #pragma clang fp contract(off)
@@ -73,6 +74,8 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
             }
             if (RULE == 4 && k > nup) k = nup + 1;  // LIBV7 stops counting there
             const bool in_band = k >= ndown && k <= nup;
+            // LIBSX:1226-1228: the floor on h after the eleventh step
+            const bool floor_h = RULE == 5 && hmin > 0.0 && it + 1 > 11 && 0.5 * s < hmin;
             if (RULE == 4 && ((k < ndown && kpre > nup) || (k > nup && kpre < ndown)))
                 coef = 0.5 * coef;
             if (RULE == 5) {
@@ -86,6 +89,11 @@ __global__ __launch_bounds__(PT) void kp_smoothing(
             if (s > smax) smax = s;
             kpre = k;
             ++it;
+            if (floor_h) {
+                s = 2.0 * hmin;
+                conv = true;
+                break;
+            }
             if (in_band) {  // before the pass limit, as the reference tests
                 conv = true;
                 break;
@@ -383,22 +391,26 @@ int plain_smoothing(const double *x, const double *y, const double *z, const dou
     if (p.h_mode == 5)
         kp_smoothing<5><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
                                                            p.nngb, p.ndown, p.nup, p.max_iter,
-                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           hstep, p.h_min, n, r0, r1, h_out, k_out,
+                                                           it_out,
                                                            counters);
     else if (p.h_mode == 4)
         kp_smoothing<4><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
                                                            p.nngb, p.ndown, p.nup, p.max_iter,
-                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           hstep, p.h_min, n, r0, r1, h_out, k_out,
+                                                           it_out,
                                                            counters);
     else if (p.h_mode >= 2)
         kp_smoothing<2><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
                                                            p.nngb, p.ndown, p.nup, p.max_iter,
-                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           hstep, p.h_min, n, r0, r1, h_out, k_out,
+                                                           it_out,
                                                            counters);
     else
         kp_smoothing<0><<<grid_rows(r0, r1), PT, 0, st>>>(x, y, z, h_prev, k_prev, divv, dt_p, dt,
                                                            p.nngb, p.ndown, p.nup, p.max_iter,
-                                                           hstep, n, r0, r1, h_out, k_out, it_out,
+                                                           hstep, p.h_min, n, r0, r1, h_out, k_out,
+                                                           it_out,
                                                            counters);
     SPHX_HIP(hipGetLastError());
     return SPHX_OK;

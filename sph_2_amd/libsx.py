@@ -599,6 +599,20 @@ def h_smooth_fast_mpi(nbeg, nend, pos, h, *, mode="mixed", device=0, **pkw):
     return h_smooth_fast(pos, h, mode=mode, device=device, nbeg=nbeg, nend=nend, **pkw)
 
 
+def h_smooth_fast_h_minimum_set(pos, h, minimum_h, *, mode="mixed", device=0, **pkw):
+    """h_smooth_fast_h_minimum_set(pos, h, minimum_h) (LIBSX:1192-1245): the
+    rule of h_smooth_fast, except that once eleven steps are taken, a count
+    made at h < minimum_h sets h = minimum_h and ends the row."""
+    return h_smooth_fast(pos, h, mode=mode, device=device, h_min=float(minimum_h), **pkw)
+
+
+def h_smooth_fast_mpi_min_h_set(nbeg, nend, pos, h, minimum_h, *, mode="mixed", device=0, **pkw):
+    """h_smooth_fast_mpi_min_h_set(nbeg, nend, pos, h, minimum_h)
+    (LIBSX:1314-1370): rows [nbeg, nend) of h_smooth_fast_h_minimum_set."""
+    return h_smooth_fast(pos, h, mode=mode, device=device, nbeg=nbeg, nend=nend,
+                         h_min=float(minimum_h), **pkw)
+
+
 def getAcc_g_smthx(pos, mass, G, epsilon, **kw):
     """getAcc_g_smthx (LIBSX:720-756): the same softened sum as getAcc_g_smth."""
     return getAcc_g_smth(pos, mass, G, epsilon, **kw)

@@ -129,6 +129,22 @@ def test_h_smooth_fast_rule(gpu, evr, mode):
                                               nup=60), ho)
 
 
+@pytest.mark.parametrize("mode", ["mixed", "fp64"])
+def test_h_smooth_fast_floor(gpu, evr, mode):
+    """h_smooth_fast_h_minimum_set and its row-range twin (LIBSX:1192-1370)
+    with a floor at the 30th percentile of the start h, which many rows hit:
+    h bit-exact against the oracle."""
+    from sph_2_amd import libsx
+
+    hmin = float(np.quantile(evr["h0"], 0.3))
+    ho, ko, ito, st = oracle.smoothing(evr["pos"], evr["h0"], p=oracle.Params(h_mode=5, h_min=hmin))
+    assert (ho == hmin).sum() > 50
+    assert np.array_equal(libsx.h_smooth_fast_h_minimum_set(evr["pos"], evr["h0"], hmin, mode=mode),
+                          ho)
+    assert np.array_equal(libsx.h_smooth_fast_mpi_min_h_set(300, 1300, evr["pos"], evr["h0"], hmin,
+                                                            mode=mode), ho[300:1300])
+
+
 @pytest.mark.parametrize("h_mode,tol", [(2, 2.1e-3), (4, 1.1e-2), (5, 6.1e-3)])
 def test_plan_reference_h_rule(gpu, h_mode, tol):
     """A plan with h_mode=2 (the reference driver's smoothing_h rule,

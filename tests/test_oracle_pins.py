@@ -439,6 +439,25 @@ def test_smoothing_relative_rule_counts_strictly():
     assert abs(h[0] - 10.03) < 1e-12 and (k[0], it[0]) == (41, 2)
 
 
+def test_smoothing_relative_rule_floor():
+    """h_smooth_fast_h_minimum_set (LIBSX:1192-1245) on the chain: from h = 19
+    (75 neighbours) h shrinks by 0.3 % a step; without a floor it needs 28
+    steps to reach 2h <= 35 (69 neighbours); with minimum_h = 18.5 the count
+    after the eleventh step is made at 19 x 0.997^11 = 18.38 < 18.5 (73
+    neighbours), so h = 18.5 there, after 12 counts."""
+    pos = _chain()
+    n = len(pos)
+    h, k, it, st = oracle.smoothing(pos, np.full(n, 19.0), p=oracle.Params(h_mode=5), rows=[100])
+    assert abs(h[0] - 19.0 * 0.997 ** 28) < 1e-12 and (k[0], it[0]) == (69, 29)
+    h, k, it, st = oracle.smoothing(pos, np.full(n, 19.0), p=oracle.Params(h_mode=5, h_min=18.5),
+                                    rows=[100])
+    assert h[0] == 18.5 and (k[0], it[0], st[0]) == (73, 12, 0)
+    # a floor the walk never goes under changes nothing
+    h, k, it, st = oracle.smoothing(pos, np.full(n, 19.0), p=oracle.Params(h_mode=5, h_min=17.0),
+                                    rows=[100])
+    assert abs(h[0] - 19.0 * 0.997 ** 28) < 1e-12 and it[0] == 29
+
+
 def test_smoothing_step_rule_walks_the_grid():
     """On an Evrard sphere from the IC guess: every row that stops inside the
     pass limit ends on the grid 2 h0 + m 0.002 h0 (|m| <= 100) with its count
