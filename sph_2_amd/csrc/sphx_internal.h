@@ -486,7 +486,7 @@ int sphx_knn_pass(Engine &E, int kth, hipStream_t st);
 
 // Exact checkpoint of one resident run (sphx_io.hip)
 #define SPHX_CKPT_MAGIC "SPHXRUN1"
-#define SPHX_CKPT_VERSION 1
+#define SPHX_CKPT_VERSION 2  // 2: sphx_params carries h_min
 #define SPHX_CKPT_ND 24  // float64 arrays: every double array of Soa
 #define SPHX_CKPT_NI 2   // int32 arrays: k, id
 struct CkptHeader {
